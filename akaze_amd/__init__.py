@@ -1,0 +1,383 @@
+"""MI355X A-KAZE: Python bindings of libakaze_amd.so (ctypes over the C-ABI).
+
+Mirrors the reference's Python module (python/python_bindings.cpp:27-40):
+AKAZEOptions with setWidth/setHeight, AKAZE with Create_Nonlinear_Scale_Space,
+Feature_Detection and Compute_Descriptors (returning (desc n x 61 uint8,
+kpts n x 7 float32 in [x, y, size, angle, response, octave, class_id] order,
+AKAZE.cpp:290-298), and Matcher.bfmatch_ returning n x 8 float32.
+
+The HIP library must be built (python -m akaze_amd.build); there is no CPU
+fallback: every call runs the HIP kernels or raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libakaze_amd.so")
+
+KP_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("size", "<f4"), ("angle", "<f4"),
+                     ("response", "<f4"), ("octave", "<i4"), ("class_id", "<i4")])
+DM_DTYPE = np.dtype([("queryIdx", "<i4"), ("trainIdx", "<i4"), ("imgIdx", "<i4"), ("distance", "<f4")])
+
+STATUS = {0: "ok", 1: "invalid argument", 2: "HIP runtime error", 3: "out of memory",
+          4: "capacity exceeded, output truncated", 5: "call order violated", 6: "option not implemented",
+          7: "no HIP device"}
+AKZ_ERR_OVERFLOW = 4
+
+
+class AkazeError(RuntimeError):
+    def __init__(self, status, where):
+        super().__init__("%s: %s (status %d)" % (where, STATUS.get(status, "?"), status))
+        self.status = status
+
+
+class COptions(ctypes.Structure):
+    _fields_ = [("omax", ctypes.c_int), ("nsublevels", ctypes.c_int), ("img_width", ctypes.c_int),
+                ("img_height", ctypes.c_int), ("soffset", ctypes.c_float), ("derivative_factor", ctypes.c_float),
+                ("sderivatives", ctypes.c_float), ("diffusivity", ctypes.c_int), ("dthreshold", ctypes.c_float),
+                ("min_dthreshold", ctypes.c_float), ("descriptor", ctypes.c_int), ("descriptor_size", ctypes.c_int),
+                ("descriptor_channels", ctypes.c_int), ("descriptor_pattern_size", ctypes.c_int),
+                ("kcontrast", ctypes.c_float), ("kcontrast_percentile", ctypes.c_float),
+                ("kcontrast_nbins", ctypes.c_int), ("maxkeypoints", ctypes.c_int), ("nms_max_waves", ctypes.c_int),
+                ("verbosity", ctypes.c_int)]
+
+
+class CTiming(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in
+                ("kcontrast", "scale", "derivatives", "detector", "extrema", "subpixel", "descriptor")]
+
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_F = ctypes.c_float
+_SZ = ctypes.c_size_t
+_U64 = ctypes.c_uint64
+_D = ctypes.c_double
+
+# every exported C-ABI symbol with its argument types (include/akaze_mi.h)
+SIGNATURES = {
+    "akz_options_default": [_P],
+    "akz_status_string": [_I],
+    "akz_device_count": [_P],
+    "akz_create": [_P, _P],
+    "akz_destroy": [_P],
+    "akz_num_levels": [_P, _P],
+    "akz_level_size": [_P, _I, _P, _P, _P, _P, _P],
+    "akz_build_scale_space": [_P, _P, _SZ],
+    "akz_build_scale_space_device": [_P, _P, _SZ],
+    "akz_detect": [_P, _P],
+    "akz_describe": [_P, _P, _P, _I, _P],
+    "akz_process_device_async": [_P, _P, _SZ],
+    "akz_wait": [_P],
+    "akz_results_device": [_P, _P, _P, _P],
+    "akz_read_results": [_P, _P, _P, _I, _P],
+    "akz_input_buffer": [_P, _P, _P],
+    "akz_set_graph_mode": [_P, _I],
+    "akz_read_level": [_P, _I, _I, _P],
+    "akz_read_kcontrast": [_P, _I, _P],
+    "akz_read_candidates": [_P, _P, _I, _P, _P, _P],
+    "akz_get_timing": [_P, _P],
+    "akz_matcher_create": [_P],
+    "akz_matcher_destroy": [_P],
+    "akz_match2nn": [_P, _P, _I, _P, _I, _SZ, _P],
+    "akz_match2nn_device": [_P, _P, _I, _P, _I, _P],
+    "akz_matcher_wait": [_P],
+    "akz_matcher_last_ms": [_P, _P],
+    "akz_dev_alloc": [_SZ, _P],
+    "akz_dev_free": [_P],
+    "akz_memcpy_h2d": [_P, _P, _SZ],
+    "akz_memcpy_d2h": [_P, _P, _SZ],
+    "akz_device_synchronize": [],
+    "akz_synth_image": [_U64, _I, _I, _P],
+    "akz_synth_homography": [_U64, _I, _I, _D, _D, _D, _D, _P],
+    "akz_synth_warp": [_P, _I, _I, _P, _D, _U64, _P],
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libakaze_amd.so is not built: run `python -m akaze_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = ctypes.c_char_p if name == "akz_status_string" else ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _check(st, where, allow=()):
+    if st != 0 and st not in allow:
+        raise AkazeError(st, where)
+    return st
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    lib().akz_device_count(ctypes.byref(n))
+    return n.value
+
+
+# ---------------------------------------------------------------- synthetic inputs
+def synth_image(seed, w, h):
+    """Seeded synthetic uint8 image (blobs, shapes, 1/f noise)."""
+    out = np.empty((h, w), np.uint8)
+    _check(lib().akz_synth_image(seed, w, h, out.ctypes.data), "akz_synth_image")
+    return out
+
+
+def synth_homography(seed, w, h, max_rot_deg=30.0, min_scale=0.7, max_scale=1.3, max_persp=1e-4):
+    H = np.empty(9, np.float64)
+    _check(lib().akz_synth_homography(seed, w, h, max_rot_deg, min_scale, max_scale, max_persp, H.ctypes.data),
+           "akz_synth_homography")
+    return H.reshape(3, 3)
+
+
+def synth_warp(img, H, noise_std=0.0, noise_seed=0):
+    img = np.ascontiguousarray(img, np.uint8)
+    Hc = np.ascontiguousarray(H, np.float64)
+    out = np.empty_like(img)
+    _check(lib().akz_synth_warp(img.ctypes.data, img.shape[1], img.shape[0], Hc.ctypes.data, noise_std, noise_seed,
+                                out.ctypes.data), "akz_synth_warp")
+    return out
+
+
+def to_float(img_u8):
+    """uint8 -> float32 in [0, 1] (the reference's input normalisation)."""
+    return img_u8.astype(np.float32) / np.float32(255.0)
+
+
+# ---------------------------------------------------------------- device memory
+class DeviceBuffer:
+    def __init__(self, nbytes):
+        p = ctypes.c_void_p()
+        _check(lib().akz_dev_alloc(nbytes, ctypes.byref(p)), "akz_dev_alloc")
+        self.ptr = p.value
+        self.nbytes = nbytes
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        _check(lib().akz_memcpy_h2d(self.ptr, arr.ctypes.data, arr.nbytes), "akz_memcpy_h2d")
+
+    def download(self, arr):
+        assert arr.flags.c_contiguous and arr.nbytes <= self.nbytes
+        _check(lib().akz_memcpy_d2h(arr.ctypes.data, self.ptr, arr.nbytes), "akz_memcpy_d2h")
+        return arr
+
+    def free(self):
+        if self.ptr and _lib is not None:
+            _lib.akz_dev_free(self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        self.free()
+
+
+# ---------------------------------------------------------------- API mirror
+class AKAZEOptions:
+    """AKAZEOptions (src/lib/AKAZEConfig.h:84-143) with the reference defaults."""
+
+    def __init__(self, **kw):
+        c = COptions()
+        lib().akz_options_default(ctypes.byref(c))
+        self._c = c
+        for k, v in kw.items():
+            setattr(self, k, v)
+
+    def __getattr__(self, k):
+        if k.startswith("_"):
+            raise AttributeError(k)
+        return getattr(self._c, k)
+
+    def __setattr__(self, k, v):
+        if k == "_c":
+            object.__setattr__(self, k, v)
+        else:
+            setattr(self._c, k, v)
+
+    def setWidth(self, w):
+        self._c.img_width = w
+
+    def setHeight(self, h):
+        self._c.img_height = h
+
+
+class AKAZE:
+    """libAKAZECU::AKAZE (src/lib/AKAZE.h:58-143) over the HIP library."""
+
+    def __init__(self, options):
+        self.options = options
+        p = ctypes.c_void_p()
+        _check(lib().akz_create(ctypes.byref(options._c), ctypes.byref(p)), "akz_create")
+        self.ctx = p.value
+        n = ctypes.c_int()
+        lib().akz_num_levels(self.ctx, ctypes.byref(n))
+        self.nlev = n.value
+        self.W, self.H = options.img_width, options.img_height
+        self.nump = 0
+
+    def close(self):
+        if getattr(self, "ctx", None) and _lib is not None:
+            _lib.akz_destroy(self.ctx)
+        self.ctx = None
+
+    def __del__(self):
+        self.close()
+
+    def level_size(self, lvl):
+        w, h, o, s = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        e = ctypes.c_float()
+        _check(lib().akz_level_size(self.ctx, lvl, ctypes.byref(w), ctypes.byref(h), ctypes.byref(o),
+                                    ctypes.byref(s), ctypes.byref(e)), "akz_level_size")
+        return w.value, h.value, o.value, s.value, e.value
+
+    # --- reference API
+    def Create_Nonlinear_Scale_Space(self, img):
+        img = np.ascontiguousarray(img, np.float32)
+        if img.shape != (self.H, self.W):
+            return -1
+        _check(lib().akz_build_scale_space(self.ctx, img.ctypes.data, img.strides[0]), "akz_build_scale_space")
+        return 0
+
+    def Feature_Detection(self):
+        n = ctypes.c_int()
+        st = lib().akz_detect(self.ctx, ctypes.byref(n))
+        _check(st, "akz_detect", allow=(AKZ_ERR_OVERFLOW,))
+        self.nump = n.value
+        self.overflow = st == AKZ_ERR_OVERFLOW
+        return self.nump
+
+    def Compute_Descriptors(self):
+        """Returns (kpts structured array, desc n x 61 uint8)."""
+        cap = max(self.nump, 1)
+        kps = np.zeros(cap, KP_DTYPE)
+        desc = np.zeros((cap, 61), np.uint8)
+        n = ctypes.c_int()
+        st = lib().akz_describe(self.ctx, kps.ctypes.data, desc.ctypes.data, cap, ctypes.byref(n))
+        _check(st, "akz_describe", allow=(AKZ_ERR_OVERFLOW,))
+        m = min(n.value, cap)
+        return kps[:m].copy(), desc[:m].copy()
+
+    def Compute_Descriptors_(self):
+        """Python-binding form (AKAZE.cpp:381-395): (desc, kpts n x 7 float32)."""
+        self.Feature_Detection()
+        kps, desc = self.Compute_Descriptors()
+        mat = np.stack([kps["x"], kps["y"], kps["size"], kps["angle"], kps["response"],
+                        kps["octave"].astype(np.float32), kps["class_id"].astype(np.float32)], axis=1)
+        return desc, mat.astype(np.float32)
+
+    def Get_Computation_Times(self):
+        t = CTiming()
+        _check(lib().akz_get_timing(self.ctx, ctypes.byref(t)), "akz_get_timing")
+        return {n: getattr(t, n) for n, _ in CTiming._fields_}
+
+    # --- debug / parity readback
+    def read_level(self, lvl, which):
+        w, h = self.level_size(lvl)[:2]
+        out = np.empty((h, w), np.float32)
+        _check(lib().akz_read_level(self.ctx, lvl, which, out.ctypes.data), "akz_read_level")
+        return out
+
+    def read_kcontrast(self, lvl):
+        k = ctypes.c_float()
+        _check(lib().akz_read_kcontrast(self.ctx, lvl, ctypes.byref(k)), "akz_read_kcontrast")
+        return np.float32(k.value)
+
+    def read_candidates(self, cap=1 << 20):
+        n = ctypes.c_int()
+        w = ctypes.c_int()
+        ls = np.zeros(self.nlev + 1, np.int32)
+        st = lib().akz_read_candidates(self.ctx, None, 0, ctypes.byref(n), ls.ctypes.data, ctypes.byref(w))
+        _check(st, "akz_read_candidates", allow=(AKZ_ERR_OVERFLOW,))
+        out = np.zeros(max(n.value, 1), KP_DTYPE)
+        _check(lib().akz_read_candidates(self.ctx, out.ctypes.data, n.value, ctypes.byref(n), ls.ctypes.data,
+                                         ctypes.byref(w)), "akz_read_candidates")
+        return out[:n.value], ls, w.value
+
+    # --- throughput path
+    def input_buffer(self):
+        p = ctypes.c_void_p()
+        pitch = ctypes.c_size_t()
+        _check(lib().akz_input_buffer(self.ctx, ctypes.byref(p), ctypes.byref(pitch)), "akz_input_buffer")
+        return p.value, pitch.value
+
+    def set_graph_mode(self, on):
+        _check(lib().akz_set_graph_mode(self.ctx, int(bool(on))), "akz_set_graph_mode")
+
+    def process_device_async(self, d_img, pitch_bytes):
+        _check(lib().akz_process_device_async(self.ctx, d_img, pitch_bytes), "akz_process_device_async")
+
+    def wait(self):
+        _check(lib().akz_wait(self.ctx), "akz_wait")
+
+    def read_results(self, cap):
+        kps = np.zeros(max(cap, 1), KP_DTYPE)
+        desc = np.zeros((max(cap, 1), 61), np.uint8)
+        n = ctypes.c_int()
+        st = lib().akz_read_results(self.ctx, kps.ctypes.data, desc.ctypes.data, cap, ctypes.byref(n))
+        _check(st, "akz_read_results", allow=(AKZ_ERR_OVERFLOW,))
+        m = min(n.value, cap)
+        return kps[:m].copy(), desc[:m].copy(), n.value
+
+    def run(self, img):
+        """Scale space + detection + description of one host image."""
+        if self.Create_Nonlinear_Scale_Space(img) != 0:
+            raise ValueError("image shape does not match the options")
+        self.Feature_Detection()
+        return self.Compute_Descriptors()
+
+
+class Matcher:
+    """libAKAZECU::Matcher (src/lib/AKAZE.h:29-55)."""
+
+    def __init__(self):
+        p = ctypes.c_void_p()
+        _check(lib().akz_matcher_create(ctypes.byref(p)), "akz_matcher_create")
+        self.m = p.value
+
+    def __del__(self):
+        if getattr(self, "m", None) and _lib is not None:
+            _lib.akz_matcher_destroy(self.m)
+        self.m = None
+
+    def match2nn(self, q, t):
+        """2-NN structured array (n_query x 2) of DMatch."""
+        q = np.ascontiguousarray(q, np.uint8)
+        t = np.ascontiguousarray(t, np.uint8)
+        if q.ndim != 2 or t.ndim != 2 or q.shape[1] < 61 or t.shape[1] < 61:
+            raise ValueError("descriptors must be n x 61 uint8")
+        if q.shape[1] != t.shape[1]:
+            t = np.ascontiguousarray(t[:, :61]); q = np.ascontiguousarray(q[:, :61])
+        out = np.zeros(max(2 * len(q), 1), DM_DTYPE)
+        _check(lib().akz_match2nn(self.m, q.ctypes.data, len(q), t.ctypes.data, len(t), q.shape[1],
+                                  out.ctypes.data), "akz_match2nn")
+        return out[:2 * len(q)].reshape(-1, 2)
+
+    def bfmatch_(self, q, t):
+        """n x 8 float32 [q, t1, 0, d1, q, t2, 0, d2] (AKAZE.cpp:82-105)."""
+        m = self.match2nn(q, t)
+        out = np.zeros((len(m), 8), np.float32)
+        for k in range(2):
+            out[:, 4 * k + 0] = m[:, k]["queryIdx"]
+            out[:, 4 * k + 1] = m[:, k]["trainIdx"]
+            out[:, 4 * k + 3] = m[:, k]["distance"]
+        return out
+
+    def match2nn_device(self, d_q64, nq, d_t64, nt, d_out):
+        _check(lib().akz_match2nn_device(self.m, d_q64, nq, d_t64, nt, d_out), "akz_match2nn_device")
+
+    def wait(self):
+        _check(lib().akz_matcher_wait(self.m), "akz_matcher_wait")
+
+    def last_ms(self):
+        ms = ctypes.c_float()
+        _check(lib().akz_matcher_last_ms(self.m, ctypes.byref(ms)), "akz_matcher_last_ms")
+        return ms.value

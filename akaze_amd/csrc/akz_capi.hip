@@ -1,0 +1,816 @@
+// C-ABI of the MI355X A-KAZE library: per-context state, device memory
+// layout and the orchestration of the pipeline stages on the context's
+// stream.  Host code; kernels live in the other translation units.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "akz_internal.h"
+#include "akz_kernels.h"
+
+akz_status akz_hip_fail(hipError_t e, const char* what, const char* file, int line) {
+  if (getenv("AKZ_VERBOSE")) fprintf(stderr, "akaze_amd: %s failed (%s) at %s:%d\n", what, hipGetErrorString(e), file, line);
+  (void)hipGetLastError();
+  return e == hipErrorOutOfMemory ? AKZ_ERR_OOM : (e == hipErrorNoDevice ? AKZ_ERR_NO_DEVICE : AKZ_ERR_HIP);
+}
+
+namespace {
+
+constexpr size_t ALIGN = 256;
+inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+inline int fround(float f) { return (int)(f + 0.5f); }
+
+// FED schedule, the published method of Grewenig et al. as fed.cpp:41-148
+// computes it (number of steps, cosine step sizes, kappa-cycle reordering
+// modulo the next prime above n).
+bool fed_is_prime(int number) {
+  if (number <= 1) return false;
+  if (number == 2 || number == 3 || number == 5 || number == 7) return true;
+  if ((number % 2) == 0 || (number % 3) == 0 || (number % 5) == 0 || (number % 7) == 0) return false;
+  int upper = (int)std::sqrt(number + 1.0);
+  for (int d = 11; d <= upper; d += 2)
+    if (number % d == 0) return false;
+  return true;
+}
+
+int fed_tau(float T, float tau_max, float* tau, int cap) {
+  const float t = T / (float)1;
+  int n = (int)(std::ceil(std::sqrt(3.0 * t / tau_max + 0.25f) - 0.5f - 1.0e-8f) + 0.5f);
+  float scale = 3.0 * t / (tau_max * (float)(n * (n + 1)));
+  if (n <= 0) return 0;
+  if (n > cap) return -1;
+  std::vector<float> tauh(n);
+  float c = 1.0f / (4.0f * (float)n + 2.0f);
+  float d = scale * tau_max / 2.0f;
+  for (int k = 0; k < n; ++k) {
+    float h = (float)std::cos(3.14159265358979323846 * (2.0f * (float)k + 1.0f) * c);
+    tauh[k] = d / (h * h);
+  }
+  if (n == 1) { tau[0] = tauh[0]; return 1; }
+  int kappa = n / 2, prime = n + 1;
+  while (!fed_is_prime(prime)) prime++;
+  for (int k = 0, l = 0; l < n; ++k, ++l) {
+    int index;
+    while ((index = ((k + 1) * kappa) % prime - 1) >= n) k++;
+    tau[l] = tauh[index];
+  }
+  return n;
+}
+
+// Gaussian taps of LowPass (cuda_akaze.cu:175-185).
+void gauss_taps(double var, int R, akz_taps* t) {
+  float sum = 0.0f;
+  for (int j = -R; j <= R; j++) {
+    t->k[j + R] = (float)expf((float)(-(double)j * j / 2.0 / var));
+    sum += t->k[j + R];
+  }
+  for (int j = -R; j <= R; j++) t->k[j + R] /= sum;
+}
+int ksize_radius(int ksize) { return ksize <= 5 ? 2 : (ksize <= 7 ? 3 : (ksize <= 9 ? 4 : 5)); }
+
+struct Level {
+  int w, h, pitch, octave, sublevel;
+  float esigma, etime;
+  int nsteps;
+  float tau[AKZ_MAX_TAU];
+  int deriv_step;
+  float border, det_size;
+  float *Lt, *Lsm, *Lx, *Ly, *det;
+};
+
+enum { EV_START, EV_CONTRAST, EV_SCALE, EV_DERIV, EV_EXTREMA, EV_NMS, EV_DESC, EV_COUNT };
+
+}  // namespace
+
+struct akz_ctx {
+  akz_options opts;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int nlev = 0;
+  Level lev[AKZ_MAX_LEVELS];
+  int octs[AKZ_MAX_LEVELS];
+  void* slab = nullptr;
+  size_t slab_bytes = 0;
+  float* img_in = nullptr;  // input staging, pitch lev[0].pitch
+  float* g = nullptr;       // conductance scratch
+  float* fed_tmp = nullptr; // FED ping-pong scratch
+  uint32_t* maxbits = nullptr;
+  int32_t* hist = nullptr;
+  uint32_t* ticket = nullptr;
+  float* kc = nullptr;
+  int32_t* seg_count = nullptr;
+  int32_t* seg_off = nullptr;
+  uint8_t* seg_slots = nullptr;
+  int32_t* scan_scratch = nullptr;
+  akz_kp* cand = nullptr;
+  int cap = 0;
+  int32_t* level_start = nullptr;
+  int32_t* ncand = nullptr;
+  int32_t* flags = nullptr;
+  int32_t* nb_count = nullptr;
+  int32_t* nb_list = nullptr;
+  int32_t *nms_mem = nullptr, *nms_minnb = nullptr, *nms_code = nullptr, *nms_act = nullptr;
+  akz_kp* kps = nullptr;
+  int32_t* nkps = nullptr;
+  int32_t* waves = nullptr;
+  uint8_t* desc = nullptr;
+  int32_t* h_small = nullptr;  // pinned: [nkps, flags, ncand_total, waves]
+  akz_det_params det{};
+  akz_desc_params dp{};
+  akz_taps taps0{}, taps1{}, taps_c{};
+  int R0 = 4;
+  hipEvent_t ev[EV_COUNT] = {};
+  bool timing_valid = false;
+  akz_timing timing{};
+  int state = 0;  // 0 created, 1 scale space, 2 detected, 3 described
+  bool graph_mode = false;
+  hipGraphExec_t graph_exec = nullptr;
+  hipGraph_t graph = nullptr;
+};
+
+struct akz_matcher {
+  hipStream_t stream = nullptr;
+  uint8_t* q = nullptr;
+  uint8_t* t = nullptr;
+  akz_dmatch* out = nullptr;
+  uint32_t* partial = nullptr;
+  size_t qcap = 0, tcap = 0, pcap = 0, ocap = 0;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+};
+
+namespace {
+
+akz_status compute_geometry(akz_ctx* c) {
+  const akz_options& o = c->opts;
+  // the derivative stencils reflect at offset s <= 4 and need levels wider than that
+  if (o.img_width < 16 || o.img_height < 16 || o.omax < 1 || o.nsublevels < 1) return AKZ_ERR_ARG;
+  if (o.img_width >= 32768 || o.img_height >= 32768) return AKZ_ERR_ARG;
+  if (o.diffusivity < 0 || o.diffusivity > 3) return AKZ_ERR_ARG;
+  if (o.descriptor != 5) return AKZ_ERR_UNSUPPORTED;  // AKAZE.cpp:428-443: only MLDB exists on the GPU
+  if (o.descriptor_size != 0 || o.descriptor_channels != 3) return AKZ_ERR_UNSUPPORTED;
+  int nlev = 0;
+  // level geometry and scales, AKAZE.cpp:151-184
+  for (int i = 0; i <= o.omax - 1; i++) {
+    float rfactor = (float)(1.0 / std::pow(2.0, (double)i));
+    int lh = (int)(o.img_height * rfactor);
+    int lw = (int)(o.img_width * rfactor);
+    if ((lw < 80 || lh < 40) && i != 0) break;
+    for (int j = 0; j < o.nsublevels; j++) {
+      if (nlev >= AKZ_MAX_LEVELS) return AKZ_ERR_ARG;
+      Level& L = c->lev[nlev];
+      L.w = lw;
+      L.h = lh;
+      L.pitch = (int)align_up((size_t)lw, 64);
+      L.octave = i;
+      L.sublevel = j;
+      L.esigma = o.soffset * powf(2.0f, (float)(j) / (float)(o.nsublevels) + i);
+      L.etime = 0.5 * (L.esigma * L.esigma);
+      L.nsteps = 0;
+      nlev++;
+    }
+  }
+  c->nlev = nlev;
+  // FED steps between consecutive levels, AKAZE.cpp:187-197
+  for (int i = 1; i < nlev; i++) {
+    float ttime = c->lev[i].etime - c->lev[i - 1].etime;
+    int n = fed_tau(ttime, 0.25f, c->lev[i].tau, AKZ_MAX_TAU);
+    if (n < 0) return AKZ_ERR_ARG;
+    c->lev[i].nsteps = n;
+  }
+  // detector parameters, AKAZE.cpp:322-364
+  float smax = 10.0 * sqrtf(2.0f);
+  for (int i = 0; i < nlev; i++) {
+    Level& L = c->lev[i];
+    float ratio = powf(2.0f, (float)L.octave);
+    L.deriv_step = fround(L.esigma * o.derivative_factor / ratio);
+    L.det_size = L.esigma * o.derivative_factor;
+    L.border = smax * fround(L.det_size / ratio);
+    c->octs[i] = L.octave;
+  }
+  return AKZ_OK;
+}
+
+struct Carver {
+  size_t off = 0;
+  template <class T>
+  size_t take(size_t count) {
+    size_t o = off;
+    off = align_up(off + sizeof(T) * count, ALIGN);
+    return o;
+  }
+};
+
+akz_status allocate(akz_ctx* c) {
+  Carver cv;
+  size_t o_lev[AKZ_MAX_LEVELS][5];
+  for (int i = 0; i < c->nlev; i++) {
+    size_t n = (size_t)c->lev[i].pitch * c->lev[i].h;
+    for (int k = 0; k < 5; k++) o_lev[i][k] = cv.take<float>(n);
+  }
+  const Level& L0 = c->lev[0];
+  size_t n0 = (size_t)L0.pitch * (L0.h + 1);
+  size_t o_img = cv.take<float>(n0), o_g = cv.take<float>(n0), o_tmp = cv.take<float>(n0);
+  size_t o_max = cv.take<uint32_t>(1), o_hist = cv.take<int32_t>(512), o_ticket = cv.take<uint32_t>(1);
+  size_t o_kc = cv.take<float>(AKZ_MAX_LEVELS);
+  // row segments of 64 pixels, level-major, raster order
+  int seg_total = 0, tiles = 0;
+  for (int i = 0; i < c->nlev; i++) {
+    Level& L = c->lev[i];
+    akz_level_desc& D = c->det.lev[i];
+    D.w = L.w; D.h = L.h; D.pitch = L.pitch; D.octave = L.octave; D.sublevel = L.sublevel;
+    D.deriv_step = L.deriv_step; D.border = L.border; D.det_size = L.det_size;
+    D.nseg_x = (L.w + 63) / 64;
+    D.seg_base = seg_total;
+    seg_total += D.nseg_x * L.h;
+    c->det.tile_start[i] = tiles;
+    tiles += D.nseg_x * ((L.h + 15) / 16);
+  }
+  c->det.tile_start[c->nlev] = tiles;
+  c->det.seg_total = seg_total;
+  c->det.nlev = c->nlev;
+  size_t o_segc = cv.take<int32_t>(seg_total + 4), o_sego = cv.take<int32_t>(seg_total + 4);
+  size_t o_slots = cv.take<uint8_t>((size_t)seg_total * 32);
+  size_t o_scan = cv.take<int32_t>((seg_total + 4095) / 4096 + 2);
+  // candidate capacity: every strict 3x3 maximum (at most one per 2x2 block)
+  size_t capz = 0;
+  for (int i = 0; i < c->nlev; i++) capz += (size_t)((c->lev[i].w + 1) / 2) * ((c->lev[i].h + 1) / 2);
+  if (c->opts.maxkeypoints > 0) capz = std::min(capz, (size_t)c->opts.maxkeypoints);
+  if (capz > (size_t)(1u << 22) - 1) capz = (1u << 22) - 1;
+  c->cap = (int)std::max<size_t>(capz, 1);
+  size_t o_cand = cv.take<akz_kp>(c->cap), o_ls = cv.take<int32_t>(AKZ_MAX_LEVELS + 1);
+  size_t o_nc = cv.take<int32_t>(4), o_flags = cv.take<int32_t>(4);
+  size_t o_nbc = cv.take<int32_t>(c->cap), o_nbl = cv.take<int32_t>((size_t)c->cap * AKZ_NMS_MAX_NB);
+  size_t o_mem = cv.take<int32_t>(c->cap), o_minnb = cv.take<int32_t>(c->cap), o_code = cv.take<int32_t>(c->cap);
+  size_t o_act = cv.take<int32_t>((size_t)c->cap * 2);
+  size_t o_kps = cv.take<akz_kp>(c->cap), o_nk = cv.take<int32_t>(4), o_waves = cv.take<int32_t>(4);
+  size_t o_desc = cv.take<uint8_t>((size_t)c->cap * AKZ_DESC_STRIDE);
+  c->slab_bytes = cv.off;
+  AKZ_HIP_TRY(hipMalloc(&c->slab, c->slab_bytes));
+  char* b = (char*)c->slab;
+  for (int i = 0; i < c->nlev; i++) {
+    Level& L = c->lev[i];
+    L.Lt = (float*)(b + o_lev[i][0]);
+    L.Lsm = (float*)(b + o_lev[i][1]);
+    L.Lx = (float*)(b + o_lev[i][2]);
+    L.Ly = (float*)(b + o_lev[i][3]);
+    L.det = (float*)(b + o_lev[i][4]);
+  }
+  c->img_in = (float*)(b + o_img);
+  c->g = (float*)(b + o_g);
+  c->fed_tmp = (float*)(b + o_tmp);
+  c->maxbits = (uint32_t*)(b + o_max);
+  c->hist = (int32_t*)(b + o_hist);
+  c->ticket = (uint32_t*)(b + o_ticket);
+  c->kc = (float*)(b + o_kc);
+  c->seg_count = (int32_t*)(b + o_segc);
+  c->seg_off = (int32_t*)(b + o_sego);
+  c->seg_slots = (uint8_t*)(b + o_slots);
+  c->scan_scratch = (int32_t*)(b + o_scan);
+  c->cand = (akz_kp*)(b + o_cand);
+  c->level_start = (int32_t*)(b + o_ls);
+  c->ncand = (int32_t*)(b + o_nc);
+  c->flags = (int32_t*)(b + o_flags);
+  c->nb_count = (int32_t*)(b + o_nbc);
+  c->nb_list = (int32_t*)(b + o_nbl);
+  c->nms_mem = (int32_t*)(b + o_mem);
+  c->nms_minnb = (int32_t*)(b + o_minnb);
+  c->nms_code = (int32_t*)(b + o_code);
+  c->nms_act = (int32_t*)(b + o_act);
+  c->kps = (akz_kp*)(b + o_kps);
+  c->nkps = (int32_t*)(b + o_nk);
+  c->waves = (int32_t*)(b + o_waves);
+  c->desc = (uint8_t*)(b + o_desc);
+  AKZ_HIP_TRY(hipMemsetAsync(c->slab, 0, c->slab_bytes, c->stream));
+  AKZ_HIP_TRY(hipHostMalloc((void**)&c->h_small, 64, hipHostMallocDefault));
+  // kernel parameter blocks
+  for (int i = 0; i < c->nlev; i++) {
+    Level& L = c->lev[i];
+    c->det.Lsm[i] = (i == 0) ? L.Lt : L.Lsm;  // level 0: Lsmooth is Lt (AKAZE.cpp:236)
+    c->det.Lx[i] = L.Lx;
+    c->det.Ly[i] = L.Ly;
+    c->det.det[i] = L.det;
+    c->dp.Lt[i] = L.Lt;
+    c->dp.Lx[i] = L.Lx;
+    c->dp.Ly[i] = L.Ly;
+    c->dp.w[i] = L.w;
+    c->dp.h[i] = L.h;
+    c->dp.pitch[i] = L.pitch;
+  }
+  c->dp.patsize = c->opts.descriptor_pattern_size;
+  c->det.thr = std::max(c->opts.dthreshold, c->opts.min_dthreshold);
+  float wgt = 10.0 / 3.0;  // HessianDeterminant, cuda_akaze.cu:706-708
+  c->det.fac1 = 1.0 / (2.0 * (wgt + 2.0));
+  c->det.fac2 = wgt * c->det.fac1;
+  return AKZ_OK;
+}
+
+// Create_Nonlinear_Scale_Space (AKAZE.cpp:211-276) on the context stream.
+void enqueue_scale_space(akz_ctx* c, const float* img, int in_pitch) {
+  hipStream_t s = c->stream;
+  Level& L0 = c->lev[0];
+  hipEventRecord(c->ev[EV_START], s);
+  // contrast factor of the input (ContrastPercentile, cuda_akaze.cu:604-649);
+  // level-0 Lx / Ly serve as blur / gradient scratch until detection
+  akz_launch_contrast_init(s, c->maxbits, c->hist, c->ticket);
+  akz_launch_gauss(s, img, in_pitch, L0.Lx, L0.pitch, L0.w, L0.h, 2, c->taps_c);
+  akz_launch_contrast_grad(s, L0.Lx, L0.pitch, L0.Ly, L0.w, L0.h, c->maxbits);
+  akz_launch_contrast_hist(s, L0.Ly, L0.pitch, L0.w, L0.h, c->maxbits, std::min(c->opts.kcontrast_nbins, 512),
+                           c->opts.kcontrast_percentile, c->hist, c->ticket, c->kc, c->nlev, c->octs);
+  // level 0: Gaussian of soffset; Lsmooth aliases Lt (AKAZE.cpp:234-236)
+  akz_launch_gauss(s, img, in_pitch, L0.Lt, L0.pitch, L0.w, L0.h, c->R0, c->taps0);
+  hipEventRecord(c->ev[EV_CONTRAST], s);
+  for (int i = 1; i < c->nlev; i++) {
+    Level& L = c->lev[i];
+    Level& P = c->lev[i - 1];
+    const int n = L.nsteps;
+    // the FED steps alternate between Lt and the scratch so that the last
+    // one lands in Lt: step 1 writes Lt when n is odd
+    float* first_dst = (n % 2 == 1) ? L.Lt : c->fed_tmp;
+    const float* base;
+    if (L.octave > P.octave) {
+      float* hs = (n % 2 == 1) ? c->fed_tmp : L.Lt;  // must differ from first_dst
+      if (n == 0) hs = L.Lt;
+      akz_launch_halfsample(s, P.Lt, P.w, P.h, P.pitch, hs, L.w, L.h, L.pitch);
+      base = hs;
+    } else {
+      base = P.Lt;  // Copy (AKAZE.cpp:258) is folded into the first step's read
+      if (n == 0) akz_launch_copy2d(s, P.Lt, P.pitch, L.Lt, L.pitch, L.w, L.h);
+    }
+    akz_launch_gauss(s, base, L.pitch, L.Lsm, L.pitch, L.w, L.h, 2, c->taps1);
+    akz_launch_flow(s, L.Lsm, L.pitch, c->g, L.w, L.h, c->opts.diffusivity, c->kc + i);
+    const float* cur = base;
+    float* dst = first_dst;
+    for (int j = 0; j < n; j++) {
+      float half_tau = 0.5 * L.tau[j];
+      akz_launch_fed_step(s, cur, c->g, dst, L.pitch, L.w, L.h, half_tau);
+      cur = dst;
+      dst = (dst == L.Lt) ? c->fed_tmp : L.Lt;
+    }
+  }
+  hipEventRecord(c->ev[EV_SCALE], s);
+}
+
+// Feature_Detection (AKAZE.cpp:316-378) + FilterExtrema (cuda_akaze.cu:1318-1402).
+void enqueue_detect(akz_ctx* c) {
+  hipStream_t s = c->stream;
+  akz_launch_hessian(s, c->det);
+  hipEventRecord(c->ev[EV_DERIV], s);
+  akz_launch_extrema(s, c->det, c->seg_count, c->seg_slots);
+  akz_launch_scan(s, c->det, c->seg_count, c->seg_off, c->level_start, c->ncand, c->cap, c->scan_scratch);
+  akz_launch_scatter(s, c->det, c->seg_count, c->seg_off, c->seg_slots, c->cand, c->cap);
+  hipEventRecord(c->ev[EV_EXTREMA], s);
+  akz_launch_neighbors(s, c->cand, c->ncand, c->level_start, c->nb_count, c->nb_list, c->flags, c->cap);
+  akz_launch_nms(s, c->cand, c->ncand, c->nb_count, c->nb_list, c->nms_mem, c->nms_minnb, c->nms_code, c->nms_act,
+                 c->kps, c->nkps, c->waves, c->opts.nms_max_waves, c->cap);
+  hipEventRecord(c->ev[EV_NMS], s);
+}
+
+// Compute_Descriptors (AKAZE.cpp:405-449): orientation, then MLDB.
+void enqueue_describe(akz_ctx* c) {
+  hipStream_t s = c->stream;
+  akz_launch_orientation(s, c->dp, c->kps, c->nkps, c->cap);
+  akz_launch_mldb(s, c->dp, c->kps, c->nkps, c->desc, c->cap);
+  hipEventRecord(c->ev[EV_DESC], s);
+}
+
+float ev_ms(akz_ctx* c, int a, int b) {
+  float ms = 0.0f;
+  if (hipEventElapsedTime(&ms, c->ev[a], c->ev[b]) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0.0f;
+  }
+  return ms;
+}
+
+// Waits for the stream and reads the small status block: kept count,
+// overflow flags, total candidates, waves.
+akz_status sync_status(akz_ctx* c, int* nk) {
+  AKZ_HIP_TRY(hipMemcpyAsync(c->h_small + 0, c->nkps, 4, hipMemcpyDeviceToHost, c->stream));
+  AKZ_HIP_TRY(hipMemcpyAsync(c->h_small + 1, c->flags, 4, hipMemcpyDeviceToHost, c->stream));
+  AKZ_HIP_TRY(hipMemcpyAsync(c->h_small + 2, c->scan_scratch + (c->det.seg_total + 4095) / 4096, 4,
+                             hipMemcpyDeviceToHost, c->stream));
+  AKZ_HIP_TRY(hipMemcpyAsync(c->h_small + 3, c->waves, 4, hipMemcpyDeviceToHost, c->stream));
+  AKZ_HIP_TRY(hipStreamSynchronize(c->stream));
+  if (nk) *nk = c->h_small[0];
+  if (c->h_small[1] != 0 || c->h_small[2] > c->cap) return AKZ_ERR_OVERFLOW;
+  return AKZ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* akz_status_string(akz_status s) {
+  switch (s) {
+    case AKZ_OK: return "ok";
+    case AKZ_ERR_ARG: return "invalid argument";
+    case AKZ_ERR_HIP: return "HIP runtime error";
+    case AKZ_ERR_OOM: return "out of memory";
+    case AKZ_ERR_OVERFLOW: return "capacity exceeded, output truncated";
+    case AKZ_ERR_STATE: return "call order violated";
+    case AKZ_ERR_UNSUPPORTED: return "option not implemented";
+    case AKZ_ERR_NO_DEVICE: return "no HIP device";
+  }
+  return "unknown status";
+}
+
+akz_status akz_options_default(akz_options* o) {
+  if (!o) return AKZ_ERR_ARG;
+  memset(o, 0, sizeof(*o));
+  o->omax = 4;
+  o->nsublevels = 4;
+  o->soffset = 1.6f;
+  o->derivative_factor = 1.5f;
+  o->sderivatives = 1.0f;
+  o->diffusivity = 1;
+  o->dthreshold = 0.001f;
+  o->min_dthreshold = 0.00001f;
+  o->descriptor = 5;
+  o->descriptor_size = 0;
+  o->descriptor_channels = 3;
+  o->descriptor_pattern_size = 10;
+  o->kcontrast = 0.001f;
+  o->kcontrast_percentile = 0.7f;
+  o->kcontrast_nbins = 300;
+  o->maxkeypoints = 0;
+  o->nms_max_waves = 10000;
+  return AKZ_OK;
+}
+
+akz_status akz_device_count(int* n) {
+  if (!n) return AKZ_ERR_ARG;
+  *n = 0;
+  hipError_t e = hipGetDeviceCount(n);
+  if (e != hipSuccess) { (void)hipGetLastError(); *n = 0; return AKZ_ERR_NO_DEVICE; }
+  return AKZ_OK;
+}
+
+akz_status akz_create(const akz_options* opts, akz_ctx** out) {
+  if (!opts || !out) return AKZ_ERR_ARG;
+  *out = nullptr;
+  akz_ctx* c = new (std::nothrow) akz_ctx();
+  if (!c) return AKZ_ERR_OOM;
+  c->opts = *opts;
+  akz_status st = compute_geometry(c);
+  if (st != AKZ_OK) { delete c; return st; }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    (void)hipGetLastError();
+    delete c;
+    return AKZ_ERR_NO_DEVICE;
+  }
+  if (hipGetDevice(&c->device) != hipSuccess) { delete c; return AKZ_ERR_HIP; }
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return AKZ_ERR_HIP; }
+  for (int i = 0; i < EV_COUNT; i++) hipEventCreate(&c->ev[i]);
+  int ksize = 2 * std::ceil((c->opts.soffset - 0.8) / 0.3) + 3;  // AKAZE.cpp:234-235
+  c->R0 = ksize_radius(ksize);
+  gauss_taps(c->opts.soffset * c->opts.soffset, c->R0, &c->taps0);
+  gauss_taps(1.0, 2, &c->taps1);
+  gauss_taps(1.0f, 2, &c->taps_c);
+  akz_describe_init_tables();
+  st = allocate(c);
+  if (st != AKZ_OK) { akz_destroy(c); return st; }
+  if (hipStreamSynchronize(c->stream) != hipSuccess) { akz_destroy(c); return AKZ_ERR_HIP; }
+  *out = c;
+  return AKZ_OK;
+}
+
+akz_status akz_destroy(akz_ctx* c) {
+  if (!c) return AKZ_ERR_ARG;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
+  if (c->graph) hipGraphDestroy(c->graph);
+  for (int i = 0; i < EV_COUNT; i++)
+    if (c->ev[i]) hipEventDestroy(c->ev[i]);
+  if (c->slab) hipFree(c->slab);
+  if (c->h_small) hipHostFree(c->h_small);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return AKZ_OK;
+}
+
+akz_status akz_num_levels(const akz_ctx* c, int* n) {
+  if (!c || !n) return AKZ_ERR_ARG;
+  *n = c->nlev;
+  return AKZ_OK;
+}
+
+akz_status akz_level_size(const akz_ctx* c, int lvl, int* w, int* h, int* octave, int* sublevel, float* esigma) {
+  if (!c || lvl < 0 || lvl >= c->nlev) return AKZ_ERR_ARG;
+  const Level& L = c->lev[lvl];
+  if (w) *w = L.w;
+  if (h) *h = L.h;
+  if (octave) *octave = L.octave;
+  if (sublevel) *sublevel = L.sublevel;
+  if (esigma) *esigma = L.esigma;
+  return AKZ_OK;
+}
+
+akz_status akz_build_scale_space(akz_ctx* c, const float* img, size_t pitch_bytes) {
+  if (!c || !img) return AKZ_ERR_ARG;
+  const Level& L0 = c->lev[0];
+  size_t sp = pitch_bytes ? pitch_bytes : sizeof(float) * L0.w;
+  if (sp < sizeof(float) * L0.w) return AKZ_ERR_ARG;
+  AKZ_HIP_TRY(hipMemcpy2DAsync(c->img_in, sizeof(float) * L0.pitch, img, sp, sizeof(float) * L0.w, L0.h,
+                               hipMemcpyHostToDevice, c->stream));
+  enqueue_scale_space(c, c->img_in, L0.pitch);
+  AKZ_HIP_TRY(hipGetLastError());
+  c->state = 1;
+  c->timing_valid = false;
+  return AKZ_OK;
+}
+
+akz_status akz_build_scale_space_device(akz_ctx* c, const float* d_img, size_t pitch_bytes) {
+  if (!c || !d_img) return AKZ_ERR_ARG;
+  const Level& L0 = c->lev[0];
+  size_t sp = pitch_bytes ? pitch_bytes : sizeof(float) * L0.w;
+  if (sp < sizeof(float) * L0.w || sp % sizeof(float)) return AKZ_ERR_ARG;
+  enqueue_scale_space(c, d_img, (int)(sp / sizeof(float)));
+  AKZ_HIP_TRY(hipGetLastError());
+  c->state = 1;
+  c->timing_valid = false;
+  return AKZ_OK;
+}
+
+akz_status akz_detect(akz_ctx* c, int* n_out) {
+  if (!c) return AKZ_ERR_ARG;
+  if (c->state < 1) return AKZ_ERR_STATE;
+  enqueue_detect(c);
+  AKZ_HIP_TRY(hipGetLastError());
+  int nk = 0;
+  akz_status st = sync_status(c, &nk);
+  if (n_out) *n_out = nk;
+  c->state = 2;
+  return st;
+}
+
+akz_status akz_describe(akz_ctx* c, akz_keypoint* kps, uint8_t* desc, int cap, int* n_out) {
+  if (!c || cap < 0) return AKZ_ERR_ARG;
+  if (c->state < 2) return AKZ_ERR_STATE;
+  enqueue_describe(c);
+  AKZ_HIP_TRY(hipGetLastError());
+  c->state = 3;
+  return akz_read_results(c, kps, desc, cap, n_out);
+}
+
+akz_status akz_read_results(akz_ctx* c, akz_keypoint* kps, uint8_t* desc, int cap, int* n_out) {
+  if (!c || cap < 0) return AKZ_ERR_ARG;
+  int nk = 0;
+  akz_status st = sync_status(c, &nk);
+  if (st != AKZ_OK && st != AKZ_ERR_OVERFLOW) return st;
+  int m = std::min(nk, cap);
+  if (kps && m > 0) AKZ_HIP_TRY(hipMemcpy(kps, c->kps, sizeof(akz_kp) * m, hipMemcpyDeviceToHost));
+  if (desc && m > 0)
+    AKZ_HIP_TRY(hipMemcpy2D(desc, 61, c->desc, AKZ_DESC_STRIDE, 61, m, hipMemcpyDeviceToHost));
+  if (n_out) *n_out = nk;
+  return (nk > cap) ? AKZ_ERR_OVERFLOW : st;
+}
+
+akz_status akz_process_device_async(akz_ctx* c, const float* d_img, size_t pitch_bytes) {
+  if (!c || !d_img) return AKZ_ERR_ARG;
+  const Level& L0 = c->lev[0];
+  size_t sp = pitch_bytes ? pitch_bytes : sizeof(float) * L0.w;
+  if (sp < sizeof(float) * L0.w || sp % sizeof(float)) return AKZ_ERR_ARG;
+  if (!c->graph_mode) {
+    enqueue_scale_space(c, d_img, (int)(sp / sizeof(float)));
+    enqueue_detect(c);
+    enqueue_describe(c);
+    AKZ_HIP_TRY(hipGetLastError());
+  } else {
+    // the captured graph reads the context's own input buffer
+    if (d_img != c->img_in || (int)(sp / sizeof(float)) != L0.pitch)
+      AKZ_HIP_TRY(hipMemcpy2DAsync(c->img_in, sizeof(float) * L0.pitch, d_img, sp, sizeof(float) * L0.w, L0.h,
+                                   hipMemcpyDeviceToDevice, c->stream));
+    if (!c->graph_exec) {
+      AKZ_HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      enqueue_scale_space(c, c->img_in, L0.pitch);
+      enqueue_detect(c);
+      enqueue_describe(c);
+      hipError_t e = hipStreamEndCapture(c->stream, &c->graph);
+      if (e != hipSuccess) return akz_hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
+      AKZ_HIP_TRY(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
+    }
+    AKZ_HIP_TRY(hipGraphLaunch(c->graph_exec, c->stream));
+  }
+  c->state = 3;
+  c->timing_valid = false;
+  return AKZ_OK;
+}
+
+akz_status akz_input_buffer(akz_ctx* c, float** d_ptr, size_t* pitch_bytes) {
+  if (!c || !d_ptr) return AKZ_ERR_ARG;
+  *d_ptr = c->img_in;
+  if (pitch_bytes) *pitch_bytes = sizeof(float) * c->lev[0].pitch;
+  return AKZ_OK;
+}
+
+akz_status akz_set_graph_mode(akz_ctx* c, int enabled) {
+  if (!c) return AKZ_ERR_ARG;
+  c->graph_mode = enabled != 0;
+  return AKZ_OK;
+}
+
+akz_status akz_wait(akz_ctx* c) {
+  if (!c) return AKZ_ERR_ARG;
+  AKZ_HIP_TRY(hipStreamSynchronize(c->stream));
+  return AKZ_OK;
+}
+
+akz_status akz_results_device(akz_ctx* c, const int32_t** d_count, const akz_keypoint** d_kps,
+                              const uint8_t** d_desc64) {
+  if (!c) return AKZ_ERR_ARG;
+  if (d_count) *d_count = c->nkps;
+  if (d_kps) *d_kps = reinterpret_cast<const akz_keypoint*>(c->kps);
+  if (d_desc64) *d_desc64 = c->desc;
+  return AKZ_OK;
+}
+
+akz_status akz_read_level(akz_ctx* c, int lvl, int which, float* dst) {
+  if (!c || !dst || lvl < 0 || lvl >= c->nlev || which < 0 || which > 4) return AKZ_ERR_ARG;
+  AKZ_HIP_TRY(hipStreamSynchronize(c->stream));
+  const Level& L = c->lev[lvl];
+  const float* src = which == 0 ? L.Lt : which == 1 ? (lvl == 0 ? L.Lt : L.Lsm) : which == 2 ? L.Lx
+                     : which == 3 ? L.Ly : L.det;
+  AKZ_HIP_TRY(hipMemcpy2D(dst, sizeof(float) * L.w, src, sizeof(float) * L.pitch, sizeof(float) * L.w, L.h,
+                          hipMemcpyDeviceToHost));
+  return AKZ_OK;
+}
+
+akz_status akz_read_kcontrast(akz_ctx* c, int lvl, float* k) {
+  if (!c || !k || lvl < 0 || lvl >= c->nlev) return AKZ_ERR_ARG;
+  AKZ_HIP_TRY(hipStreamSynchronize(c->stream));
+  AKZ_HIP_TRY(hipMemcpy(k, c->kc + lvl, sizeof(float), hipMemcpyDeviceToHost));
+  return AKZ_OK;
+}
+
+akz_status akz_read_candidates(akz_ctx* c, akz_keypoint* cand, int cap, int* n_out, int32_t* level_start,
+                               int* waves) {
+  if (!c || cap < 0) return AKZ_ERR_ARG;
+  if (c->state < 2) return AKZ_ERR_STATE;
+  AKZ_HIP_TRY(hipStreamSynchronize(c->stream));
+  int n = 0, w = 0;
+  AKZ_HIP_TRY(hipMemcpy(&n, c->ncand, 4, hipMemcpyDeviceToHost));
+  AKZ_HIP_TRY(hipMemcpy(&w, c->waves, 4, hipMemcpyDeviceToHost));
+  int m = std::min(n, cap);
+  if (cand && m > 0) AKZ_HIP_TRY(hipMemcpy(cand, c->cand, sizeof(akz_kp) * m, hipMemcpyDeviceToHost));
+  if (level_start) AKZ_HIP_TRY(hipMemcpy(level_start, c->level_start, 4 * (c->nlev + 1), hipMemcpyDeviceToHost));
+  if (n_out) *n_out = n;
+  if (waves) *waves = w;
+  return n > cap ? AKZ_ERR_OVERFLOW : AKZ_OK;
+}
+
+akz_status akz_get_timing(akz_ctx* c, akz_timing* t) {
+  if (!c || !t) return AKZ_ERR_ARG;
+  AKZ_HIP_TRY(hipStreamSynchronize(c->stream));
+  akz_timing r{};
+  if (c->state >= 1) {
+    r.kcontrast = ev_ms(c, EV_START, EV_CONTRAST);
+    r.scale = ev_ms(c, EV_START, EV_SCALE);
+  }
+  if (c->state >= 2) {
+    r.derivatives = ev_ms(c, EV_SCALE, EV_DERIV);
+    r.extrema = ev_ms(c, EV_DERIV, EV_NMS);
+    r.detector = ev_ms(c, EV_SCALE, EV_NMS);
+    r.subpixel = 0.0;  // folded into the extrema kernels, as in the reference
+  }
+  if (c->state >= 3) r.descriptor = ev_ms(c, EV_NMS, EV_DESC);
+  *t = r;
+  return AKZ_OK;
+}
+
+// ---- matcher ----
+akz_status akz_matcher_create(akz_matcher** out) {
+  if (!out) return AKZ_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) { (void)hipGetLastError(); return AKZ_ERR_NO_DEVICE; }
+  akz_matcher* m = new (std::nothrow) akz_matcher();
+  if (!m) return AKZ_ERR_OOM;
+  if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess) { delete m; return AKZ_ERR_HIP; }
+  hipEventCreate(&m->e0);
+  hipEventCreate(&m->e1);
+  *out = m;
+  return AKZ_OK;
+}
+
+akz_status akz_matcher_destroy(akz_matcher* m) {
+  if (!m) return AKZ_ERR_ARG;
+  hipStreamSynchronize(m->stream);
+  if (m->q) hipFree(m->q);
+  if (m->t) hipFree(m->t);
+  if (m->out) hipFree(m->out);
+  if (m->partial) hipFree(m->partial);
+  if (m->e0) hipEventDestroy(m->e0);
+  if (m->e1) hipEventDestroy(m->e1);
+  hipStreamDestroy(m->stream);
+  delete m;
+  return AKZ_OK;
+}
+
+static akz_status grow(void** p, size_t* cap, size_t need) {
+  if (need <= *cap) return AKZ_OK;
+  if (*p) hipFree(*p);
+  *p = nullptr;
+  *cap = 0;
+  AKZ_HIP_TRY(hipMalloc(p, need));
+  *cap = need;
+  return AKZ_OK;
+}
+
+akz_status akz_match2nn_device(akz_matcher* m, const uint8_t* d_q64, int nq, const uint8_t* d_t64, int nt,
+                               akz_dmatch* d_out) {
+  if (!m || nq < 0 || nt < 0 || (nq > 0 && (!d_q64 || !d_out)) || (nt > 0 && !d_t64)) return AKZ_ERR_ARG;
+  if (nt >= (1 << 22)) return AKZ_ERR_UNSUPPORTED;
+  if (nq == 0) return AKZ_OK;
+  int ns = akz_match_splits(nq, nt);
+  akz_status st = grow((void**)&m->partial, &m->pcap, sizeof(uint32_t) * 2 * (size_t)nq * ns);
+  if (st != AKZ_OK) return st;
+  hipEventRecord(m->e0, m->stream);
+  akz_launch_match(m->stream, d_q64, nq, d_t64, nt, m->partial, ns);
+  akz_launch_match_merge(m->stream, m->partial, nq, ns, d_out);
+  hipEventRecord(m->e1, m->stream);
+  AKZ_HIP_TRY(hipGetLastError());
+  return AKZ_OK;
+}
+
+akz_status akz_match2nn(akz_matcher* m, const uint8_t* q, int nq, const uint8_t* t, int nt, size_t stride,
+                        akz_dmatch* out) {
+  if (!m || nq < 0 || nt < 0 || stride < 61 || (nq > 0 && (!q || !out)) || (nt > 0 && !t)) return AKZ_ERR_ARG;
+  if (nq == 0) return AKZ_OK;
+  akz_status st;
+  if ((st = grow((void**)&m->q, &m->qcap, (size_t)nq * 64)) != AKZ_OK) return st;
+  if ((st = grow((void**)&m->t, &m->tcap, (size_t)std::max(nt, 1) * 64)) != AKZ_OK) return st;
+  if ((st = grow((void**)&m->out, &m->ocap, sizeof(akz_dmatch) * 2 * (size_t)nq)) != AKZ_OK) return st;
+  // 61-byte rows into zero-padded 64-byte rows (AKAZE.cpp:39-60)
+  AKZ_HIP_TRY(hipMemsetAsync(m->q, 0, (size_t)nq * 64, m->stream));
+  AKZ_HIP_TRY(hipMemcpy2DAsync(m->q, 64, q, stride, 61, nq, hipMemcpyHostToDevice, m->stream));
+  if (nt > 0) {
+    AKZ_HIP_TRY(hipMemsetAsync(m->t, 0, (size_t)nt * 64, m->stream));
+    AKZ_HIP_TRY(hipMemcpy2DAsync(m->t, 64, t, stride, 61, nt, hipMemcpyHostToDevice, m->stream));
+  }
+  st = akz_match2nn_device(m, m->q, nq, m->t, nt, m->out);
+  if (st != AKZ_OK) return st;
+  AKZ_HIP_TRY(hipMemcpyAsync(out, m->out, sizeof(akz_dmatch) * 2 * (size_t)nq, hipMemcpyDeviceToHost, m->stream));
+  AKZ_HIP_TRY(hipStreamSynchronize(m->stream));
+  return AKZ_OK;
+}
+
+akz_status akz_matcher_wait(akz_matcher* m) {
+  if (!m) return AKZ_ERR_ARG;
+  AKZ_HIP_TRY(hipStreamSynchronize(m->stream));
+  return AKZ_OK;
+}
+
+akz_status akz_matcher_last_ms(akz_matcher* m, float* ms) {
+  if (!m || !ms) return AKZ_ERR_ARG;
+  AKZ_HIP_TRY(hipEventSynchronize(m->e1));
+  AKZ_HIP_TRY(hipEventElapsedTime(ms, m->e0, m->e1));
+  return AKZ_OK;
+}
+
+// ---- device memory helpers ----
+akz_status akz_dev_alloc(size_t bytes, void** ptr) {
+  if (!ptr) return AKZ_ERR_ARG;
+  AKZ_HIP_TRY(hipMalloc(ptr, bytes ? bytes : 1));
+  return AKZ_OK;
+}
+akz_status akz_dev_free(void* ptr) {
+  if (ptr) AKZ_HIP_TRY(hipFree(ptr));
+  return AKZ_OK;
+}
+akz_status akz_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+  AKZ_HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return AKZ_OK;
+}
+akz_status akz_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+  AKZ_HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return AKZ_OK;
+}
+akz_status akz_device_synchronize(void) {
+  AKZ_HIP_TRY(hipDeviceSynchronize());
+  return AKZ_OK;
+}
+
+// ---- synthetic inputs ----
+akz_status akz_synth_image(uint64_t seed, int w, int h, uint8_t* out) {
+  if (!out || w <= 0 || h <= 0) return AKZ_ERR_ARG;
+  akz_synth_render(seed, w, h, out);
+  return AKZ_OK;
+}
+akz_status akz_synth_homography(uint64_t seed, int w, int h, double max_rot_deg, double min_scale, double max_scale,
+                                double max_persp, double* H9) {
+  if (!H9 || w <= 0 || h <= 0) return AKZ_ERR_ARG;
+  akz_synth_homography_impl(seed, w, h, max_rot_deg, min_scale, max_scale, max_persp, H9);
+  return AKZ_OK;
+}
+akz_status akz_synth_warp(const uint8_t* src, int w, int h, const double* H9, double noise_std, uint64_t noise_seed,
+                          uint8_t* out) {
+  if (!src || !out || !H9 || w <= 0 || h <= 0) return AKZ_ERR_ARG;
+  return akz_synth_warp_u8(src, w, h, H9, noise_std, noise_seed, out) == 0 ? AKZ_OK : AKZ_ERR_ARG;
+}
+
+}  // extern "C"

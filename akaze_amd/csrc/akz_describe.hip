@@ -1,0 +1,234 @@
+// Orientation and MLDB descriptor kernels: one wavefront per keypoint.
+//
+// Both kernels are persistent-style (grid-stride over the keypoint count read
+// from device memory), so they are launched with a fixed grid and need no host
+// knowledge of how many keypoints the detector kept.
+#include "akz_kernels.h"
+
+namespace {
+
+// MLDB comparison pairs of InitCompareIndices (cuda_akaze.cu:2034-2111): for
+// the 2x2, 3x3 and 4x4 grids in turn, for channel 0, 1, 2, every cell pair
+// j < i compares vals[3j+ch] > vals[3i+ch]; 6+36+120 pairs x 3 = 486 bits.
+struct CmpTables {
+  int16_t a[488], b[488];
+};
+
+constexpr CmpTables make_cmp_tables() {
+  CmpTables t{};
+  int c = 0;
+  const int lo[3] = {0, 4, 13}, hi[3] = {4, 13, 29};
+  for (int g = 0; g < 3; g++)
+    for (int ch = 0; ch < 3; ch++)
+      for (int j = lo[g]; j < hi[g] - 1; j++)
+        for (int i = j + 1; i < hi[g]; i++) {
+          t.a[c] = (int16_t)(3 * j + ch);
+          t.b[c] = (int16_t)(3 * i + ch);
+          c++;
+        }
+  for (; c < 488; c++) t.a[c] = t.b[c] = 0;
+  return t;
+}
+
+__constant__ CmpTables d_cmp = make_cmp_tables();
+
+// exp(-r2 / 12.5) for r2 = 0..35 (cuda_akaze.cu:2133), rounded once on the host
+struct OrientWeights { float w[36]; };
+
+constexpr int OR_WAVES = 4;
+
+// Dominant orientation (FindOrientation, cuda_akaze.cu:2114-2167).  The 109
+// samples of the radius-6 disc are distributed over the lanes; their bin and
+// weighted derivatives go to LDS, then lane b (b < 42) sums bin b in sample
+// order, so the float sums do not depend on scheduling.
+__global__ __launch_bounds__(64 * OR_WAVES) void k_orientation(akz_desc_params p, akz_kp* __restrict__ kps,
+                                                               const int32_t* nkps_p, OrientWeights wt) {
+  __shared__ int s_bin[OR_WAVES][208];
+  __shared__ float s_dx[OR_WAVES][208], s_dy[OR_WAVES][208];
+  __shared__ float s_rx[OR_WAVES][42], s_ry[OR_WAVES][42];
+  __shared__ float s_ex[OR_WAVES][42], s_ey[OR_WAVES][42];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int n = *nkps_p;
+  // the loop bound is uniform over the block so every barrier is reached by all waves
+  for (int base = blockIdx.x * OR_WAVES; base < n; base += gridDim.x * OR_WAVES) {
+    const int kp = base + wv;
+    const bool active = kp < n;
+    if (active) {
+      akz_kp k = kps[kp];
+      const int lev = k.class_id;
+      const float* dxd = p.Lx[lev];
+      const float* dyd = p.Ly[lev];
+      const int pitch = p.pitch[lev], w = p.w[lev], h = p.h[lev];
+      const int octave = k.octave;
+      const int step = (int)(0.5f * k.size + 0.5f) >> octave;
+      const int x = (int)(k.x + 0.5f) >> octave;
+      const int y = (int)(k.y + 0.5f) >> octave;
+      for (int tx = lane; tx < 208; tx += 64) {
+        int i = (tx & 15) - 6, j = (tx / 16) - 6;
+        int r2 = i * i + j * j;
+        int bin = -1;
+        float dx = 0.0f, dy = 0.0f;
+        if (r2 < 36) {
+          float gw = wt.w[r2];
+          int xs = akz_iclamp(x + step * i, 0, w - 1), ys = akz_iclamp(y + step * j, 0, h - 1);
+          size_t pos = (size_t)ys * pitch + xs;
+          dx = gw * dxd[pos];
+          dy = gw * dyd[pos];
+          float angle = akz_atan2f(dy, dx);
+          bin = akz_imax(akz_imin((int)(angle * (21 / 3.14159265358979323846)) + 21, 41), 0);
+        }
+        s_bin[wv][tx] = bin;
+        s_dx[wv][tx] = dx;
+        s_dy[wv][tx] = dy;
+      }
+    }
+    __syncthreads();
+    if (active && lane < 42) {
+      float rx = 0.0f, ry = 0.0f;
+      for (int tx = 0; tx < 208; tx++)
+        if (s_bin[wv][tx] == lane) { rx += s_dx[wv][tx]; ry += s_dy[wv][tx]; }
+      s_rx[wv][lane] = rx;
+      s_ry[wv][lane] = ry;
+    }
+    __syncthreads();
+    if (active && lane < 42) {
+      float ex = s_rx[wv][lane], ey = s_ry[wv][lane];
+      for (int q = lane + 1; q < lane + 7; q++) {
+        ex += s_rx[wv][q < 42 ? q : q - 42];
+        ey += s_ry[wv][q < 42 ? q : q - 42];
+      }
+      s_ex[wv][lane] = ex;
+      s_ey[wv][lane] = ey;
+    }
+    __syncthreads();
+    if (active && lane == 0) {
+      float maxr = 0.0f;
+      int maxk = 0;
+      for (int q = 0; q < 42; q++) {
+        float r = s_ex[wv][q] * s_ex[wv][q] + s_ey[wv][q] * s_ey[wv][q];
+        if (r > maxr) { maxr = r; maxk = q; }
+      }
+      float angle = akz_atan2f(s_ey[wv][maxk], s_ex[wv][maxk]);
+      kps[kp].angle = (angle < 0.0f ? angle + 2.0f * 3.14159265358979323846 : angle);
+    }
+    __syncthreads();
+  }
+}
+
+// MLDB descriptor (ExtractDescriptors + BuildDescriptor, cuda_akaze.cu:
+// 1420-1533, 1786-1807).  Lane l accumulates the samples i = l, l+64, ... of
+// the 21x21 rotated grid into its own 87 cell sums (LDS, value-major so the
+// lanes hit distinct banks); the 64 partials of each value are then combined
+// as the reference does: p[t] + p[t+32], then a pairwise tree over 32.
+constexpr int MLDB_WAVES = 1;
+
+__global__ __launch_bounds__(64 * MLDB_WAVES) void k_mldb(akz_desc_params p, const akz_kp* __restrict__ kps,
+                                                          const int32_t* nkps_p, uint8_t* __restrict__ desc) {
+  __shared__ float acc[87 * 64];
+  __shared__ float vals[88];
+  const int lane = threadIdx.x & 63;
+  const int n = *nkps_p;
+  const int size2 = p.patsize;
+  const int size3 = (int)ceilf(2.0f * p.patsize / 3.0f);
+  const int size4 = (int)ceilf(0.5f * p.patsize);
+  const int winsize = akz_imax(3 * size3, 4 * size4);
+  for (int kp = blockIdx.x; kp < n; kp += gridDim.x) {
+    const akz_kp k = kps[kp];
+    const int lev = k.class_id;
+    const float* imd = p.Lt[lev];
+    const float* dxd = p.Lx[lev];
+    const float* dyd = p.Ly[lev];
+    const int pitch = p.pitch[lev], w = p.w[lev], h = p.h[lev];
+    const float iratio = 1.0f / (1 << k.octave);
+    const int scale = (int)(0.5f * k.size * iratio + 0.5f);
+    const float xf = k.x * iratio;
+    const float yf = k.y * iratio;
+    float co, si;
+    akz_sincosf(k.angle, &si, &co);
+    for (int v = 0; v < 87; v++) acc[v * 64 + lane] = 0.0f;
+    for (int i = lane; i < winsize * winsize; i += 64) {
+      int y = i / winsize;
+      int x = i - winsize * y;
+      int m = akz_imax(x, y);
+      int l = x - size2;
+      int kk = y - size2;
+      int xp = (int)(xf + scale * (kk * co - l * si) + 0.5f);
+      int yp = (int)(yf + scale * (kk * si + l * co) + 0.5f);
+      xp = akz_iclamp(xp, 0, w - 1);
+      yp = akz_iclamp(yp, 0, h - 1);
+      size_t pos = (size_t)yp * pitch + xp;
+      float im = imd[pos], dx = dxd[pos], dy = dyd[pos];
+      float rx = -dx * si + dy * co;
+      float ry = dx * co + dy * si;
+      if (m < 2 * size2) {
+        int x2 = (x < size2 ? 0 : 1), y2 = (y < size2 ? 0 : 1);
+        int c = 3 * (y2 * 2 + x2);
+        acc[(c + 0) * 64 + lane] += im;
+        acc[(c + 1) * 64 + lane] += rx;
+        acc[(c + 2) * 64 + lane] += ry;
+      }
+      if (m < 3 * size3) {
+        int x3 = (x < size3 ? 0 : (x < 2 * size3 ? 1 : 2));
+        int y3 = (y < size3 ? 0 : (y < 2 * size3 ? 1 : 2));
+        int c = 3 * (4 + y3 * 3 + x3);
+        acc[(c + 0) * 64 + lane] += im;
+        acc[(c + 1) * 64 + lane] += rx;
+        acc[(c + 2) * 64 + lane] += ry;
+      }
+      if (m < 4 * size4) {
+        int x4 = (x < 2 * size4 ? (x < size4 ? 0 : 1) : (x < 3 * size4 ? 2 : 3));
+        int y4 = (y < 2 * size4 ? (y < size4 ? 0 : 1) : (y < 3 * size4 ? 2 : 3));
+        int c = 3 * (13 + y4 * 4 + x4);
+        acc[(c + 0) * 64 + lane] += im;
+        acc[(c + 1) * 64 + lane] += rx;
+        acc[(c + 2) * 64 + lane] += ry;
+      }
+    }
+    __syncthreads();
+    for (int v = lane; v < 87; v += 64) {
+      float a[32];
+#pragma unroll
+      for (int t = 0; t < 32; t++) a[t] = acc[v * 64 + t] + acc[v * 64 + t + 32];
+#pragma unroll
+      for (int s = 1; s < 32; s <<= 1)
+#pragma unroll
+        for (int t = 0; t + s < 32; t += 2 * s) a[t] = a[t] + a[t + s];
+      vals[v] = a[0];
+    }
+    __syncthreads();
+    uint8_t byte = 0;
+    if (lane < 61) {
+      const int nb = (lane == 60 ? 6 : 8);
+      for (int b = 0; b < nb; b++) {
+        int c = lane * 8 + b;
+        byte |= (uint8_t)((vals[d_cmp.a[c]] > vals[d_cmp.b[c]] ? 1 : 0) << b);
+      }
+    }
+    desc[(size_t)kp * AKZ_DESC_STRIDE + lane] = byte;
+    __syncthreads();
+  }
+}
+
+OrientWeights g_weights;
+bool g_weights_ready = false;
+
+}  // namespace
+
+void akz_describe_init_tables() {
+  if (g_weights_ready) return;
+  for (int r2 = 0; r2 < 36; r2++)
+    g_weights.w[r2] = (float)exp((double)((float)(-r2) / (2.5f * 2.5f * 2.0f)));
+  g_weights_ready = true;
+}
+
+void akz_launch_orientation(hipStream_t s, const akz_desc_params& p, akz_kp* kps, const int32_t* nkps, int cap) {
+  int blocks = (cap + OR_WAVES - 1) / OR_WAVES;
+  blocks = blocks < 2048 ? (blocks > 0 ? blocks : 1) : 2048;
+  hipLaunchKernelGGL(k_orientation, dim3(blocks), dim3(64 * OR_WAVES), 0, s, p, kps, nkps, g_weights);
+}
+
+void akz_launch_mldb(hipStream_t s, const akz_desc_params& p, const akz_kp* kps, const int32_t* nkps, uint8_t* desc64,
+                     int cap) {
+  int blocks = cap < 4096 ? (cap > 0 ? cap : 1) : 4096;
+  hipLaunchKernelGGL(k_mldb, dim3(blocks), dim3(64 * MLDB_WAVES), 0, s, p, kps, nkps, desc64);
+}

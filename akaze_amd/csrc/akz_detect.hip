@@ -1,0 +1,508 @@
+// Detector kernels: Hessian response, in-level extrema with subpixel fit,
+// ordered compaction, neighbour search and wave-synchronous suppression.
+//
+// All levels are handled by one launch per stage: blocks map to (level, tile)
+// through the prefix table in akz_det_params.  Candidate order is
+// (level, y, x), the order the reference establishes with its per-level
+// bitonic sort (cuda_akaze.cu:882-939), produced here without sorting: each
+// 64-pixel row segment is one wavefront whose ballot gives in-segment ranks,
+// and an exclusive scan over the segments in raster order gives offsets.
+#include "akz_kernels.h"
+
+namespace {
+
+__device__ __forceinline__ int find_level(const akz_det_params& p, int b) {
+  int l = 0;
+  while (l + 1 < p.nlev && b >= p.tile_start[l + 1]) l++;
+  return l;
+}
+
+// Derivate (cuda_akaze.cu:651-671): first derivatives at integer step s,
+// reflect-101 at offset s.
+__device__ __forceinline__ void deriv_xy(const float* f, int pitch, int w, int h, int x, int y, int s, float fac1,
+                                         float fac2, float* dx, float* dy) {
+  int xl = (x < s ? s - x : x - s);
+  int xh = (x >= w - s ? 2 * w - x - s - 2 : x + s);
+  int yl = (y < s ? s - y : y - s);
+  int yh = (y >= h - s ? 2 * h - y - s - 2 : y + s);
+  float ul = f[(size_t)yl * pitch + xl], ur = f[(size_t)yl * pitch + xh];
+  float ll = f[(size_t)yh * pitch + xl], lr = f[(size_t)yh * pitch + xh];
+  if (dx) {
+    float cl = f[(size_t)y * pitch + xl], cr = f[(size_t)y * pitch + xh];
+    *dx = fac1 * (ur + lr - ul - ll) + fac2 * (cr - cl);
+  }
+  if (dy) {
+    float uc = f[(size_t)yl * pitch + x], lc = f[(size_t)yh * pitch + x];
+    *dy = fac1 * (lr + ll - ur - ul) + fac2 * (lc - uc);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_deriv(akz_det_params p) {
+  const int l = find_level(p, blockIdx.x);
+  const akz_level_desc& L = p.lev[l];
+  int t = blockIdx.x - p.tile_start[l];
+  int tiles_x = (L.w + 63) / 64;
+  int x = (t % tiles_x) * 64 + (threadIdx.x & 63);
+  int y0 = (t / tiles_x) * 16;
+  if (x >= L.w) return;
+  for (int y = y0 + (threadIdx.x >> 6); y < y0 + 16 && y < L.h; y += 4) {
+    float dx, dy;
+    deriv_xy(p.Lsm[l], L.pitch, L.w, L.h, x, y, L.deriv_step, p.fac1, p.fac2, &dx, &dy);
+    p.Lx[l][(size_t)y * L.pitch + x] = dx;
+    p.Ly[l][(size_t)y * L.pitch + x] = dy;
+  }
+}
+
+// HessianDeterminant (cuda_akaze.cu:673-701): det = Lxx*Lyy - Lxy^2 with the
+// same stencil applied to Lx and Ly.
+__global__ __launch_bounds__(256) void k_hess(akz_det_params p) {
+  const int l = find_level(p, blockIdx.x);
+  const akz_level_desc& L = p.lev[l];
+  int t = blockIdx.x - p.tile_start[l];
+  int tiles_x = (L.w + 63) / 64;
+  int x = (t % tiles_x) * 64 + (threadIdx.x & 63);
+  int y0 = (t / tiles_x) * 16;
+  if (x >= L.w) return;
+  for (int y = y0 + (threadIdx.x >> 6); y < y0 + 16 && y < L.h; y += 4) {
+    float lxx, lyx, lyy;
+    deriv_xy(p.Lx[l], L.pitch, L.w, L.h, x, y, L.deriv_step, p.fac1, p.fac2, &lxx, &lyx);
+    deriv_xy(p.Ly[l], L.pitch, L.w, L.h, x, y, L.deriv_step, p.fac1, p.fac2, nullptr, &lyy);
+    p.det[l][(size_t)y * L.pitch + x] = lxx * lyy - lyx * lyx;
+  }
+}
+
+// FindExtrema predicate (cuda_akaze.cu:734-743): inside the border, above the
+// threshold and strictly greater than the 8 in-level neighbours.
+__device__ __forceinline__ bool is_extremum(const float* imd, int pitch, int w, int h, int x, int y, float border,
+                                            float thr) {
+  if (x >= w || y >= h) return false;
+  int left_x = (int)(x - border + 0.5f) - 1;
+  int right_x = (int)(x + border + 0.5f) + 1;
+  int up_y = (int)(y - border + 0.5f) - 1;
+  int down_y = (int)(y + border + 0.5f) + 1;
+  if (left_x < 0 || right_x >= w || up_y < 0 || down_y >= h) return false;
+  size_t p = (size_t)y * pitch + x;
+  float v = imd[p];
+  return v > thr && v > imd[p - pitch - 1] && v > imd[p + pitch + 1] && v > imd[p + pitch - 1] &&
+         v > imd[p - pitch + 1] && v > imd[p - 1] && v > imd[p + 1] && v > imd[p + pitch] && v > imd[p - pitch];
+}
+
+// One wavefront per 64-pixel row segment: the ballot of the predicate gives
+// the segment's count and each candidate's rank within it.
+__global__ __launch_bounds__(256) void k_extrema(akz_det_params p, int32_t* __restrict__ seg_count,
+                                                 uint8_t* __restrict__ seg_slots) {
+  const int l = find_level(p, blockIdx.x);
+  const akz_level_desc& L = p.lev[l];
+  int t = blockIdx.x - p.tile_start[l];
+  int tx = t % L.nseg_x;
+  int lane = threadIdx.x & 63;
+  int x = tx * 64 + lane;
+  int y0 = (t / L.nseg_x) * 16;
+  for (int y = y0 + (threadIdx.x >> 6); y < y0 + 16 && y < L.h; y += 4) {
+    bool pred = is_extremum(p.det[l], L.pitch, L.w, L.h, x, y, L.border, p.thr);
+    uint64_t mask = __ballot(pred);
+    int seg = L.seg_base + y * L.nseg_x + tx;
+    if (lane == 0) seg_count[seg] = __popcll(mask);
+    if (pred) {
+      int rank = __popcll(mask & ((1ull << lane) - 1ull));
+      seg_slots[(size_t)seg * 32 + rank] = (uint8_t)lane;
+    }
+  }
+}
+
+// Exclusive scan of the segment counts in (level, y, segment) order: block
+// sums (k_scan_reduce), a scan of the block sums (k_scan_top), then each
+// block scans its 4096 counts with its carry (k_scan_down).  Each thread owns
+// four consecutive counts, read as one int4.
+constexpr int SCAN_T = 1024, SCAN_PER = 4, SCAN_BLK = SCAN_T * SCAN_PER;
+
+__device__ __forceinline__ int4 load4(const int32_t* a, int i, int total) {
+  if (i + 3 < total) return *reinterpret_cast<const int4*>(a + i);
+  int4 v = make_int4(0, 0, 0, 0);
+  if (i < total) v.x = a[i];
+  if (i + 1 < total) v.y = a[i + 1];
+  if (i + 2 < total) v.z = a[i + 2];
+  return v;
+}
+
+// block-wide exclusive scan of one value per thread; returns the block total
+__device__ __forceinline__ int block_exclusive_scan(int v, int* excl, int32_t* s_w) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int inc = v;
+  for (int d = 1; d < 64; d <<= 1) {
+    int o = __shfl_up(inc, d, 64);
+    if (lane >= d) inc += o;
+  }
+  if (lane == 63) s_w[wv] = inc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int run = 0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); i++) { int tv = s_w[i]; s_w[i] = run; run += tv; }
+    s_w[16] = run;
+  }
+  __syncthreads();
+  *excl = s_w[wv] + inc - v;
+  int tot = s_w[16];
+  __syncthreads();
+  return tot;
+}
+
+__global__ __launch_bounds__(SCAN_T) void k_scan_reduce(const int32_t* __restrict__ cnt, int total,
+                                                        int32_t* __restrict__ bsum) {
+  __shared__ int32_t s_w[17];
+  int i = blockIdx.x * SCAN_BLK + threadIdx.x * SCAN_PER;
+  int4 v = load4(cnt, i, total);
+  int ex;
+  int tot = block_exclusive_scan(v.x + v.y + v.z + v.w, &ex, s_w);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(SCAN_T) void k_scan_top(int32_t* bsum, int nblk, int32_t* ncand, int32_t* total_out,
+                                                     int cap) {
+  __shared__ int32_t s_w[17];
+  int carry = 0;
+  for (int base = 0; base < nblk; base += SCAN_T) {
+    int i = base + threadIdx.x;
+    int v = i < nblk ? bsum[i] : 0;
+    int ex;
+    int tot = block_exclusive_scan(v, &ex, s_w);
+    if (i < nblk) bsum[i] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) {
+    *total_out = carry;
+    *ncand = akz_imin(carry, cap);
+  }
+}
+
+__global__ __launch_bounds__(SCAN_T) void k_scan_down(const int32_t* __restrict__ cnt, int total,
+                                                      const int32_t* __restrict__ bsum, int32_t* __restrict__ off,
+                                                      akz_det_params p, int32_t* level_start,
+                                                      const int32_t* total_p) {
+  __shared__ int32_t s_w[17];
+  int i = blockIdx.x * SCAN_BLK + threadIdx.x * SCAN_PER;
+  int4 v = load4(cnt, i, total);
+  int ex;
+  block_exclusive_scan(v.x + v.y + v.z + v.w, &ex, s_w);
+  int o0 = bsum[blockIdx.x] + ex;
+  int4 o = make_int4(o0, o0 + v.x, o0 + v.x + v.y, o0 + v.x + v.y + v.z);
+  if (i + 3 < total) {
+    *reinterpret_cast<int4*>(off + i) = o;
+  } else {
+    if (i < total) off[i] = o.x;
+    if (i + 1 < total) off[i + 1] = o.y;
+    if (i + 2 < total) off[i + 2] = o.z;
+  }
+  // level start offsets: the block holding a level's first segment writes it
+  for (int l = 0; l < p.nlev; l++) {
+    int sb = p.lev[l].seg_base;
+    if (sb >= i && sb < i + SCAN_PER && sb < total) {
+      int k = sb - i;
+      level_start[l] = k == 0 ? o.x : (k == 1 ? o.y : (k == 2 ? o.z : o.w));
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) level_start[p.nlev] = *total_p;
+}
+
+// Materialise candidates in keypoint layout (FindExtrema, cuda_akaze.cu:744-772).
+__global__ __launch_bounds__(256) void k_scatter(akz_det_params p, const int32_t* __restrict__ seg_count,
+                                                 const int32_t* __restrict__ seg_off,
+                                                 const uint8_t* __restrict__ seg_slots, akz_kp* __restrict__ cand,
+                                                 int cap) {
+  int seg = blockIdx.x * 256 + threadIdx.x;
+  if (seg >= p.seg_total) return;
+  int c = seg_count[seg];
+  if (c == 0) return;
+  int l = 0;
+  while (l + 1 < p.nlev && seg >= p.lev[l + 1].seg_base) l++;
+  const akz_level_desc& L = p.lev[l];
+  int ls = seg - L.seg_base;
+  int y = ls / L.nseg_x, tx = ls - y * L.nseg_x;
+  int base = seg_off[seg];
+  const float* imd = p.det[l];
+  const size_t pitch = L.pitch;
+  for (int r = 0; r < c; r++) {
+    int idx = base + r;
+    if (idx >= cap) return;
+    int x = tx * 64 + seg_slots[(size_t)seg * 32 + r];
+    size_t q = (size_t)y * pitch + x;
+    float v = imd[q];
+    float dx = 0.5f * (imd[q + 1] - imd[q - 1]);
+    float dy = 0.5f * (imd[q + pitch] - imd[q - pitch]);
+    float dxx = imd[q + 1] + imd[q - 1] - 2.0f * v;
+    float dyy = imd[q + pitch] + imd[q - pitch] - 2.0f * v;
+    float dxy = 0.25f * (imd[q + pitch + 1] + imd[q - pitch - 1] - imd[q + pitch - 1] - imd[q - pitch + 1]);
+    float det = dxx * dyy - dxy * dxy;
+    float idet = (det != 0.0f ? 1.0f / det : 0.0f);
+    float dst0 = idet * (dxy * dy - dyy * dx);
+    float dst1 = idet * (dxy * dx - dxx * dy);
+    bool weak = !(dst0 >= -1.0f && dst0 <= 1.0f && dst1 >= -1.0f && dst1 <= 1.0f);
+    akz_kp k;
+    k.response = v;
+    k.size = (weak ? -1 : 1) * 2.0 * L.det_size;
+    float octsub = (dst0 < 0 ? -1 : 1) * (L.octave + fabsf(dst0));
+    k.octave = akz_f2bits(weak ? (float)L.octave : octsub);
+    k.class_id = l;
+    int ratio = (1 << L.octave);
+    k.x = ratio * (x);
+    k.y = ratio * (y);
+    k.angle = dst1;
+    cand[idx] = k;
+  }
+}
+
+// Neighbour lists of FindNeighbors (cuda_akaze.cu:1005-1070): earlier
+// candidates of the same level and all candidates of the previous level
+// within half the candidate's size.  Weak candidates (negative size) get
+// none, as the reference's early break gives.  The previous level is entered
+// at the last candidate whose y is within reach (binary search) instead of
+// being walked from its end.
+__global__ __launch_bounds__(256) void k_neighbors(const akz_kp* __restrict__ cand, const int32_t* ncand_p,
+                                                   const int32_t* __restrict__ level_start, int32_t* nb_count,
+                                                   int32_t* nb_list, int32_t* flags) {
+  const int n = *ncand_p;
+  for (int c = blockIdx.x * 256 + threadIdx.x; c < n; c += gridDim.x * 256) {
+    const akz_kp k = cand[c];
+    const int lvl = k.class_id;
+    const int start = level_start[lvl];
+    const float size = k.size;
+    const float half = size * .5f;
+    const double r2 = (double)(size * size) * 0.25;
+    int cnt = 0;
+    int32_t* out = nb_list + (size_t)c * AKZ_NMS_MAX_NB;
+    for (int i = c - 1; i >= start; i--) {
+      const akz_kp q = cand[i];
+      if (k.y - q.y > half) break;
+      float dist = (k.x - q.x) * (k.x - q.x) + (k.y - q.y) * (k.y - q.y);
+      if ((double)dist < r2) {
+        if (cnt < AKZ_NMS_MAX_NB) out[cnt] = i;
+        cnt++;
+      }
+    }
+    if (lvl > 0) {
+      int lo = level_start[lvl - 1], hi = start;
+      // first index in [lo, hi) with q.y - k.y > half (monotone in sorted order)
+      int a = lo, b = hi;
+      while (a < b) {
+        int m = (a + b) >> 1;
+        if (cand[m].y - k.y > half) b = m; else a = m + 1;
+      }
+      for (int i = a - 1; i >= lo; i--) {
+        const akz_kp q = cand[i];
+        if (k.y - q.y > half) break;
+        float dist = (k.x - q.x) * (k.x - q.x) + (k.y - q.y) * (k.y - q.y);
+        if ((double)dist < r2) {
+          if (cnt < AKZ_NMS_MAX_NB) out[cnt] = i;
+          cnt++;
+        }
+      }
+    }
+    if (cnt > AKZ_NMS_MAX_NB) {
+      atomicOr(flags, 1);
+      cnt = AKZ_NMS_MAX_NB;
+    }
+    nb_count[c] = cnt;
+  }
+}
+
+constexpr int NMS_THREADS = 1024;
+constexpr int NMS_LDS_N = 8192;
+constexpr int CODE_WAIT = -1, CODE_ADD = -2;
+
+// Emitted keypoint of sortFiltered_kernel (cuda_akaze.cu:1271-1286): the
+// subpixel offsets are restored from the float bits kept in `octave`.
+__device__ __forceinline__ akz_kp emit(const akz_kp& ok) {
+  float octf = akz_bits2f(ok.octave);
+  float octsub = fabsf(octf);
+  int octave = (int)octsub;
+  float subp = (octf < 0 ? -1 : 1) * (octsub - octave);
+  float ratio = 1 << octave;
+  akz_kp t;
+  t.y = ratio * ((int)(0.5f + ok.y / ratio) + ok.angle);
+  t.x = ratio * ((int)(0.5f + ok.x / ratio) + subp);
+  t.class_id = ok.class_id;
+  t.octave = octave;
+  t.response = ok.response;
+  t.size = ok.size;
+  t.angle = 0.0f;
+  return t;
+}
+
+// Wave-synchronous greedy suppression of FilterExtrema_kernel
+// (cuda_akaze.cu:1072-1224) in one workgroup.  Per wave: a candidate whose
+// neighbours are all processed is "ready"; a ready candidate with no kept
+// neighbour is kept; otherwise it contends for its kept neighbour with the
+// smallest member value, the lowest contending index wins (atomicMin here,
+// a racy min in the reference), and the winner replaces that neighbour if its
+// response is higher, else is dropped.  Member values live in LDS when the
+// candidate count allows, in global memory otherwise.  Only unprocessed
+// candidates are revisited (compacted active list).  Then the output of
+// sortFiltered_kernel (:1227-1316): per 2048-slot chunk, the kept member
+// values whose point is strong, ascending.
+__global__ __launch_bounds__(NMS_THREADS) void k_nms(const akz_kp* __restrict__ cand, const int32_t* ncand_p,
+                                                     const int32_t* __restrict__ nb_count,
+                                                     const int32_t* __restrict__ nb_list, int32_t* g_mem,
+                                                     int32_t* g_minnb, int32_t* g_code, int32_t* g_act,
+                                                     akz_kp* __restrict__ out, int32_t* nout, int32_t* waves_out,
+                                                     int max_waves, int cap) {
+  __shared__ int32_t s_mem[NMS_LDS_N];
+  __shared__ int32_t s_minnb[NMS_LDS_N];
+  __shared__ int32_t s_sort[2048];
+  __shared__ int32_t s_nact[2];
+  __shared__ int32_t s_run;
+  const int t = threadIdx.x;
+  const int n = *ncand_p;
+  const bool lds = n <= NMS_LDS_N;
+  int32_t* mem = lds ? s_mem : g_mem;
+  int32_t* minnb = lds ? s_minnb : g_minnb;
+  int32_t* act[2] = {g_act, g_act + cap};
+  for (int i = t; i < n; i += NMS_THREADS) {
+    mem[i] = -1;
+    minnb[i] = INT32_MAX;
+    act[0][i] = i;
+  }
+  if (t == 0) { s_nact[0] = n; s_nact[1] = 0; s_run = 0; }
+  __syncthreads();
+  int cur = 0, wave = 0;
+  while (wave < max_waves) {
+    const int na = s_nact[cur];
+    if (na == 0) break;
+    const int32_t* A = act[cur];
+    // phase 1: classify ready candidates and arbitrate contention
+    for (int j = t; j < na; j += NMS_THREADS) {
+      int c = A[j];
+      int e = nb_count[c];
+      const int32_t* nb = nb_list + (size_t)c * AKZ_NMS_MAX_NB;
+      bool ready = true, anykept = false;
+      int best = -1, bestv = INT32_MAX;
+      for (int k = 0; k < e; k++) {
+        int q = nb[k];
+        int m = mem[q];
+        if (m == -1) { ready = false; break; }
+        if (m >= 0) {
+          anykept = true;
+          if (m < bestv) { bestv = m; best = q; }
+        }
+      }
+      int code = CODE_WAIT;
+      if (ready) {
+        if (!anykept) code = CODE_ADD;
+        else { code = best; atomicMin(&minnb[best], c); }
+      }
+      g_code[c] = code;
+    }
+    __syncthreads();
+    // phase 2: additions and replacements
+    for (int j = t; j < na; j += NMS_THREADS) {
+      int c = A[j];
+      int code = g_code[c];
+      if (code == CODE_ADD) {
+        mem[c] = c;
+      } else if (code >= 0 && minnb[code] == c) {
+        if (cand[c].response > cand[code].response) {
+          mem[c] = code;
+          mem[code] = -2;
+        } else {
+          mem[c] = -2;
+        }
+      }
+    }
+    __syncthreads();
+    // phase 3: reset arbitration slots, compact the still-unprocessed list
+    const int nxt = cur ^ 1;
+    for (int j = t; j < na; j += NMS_THREADS) {
+      int c = A[j];
+      int code = g_code[c];
+      if (code >= 0) minnb[code] = INT32_MAX;
+      if (mem[c] == -1) {
+        int pos = atomicAdd(&s_nact[nxt], 1);
+        act[nxt][pos] = c;
+      }
+    }
+    __syncthreads();
+    if (t == 0) s_nact[cur] = 0;
+    cur = nxt;
+    wave++;
+    __syncthreads();
+  }
+  if (t == 0) *waves_out = wave;
+  // output chunks
+  for (int base = 0; base < n; base += 2048) {
+    for (int e = t; e < 2048; e += NMS_THREADS) {
+      int i = base + e;
+      int v = (i < n) ? mem[i] : -1;
+      bool ok = v >= 0 && !(cand[v].size < 0);
+      s_sort[e] = ok ? v : INT32_MAX;
+    }
+    __syncthreads();
+    for (int k = 2; k <= 2048; k <<= 1) {
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        int a = ((t & ~(j - 1)) << 1) | (t & (j - 1));
+        int b = a | j;
+        int va = s_sort[a], vb = s_sort[b];
+        bool asc = (a & k) == 0;
+        if ((va > vb) == asc) { s_sort[a] = vb; s_sort[b] = va; }
+        __syncthreads();
+      }
+    }
+    int cnt = __syncthreads_count((s_sort[t] != INT32_MAX) ? 1 : 0) +
+              __syncthreads_count((s_sort[t + 1024] != INT32_MAX) ? 1 : 0);
+    const int run = s_run;
+    for (int e = t; e < 2048; e += NMS_THREADS) {
+      int v = s_sort[e];
+      if (v != INT32_MAX && run + e < cap) out[run + e] = emit(cand[v]);
+    }
+    __syncthreads();
+    if (t == 0) s_run = run + cnt;
+    __syncthreads();
+  }
+  if (t == 0) *nout = akz_imin(s_run, cap);
+}
+
+inline int grid_stride_blocks(int cap) {
+  int b = (cap + 255) / 256;
+  return b < 2048 ? (b > 0 ? b : 1) : 2048;
+}
+
+}  // namespace
+
+void akz_launch_hessian(hipStream_t s, const akz_det_params& p) {
+  int nb = p.tile_start[p.nlev];
+  hipLaunchKernelGGL(k_deriv, dim3(nb), dim3(256), 0, s, p);
+  hipLaunchKernelGGL(k_hess, dim3(nb), dim3(256), 0, s, p);
+}
+
+void akz_launch_extrema(hipStream_t s, const akz_det_params& p, int32_t* seg_count, uint8_t* seg_slots) {
+  hipLaunchKernelGGL(k_extrema, dim3(p.tile_start[p.nlev]), dim3(256), 0, s, p, seg_count, seg_slots);
+}
+
+void akz_launch_scan(hipStream_t s, const akz_det_params& p, const int32_t* seg_count, int32_t* seg_off,
+                     int32_t* level_start, int32_t* ncand, int cap, int32_t* scratch) {
+  // scratch: block sums (ceil(seg_total/4096) ints) followed by the total
+  int nblk = (p.seg_total + SCAN_BLK - 1) / SCAN_BLK;
+  int32_t* bsum = scratch;
+  int32_t* total = scratch + nblk;
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nblk), dim3(SCAN_T), 0, s, seg_count, p.seg_total, bsum);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_T), 0, s, bsum, nblk, ncand, total, cap);
+  hipLaunchKernelGGL(k_scan_down, dim3(nblk), dim3(SCAN_T), 0, s, seg_count, p.seg_total, bsum, seg_off, p,
+                     level_start, total);
+}
+
+void akz_launch_scatter(hipStream_t s, const akz_det_params& p, const int32_t* seg_count, const int32_t* seg_off,
+                        const uint8_t* seg_slots, akz_kp* cand, int cap) {
+  hipLaunchKernelGGL(k_scatter, dim3((p.seg_total + 255) / 256), dim3(256), 0, s, p, seg_count, seg_off, seg_slots,
+                     cand, cap);
+}
+
+void akz_launch_neighbors(hipStream_t s, const akz_kp* cand, const int32_t* ncand, const int32_t* level_start,
+                          int32_t* nb_count, int32_t* nb_list, int32_t* flags, int cap) {
+  hipLaunchKernelGGL(k_neighbors, dim3(grid_stride_blocks(cap)), dim3(256), 0, s, cand, ncand, level_start, nb_count,
+                     nb_list, flags);
+}
+
+void akz_launch_nms(hipStream_t s, const akz_kp* cand, const int32_t* ncand, const int32_t* nb_count,
+                    const int32_t* nb_list, int32_t* mem, int32_t* minnb, int32_t* target, int32_t* act,
+                    akz_kp* out, int32_t* nout, int32_t* waves, int max_waves, int cap) {
+  hipLaunchKernelGGL(k_nms, dim3(1), dim3(NMS_THREADS), 0, s, cand, ncand, nb_count, nb_list, mem, minnb, target,
+                     act, out, nout, waves, max_waves, cap);
+}

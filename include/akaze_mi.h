@@ -1,0 +1,181 @@
+/*
+ * akaze_mi.h -- C ABI of the MI355X A-KAZE library (libakaze_amd.so).
+ *
+ * Plain C types, plain pointers and sizes.  Every entry point returns an
+ * akz_status; none of them exits the process (the reference's safeCall calls
+ * exit(-1), src/lib/cudautils.h:15-21).  All state lives in a per-context
+ * object with its own HIP stream (the reference keeps module-global device
+ * symbols and one global copy stream, src/lib/cuda_akaze.cu:21-28).
+ *
+ * The functions replace the reference's kernel-wrapper layer
+ * (src/lib/cuda_akaze.h:4-28) and carry what its public C++ classes need
+ * (src/lib/AKAZE.h:29-143).  The C++ classes themselves are in
+ * include/akaze/AKAZE.h.
+ */
+#ifndef AKAZE_MI_H
+#define AKAZE_MI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  AKZ_OK = 0,
+  AKZ_ERR_ARG = 1,        /* invalid argument */
+  AKZ_ERR_HIP = 2,        /* HIP runtime error */
+  AKZ_ERR_OOM = 3,        /* device or host allocation failed */
+  AKZ_ERR_OVERFLOW = 4,   /* a capacity was exceeded: output truncated */
+  AKZ_ERR_STATE = 5,      /* call order violated */
+  AKZ_ERR_UNSUPPORTED = 6,/* option not implemented */
+  AKZ_ERR_NO_DEVICE = 7   /* no HIP device visible */
+} akz_status;
+
+/* AKAZEOptions, src/lib/AKAZEConfig.h:84-143 (fields on the GPU path). */
+typedef struct {
+  int omax;                     /* octaves, default 4 */
+  int nsublevels;               /* sublevels per octave, default 4 */
+  int img_width, img_height;    /* image size fixed at creation (AKAZEConfig.h:113-114) */
+  float soffset;                /* 1.6 */
+  float derivative_factor;      /* 1.5 */
+  float sderivatives;           /* 1.0 (unused by the path, kept for layout) */
+  int diffusivity;              /* 0 PM_G1, 1 PM_G2 (default), 2 WEICKERT, 3 CHARBONNIER */
+  float dthreshold;             /* 0.001 */
+  float min_dthreshold;         /* 0.00001 */
+  int descriptor;               /* 5 = MLDB (the only descriptor the reference implements) */
+  int descriptor_size;          /* 0 = full 486 bits */
+  int descriptor_channels;      /* 3 */
+  int descriptor_pattern_size;  /* 10 */
+  float kcontrast;              /* output of the contrast stage */
+  float kcontrast_percentile;   /* 0.7 */
+  int kcontrast_nbins;          /* 300, at most 512 */
+  int maxkeypoints;             /* 0 = capacity for every possible candidate */
+  int nms_max_waves;            /* 10000, cuda_akaze.cu:1100 */
+  int verbosity;
+} akz_options;
+
+/* cv::KeyPoint wire layout, 28 bytes (device arrays are memcpy'd into
+ * std::vector<cv::KeyPoint> at src/lib/cuda_akaze.cu:1405-1411). */
+typedef struct {
+  float x, y, size, angle, response;
+  int32_t octave, class_id;
+} akz_keypoint;
+
+/* cv::DMatch wire layout, 16 bytes (src/lib/AKAZE.cpp:62-77). */
+typedef struct {
+  int32_t queryIdx, trainIdx, imgIdx;
+  float distance;
+} akz_dmatch;
+
+/* AKAZETiming, src/lib/AKAZEConfig.h:61-80 (milliseconds, GPU event time). */
+typedef struct {
+  double kcontrast, scale, derivatives, detector, extrema, subpixel, descriptor;
+} akz_timing;
+
+typedef struct akz_ctx akz_ctx;
+typedef struct akz_matcher akz_matcher;
+
+/* Defaults of AKAZEOptions::AKAZEOptions(), AKAZEConfig.h:86-111. */
+akz_status akz_options_default(akz_options* opts);
+const char* akz_status_string(akz_status s);
+akz_status akz_device_count(int* n);
+
+/* AKAZE::AKAZE + Allocate_Memory_Evolution (AKAZE.cpp:126-208; AllocBuffers,
+ * cuda_akaze.cu:479-538).  Level geometry, FED schedule and one device slab. */
+akz_status akz_create(const akz_options* opts, akz_ctx** out);
+/* AKAZE::~AKAZE (AKAZE.cpp:140-143; FreeBuffers, cuda_akaze.cu:541). */
+akz_status akz_destroy(akz_ctx* ctx);
+/* Level count after the <80x40 cut (AKAZE.cpp:157-160) and each level's size. */
+akz_status akz_num_levels(const akz_ctx* ctx, int* nlev);
+akz_status akz_level_size(const akz_ctx* ctx, int lvl, int* w, int* h, int* octave, int* sublevel,
+                          float* esigma);
+
+/* AKAZE::Create_Nonlinear_Scale_Space (AKAZE.cpp:211-276): contrast factor,
+ * Gaussian, conductance, FED cycles and downsampling.  `img` is a host
+ * float32 image in [0,1], `pitch_bytes` apart per row (0 = dense). */
+akz_status akz_build_scale_space(akz_ctx* ctx, const float* img, size_t pitch_bytes);
+/* Same, for an image already in device memory.  Returns without waiting. */
+akz_status akz_build_scale_space_device(akz_ctx* ctx, const float* d_img, size_t pitch_bytes);
+
+/* AKAZE::Feature_Detection (AKAZE.cpp:316-378): Hessian response, in-level
+ * extrema with subpixel fit, and the cross-level suppression of FilterExtrema
+ * (cuda_akaze.cu:1318-1402).  Writes the number of kept keypoints. */
+akz_status akz_detect(akz_ctx* ctx, int* n_out);
+/* AKAZE::Compute_Descriptors (AKAZE.cpp:405-449): orientation
+ * (FindOrientation, cuda_akaze.cu:2114-2188) and 486-bit MLDB descriptors
+ * (ExtractDescriptors + BuildDescriptor, :1420-1533, :1786-1837).  Copies at
+ * most `cap` keypoints (28 B each) and descriptors (61 B each) to the host. */
+akz_status akz_describe(akz_ctx* ctx, akz_keypoint* kps, uint8_t* desc, int cap, int* n_out);
+
+/* Asynchronous whole-image pipeline for throughput: scale space + detection +
+ * description of a device-resident image, results left on the device.  No
+ * host synchronisation; `akz_wait` waits for the context's stream. */
+akz_status akz_process_device_async(akz_ctx* ctx, const float* d_img, size_t pitch_bytes);
+akz_status akz_wait(akz_ctx* ctx);
+/* Device pointers of the results of the last pipeline run (count, keypoints
+ * and 64-byte-padded descriptor rows); valid until the next run. */
+akz_status akz_results_device(akz_ctx* ctx, const int32_t** d_count, const akz_keypoint** d_kps,
+                              const uint8_t** d_desc64);
+/* Host copy of results after akz_process_device_async + akz_wait. */
+akz_status akz_read_results(akz_ctx* ctx, akz_keypoint* kps, uint8_t* desc, int cap, int* n_out);
+/* The context's own device input buffer (level-0 size, `pitch_bytes` per
+ * row).  An image written there is processed by akz_process_device_async
+ * without a device-to-device copy. */
+akz_status akz_input_buffer(akz_ctx* ctx, float** d_ptr, size_t* pitch_bytes);
+/* Capture the asynchronous pipeline into a HIP graph on the next call (1) or
+ * run it as eager launches (0). */
+akz_status akz_set_graph_mode(akz_ctx* ctx, int enabled);
+
+/* Debug readback of a level image (AKAZE::Save_Scale_Space reads Lt back,
+ * AKAZE.cpp:453-463).  which: 0 Lt, 1 Lsmooth, 2 Lx, 3 Ly, 4 Ldet.
+ * dst is dense w*h floats. */
+akz_status akz_read_level(akz_ctx* ctx, int lvl, int which, float* dst);
+/* Contrast factor used at level `lvl` (options_.kcontrast after the 0.75 per
+ * octave decay, AKAZE.cpp:254-256). */
+akz_status akz_read_kcontrast(akz_ctx* ctx, int lvl, float* k);
+/* Detector candidates before suppression in (level, y, x) order, candidate
+ * count per level, and suppression wave count: the state FilterExtrema
+ * works on (cuda_akaze.cu:1318-1402). */
+akz_status akz_read_candidates(akz_ctx* ctx, akz_keypoint* cand, int cap, int* n_out, int32_t* level_start,
+                               int* waves);
+/* AKAZE::Get_Computation_Times (AKAZE.h:139-141). */
+akz_status akz_get_timing(akz_ctx* ctx, akz_timing* t);
+
+/* ---- Brute-force Hamming 2-NN matcher: Matcher (AKAZE.h:29-55, AKAZE.cpp:
+ * 36-121) and MatchDescriptors (cuda_akaze.cu:1842-2031). ---- */
+akz_status akz_matcher_create(akz_matcher** out);
+akz_status akz_matcher_destroy(akz_matcher* m);
+/* Matcher::bfmatch: q is nq x 61 bytes, t is nt x 61 bytes (row stride
+ * `stride` bytes, >= 61).  Writes 2*nq DMatches: best then second best by
+ * (distance, trainIdx). */
+akz_status akz_match2nn(akz_matcher* m, const uint8_t* q, int nq, const uint8_t* t, int nt, size_t stride,
+                        akz_dmatch* out);
+/* Device-resident variant: 64-byte rows (61 used, 3 zero), output in device
+ * memory; asynchronous on the matcher's stream. */
+akz_status akz_match2nn_device(akz_matcher* m, const uint8_t* d_q64, int nq, const uint8_t* d_t64, int nt,
+                               akz_dmatch* d_out);
+akz_status akz_matcher_wait(akz_matcher* m);
+/* Average duration in ms of the matcher kernels of the last call (HIP events). */
+akz_status akz_matcher_last_ms(akz_matcher* m, float* ms);
+
+/* ---- Device memory helpers (callers without their own HIP runtime) ---- */
+akz_status akz_dev_alloc(size_t bytes, void** ptr);
+akz_status akz_dev_free(void* ptr);
+akz_status akz_memcpy_h2d(void* dst, const void* src, size_t bytes);
+akz_status akz_memcpy_d2h(void* dst, const void* src, size_t bytes);
+akz_status akz_device_synchronize(void);
+
+/* ---- Seeded synthetic inputs (no reference counterpart: the benchmark
+ * inputs).  uint8 image, homography about the image centre, warp. ---- */
+akz_status akz_synth_image(uint64_t seed, int w, int h, uint8_t* out);
+akz_status akz_synth_homography(uint64_t seed, int w, int h, double max_rot_deg, double min_scale,
+                                double max_scale, double max_persp, double* H9);
+akz_status akz_synth_warp(const uint8_t* src, int w, int h, const double* H9, double noise_std,
+                          uint64_t noise_seed, uint8_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
