@@ -41,7 +41,7 @@ class COptions(ctypes.Structure):
                 ("descriptor_channels", ctypes.c_int), ("descriptor_pattern_size", ctypes.c_int),
                 ("kcontrast", ctypes.c_float), ("kcontrast_percentile", ctypes.c_float),
                 ("kcontrast_nbins", ctypes.c_int), ("maxkeypoints", ctypes.c_int), ("nms_max_waves", ctypes.c_int),
-                ("verbosity", ctypes.c_int)]
+                ("verbosity", ctypes.c_int), ("fed_steps_per_launch", ctypes.c_int)]
 
 
 class CTiming(ctypes.Structure):
@@ -79,6 +79,8 @@ SIGNATURES = {
     "akz_read_kcontrast": [_P, _I, _P],
     "akz_read_candidates": [_P, _P, _I, _P, _P, _P],
     "akz_get_timing": [_P, _P],
+    "akz_profile_fed": [_P, _P, _SZ, _P, _P, _P],
+    "akz_copy_kernel_run": [_P, _P, _SZ, _I, _P],
     "akz_matcher_create": [_P],
     "akz_matcher_destroy": [_P],
     "akz_match2nn": [_P, _P, _I, _P, _I, _SZ, _P],
@@ -317,6 +319,13 @@ class AKAZE:
 
     def wait(self):
         _check(lib().akz_wait(self.ctx), "akz_wait")
+
+    def profile_fed(self, d_img, pitch_bytes):
+        """(FED ms per image, algorithmic bytes, launches) from HIP events on the context stream."""
+        ms = ctypes.c_float(); by = ctypes.c_double(); n = ctypes.c_int()
+        _check(lib().akz_profile_fed(self.ctx, d_img, pitch_bytes, ctypes.byref(ms), ctypes.byref(by),
+                                     ctypes.byref(n)), "akz_profile_fed")
+        return ms.value, by.value, n.value
 
     def read_results(self, cap):
         kps = np.zeros(max(cap, 1), KP_DTYPE)

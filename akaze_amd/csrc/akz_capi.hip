@@ -100,7 +100,7 @@ struct akz_ctx {
   float* fed_tmp = nullptr; // FED ping-pong scratch
   uint32_t* maxbits = nullptr;
   int32_t* hist = nullptr;
-  uint32_t* ticket = nullptr;
+  uint32_t* blockmax = nullptr;
   float* kc = nullptr;
   int32_t* seg_count = nullptr;
   int32_t* seg_off = nullptr;
@@ -117,9 +117,14 @@ struct akz_ctx {
   akz_kp* kps = nullptr;
   int32_t* nkps = nullptr;
   int32_t* waves = nullptr;
+  int32_t* total = nullptr;
+  int32_t* status = nullptr;
   uint8_t* desc = nullptr;
-  int32_t* h_small = nullptr;  // pinned: [nkps, flags, ncand_total, waves]
+  int32_t* h_small = nullptr;  // pinned copy of the status block
+  uint8_t* h_stage = nullptr;  // pinned staging for keypoints + 64-byte descriptor rows
+  size_t h_stage_bytes = 0;
   akz_det_params det{};
+  akz_det_params det32{};  // same, with the 64 x akz_detect_tile_rows() tiling of the fused detector
   akz_desc_params dp{};
   akz_taps taps0{}, taps1{}, taps_c{};
   int R0 = 4;
@@ -128,8 +133,13 @@ struct akz_ctx {
   akz_timing timing{};
   int state = 0;  // 0 created, 1 scale space, 2 detected, 3 described
   bool graph_mode = false;
+  bool capturing = false;
   hipGraphExec_t graph_exec = nullptr;
   hipGraph_t graph = nullptr;
+  // optional per-level events around the FED launches (akz_profile_fed)
+  bool fed_profile = false;
+  int fed_block = 1;  // FED steps per launch
+  hipEvent_t fed_ev[2 * AKZ_MAX_LEVELS] = {};
 };
 
 struct akz_matcher {
@@ -149,9 +159,11 @@ akz_status compute_geometry(akz_ctx* c) {
   // the derivative stencils reflect at offset s <= 4 and need levels wider than that
   if (o.img_width < 16 || o.img_height < 16 || o.omax < 1 || o.nsublevels < 1) return AKZ_ERR_ARG;
   if (o.img_width >= 32768 || o.img_height >= 32768) return AKZ_ERR_ARG;
-  if (o.diffusivity < 0 || o.diffusivity > 3) return AKZ_ERR_ARG;
+  if (o.diffusivity < 0 || o.diffusivity > 3 || o.fed_steps_per_launch < 0) return AKZ_ERR_ARG;
   if (o.descriptor != 5) return AKZ_ERR_UNSUPPORTED;  // AKAZE.cpp:428-443: only MLDB exists on the GPU
   if (o.descriptor_size != 0 || o.descriptor_channels != 3) return AKZ_ERR_UNSUPPORTED;
+  // the MLDB grid (max(3*ceil(2p/3), 4*ceil(p/2)))^2 must fit 7 x 64 samples per wavefront
+  if (o.descriptor_pattern_size < 1 || o.descriptor_pattern_size > 10) return AKZ_ERR_UNSUPPORTED;
   int nlev = 0;
   // level geometry and scales, AKAZE.cpp:151-184
   for (int i = 0; i <= o.omax - 1; i++) {
@@ -214,7 +226,8 @@ akz_status allocate(akz_ctx* c) {
   const Level& L0 = c->lev[0];
   size_t n0 = (size_t)L0.pitch * (L0.h + 1);
   size_t o_img = cv.take<float>(n0), o_g = cv.take<float>(n0), o_tmp = cv.take<float>(n0);
-  size_t o_max = cv.take<uint32_t>(1), o_hist = cv.take<int32_t>(512), o_ticket = cv.take<uint32_t>(1);
+  size_t o_max = cv.take<uint32_t>(1), o_hist = cv.take<int32_t>(512 * 512 + 512);
+  size_t o_bmax = cv.take<uint32_t>(2048);
   size_t o_kc = cv.take<float>(AKZ_MAX_LEVELS);
   // row segments of 64 pixels, level-major, raster order
   int seg_total = 0, tiles = 0;
@@ -232,6 +245,11 @@ akz_status allocate(akz_ctx* c) {
   c->det.tile_start[c->nlev] = tiles;
   c->det.seg_total = seg_total;
   c->det.nlev = c->nlev;
+  const int dr = akz_detect_tile_rows();
+  for (int i = 0, t32 = 0; i <= c->nlev; i++) {
+    c->det32.tile_start[i] = t32;
+    if (i < c->nlev) t32 += c->det.lev[i].nseg_x * ((c->lev[i].h + dr - 1) / dr);
+  }
   size_t o_segc = cv.take<int32_t>(seg_total + 4), o_sego = cv.take<int32_t>(seg_total + 4);
   size_t o_slots = cv.take<uint8_t>((size_t)seg_total * 32);
   size_t o_scan = cv.take<int32_t>((seg_total + 4095) / 4096 + 2);
@@ -242,11 +260,11 @@ akz_status allocate(akz_ctx* c) {
   if (capz > (size_t)(1u << 22) - 1) capz = (1u << 22) - 1;
   c->cap = (int)std::max<size_t>(capz, 1);
   size_t o_cand = cv.take<akz_kp>(c->cap), o_ls = cv.take<int32_t>(AKZ_MAX_LEVELS + 1);
-  size_t o_nc = cv.take<int32_t>(4), o_flags = cv.take<int32_t>(4);
+  size_t o_nc = cv.take<int32_t>(4), o_status = cv.take<int32_t>(4);
   size_t o_nbc = cv.take<int32_t>(c->cap), o_nbl = cv.take<int32_t>((size_t)c->cap * AKZ_NMS_MAX_NB);
   size_t o_mem = cv.take<int32_t>(c->cap), o_minnb = cv.take<int32_t>(c->cap), o_code = cv.take<int32_t>(c->cap);
   size_t o_act = cv.take<int32_t>((size_t)c->cap * 2);
-  size_t o_kps = cv.take<akz_kp>(c->cap), o_nk = cv.take<int32_t>(4), o_waves = cv.take<int32_t>(4);
+  size_t o_kps = cv.take<akz_kp>(c->cap);
   size_t o_desc = cv.take<uint8_t>((size_t)c->cap * AKZ_DESC_STRIDE);
   c->slab_bytes = cv.off;
   AKZ_HIP_TRY(hipMalloc(&c->slab, c->slab_bytes));
@@ -264,7 +282,7 @@ akz_status allocate(akz_ctx* c) {
   c->fed_tmp = (float*)(b + o_tmp);
   c->maxbits = (uint32_t*)(b + o_max);
   c->hist = (int32_t*)(b + o_hist);
-  c->ticket = (uint32_t*)(b + o_ticket);
+  c->blockmax = (uint32_t*)(b + o_bmax);
   c->kc = (float*)(b + o_kc);
   c->seg_count = (int32_t*)(b + o_segc);
   c->seg_off = (int32_t*)(b + o_sego);
@@ -273,7 +291,12 @@ akz_status allocate(akz_ctx* c) {
   c->cand = (akz_kp*)(b + o_cand);
   c->level_start = (int32_t*)(b + o_ls);
   c->ncand = (int32_t*)(b + o_nc);
-  c->flags = (int32_t*)(b + o_flags);
+  // status block read back in one copy: kept count, overflow flags, total candidates, waves
+  c->status = (int32_t*)(b + o_status);
+  c->nkps = c->status + 0;
+  c->flags = c->status + 1;
+  c->total = c->status + 2;
+  c->waves = c->status + 3;
   c->nb_count = (int32_t*)(b + o_nbc);
   c->nb_list = (int32_t*)(b + o_nbl);
   c->nms_mem = (int32_t*)(b + o_mem);
@@ -281,8 +304,6 @@ akz_status allocate(akz_ctx* c) {
   c->nms_code = (int32_t*)(b + o_code);
   c->nms_act = (int32_t*)(b + o_act);
   c->kps = (akz_kp*)(b + o_kps);
-  c->nkps = (int32_t*)(b + o_nk);
-  c->waves = (int32_t*)(b + o_waves);
   c->desc = (uint8_t*)(b + o_desc);
   AKZ_HIP_TRY(hipMemsetAsync(c->slab, 0, c->slab_bytes, c->stream));
   AKZ_HIP_TRY(hipHostMalloc((void**)&c->h_small, 64, hipHostMallocDefault));
@@ -305,34 +326,42 @@ akz_status allocate(akz_ctx* c) {
   float wgt = 10.0 / 3.0;  // HessianDeterminant, cuda_akaze.cu:706-708
   c->det.fac1 = 1.0 / (2.0 * (wgt + 2.0));
   c->det.fac2 = wgt * c->det.fac1;
+  int t32[AKZ_MAX_LEVELS + 1];
+  for (int i = 0; i <= c->nlev; i++) t32[i] = c->det32.tile_start[i];
+  c->det32 = c->det;
+  for (int i = 0; i <= c->nlev; i++) c->det32.tile_start[i] = t32[i];
   return AKZ_OK;
 }
 
 // Create_Nonlinear_Scale_Space (AKAZE.cpp:211-276) on the context stream.
+inline void record(akz_ctx* c, int e) {
+  if (!c->capturing) hipEventRecord(c->ev[e], c->stream);
+}
+
 void enqueue_scale_space(akz_ctx* c, const float* img, int in_pitch) {
   hipStream_t s = c->stream;
   Level& L0 = c->lev[0];
-  hipEventRecord(c->ev[EV_START], s);
+  record(c, EV_START);
   // contrast factor of the input (ContrastPercentile, cuda_akaze.cu:604-649);
   // level-0 Lx / Ly serve as blur / gradient scratch until detection
-  akz_launch_contrast_init(s, c->maxbits, c->hist, c->ticket);
   akz_launch_gauss(s, img, in_pitch, L0.Lx, L0.pitch, L0.w, L0.h, 2, c->taps_c);
-  akz_launch_contrast_grad(s, L0.Lx, L0.pitch, L0.Ly, L0.w, L0.h, c->maxbits);
-  akz_launch_contrast_hist(s, L0.Ly, L0.pitch, L0.w, L0.h, c->maxbits, std::min(c->opts.kcontrast_nbins, 512),
-                           c->opts.kcontrast_percentile, c->hist, c->ticket, c->kc, c->nlev, c->octs);
+  akz_launch_contrast(s, L0.Lx, L0.pitch, L0.Ly, L0.w, L0.h, std::min(c->opts.kcontrast_nbins, 512),
+                      c->opts.kcontrast_percentile, c->maxbits, c->blockmax, c->hist, c->kc, c->nlev, c->octs);
   // level 0: Gaussian of soffset; Lsmooth aliases Lt (AKAZE.cpp:234-236)
   akz_launch_gauss(s, img, in_pitch, L0.Lt, L0.pitch, L0.w, L0.h, c->R0, c->taps0);
-  hipEventRecord(c->ev[EV_CONTRAST], s);
+  record(c, EV_CONTRAST);
   for (int i = 1; i < c->nlev; i++) {
     Level& L = c->lev[i];
     Level& P = c->lev[i - 1];
     const int n = L.nsteps;
-    // the FED steps alternate between Lt and the scratch so that the last
-    // one lands in Lt: step 1 writes Lt when n is odd
-    float* first_dst = (n % 2 == 1) ? L.Lt : c->fed_tmp;
+    // FED launches alternate between Lt and the scratch so that the last one
+    // lands in Lt; each launch runs up to `bs` steps (bs = 1: one per launch)
+    const int bs = c->fed_block;
+    const int nl = (n + bs - 1) / bs;
+    float* first_dst = (nl % 2 == 1) ? L.Lt : c->fed_tmp;
     const float* base;
     if (L.octave > P.octave) {
-      float* hs = (n % 2 == 1) ? c->fed_tmp : L.Lt;  // must differ from first_dst
+      float* hs = (nl % 2 == 1) ? c->fed_tmp : L.Lt;  // must differ from first_dst
       if (n == 0) hs = L.Lt;
       akz_launch_halfsample(s, P.Lt, P.w, P.h, P.pitch, hs, L.w, L.h, L.pitch);
       base = hs;
@@ -340,33 +369,47 @@ void enqueue_scale_space(akz_ctx* c, const float* img, int in_pitch) {
       base = P.Lt;  // Copy (AKAZE.cpp:258) is folded into the first step's read
       if (n == 0) akz_launch_copy2d(s, P.Lt, P.pitch, L.Lt, L.pitch, L.w, L.h);
     }
-    akz_launch_gauss(s, base, L.pitch, L.Lsm, L.pitch, L.w, L.h, 2, c->taps1);
-    akz_launch_flow(s, L.Lsm, L.pitch, c->g, L.w, L.h, c->opts.diffusivity, c->kc + i);
+    akz_launch_smooth_flow(s, base, L.pitch, L.Lsm, c->g, L.w, L.h, c->opts.diffusivity, c->kc + i, c->taps1);
     const float* cur = base;
     float* dst = first_dst;
-    for (int j = 0; j < n; j++) {
-      float half_tau = 0.5 * L.tau[j];
-      akz_launch_fed_step(s, cur, c->g, dst, L.pitch, L.w, L.h, half_tau);
+    if (c->fed_profile) hipEventRecord(c->fed_ev[2 * i], s);
+    for (int j = 0; j < n; j += bs) {
+      int k = std::min(bs, n - j);
+      float ht[AKZ_MAX_TAU];
+      for (int q = 0; q < k; q++) ht[q] = 0.5 * L.tau[j + q];
+      if (bs == 1)
+        akz_launch_fed_step(s, cur, c->g, dst, L.pitch, L.w, L.h, ht[0]);
+      else
+        akz_launch_fed_multi(s, cur, c->g, dst, L.pitch, L.w, L.h, ht, k);
       cur = dst;
       dst = (dst == L.Lt) ? c->fed_tmp : L.Lt;
     }
+    if (c->fed_profile) hipEventRecord(c->fed_ev[2 * i + 1], s);
   }
-  hipEventRecord(c->ev[EV_SCALE], s);
+  record(c, EV_SCALE);
 }
 
 // Feature_Detection (AKAZE.cpp:316-378) + FilterExtrema (cuda_akaze.cu:1318-1402).
 void enqueue_detect(akz_ctx* c) {
   hipStream_t s = c->stream;
-  akz_launch_hessian(s, c->det);
-  hipEventRecord(c->ev[EV_DERIV], s);
-  akz_launch_extrema(s, c->det, c->seg_count, c->seg_slots);
-  akz_launch_scan(s, c->det, c->seg_count, c->seg_off, c->level_start, c->ncand, c->cap, c->scan_scratch);
+  int smax = 0;
+  for (int i = 0; i < c->nlev; i++) smax = std::max(smax, c->lev[i].deriv_step);
+  if (smax <= akz_detect_max_step()) {
+    akz_launch_detect_fused(s, c->det32, c->seg_count, c->seg_slots);
+    record(c, EV_DERIV);
+  } else {
+    akz_launch_hessian(s, c->det);
+    record(c, EV_DERIV);
+    akz_launch_extrema(s, c->det, c->seg_count, c->seg_slots);
+  }
+  akz_launch_scan(s, c->det, c->seg_count, c->seg_off, c->level_start, c->ncand, c->cap, c->scan_scratch,
+                  c->total, c->flags);
   akz_launch_scatter(s, c->det, c->seg_count, c->seg_off, c->seg_slots, c->cand, c->cap);
-  hipEventRecord(c->ev[EV_EXTREMA], s);
+  record(c, EV_EXTREMA);
   akz_launch_neighbors(s, c->cand, c->ncand, c->level_start, c->nb_count, c->nb_list, c->flags, c->cap);
   akz_launch_nms(s, c->cand, c->ncand, c->nb_count, c->nb_list, c->nms_mem, c->nms_minnb, c->nms_code, c->nms_act,
                  c->kps, c->nkps, c->waves, c->opts.nms_max_waves, c->cap);
-  hipEventRecord(c->ev[EV_NMS], s);
+  record(c, EV_NMS);
 }
 
 // Compute_Descriptors (AKAZE.cpp:405-449): orientation, then MLDB.
@@ -374,7 +417,7 @@ void enqueue_describe(akz_ctx* c) {
   hipStream_t s = c->stream;
   akz_launch_orientation(s, c->dp, c->kps, c->nkps, c->cap);
   akz_launch_mldb(s, c->dp, c->kps, c->nkps, c->desc, c->cap);
-  hipEventRecord(c->ev[EV_DESC], s);
+  record(c, EV_DESC);
 }
 
 float ev_ms(akz_ctx* c, int a, int b) {
@@ -389,11 +432,7 @@ float ev_ms(akz_ctx* c, int a, int b) {
 // Waits for the stream and reads the small status block: kept count,
 // overflow flags, total candidates, waves.
 akz_status sync_status(akz_ctx* c, int* nk) {
-  AKZ_HIP_TRY(hipMemcpyAsync(c->h_small + 0, c->nkps, 4, hipMemcpyDeviceToHost, c->stream));
-  AKZ_HIP_TRY(hipMemcpyAsync(c->h_small + 1, c->flags, 4, hipMemcpyDeviceToHost, c->stream));
-  AKZ_HIP_TRY(hipMemcpyAsync(c->h_small + 2, c->scan_scratch + (c->det.seg_total + 4095) / 4096, 4,
-                             hipMemcpyDeviceToHost, c->stream));
-  AKZ_HIP_TRY(hipMemcpyAsync(c->h_small + 3, c->waves, 4, hipMemcpyDeviceToHost, c->stream));
+  AKZ_HIP_TRY(hipMemcpyAsync(c->h_small, c->status, 16, hipMemcpyDeviceToHost, c->stream));
   AKZ_HIP_TRY(hipStreamSynchronize(c->stream));
   if (nk) *nk = c->h_small[0];
   if (c->h_small[1] != 0 || c->h_small[2] > c->cap) return AKZ_ERR_OVERFLOW;
@@ -468,6 +507,8 @@ akz_status akz_create(const akz_options* opts, akz_ctx** out) {
   for (int i = 0; i < EV_COUNT; i++) hipEventCreate(&c->ev[i]);
   int ksize = 2 * std::ceil((c->opts.soffset - 0.8) / 0.3) + 3;  // AKAZE.cpp:234-235
   c->R0 = ksize_radius(ksize);
+  c->fed_block = c->opts.fed_steps_per_launch > 0 ? std::min(c->opts.fed_steps_per_launch, akz_fed_block_steps())
+                                                 : akz_fed_block_steps();
   gauss_taps(c->opts.soffset * c->opts.soffset, c->R0, &c->taps0);
   gauss_taps(1.0, 2, &c->taps1);
   gauss_taps(1.0f, 2, &c->taps_c);
@@ -486,8 +527,11 @@ akz_status akz_destroy(akz_ctx* c) {
   if (c->graph) hipGraphDestroy(c->graph);
   for (int i = 0; i < EV_COUNT; i++)
     if (c->ev[i]) hipEventDestroy(c->ev[i]);
+  for (int i = 0; i < 2 * AKZ_MAX_LEVELS; i++)
+    if (c->fed_ev[i]) hipEventDestroy(c->fed_ev[i]);
   if (c->slab) hipFree(c->slab);
   if (c->h_small) hipHostFree(c->h_small);
+  if (c->h_stage) hipHostFree(c->h_stage);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return AKZ_OK;
@@ -563,9 +607,24 @@ akz_status akz_read_results(akz_ctx* c, akz_keypoint* kps, uint8_t* desc, int ca
   akz_status st = sync_status(c, &nk);
   if (st != AKZ_OK && st != AKZ_ERR_OVERFLOW) return st;
   int m = std::min(nk, cap);
-  if (kps && m > 0) AKZ_HIP_TRY(hipMemcpy(kps, c->kps, sizeof(akz_kp) * m, hipMemcpyDeviceToHost));
-  if (desc && m > 0)
-    AKZ_HIP_TRY(hipMemcpy2D(desc, 61, c->desc, AKZ_DESC_STRIDE, 61, m, hipMemcpyDeviceToHost));
+  if (m > 0 && (kps || desc)) {
+    // one contiguous copy each into pinned staging, then repack 64-byte rows to 61
+    size_t kb = sizeof(akz_kp) * (size_t)m, db = (size_t)AKZ_DESC_STRIDE * m;
+    if (c->h_stage_bytes < kb + db) {
+      if (c->h_stage) hipHostFree(c->h_stage);
+      c->h_stage = nullptr;
+      c->h_stage_bytes = 0;
+      size_t want = std::max(kb + db, (size_t)1 << 20);
+      AKZ_HIP_TRY(hipHostMalloc((void**)&c->h_stage, want, hipHostMallocDefault));
+      c->h_stage_bytes = want;
+    }
+    if (kps) AKZ_HIP_TRY(hipMemcpyAsync(c->h_stage, c->kps, kb, hipMemcpyDeviceToHost, c->stream));
+    if (desc) AKZ_HIP_TRY(hipMemcpyAsync(c->h_stage + kb, c->desc, db, hipMemcpyDeviceToHost, c->stream));
+    AKZ_HIP_TRY(hipStreamSynchronize(c->stream));
+    if (kps) memcpy(kps, c->h_stage, kb);
+    if (desc)
+      for (int i = 0; i < m; i++) memcpy(desc + (size_t)i * 61, c->h_stage + kb + (size_t)i * AKZ_DESC_STRIDE, 61);
+  }
   if (n_out) *n_out = nk;
   return (nk > cap) ? AKZ_ERR_OVERFLOW : st;
 }
@@ -587,9 +646,11 @@ akz_status akz_process_device_async(akz_ctx* c, const float* d_img, size_t pitch
                                    hipMemcpyDeviceToDevice, c->stream));
     if (!c->graph_exec) {
       AKZ_HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      c->capturing = true;
       enqueue_scale_space(c, c->img_in, L0.pitch);
       enqueue_detect(c);
       enqueue_describe(c);
+      c->capturing = false;
       hipError_t e = hipStreamEndCapture(c->stream, &c->graph);
       if (e != hipSuccess) return akz_hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
       AKZ_HIP_TRY(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
@@ -679,6 +740,59 @@ akz_status akz_get_timing(akz_ctx* c, akz_timing* t) {
   }
   if (c->state >= 3) r.descriptor = ev_ms(c, EV_NMS, EV_DESC);
   *t = r;
+  return AKZ_OK;
+}
+
+akz_status akz_profile_fed(akz_ctx* c, const float* d_img, size_t pitch_bytes, float* fed_ms, double* fed_bytes,
+                           int* fed_launches) {
+  if (!c || !d_img) return AKZ_ERR_ARG;
+  const Level& L0 = c->lev[0];
+  size_t sp = pitch_bytes ? pitch_bytes : sizeof(float) * L0.w;
+  if (sp < sizeof(float) * L0.w || sp % sizeof(float)) return AKZ_ERR_ARG;
+  for (int i = 0; i < 2 * c->nlev; i++)
+    if (!c->fed_ev[i]) AKZ_HIP_TRY(hipEventCreate(&c->fed_ev[i]));
+  c->fed_profile = true;
+  enqueue_scale_space(c, d_img, (int)(sp / sizeof(float)));
+  c->fed_profile = false;
+  enqueue_detect(c);
+  enqueue_describe(c);
+  AKZ_HIP_TRY(hipGetLastError());
+  AKZ_HIP_TRY(hipStreamSynchronize(c->stream));
+  c->state = 3;
+  float total = 0.0f;
+  double bytes = 0.0;
+  int launches = 0;
+  for (int i = 1; i < c->nlev; i++) {
+    float ms = 0.0f;
+    AKZ_HIP_TRY(hipEventElapsedTime(&ms, c->fed_ev[2 * i], c->fed_ev[2 * i + 1]));
+    total += ms;
+    // algorithmic traffic of one step: read L and g, write L (4 B each per pixel)
+    bytes += 12.0 * c->lev[i].w * c->lev[i].h * c->lev[i].nsteps;
+    launches += (c->lev[i].nsteps + c->fed_block - 1) / c->fed_block;
+  }
+  if (fed_ms) *fed_ms = total;
+  if (fed_bytes) *fed_bytes = bytes;
+  if (fed_launches) *fed_launches = launches;
+  return AKZ_OK;
+}
+
+akz_status akz_copy_kernel_run(const float* d_src, float* d_dst, size_t nfloats, int iters, float* avg_ms) {
+  if (!d_src || !d_dst || iters < 1 || nfloats == 0) return AKZ_ERR_ARG;
+  int w = 4096;
+  int h = (int)((nfloats + w - 1) / w);
+  if ((size_t)w * h != nfloats) return AKZ_ERR_ARG;
+  hipEvent_t a, b;
+  AKZ_HIP_TRY(hipEventCreate(&a));
+  AKZ_HIP_TRY(hipEventCreate(&b));
+  AKZ_HIP_TRY(hipEventRecord(a, nullptr));
+  for (int i = 0; i < iters; i++) akz_launch_copy2d(nullptr, d_src, w, d_dst, w, w, h);
+  AKZ_HIP_TRY(hipEventRecord(b, nullptr));
+  AKZ_HIP_TRY(hipEventSynchronize(b));
+  float ms = 0.0f;
+  AKZ_HIP_TRY(hipEventElapsedTime(&ms, a, b));
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  if (avg_ms) *avg_ms = ms / iters;
   return AKZ_OK;
 }
 

@@ -63,17 +63,33 @@ __global__ __launch_bounds__(64 * OR_WAVES) void k_orientation(akz_desc_params p
       const int step = (int)(0.5f * k.size + 0.5f) >> octave;
       const int x = (int)(k.x + 0.5f) >> octave;
       const int y = (int)(k.y + 0.5f) >> octave;
-      for (int tx = lane; tx < 208; tx += 64) {
-        int i = (tx & 15) - 6, j = (tx / 16) - 6;
-        int r2 = i * i + j * j;
+      // the lane's 4 sample slots: all gathers are issued before any is used
+      float gxv[4], gyv[4], gwv[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const int tx = lane + 64 * k;
+        const int i = (tx & 15) - 6, j = (tx / 16) - 6;
+        const int r2 = i * i + j * j;
+        gwv[k] = 0.0f;
+        gxv[k] = gyv[k] = 0.0f;
+        if (tx < 208 && r2 < 36) {
+          gwv[k] = wt.w[r2];
+          const int xs = akz_iclamp(x + step * i, 0, w - 1), ys = akz_iclamp(y + step * j, 0, h - 1);
+          const size_t pos = (size_t)ys * pitch + xs;
+          gxv[k] = dxd[pos];
+          gyv[k] = dyd[pos];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const int tx = lane + 64 * k;
+        if (tx >= 208) break;
+        const int i = (tx & 15) - 6, j = (tx / 16) - 6;
         int bin = -1;
         float dx = 0.0f, dy = 0.0f;
-        if (r2 < 36) {
-          float gw = wt.w[r2];
-          int xs = akz_iclamp(x + step * i, 0, w - 1), ys = akz_iclamp(y + step * j, 0, h - 1);
-          size_t pos = (size_t)ys * pitch + xs;
-          dx = gw * dxd[pos];
-          dy = gw * dyd[pos];
+        if (i * i + j * j < 36) {
+          dx = gwv[k] * gxv[k];
+          dy = gwv[k] * gyv[k];
           float angle = akz_atan2f(dy, dx);
           bin = akz_imax(akz_imin((int)(angle * (21 / 3.14159265358979323846)) + 21, 41), 0);
         }
@@ -146,28 +162,45 @@ __global__ __launch_bounds__(64 * MLDB_WAVES) void k_mldb(akz_desc_params p, con
     float co, si;
     akz_sincosf(k.angle, &si, &co);
     for (int v = 0; v < 87; v++) acc[v * 64 + lane] = 0.0f;
-    for (int i = lane; i < winsize * winsize; i += 64) {
-      int y = i / winsize;
-      int x = i - winsize * y;
-      int m = akz_imax(x, y);
-      int l = x - size2;
-      int kk = y - size2;
-      int xp = (int)(xf + scale * (kk * co - l * si) + 0.5f);
-      int yp = (int)(yf + scale * (kk * si + l * co) + 0.5f);
-      xp = akz_iclamp(xp, 0, w - 1);
-      yp = akz_iclamp(yp, 0, h - 1);
-      size_t pos = (size_t)yp * pitch + xp;
-      float im = imd[pos], dx = dxd[pos], dy = dyd[pos];
-      float rx = -dx * si + dy * co;
-      float ry = dx * co + dy * si;
-      if (m < 2 * size2) {
+    // the lane's samples i = lane + 64m (m < 7 covers the 21x21 grid): all
+    // gathers are issued first, then accumulated in increasing i
+    constexpr int MS = 7;
+    float imv[MS], dxv[MS], dyv[MS];
+#pragma unroll
+    for (int m = 0; m < MS; m++) {
+      const int i = lane + 64 * m;
+      imv[m] = dxv[m] = dyv[m] = 0.0f;
+      if (i < winsize * winsize) {
+        const int y = i / winsize, x = i - winsize * y;
+        const int l = x - size2, kk = y - size2;
+        int xp = (int)(xf + scale * (kk * co - l * si) + 0.5f);
+        int yp = (int)(yf + scale * (kk * si + l * co) + 0.5f);
+        xp = akz_iclamp(xp, 0, w - 1);
+        yp = akz_iclamp(yp, 0, h - 1);
+        const size_t pos = (size_t)yp * pitch + xp;
+        imv[m] = imd[pos];
+        dxv[m] = dxd[pos];
+        dyv[m] = dyd[pos];
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < MS; m++) {
+      const int i = lane + 64 * m;
+      if (i >= winsize * winsize) break;
+      const int y = i / winsize;
+      const int x = i - winsize * y;
+      const int mm = akz_imax(x, y);
+      const float im = imv[m], dx = dxv[m], dy = dyv[m];
+      const float rx = -dx * si + dy * co;
+      const float ry = dx * co + dy * si;
+      if (mm < 2 * size2) {
         int x2 = (x < size2 ? 0 : 1), y2 = (y < size2 ? 0 : 1);
         int c = 3 * (y2 * 2 + x2);
         acc[(c + 0) * 64 + lane] += im;
         acc[(c + 1) * 64 + lane] += rx;
         acc[(c + 2) * 64 + lane] += ry;
       }
-      if (m < 3 * size3) {
+      if (mm < 3 * size3) {
         int x3 = (x < size3 ? 0 : (x < 2 * size3 ? 1 : 2));
         int y3 = (y < size3 ? 0 : (y < 2 * size3 ? 1 : 2));
         int c = 3 * (4 + y3 * 3 + x3);
@@ -175,7 +208,7 @@ __global__ __launch_bounds__(64 * MLDB_WAVES) void k_mldb(akz_desc_params p, con
         acc[(c + 1) * 64 + lane] += rx;
         acc[(c + 2) * 64 + lane] += ry;
       }
-      if (m < 4 * size4) {
+      if (mm < 4 * size4) {
         int x4 = (x < 2 * size4 ? (x < size4 ? 0 : 1) : (x < 3 * size4 ? 2 : 3));
         int y4 = (y < 2 * size4 ? (y < size4 ? 0 : 1) : (y < 3 * size4 ? 2 : 3));
         int c = 3 * (13 + y4 * 4 + x4);

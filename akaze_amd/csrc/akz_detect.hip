@@ -110,6 +110,141 @@ __global__ __launch_bounds__(256) void k_extrema(akz_det_params p, int32_t* __re
   }
 }
 
+// Derivate + HessianDeterminant + FindExtrema (cuda_akaze.cu:651-774) fused
+// per 64x32 tile of every level in one launch.  Lsmooth is loaded with a
+// (2s+1)-pixel halo, Lx and Ly are computed on an (s+1)-pixel halo and the
+// determinant on a one-pixel ring, all in LDS (rows padded to 96 floats, so a
+// wavefront's lanes walk one row and nothing is divided).  Every stencil
+// addresses its reflect-101 neighbours through image coordinates, which stay
+// inside the loaded regions.  Lx, Ly and det are written for the tile's own
+// pixels and each 64-pixel row is one wavefront whose ballot gives the
+// candidate count and ranks, as in k_extrema.
+constexpr int DF_SMAX = 4, DF_TH = 32;
+constexpr int DF_H1 = 2 * DF_SMAX + 1, DF_H2 = DF_SMAX + 1;
+constexpr int DF_RW = 96;                       // LDS row stride, >= 64 + 2*DF_H1
+constexpr int DF_R1 = DF_TH + 2 * DF_H1, DF_R2 = DF_TH + 2 * DF_H2, DF_RD = DF_TH + 2;
+
+__global__ __launch_bounds__(512) void k_detect_fused(akz_det_params p, int32_t* __restrict__ seg_count,
+                                                      uint8_t* __restrict__ seg_slots) {
+  __shared__ float s_l[DF_R1 * DF_RW];
+  __shared__ float s_x[DF_R2 * DF_RW];
+  __shared__ float s_y[DF_R2 * DF_RW];
+  __shared__ float s_d[DF_RD * DF_RW];
+  const int l = find_level(p, blockIdx.x);
+  const akz_level_desc& L = p.lev[l];
+  const int t = blockIdx.x - p.tile_start[l];
+  const int tx = t % L.nseg_x;
+  const int x0 = tx * 64, y0 = (t / L.nseg_x) * DF_TH;
+  const int w = L.w, h = L.h, pitch = L.pitch, s = L.deriv_step;
+  const int H1 = 2 * s + 1, H2 = s + 1;
+  const int W1 = 64 + 2 * H1, W2 = 64 + 2 * H2;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  constexpr int NW = 8;
+  const float* lsm = p.Lsm[l];
+  // Lsmooth rows [y0-H1, y0+TH+H1), columns [x0-H1, x0+64+H1): every load of
+  // a thread is issued before any is written to LDS
+  {
+    constexpr int ROWS = (DF_TH + 2 * DF_H1 + NW - 1) / NW, COLS = 2;
+    float v[ROWS][COLS];
+#pragma unroll
+    for (int k = 0; k < ROWS; k++) {
+      const int r = wv + k * NW, gy = y0 - H1 + r;
+#pragma unroll
+      for (int q = 0; q < COLS; q++) {
+        const int c = lane + q * 64, gx = x0 - H1 + c;
+        if (r < DF_TH + 2 * H1 && c < W1 && gy >= 0 && gy < h && gx >= 0 && gx < w)
+          v[k][q] = lsm[(size_t)gy * pitch + gx];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < ROWS; k++) {
+      const int r = wv + k * NW, gy = y0 - H1 + r;
+#pragma unroll
+      for (int q = 0; q < COLS; q++) {
+        const int c = lane + q * 64, gx = x0 - H1 + c;
+        if (r < DF_TH + 2 * H1 && c < W1 && gy >= 0 && gy < h && gx >= 0 && gx < w) s_l[r * DF_RW + c] = v[k][q];
+      }
+    }
+  }
+  __syncthreads();
+  const float fac1 = p.fac1, fac2 = p.fac2;
+  // first derivatives on rows [y0-H2, y0+TH+H2), columns [x0-H2, x0+64+H2):
+  // 64 columns per row-pass, then the 2*H2 leftover columns as one flat list
+  auto deriv_at = [&](int r, int c) {
+    const int y = y0 - H2 + r, x = x0 - H2 + c;
+    if (x < 0 || x >= w || y < 0 || y >= h) return;
+    const int yl = ((y < s ? s - y : y - s) - (y0 - H1)) * DF_RW;
+    const int yh = ((y >= h - s ? 2 * h - y - s - 2 : y + s) - (y0 - H1)) * DF_RW;
+    const int yc = (y - (y0 - H1)) * DF_RW;
+    const int xl = (x < s ? s - x : x - s) - (x0 - H1);
+    const int xh = (x >= w - s ? 2 * w - x - s - 2 : x + s) - (x0 - H1);
+    const int xc = x - (x0 - H1);
+    float ul = s_l[yl + xl], ur = s_l[yl + xh], ll = s_l[yh + xl], lr = s_l[yh + xh];
+    float cl = s_l[yc + xl], cr = s_l[yc + xh];
+    float uc = s_l[yl + xc], lc = s_l[yh + xc];
+    s_x[r * DF_RW + c] = fac1 * (ur + lr - ul - ll) + fac2 * (cr - cl);
+    s_y[r * DF_RW + c] = fac1 * (lr + ll - ur - ul) + fac2 * (lc - uc);
+  };
+  const int R2n = DF_TH + 2 * H2, E2 = 2 * H2;
+  for (int r = wv; r < R2n; r += NW) deriv_at(r, lane);
+  for (int k = threadIdx.x; k < R2n * E2; k += 64 * NW) deriv_at(k / E2, 64 + k % E2);
+  __syncthreads();
+  // determinant on rows [y0-1, y0+TH+1), columns [x0-1, x0+65)
+  auto det_at = [&](int r, int c) {
+    const int y = y0 - 1 + r, x = x0 - 1 + c;
+    if (x < 0 || x >= w || y < 0 || y >= h) return;
+    const int yl = ((y < s ? s - y : y - s) - (y0 - H2)) * DF_RW;
+    const int yh = ((y >= h - s ? 2 * h - y - s - 2 : y + s) - (y0 - H2)) * DF_RW;
+    const int yc = (y - (y0 - H2)) * DF_RW;
+    const int xl = (x < s ? s - x : x - s) - (x0 - H2);
+    const int xh = (x >= w - s ? 2 * w - x - s - 2 : x + s) - (x0 - H2);
+    const int xc = x - (x0 - H2);
+    float ul = s_x[yl + xl], ur = s_x[yl + xh], ll = s_x[yh + xl], lr = s_x[yh + xh];
+    float cl = s_x[yc + xl], cr = s_x[yc + xh];
+    float lxx = fac1 * (ur + lr - ul - ll) + fac2 * (cr - cl);
+    float uc = s_x[yl + xc], lc = s_x[yh + xc];
+    float lyx = fac1 * (lr + ll - ur - ul) + fac2 * (lc - uc);
+    ul = s_y[yl + xl]; ur = s_y[yl + xh]; ll = s_y[yh + xl]; lr = s_y[yh + xh];
+    uc = s_y[yl + xc]; lc = s_y[yh + xc];
+    float lyy = fac1 * (lr + ll - ur - ul) + fac2 * (lc - uc);
+    s_d[r * DF_RW + c] = lxx * lyy - lyx * lyx;
+  };
+  for (int r = wv; r < DF_RD; r += NW) det_at(r, lane);
+  for (int k = threadIdx.x; k < DF_RD * 2; k += 64 * NW) det_at(k >> 1, 64 + (k & 1));
+  __syncthreads();
+  const int x = x0 + lane;
+  float* Lx = p.Lx[l];
+  float* Ly = p.Ly[l];
+  float* D = p.det[l];
+  for (int r = wv; r < DF_TH; r += NW) {
+    const int y = y0 + r;
+    if (y >= h) break;
+    bool pred = false;
+    if (x < w) {
+      const int i2 = (r + H2) * DF_RW + lane + H2;
+      const int id = (r + 1) * DF_RW + lane + 1;
+      const size_t o = (size_t)y * pitch + x;
+      Lx[o] = s_x[i2];
+      Ly[o] = s_y[i2];
+      const float v = s_d[id];
+      D[o] = v;
+      int left_x = (int)(x - L.border + 0.5f) - 1;
+      int right_x = (int)(x + L.border + 0.5f) + 1;
+      int up_y = (int)(y - L.border + 0.5f) - 1;
+      int down_y = (int)(y + L.border + 0.5f) + 1;
+      if (!(left_x < 0 || right_x >= w || up_y < 0 || down_y >= h)) {
+        const int R = DF_RW;
+        pred = v > p.thr && v > s_d[id - R - 1] && v > s_d[id + R + 1] && v > s_d[id + R - 1] &&
+               v > s_d[id - R + 1] && v > s_d[id - 1] && v > s_d[id + 1] && v > s_d[id + R] && v > s_d[id - R];
+      }
+    }
+    uint64_t mask = __ballot(pred);
+    int seg = L.seg_base + y * L.nseg_x + tx;
+    if (lane == 0) seg_count[seg] = __popcll(mask);
+    if (pred) seg_slots[(size_t)seg * 32 + __popcll(mask & ((1ull << lane) - 1ull))] = (uint8_t)lane;
+  }
+}
+
 // Exclusive scan of the segment counts in (level, y, segment) order: block
 // sums (k_scan_reduce), a scan of the block sums (k_scan_top), then each
 // block scans its 4096 counts with its carry (k_scan_down).  Each thread owns
@@ -158,7 +293,7 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_reduce(const int32_t* __restric
 }
 
 __global__ __launch_bounds__(SCAN_T) void k_scan_top(int32_t* bsum, int nblk, int32_t* ncand, int32_t* total_out,
-                                                     int cap) {
+                                                     int cap, int32_t* flags) {
   __shared__ int32_t s_w[17];
   int carry = 0;
   for (int base = 0; base < nblk; base += SCAN_T) {
@@ -172,6 +307,7 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_top(int32_t* bsum, int nblk, in
   if (threadIdx.x == 0) {
     *total_out = carry;
     *ncand = akz_imin(carry, cap);
+    *flags = 0;  // overflow flags of this detection run (set by k_neighbors)
   }
 }
 
@@ -308,6 +444,29 @@ __global__ __launch_bounds__(256) void k_neighbors(const akz_kp* __restrict__ ca
 constexpr int NMS_THREADS = 1024;
 constexpr int NMS_LDS_N = 8192;
 constexpr int CODE_WAIT = -1, CODE_ADD = -2;
+constexpr int NMS_BITMAP_WORDS = 2048;  // bitmap ranking covers up to 65536 candidates
+
+// block-wide exclusive scan for the 1024-thread suppression kernel
+__device__ __forceinline__ int nms_block_scan(int v, int* excl, int32_t* s_w) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int inc = v;
+  for (int d = 1; d < 64; d <<= 1) {
+    int o = __shfl_up(inc, d, 64);
+    if (lane >= d) inc += o;
+  }
+  if (lane == 63) s_w[wv] = inc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int run = 0;
+    for (int i = 0; i < NMS_THREADS / 64; i++) { int tv = s_w[i]; s_w[i] = run; run += tv; }
+    s_w[16] = run;
+  }
+  __syncthreads();
+  *excl = s_w[wv] + inc - v;
+  int tot = s_w[16];
+  __syncthreads();
+  return tot;
+}
 
 // Emitted keypoint of sortFiltered_kernel (cuda_akaze.cu:1271-1286): the
 // subpixel offsets are restored from the float bits kept in `octave`.
@@ -348,6 +507,9 @@ __global__ __launch_bounds__(NMS_THREADS) void k_nms(const akz_kp* __restrict__ 
   __shared__ int32_t s_mem[NMS_LDS_N];
   __shared__ int32_t s_minnb[NMS_LDS_N];
   __shared__ int32_t s_sort[2048];
+  __shared__ uint32_t s_bits[NMS_BITMAP_WORDS];
+  __shared__ int32_t s_pre[NMS_BITMAP_WORDS];
+  __shared__ int32_t s_w[17];
   __shared__ int32_t s_nact[2];
   __shared__ int32_t s_run;
   const int t = threadIdx.x;
@@ -426,35 +588,70 @@ __global__ __launch_bounds__(NMS_THREADS) void k_nms(const akz_kp* __restrict__ 
     __syncthreads();
   }
   if (t == 0) *waves_out = wave;
-  // output chunks
+  // output chunks: the member values of a chunk are distinct candidate
+  // indices, so a bitmap over [0, n) and a prefix popcount of its words give
+  // every value its rank in ascending order (the order of the reference's
+  // per-chunk sort); candidate counts beyond the bitmap use a bitonic sort
+  const int nwords = (n + 31) >> 5;
   for (int base = 0; base < n; base += 2048) {
-    for (int e = t; e < 2048; e += NMS_THREADS) {
-      int i = base + e;
-      int v = (i < n) ? mem[i] : -1;
-      bool ok = v >= 0 && !(cand[v].size < 0);
-      s_sort[e] = ok ? v : INT32_MAX;
-    }
-    __syncthreads();
-    for (int k = 2; k <= 2048; k <<= 1) {
-      for (int j = k >> 1; j > 0; j >>= 1) {
-        int a = ((t & ~(j - 1)) << 1) | (t & (j - 1));
-        int b = a | j;
-        int va = s_sort[a], vb = s_sort[b];
-        bool asc = (a & k) == 0;
-        if ((va > vb) == asc) { s_sort[a] = vb; s_sort[b] = va; }
-        __syncthreads();
+    if (nwords <= NMS_BITMAP_WORDS) {
+      for (int i = t; i < nwords; i += NMS_THREADS) s_bits[i] = 0u;
+      __syncthreads();
+      for (int e = t; e < 2048; e += NMS_THREADS) {
+        int i = base + e;
+        int v = (i < n) ? mem[i] : -1;
+        if (v >= 0 && !(cand[v].size < 0)) atomicOr(&s_bits[v >> 5], 1u << (v & 31));
       }
+      __syncthreads();
+      // exclusive prefix of word popcounts, two words per thread
+      int c0 = (2 * t < nwords) ? __popc(s_bits[2 * t]) : 0;
+      int c1 = (2 * t + 1 < nwords) ? __popc(s_bits[2 * t + 1]) : 0;
+      int ex;
+      int tot = nms_block_scan(c0 + c1, &ex, s_w);
+      if (2 * t < nwords) s_pre[2 * t] = ex;
+      if (2 * t + 1 < nwords) s_pre[2 * t + 1] = ex + c0;
+      __syncthreads();
+      const int run = s_run;
+      for (int e = t; e < 2048; e += NMS_THREADS) {
+        int i = base + e;
+        int v = (i < n) ? mem[i] : -1;
+        if (v >= 0 && !(cand[v].size < 0)) {
+          int rank = s_pre[v >> 5] + __popc(s_bits[v >> 5] & ((1u << (v & 31)) - 1u));
+          if (run + rank < cap) out[run + rank] = emit(cand[v]);
+        }
+      }
+      __syncthreads();
+      if (t == 0) s_run = run + tot;
+      __syncthreads();
+    } else {
+      for (int e = t; e < 2048; e += NMS_THREADS) {
+        int i = base + e;
+        int v = (i < n) ? mem[i] : -1;
+        bool ok = v >= 0 && !(cand[v].size < 0);
+        s_sort[e] = ok ? v : INT32_MAX;
+      }
+      __syncthreads();
+      for (int k = 2; k <= 2048; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+          int a = ((t & ~(j - 1)) << 1) | (t & (j - 1));
+          int b = a | j;
+          int va = s_sort[a], vb = s_sort[b];
+          bool asc = (a & k) == 0;
+          if ((va > vb) == asc) { s_sort[a] = vb; s_sort[b] = va; }
+          __syncthreads();
+        }
+      }
+      int cnt = __syncthreads_count((s_sort[t] != INT32_MAX) ? 1 : 0) +
+                __syncthreads_count((s_sort[t + 1024] != INT32_MAX) ? 1 : 0);
+      const int run = s_run;
+      for (int e = t; e < 2048; e += NMS_THREADS) {
+        int v = s_sort[e];
+        if (v != INT32_MAX && run + e < cap) out[run + e] = emit(cand[v]);
+      }
+      __syncthreads();
+      if (t == 0) s_run = run + cnt;
+      __syncthreads();
     }
-    int cnt = __syncthreads_count((s_sort[t] != INT32_MAX) ? 1 : 0) +
-              __syncthreads_count((s_sort[t + 1024] != INT32_MAX) ? 1 : 0);
-    const int run = s_run;
-    for (int e = t; e < 2048; e += NMS_THREADS) {
-      int v = s_sort[e];
-      if (v != INT32_MAX && run + e < cap) out[run + e] = emit(cand[v]);
-    }
-    __syncthreads();
-    if (t == 0) s_run = run + cnt;
-    __syncthreads();
   }
   if (t == 0) *nout = akz_imin(s_run, cap);
 }
@@ -472,18 +669,26 @@ void akz_launch_hessian(hipStream_t s, const akz_det_params& p) {
   hipLaunchKernelGGL(k_hess, dim3(nb), dim3(256), 0, s, p);
 }
 
+int akz_detect_max_step() { return DF_SMAX; }
+
+int akz_detect_tile_rows() { return DF_TH; }
+
+void akz_launch_detect_fused(hipStream_t s, const akz_det_params& p32, int32_t* seg_count, uint8_t* seg_slots) {
+  hipLaunchKernelGGL(k_detect_fused, dim3(p32.tile_start[p32.nlev]), dim3(512), 0, s, p32, seg_count, seg_slots);
+}
+
 void akz_launch_extrema(hipStream_t s, const akz_det_params& p, int32_t* seg_count, uint8_t* seg_slots) {
   hipLaunchKernelGGL(k_extrema, dim3(p.tile_start[p.nlev]), dim3(256), 0, s, p, seg_count, seg_slots);
 }
 
 void akz_launch_scan(hipStream_t s, const akz_det_params& p, const int32_t* seg_count, int32_t* seg_off,
-                     int32_t* level_start, int32_t* ncand, int cap, int32_t* scratch) {
-  // scratch: block sums (ceil(seg_total/4096) ints) followed by the total
+                     int32_t* level_start, int32_t* ncand, int cap, int32_t* scratch, int32_t* total,
+                     int32_t* flags) {
+  // scratch: block sums (ceil(seg_total/4096) ints)
   int nblk = (p.seg_total + SCAN_BLK - 1) / SCAN_BLK;
   int32_t* bsum = scratch;
-  int32_t* total = scratch + nblk;
   hipLaunchKernelGGL(k_scan_reduce, dim3(nblk), dim3(SCAN_T), 0, s, seg_count, p.seg_total, bsum);
-  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_T), 0, s, bsum, nblk, ncand, total, cap);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_T), 0, s, bsum, nblk, ncand, total, cap, flags);
   hipLaunchKernelGGL(k_scan_down, dim3(nblk), dim3(SCAN_T), 0, s, seg_count, p.seg_total, bsum, seg_off, p,
                      level_start, total);
 }

@@ -8,6 +8,30 @@
 
 namespace {
 
+// Copies an AH x AW window (top-left at image (x0, y0), coordinates clamped
+// into the image) into LDS.  Each thread first issues all of its loads, then
+// writes them, so a block waits for one memory latency instead of one per
+// element.
+template <int AH, int AW, int NT>
+__device__ __forceinline__ void load_window(float* __restrict__ dst, const float* __restrict__ src, int pitch,
+                                            int x0, int y0, int w, int h) {
+  constexpr int PER = (AH * AW + NT - 1) / NT;
+  float v[PER];
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const int idx = threadIdx.x + k * NT;
+    if (idx < AH * AW) {
+      const int r = idx / AW, c = idx - r * AW;
+      v[k] = src[(size_t)akz_iclamp(y0 + r, 0, h - 1) * pitch + akz_iclamp(x0 + c, 0, w - 1)];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const int idx = threadIdx.x + k * NT;
+    if (idx < AH * AW) dst[idx] = v[k];
+  }
+}
+
 constexpr int GT_X = 64;   // Gaussian tile width (one wavefront per row)
 constexpr int GT_Y = 16;   // Gaussian tile height
 constexpr int GT_THREADS = 256;
@@ -25,12 +49,7 @@ __global__ __launch_bounds__(GT_THREADS) void k_gauss(const float* __restrict__ 
   __shared__ float B[AH][GT_X];
   const int x0 = blockIdx.x * GT_X, y0 = blockIdx.y * GT_Y;
   const int tid = threadIdx.x;
-  for (int idx = tid; idx < AH * AW; idx += GT_THREADS) {
-    int r = idx / AW, c = idx - r * AW;
-    int gy = akz_iclamp(y0 - R + r, 0, h - 1);
-    int gx = akz_iclamp(x0 - R + c, 0, w - 1);
-    A[r][c] = in[(size_t)gy * in_pitch + gx];
-  }
+  load_window<AH, AW, GT_THREADS>(&A[0][0], in, in_pitch, x0 - R, y0 - R, w, h);
   __syncthreads();
   for (int idx = tid; idx < AH * GT_X; idx += GT_THREADS) {
     int r = idx / GT_X, c = idx - r * GT_X;
@@ -52,83 +71,163 @@ __global__ __launch_bounds__(GT_THREADS) void k_gauss(const float* __restrict__ 
   }
 }
 
-__global__ void k_contrast_init(uint32_t* maxbits, int32_t* hist, uint32_t* ticket) {
-  int t = threadIdx.x;
-  for (int i = t; i < 512; i += blockDim.x) hist[i] = 0;
-  if (t == 0) { *maxbits = 0u; *ticket = 0u; }
-}
+constexpr int CT_GRAD_BLOCKS = 2048;  // grid-stride block counts of the contrast kernels
+constexpr int CT_HIST_BLOCKS = 512;
+constexpr int CT_HIST_THREADS = 512;
 
 // Gradient magnitude of the sigma=1 blur with the unnormalised Scharr pair of
 // MaxContrast (cuda_akaze.cu:552-579): the 3x3 window with top-left (x, y)
-// gives the value stored at (x+1, y+1); interior pixels only.  The maximum
-// is reduced per wavefront and merged with one atomicMax on the float bits.
+// gives the value stored at (x+1, y+1); interior pixels only.  Each block
+// writes its maximum (float bits; all values are >= 0) to blockmax and the
+// single-block k_contrast_max reduces them.  No global atomics: device-scope
+// atomics on one address serialise across the whole chip.
 __global__ __launch_bounds__(256) void k_contrast_grad(const float* __restrict__ blur, int pitch,
-                                                       float* __restrict__ grad, int w, int h, uint32_t* maxbits) {
-  int x = blockIdx.x * 64 + (threadIdx.x & 63);
-  int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-  float gv = 0.0f;
-  if (x < w - 2 && y < h - 2) {
-    const float* b = blur + (size_t)y * pitch + x;
-    const int W = pitch;
-    float dx = 3.0f * (b[0] - b[2] + b[2 * W] - b[2 * W + 2]) + 10.0f * (b[W] - b[W + 2]);
-    float dy = 3.0f * (b[0] + b[2] - b[2 * W] - b[2 * W + 2]) + 10.0f * (b[1] - b[2 * W + 1]);
-    gv = sqrtf(dx * dx + dy * dy);
-    grad[(size_t)(y + 1) * pitch + (x + 1)] = gv;
+                                                       float* __restrict__ grad, int w, int h,
+                                                       uint32_t* __restrict__ blockmax) {
+  __shared__ uint32_t s_w[4];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int tiles_x = (w - 2 + 63) / 64, tiles_y = (h - 2 + 3) / 4;
+  uint32_t bits = 0u;
+  for (int t = blockIdx.x; t < tiles_x * tiles_y; t += gridDim.x) {
+    int x = (t % tiles_x) * 64 + lane;
+    int y = (t / tiles_x) * 4 + wv;
+    if (x < w - 2 && y < h - 2) {
+      const float* b = blur + (size_t)y * pitch + x;
+      const int W = pitch;
+      float dx = 3.0f * (b[0] - b[2] + b[2 * W] - b[2 * W + 2]) + 10.0f * (b[W] - b[W + 2]);
+      float dy = 3.0f * (b[0] + b[2] - b[2 * W] - b[2 * W + 2]) + 10.0f * (b[1] - b[2 * W + 1]);
+      float gv = sqrtf(dx * dx + dy * dy);
+      grad[(size_t)(y + 1) * pitch + (x + 1)] = gv;
+      uint32_t gb = __float_as_uint(gv);
+      bits = bits > gb ? bits : gb;
+    }
   }
-  uint32_t bits = __float_as_uint(gv);
   for (int off = 32; off > 0; off >>= 1) {
     uint32_t o = __shfl_xor(bits, off, 64);
     bits = bits > o ? bits : o;
   }
-  if ((threadIdx.x & 63) == 0 && bits) atomicMax(maxbits, bits);
+  if (lane == 0) s_w[wv] = bits;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t m = s_w[0];
+    for (int i = 1; i < 4; i++) m = m > s_w[i] ? m : s_w[i];
+    blockmax[blockIdx.x] = m;
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_contrast_max(const uint32_t* __restrict__ blockmax, int nb,
+                                                       uint32_t* maxbits) {
+  __shared__ uint32_t s_w[16];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  uint32_t m = 0u;
+  for (int i = threadIdx.x; i < nb; i += 1024) m = m > blockmax[i] ? m : blockmax[i];
+  for (int off = 32; off > 0; off >>= 1) {
+    uint32_t o = __shfl_xor(m, off, 64);
+    m = m > o ? m : o;
+  }
+  if (lane == 0) s_w[wv] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t r = s_w[0];
+    for (int i = 1; i < 16; i++) r = r > s_w[i] ? r : s_w[i];
+    *maxbits = r;
+  }
 }
 
 struct akz_octaves { int nlev; int oct[AKZ_MAX_LEVELS]; };
 
-// Histogram of grad/max over interior pixels (HistContrast, :581-602) and the
-// percentile search of ContrastPercentile (:637-642), done by the last block
-// to finish so that no value round-trips through the host.
-__global__ __launch_bounds__(256) void k_contrast_hist(const float* __restrict__ grad, int pitch, int w, int h,
-                                                       const uint32_t* maxbits, int nbins, float perc,
-                                                       int32_t* hist, uint32_t* ticket, float* kc,
-                                                       akz_octaves octs) {
-  __shared__ int32_t lh[512];
-  __shared__ bool last;
-  for (int i = threadIdx.x; i < 512; i += 256) lh[i] = 0;
+// Histogram of grad/max over interior pixels (HistContrast, :581-602).  Bins
+// are kept per lane (LDS hist[bin][lane]) so the lanes of a wavefront never
+// collide on a bin; each block stores its bin totals in hist_part.
+__global__ __launch_bounds__(CT_HIST_THREADS) void k_contrast_hist(const float* __restrict__ grad, int pitch,
+                                                                   int w, int h, const uint32_t* maxbits, int nbins,
+                                                                   int32_t* __restrict__ hist_part) {
+  __shared__ int32_t lh[300 * 64];  // bins >= 300 (nbins <= 512) go to `tail`
+  __shared__ int32_t tail[212];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  constexpr int NW = CT_HIST_THREADS / 64;
+  for (int i = threadIdx.x; i < 300 * 64; i += CT_HIST_THREADS) lh[i] = 0;
+  for (int i = threadIdx.x; i < 212; i += CT_HIST_THREADS) tail[i] = 0;
   __syncthreads();
-  const float maxv = __uint_as_float(__hip_atomic_load(maxbits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  const float maxv = __uint_as_float(*maxbits);
   const float imaxval = 1.0f / maxv;
-  const int y0 = blockIdx.y * 16;
-  const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-  if (x > 0 && x < w - 1) {
-    for (int y = y0 + (threadIdx.x >> 6); y < y0 + 16; y += 4) {
-      if (y > 0 && y < h - 1) {
-        float t = nbins * grad[(size_t)y * pitch + x] * imaxval;
-        int idx = (t != t) ? 0 : (int)t;
-        idx = akz_imin(idx, nbins - 1);
-        atomicAdd(&lh[idx], 1);
-      }
+  const int tiles_x = (w - 2 + 63) / 64, tiles_y = (h - 2 + NW - 1) / NW;
+  for (int t = blockIdx.x; t < tiles_x * tiles_y; t += gridDim.x) {
+    int x = 1 + (t % tiles_x) * 64 + lane;
+    int y = 1 + (t / tiles_x) * NW + wv;
+    if (x < w - 1 && y < h - 1) {
+      float v = nbins * grad[(size_t)y * pitch + x] * imaxval;
+      int idx = (v != v) ? 0 : (int)v;
+      idx = akz_imin(idx, nbins - 1);
+      if (idx < 300) atomicAdd(&lh[idx * 64 + lane], 1);
+      else atomicAdd(&tail[idx - 300], 1);
     }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < nbins; i += 256)
-    if (lh[i] > 0) atomicAdd(&hist[i], lh[i]);
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t nb = gridDim.x * gridDim.y;
-    last = (atomicAdd(ticket, 1u) == nb - 1);
+  for (int i = threadIdx.x; i < nbins; i += CT_HIST_THREADS) {
+    int sum = 0;
+    if (i < 300) {
+      for (int l = 0; l < 64; l++) sum += lh[i * 64 + ((l + i) & 63)];
+    } else {
+      sum = tail[i - 300];
+    }
+    hist_part[(size_t)i * CT_HIST_BLOCKS + blockIdx.x] = sum;  // bin-major for the final reduction
   }
+}
+
+// Percentile search of ContrastPercentile (:637-642) over the summed block
+// histograms, on the device so no value round-trips through the host; the
+// per-level factor decays by 0.75 at each new octave (AKAZE.cpp:254-256).
+// Counts are integers: the result does not depend on the summation order.
+// Bin totals over the block histograms: one wavefront per bin, lanes read
+// consecutive block partials, a single round of loads per lane.
+__global__ __launch_bounds__(64) void k_contrast_bins(const int32_t* __restrict__ hist_part, int nb,
+                                                      int32_t* __restrict__ tot) {
+  const int lane = threadIdx.x, i = blockIdx.x;
+  const int32_t* row = hist_part + (size_t)i * CT_HIST_BLOCKS;
+  int sum = 0;
+#pragma unroll
+  for (int k = 0; k < CT_HIST_BLOCKS / 64; k++) {
+    const int b = lane + 64 * k;
+    if (b < nb) sum += row[b];
+  }
+  for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
+  if (lane == 0) tot[i] = sum;
+}
+
+__global__ __launch_bounds__(1024) void k_contrast_finish(const int32_t* __restrict__ bin_tot, int nbins,
+                                                          const uint32_t* maxbits, int w, int h, float perc, float* kc,
+                                                          akz_octaves octs) {
+  __shared__ int32_t tot[512];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int i = threadIdx.x; i < nbins; i += 1024) tot[i] = bin_tot[i];
   __syncthreads();
-  if (!last || threadIdx.x != 0) return;
-  __threadfence();
-  int npoints = (w - 2) * (h - 2);
-  int nthreshold = (int)(npoints * perc);
-  int k = 0, nelements = 0;
-  for (k = 0; nelements < nthreshold && k < nbins; k++)
-    nelements += __hip_atomic_load(&hist[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  float contrast = (nelements < nthreshold ? 0.03f : maxv * ((float)k / nbins));
-  // options_.kcontrast decays by 0.75 at each new octave (AKAZE.cpp:254-256)
+  // percentile: the loop of ContrastPercentile stops at k = 1 + the first bin
+  // whose cumulative count reaches nthreshold (k = 0 when nthreshold <= 0);
+  // found here with a block prefix sum over the bins
+  __shared__ int32_t s_w[16];
+  __shared__ int s_k;
+  const int npoints = (w - 2) * (h - 2);
+  const int nthreshold = (int)(npoints * perc);
+  if (threadIdx.x == 0) s_k = -1;
+  const int v = (threadIdx.x < nbins) ? tot[threadIdx.x] : 0;
+  int inc = v;
+  for (int d = 1; d < 64; d <<= 1) {
+    int o = __shfl_up(inc, d, 64);
+    if (lane >= d) inc += o;
+  }
+  if (lane == 63) s_w[wv] = inc;
+  __syncthreads();
+  int before = 0;
+  for (int i = 0; i < wv; i++) before += s_w[i];
+  inc += before;
+  if (threadIdx.x < nbins && inc >= nthreshold && inc - v < nthreshold) s_k = threadIdx.x + 1;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  const float maxv = __uint_as_float(*maxbits);
+  float contrast;
+  if (nthreshold <= 0) contrast = maxv * ((float)0 / nbins);
+  else contrast = (s_k < 0) ? 0.03f : maxv * ((float)s_k / nbins);
   kc[0] = contrast;
   for (int i = 1; i < octs.nlev; i++) {
     if (octs.oct[i] > octs.oct[i - 1]) contrast = contrast * 0.75;
@@ -136,34 +235,69 @@ __global__ __launch_bounds__(256) void k_contrast_hist(const float* __restrict__
   }
 }
 
-// Conductance of Flow (cuda_akaze.cu:240-270): unnormalised Scharr of Lsmooth
-// with reflect-101 borders, then the diffusivity function.
-__global__ __launch_bounds__(256) void k_flow(const float* __restrict__ L, int pitch, float* __restrict__ g, int w,
-                                              int h, int type, const float* kcp) {
-  int x = blockIdx.x * 64 + (threadIdx.x & 63);
-  int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-  if (x >= w || y >= h) return;
-  float kcontrast = *kcp;
-  float invk = 1.0f / (kcontrast * kcontrast);
-  int ym = akz_reflect101(y - 1, h), yp = akz_reflect101(y + 1, h);
-  int xm = akz_reflect101(x - 1, w), xp = akz_reflect101(x + 1, w);
-  const float* rm = L + (size_t)ym * pitch;
-  const float* r0 = L + (size_t)y * pitch;
-  const float* rp = L + (size_t)yp * pitch;
-  float ul = rm[xm], ur = rm[xp], ll = rp[xm], lr = rp[xp];
-  float lx = 3.0f * (lr - ll + ur - ul) + 10.0f * (r0[xp] - r0[xm]);
-  float ly = 3.0f * (lr + ll - ur - ul) + 10.0f * (rp[x] - rm[x]);
-  float dif2 = invk * (lx * lx + ly * ly);
-  float r;
-  if (type == 0)
-    r = expf(-dif2);
-  else if (type == 1)
-    r = 1.0f / (1.0f + dif2);
-  else if (type == 2)
-    r = 1.0f - (float)exp(-3.315 / (dif2 * dif2 * dif2 * dif2));
-  else
-    r = 1.0f / sqrtf(1.0f + dif2);
-  g[(size_t)y * pitch + x] = r;
+// Lsmooth and conductance of one level in one pass: the sigma=1 Gaussian of
+// LowPass (5 taps, replicate border, row pass then column pass) fused with
+// Flow (unnormalised Scharr of Lsmooth, reflect-101, then the diffusivity),
+// AKAZE.cpp:260-261.  The tile keeps Lsmooth for its 64x16 pixels plus a
+// one-pixel ring in LDS; flow neighbours are addressed through their
+// reflected image coordinates, which always fall inside that region.
+__global__ __launch_bounds__(256) void k_smooth_flow(const float* __restrict__ in, int pitch,
+                                                     float* __restrict__ lsm, float* __restrict__ g, int w, int h,
+                                                     int type, const float* kcp, akz_taps taps) {
+  constexpr int TW = 64, TH = 16, R = 2;
+  constexpr int AW = TW + 2 + 2 * R, AH = TH + 2 + 2 * R;  // input region
+  constexpr int CW = TW + 2, CH = TH + 2;                  // Lsmooth region
+  __shared__ float A[AH][AW];
+  __shared__ float B[AH][CW];
+  __shared__ float C[CH][CW];
+  const int tid = threadIdx.x;
+  const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
+  load_window<AH, AW, 256>(&A[0][0], in, pitch, x0 - 1 - R, y0 - 1 - R, w, h);
+  __syncthreads();
+  for (int idx = tid; idx < AH * CW; idx += 256) {
+    int r = idx / CW, c = idx - r * CW;
+    float sum = 0.0f;
+#pragma unroll
+    for (int i = 0; i <= 2 * R; i++) sum += A[r][c + i] * taps.k[i];
+    B[r][c] = sum;
+  }
+  __syncthreads();
+  for (int idx = tid; idx < CH * CW; idx += 256) {
+    int r = idx / CW, c = idx - r * CW;
+    float sum = 0.0f;
+#pragma unroll
+    for (int i = 0; i <= 2 * R; i++) sum += B[r + i][c] * taps.k[i];
+    C[r][c] = sum;
+  }
+  __syncthreads();
+  const float kcontrast = *kcp;
+  const float invk = 1.0f / (kcontrast * kcontrast);
+  const int x = x0 + (tid & 63);
+  if (x >= w) return;
+  const int cx = x - x0 + 1;
+  const int cxm = akz_reflect101(x - 1, w) - x0 + 1, cxp = akz_reflect101(x + 1, w) - x0 + 1;
+  for (int r = tid >> 6; r < TH; r += 4) {
+    const int y = y0 + r;
+    if (y >= h) break;
+    const int cy = r + 1;
+    const int cym = akz_reflect101(y - 1, h) - y0 + 1, cyp = akz_reflect101(y + 1, h) - y0 + 1;
+    const float v = C[cy][cx];
+    lsm[(size_t)y * pitch + x] = v;
+    float ul = C[cym][cxm], ur = C[cym][cxp], ll = C[cyp][cxm], lr = C[cyp][cxp];
+    float lx = 3.0f * (lr - ll + ur - ul) + 10.0f * (C[cy][cxp] - C[cy][cxm]);
+    float ly = 3.0f * (lr + ll - ur - ul) + 10.0f * (C[cyp][cx] - C[cym][cx]);
+    float dif2 = invk * (lx * lx + ly * ly);
+    float res;
+    if (type == 0)
+      res = expf(-dif2);
+    else if (type == 1)
+      res = 1.0f / (1.0f + dif2);
+    else if (type == 2)
+      res = 1.0f - (float)exp(-3.315 / (dif2 * dif2 * dif2 * dif2));
+    else
+      res = 1.0f / sqrtf(1.0f + dif2);
+    g[(size_t)y * pitch + x] = res;
+  }
 }
 
 // One FED step (NLDStep + NLDUpdate, cuda_akaze.cu:289-345), out of place:
@@ -185,6 +319,96 @@ __global__ __launch_bounds__(256) void k_fed_step(const float* __restrict__ L, c
   float ypos = (f0 + g[s]) * (L[s] - i0);
   float yneg = (f0 + g[n]) * (i0 - L[n]);
   out[c] = i0 + stepsize * (xpos - xneg + ypos - yneg);
+}
+
+// FED steps with temporal blocking.  A block holds an RW x RH region of L
+// and g in LDS: its output tile plus a K-pixel halo.  Step s updates the part
+// of the region still exact after s steps (it shrinks by one pixel per side
+// and step, except along image borders, where neighbours are clamped to the
+// pixel itself as in NLDStep), ping-ponging between two LDS buffers.  Per
+// pixel and step the arithmetic is k_fed_step's, so the result equals one
+// launch per step bit for bit.  RW and RH are powers of two, so the flattened
+// region index splits into row and column with shifts.  Region shapes per
+// level size are chosen from measurements (fed_config below).
+constexpr int FB_K = 8;  // most steps per launch (the halo of every variant)
+struct akz_fed_steps { float half_tau[FB_K]; int n; };
+
+template <int RW, int RH, int NT>
+__global__ __launch_bounds__(NT) void k_fed_multi(const float* __restrict__ L, const float* __restrict__ g,
+                                                  float* __restrict__ out, int pitch, int w, int h,
+                                                  akz_fed_steps st) {
+  constexpr int K = FB_K, TW = RW - 2 * K, TH = RH - 2 * K;
+  constexpr int LW = (RW == 64) ? 6 : (RW == 32 ? 5 : 4);
+  static_assert((1 << LW) == RW, "power-of-two region width");
+  __shared__ float sg[RH * RW];
+  __shared__ float sa[RH * RW];
+  __shared__ float sb[RH * RW];
+  const int tid = threadIdx.x;
+  const int x0 = blockIdx.x * TW - K, y0 = blockIdx.y * TH - K;
+  load_window<RH, RW, NT>(sa, L, pitch, x0, y0, w, h);
+  load_window<RH, RW, NT>(sg, g, pitch, x0, y0, w, h);
+  __syncthreads();
+  float* src = sa;
+  float* dst = sb;
+  for (int s = 1; s <= st.n; s++) {
+    const float stepsize = st.half_tau[s - 1];
+    const float* __restrict__ cs = src;
+    float* __restrict__ cd = dst;
+    for (int idx = s * RW + tid; idx < (RH - s) * RW; idx += NT) {
+      const int r = idx >> LW, c = idx & (RW - 1);
+      const int gx = x0 + c, gy = y0 + r;
+      if (c < s || c >= RW - s || gx < 0 || gy < 0 || gx >= w || gy >= h) continue;
+      const int e = (gx < w - 1) ? idx + 1 : idx, wi = (gx > 0) ? idx - 1 : idx;
+      const int so = (gy < h - 1) ? idx + RW : idx, no = (gy > 0) ? idx - RW : idx;
+      const float i0 = cs[idx], f0 = sg[idx];
+      const float le = cs[e], lw = cs[wi], ls = cs[so], ln = cs[no];
+      const float ge = sg[e], gw = sg[wi], gs = sg[so], gn = sg[no];
+      float xpos = (f0 + ge) * (le - i0);
+      float xneg = (f0 + gw) * (i0 - lw);
+      float ypos = (f0 + gs) * (ls - i0);
+      float yneg = (f0 + gn) * (i0 - ln);
+      cd[idx] = i0 + stepsize * (xpos - xneg + ypos - yneg);
+    }
+    __syncthreads();
+    float* t = src;
+    src = dst;
+    dst = t;
+  }
+  for (int idx = tid; idx < TH * TW; idx += NT) {
+    const int r = K + idx / TW, c = K + idx % TW;
+    const int gx = x0 + c, gy = y0 + r;
+    if (gx < w && gy < h) out[(size_t)gy * pitch + gx] = src[r * RW + c];
+  }
+}
+
+// One FED step, LDS-tiled: a 64x16 output tile plus a one-pixel halo of L and
+// g is read once with coalesced rows; arithmetic as k_fed_step.
+__global__ __launch_bounds__(256) void k_fed_tile(const float* __restrict__ L, const float* __restrict__ g,
+                                                  float* __restrict__ out, int pitch, int w, int h,
+                                                  float stepsize) {
+  constexpr int TW = 64, TH = 16, SW = TW + 2, SH = TH + 2;
+  __shared__ float sl[SH * SW], sg[SH * SW];
+  const int tid = threadIdx.x;
+  const int x0 = blockIdx.x * TW - 1, y0 = blockIdx.y * TH - 1;
+  load_window<SH, SW, 256>(sl, L, pitch, x0, y0, w, h);
+  load_window<SH, SW, 256>(sg, g, pitch, x0, y0, w, h);
+  __syncthreads();
+  const int c = 1 + (tid & 63);
+  const int gx = x0 + c;
+  if (gx >= w) return;
+  const int de = (gx < w - 1) ? 1 : 0, dw = (gx > 0) ? 1 : 0;
+  for (int r = 1 + (tid >> 6); r <= TH; r += 4) {
+    const int gy = y0 + r;
+    if (gy >= h) break;
+    const int i = r * SW + c;
+    const int so = (gy < h - 1) ? i + SW : i, no = (gy > 0) ? i - SW : i;
+    const float i0 = sl[i], f0 = sg[i];
+    float xpos = (f0 + sg[i + de]) * (sl[i + de] - i0);
+    float xneg = (f0 + sg[i - dw]) * (i0 - sl[i - dw]);
+    float ypos = (f0 + sg[so]) * (sl[so] - i0);
+    float yneg = (f0 + sg[no]) * (i0 - sl[no]);
+    out[(size_t)gy * pitch + gx] = i0 + stepsize * (xpos - xneg + ypos - yneg);
+  }
 }
 
 // 2x2 box mean for exact halving (HalfSample2, cuda_akaze.cu:432-440).
@@ -249,33 +473,98 @@ void akz_launch_gauss(hipStream_t s, const float* in, int in_pitch, float* out, 
   }
 }
 
-void akz_launch_contrast_init(hipStream_t s, uint32_t* maxbits, int32_t* hist, uint32_t* ticket) {
-  hipLaunchKernelGGL(k_contrast_init, dim3(1), dim3(256), 0, s, maxbits, hist, ticket);
-}
-
-void akz_launch_contrast_grad(hipStream_t s, const float* blur, int pitch, float* grad, int w, int h,
-                              uint32_t* maxbits) {
-  hipLaunchKernelGGL(k_contrast_grad, grid64x4(w, h), dim3(256), 0, s, blur, pitch, grad, w, h, maxbits);
-}
-
-void akz_launch_contrast_hist(hipStream_t s, const float* grad, int pitch, int w, int h, const uint32_t* maxbits,
-                              int nbins, float perc, int32_t* hist, uint32_t* ticket, float* kc, int nlev,
-                              const int* octaves_host) {
+void akz_launch_contrast(hipStream_t s, const float* blur, int pitch, float* grad, int w, int h, int nbins,
+                         float perc, uint32_t* maxbits, uint32_t* blockmax, int32_t* hist_part, float* kc, int nlev,
+                         const int* octaves_host) {
   akz_octaves o;
   o.nlev = nlev;
   for (int i = 0; i < nlev; i++) o.oct[i] = octaves_host[i];
-  dim3 grid((w + 63) / 64, (h + 15) / 16);
-  hipLaunchKernelGGL(k_contrast_hist, grid, dim3(256), 0, s, grad, pitch, w, h, maxbits, nbins, perc, hist, ticket,
-                     kc, o);
+  int gtiles = ((w - 2 + 63) / 64) * ((h - 2 + 3) / 4);
+  int gb = gtiles < CT_GRAD_BLOCKS ? gtiles : CT_GRAD_BLOCKS;
+  hipLaunchKernelGGL(k_contrast_grad, dim3(gb), dim3(256), 0, s, blur, pitch, grad, w, h, blockmax);
+  hipLaunchKernelGGL(k_contrast_max, dim3(1), dim3(1024), 0, s, blockmax, gb, maxbits);
+  int htiles = ((w - 2 + 63) / 64) * ((h - 2 + CT_HIST_THREADS / 64 - 1) / (CT_HIST_THREADS / 64));
+  int hb = htiles < CT_HIST_BLOCKS ? htiles : CT_HIST_BLOCKS;
+  hipLaunchKernelGGL(k_contrast_hist, dim3(hb), dim3(CT_HIST_THREADS), 0, s, grad, pitch, w, h, maxbits, nbins,
+                     hist_part);
+  int32_t* bin_tot = hist_part + (size_t)512 * CT_HIST_BLOCKS;
+  hipLaunchKernelGGL(k_contrast_bins, dim3(nbins), dim3(64), 0, s, hist_part, hb, bin_tot);
+  hipLaunchKernelGGL(k_contrast_finish, dim3(1), dim3(1024), 0, s, bin_tot, nbins, maxbits, w, h, perc, kc, o);
 }
 
-void akz_launch_flow(hipStream_t s, const float* Lsm, int pitch, float* g, int w, int h, int type, const float* kc) {
-  hipLaunchKernelGGL(k_flow, grid64x4(w, h), dim3(256), 0, s, Lsm, pitch, g, w, h, type, kc);
+void akz_launch_smooth_flow(hipStream_t s, const float* in, int pitch, float* lsm, float* g, int w, int h, int type,
+                            const float* kc, const akz_taps& taps) {
+  hipLaunchKernelGGL(k_smooth_flow, dim3((w + 63) / 64, (h + 15) / 16), dim3(256), 0, s, in, pitch, lsm, g, w, h,
+                     type, kc, taps);
 }
+
 
 void akz_launch_fed_step(hipStream_t s, const float* Lin, const float* g, float* Lout, int pitch, int w, int h,
                          float half_tau) {
   hipLaunchKernelGGL(k_fed_step, grid64x4(w, h), dim3(256), 0, s, Lin, g, Lout, pitch, w, h, half_tau);
+}
+
+int akz_fed_block_steps() { return FB_K; }
+
+// Kernel variant per FED launch: 0 k_fed_step, 1 k_fed_tile; blocked regions
+// (tile = region minus the 8-pixel halo): 2 64x64/512 threads, 3 32x32/256,
+// 4 64x32/256, 5 32x32/1024, 6 64x64/1024, 7 64x48/512, 8 32x48/512.
+void akz_launch_fed_variant(hipStream_t s, int variant, const float* Lin, const float* g, float* Lout, int pitch,
+                            int w, int h, const float* half_tau, int n) {
+  if (variant <= 1) {
+    for (int j = 0; j < n; j++) {
+      const float* src = (j == 0) ? Lin : Lout;
+      if (variant == 0) {
+        hipLaunchKernelGGL(k_fed_step, dim3((w + 63) / 64, (h + 3) / 4), dim3(256), 0, s, src, g, Lout, pitch, w, h,
+                           half_tau[j]);
+      } else {
+        hipLaunchKernelGGL(k_fed_tile, dim3((w + 63) / 64, (h + 15) / 16), dim3(256), 0, s, src, g, Lout, pitch, w,
+                           h, half_tau[j]);
+      }
+    }
+    return;
+  }
+  akz_fed_steps st;
+  st.n = n;
+  for (int i = 0; i < FB_K; i++) st.half_tau[i] = i < n ? half_tau[i] : 0.0f;
+  if (variant == 2) {
+    dim3 grid((w + 47) / 48, (h + 47) / 48);
+    hipLaunchKernelGGL((k_fed_multi<64, 64, 512>), grid, dim3(512), 0, s, Lin, g, Lout, pitch, w, h, st);
+  } else if (variant == 3) {
+    dim3 grid((w + 15) / 16, (h + 15) / 16);
+    hipLaunchKernelGGL((k_fed_multi<32, 32, 256>), grid, dim3(256), 0, s, Lin, g, Lout, pitch, w, h, st);
+  } else if (variant == 5) {
+    dim3 grid((w + 15) / 16, (h + 15) / 16);
+    hipLaunchKernelGGL((k_fed_multi<32, 32, 1024>), grid, dim3(1024), 0, s, Lin, g, Lout, pitch, w, h, st);
+  } else if (variant == 6) {
+    dim3 grid((w + 47) / 48, (h + 47) / 48);
+    hipLaunchKernelGGL((k_fed_multi<64, 64, 1024>), grid, dim3(1024), 0, s, Lin, g, Lout, pitch, w, h, st);
+  } else if (variant == 7) {
+    dim3 grid((w + 47) / 48, (h + 31) / 32);
+    hipLaunchKernelGGL((k_fed_multi<64, 48, 512>), grid, dim3(512), 0, s, Lin, g, Lout, pitch, w, h, st);
+  } else if (variant == 8) {
+    dim3 grid((w + 15) / 16, (h + 31) / 32);
+    hipLaunchKernelGGL((k_fed_multi<32, 48, 512>), grid, dim3(512), 0, s, Lin, g, Lout, pitch, w, h, st);
+  } else {
+    dim3 grid((w + 47) / 48, (h + 15) / 16);
+    hipLaunchKernelGGL((k_fed_multi<64, 32, 256>), grid, dim3(256), 0, s, Lin, g, Lout, pitch, w, h, st);
+  }
+}
+
+// Region shape by level size, from per-size timings of the variants on
+// MI355X (8 steps per launch, us per step):
+//   3840x2160: 64x48/512t 13.2, 64x64/512t 14.1;  1920x1080: 64x64/1024t 4.34;
+//   960x540: 64x64/1024t 1.70;  480x270: 32x32/1024t 0.97;  240x135: 0.73.
+int akz_fed_variant_for(int w, int h) {
+  size_t px = (size_t)w * h;
+  if (px >= 4000000) return 7;
+  if (px >= 300000) return 6;
+  return 5;
+}
+
+void akz_launch_fed_multi(hipStream_t s, const float* Lin, const float* g, float* Lout, int pitch, int w, int h,
+                          const float* half_tau, int n) {
+  akz_launch_fed_variant(s, akz_fed_variant_for(w, h), Lin, g, Lout, pitch, w, h, half_tau, n);
 }
 
 void akz_launch_halfsample(hipStream_t s, const float* in, int iw, int ih, int ipitch, float* out, int ow, int oh,

@@ -174,11 +174,14 @@ int akz_synth_warp_u8(const uint8_t* src, int w, int h, const double* H, double 
       double sx = (Hi[0] * u + Hi[1] * v + Hi[2]) / zz - 0.5;
       double sy = (Hi[3] * u + Hi[4] * v + Hi[5]) / zz - 0.5;
       double val = 0.0;
-      int ix = (int)std::floor(sx), iy = (int)std::floor(sy);
-      if (ix >= 0 && iy >= 0 && ix + 1 < w && iy + 1 < h) {
+      // bilinear inside [0, w-1] x [0, h-1]; constant 0 outside
+      if (sx >= 0.0 && sy >= 0.0 && sx <= w - 1 && sy <= h - 1) {
+        int ix = (int)std::floor(sx), iy = (int)std::floor(sy);
+        int ix1 = std::min(ix + 1, w - 1), iy1 = std::min(iy + 1, h - 1);
         double fx = sx - ix, fy = sy - iy;
-        const uint8_t* p = src + (size_t)iy * w + ix;
-        val = (p[0] * (1 - fx) + p[1] * fx) * (1 - fy) + (p[w] * (1 - fx) + p[w + 1] * fx) * fy;
+        const uint8_t* r0 = src + (size_t)iy * w;
+        const uint8_t* r1 = src + (size_t)iy1 * w;
+        val = (r0[ix] * (1 - fx) + r0[ix1] * fx) * (1 - fy) + (r1[ix] * (1 - fx) + r1[ix1] * fx) * fy;
       }
       if (noise_std > 0) val += noise_std * rng.normal();
       val = val < 0 ? 0 : (val > 255 ? 255 : val);

@@ -54,6 +54,7 @@ typedef struct {
   int maxkeypoints;             /* 0 = capacity for every possible candidate */
   int nms_max_waves;            /* 10000, cuda_akaze.cu:1100 */
   int verbosity;
+  int fed_steps_per_launch;     /* 0 = library choice; 1 = one FED step per kernel launch */
 } akz_options;
 
 /* cv::KeyPoint wire layout, 28 bytes (device arrays are memcpy'd into
@@ -140,6 +141,15 @@ akz_status akz_read_kcontrast(akz_ctx* ctx, int lvl, float* k);
  * works on (cuda_akaze.cu:1318-1402). */
 akz_status akz_read_candidates(akz_ctx* ctx, akz_keypoint* cand, int cap, int* n_out, int32_t* level_start,
                                int* waves);
+/* Measurement: runs the whole pipeline on `d_img` with HIP events around each
+ * level's FED launches and returns their summed time, the algorithmic bytes of
+ * those launches (12 B per pixel and step) and the launch count.  Waits. */
+akz_status akz_profile_fed(akz_ctx* ctx, const float* d_img, size_t pitch_bytes, float* fed_ms, double* fed_bytes,
+                           int* fed_launches);
+/* Measurement: a plain device copy of `nfloats` floats (a multiple of 4096),
+ * `iters` times on the null stream; average ms per launch.  Used to calibrate
+ * hardware byte counters against a kernel of known traffic. */
+akz_status akz_copy_kernel_run(const float* d_src, float* d_dst, size_t nfloats, int iters, float* avg_ms);
 /* AKAZE::Get_Computation_Times (AKAZE.h:139-141). */
 akz_status akz_get_timing(akz_ctx* ctx, akz_timing* t);
 

@@ -1,0 +1,91 @@
+"""C-ABI library checks that need no GPU: exports, argument validation,
+the C++ API header, and the synthetic input generator."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import akaze_amd as A
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for root, _, files in os.walk(os.path.join(REPO, "include")):
+        for f in files:
+            if f.endswith(".h"):
+                txt = open(os.path.join(root, f)).read()
+                names |= set(re.findall(r"^\s*(?:akz_status|const char\*)\s+(akz_\w+)\s*\(", txt, re.M))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    L = A.lib()
+    names = declared_symbols()
+    assert len(names) >= 30
+    for n in sorted(names):
+        assert hasattr(L, n), n
+    assert names == set(A.SIGNATURES), names ^ set(A.SIGNATURES)
+
+
+def test_status_strings_and_defaults():
+    L = A.lib()
+    assert L.akz_status_string(0) == b"ok"
+    o = A.AKAZEOptions()
+    assert (o.omax, o.nsublevels, o.descriptor, o.kcontrast_nbins) == (4, 4, 5, 300)
+    assert abs(o.soffset - 1.6) < 1e-6 and abs(o.dthreshold - 0.001) < 1e-9
+
+
+def test_create_validates_arguments_before_touching_the_device():
+    L = A.lib()
+    p = ctypes.c_void_p()
+    bad = [dict(img_width=8, img_height=480), dict(img_width=640, img_height=480, diffusivity=7),
+           dict(img_width=640, img_height=480, descriptor=3)]
+    for kw in bad:
+        o = A.AKAZEOptions(**kw)
+        st = L.akz_create(ctypes.byref(o._c), ctypes.byref(p))
+        assert st in (1, 6), (kw, st)
+    assert L.akz_create(None, ctypes.byref(p)) == 1
+
+
+@pytest.mark.skipif(A.device_count() > 0, reason="checks the no-device path")
+def test_no_device_is_an_error_not_a_fallback():
+    o = A.AKAZEOptions(img_width=640, img_height=480)
+    with pytest.raises(A.AkazeError) as e:
+        A.AKAZE(o)
+    assert e.value.status == 7
+    with pytest.raises(A.AkazeError):
+        A.Matcher()
+
+
+def test_cpp_api_compiles_and_links(tmp_path):
+    exe = tmp_path / "api_check"
+    r = subprocess.run(["g++", "-std=c++17", "-O1", "-I", os.path.join(REPO, "include"),
+                        os.path.join(REPO, "tests", "cpp", "api_check.cpp"), "-L", os.path.join(REPO, "akaze_amd"),
+                        "-lakaze_amd", "-Wl,-rpath," + os.path.join(REPO, "akaze_amd"), "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    if A.device_count() == 0:
+        r = subprocess.run([str(exe)], capture_output=True, text=True)
+        assert r.returncode == 10, r.stdout + r.stderr
+
+
+def test_synthetic_generator_is_seeded():
+    a = A.synth_image(11, 640, 480)
+    b = A.synth_image(11, 640, 480)
+    c = A.synth_image(12, 640, 480)
+    assert np.array_equal(a, b) and not np.array_equal(a, c)
+    assert a.dtype == np.uint8 and 40 < a.mean() < 220 and a.std() > 10
+    H = A.synth_homography(3, 640, 480)
+    assert np.all(np.abs(H[2, :2]) <= 1e-4)  # perspective terms; H[2,2] absorbs the centring
+    s = np.sqrt(abs(np.linalg.det(H[:2, :2])))
+    assert 0.7 <= s <= 1.3
+    Hid = np.eye(3)
+    assert np.array_equal(A.synth_warp(a, Hid), a)
+    Hr = A.synth_homography(4, 640, 480, max_persp=0.0)
+    c0 = Hr @ np.array([320.0, 240.0, 1.0])
+    assert np.allclose(c0[:2] / c0[2], [320, 240])

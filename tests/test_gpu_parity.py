@@ -14,12 +14,12 @@ import oracle_lib as O
 
 pytestmark = pytest.mark.gpu
 
-SIZES = [(640, 480, 0), (1920, 1080, 1), (900, 675, 2), (320, 240, 3)]
+SIZES = [(640, 480, 0), (1920, 1080, 1), (900, 675, 2), (320, 240, 3), (3840, 2160, 4)]
 
 
-def run_both(A, W, H, seed):
+def run_both(A, W, H, seed, fed_steps=0):
     img = A.to_float(A.synth_image(seed, W, H))
-    opts = A.AKAZEOptions(img_width=W, img_height=H)
+    opts = A.AKAZEOptions(img_width=W, img_height=H, fed_steps_per_launch=fed_steps)
     ak = A.AKAZE(opts)
     assert ak.Create_Nonlinear_Scale_Space(img) == 0
     orc = O.Pipeline(W, H)
@@ -31,10 +31,12 @@ def rel_err(a, b):
     return float(np.max(np.abs(a.astype(np.float64) - b)) / max(np.max(np.abs(b)), 1e-30))
 
 
+@pytest.mark.parametrize("fed_steps", [0, 1])
 @pytest.mark.parametrize("W,H,seed", SIZES)
-def test_scale_space_levels(gpu_lib, W, H, seed):
+def test_scale_space_levels(gpu_lib, W, H, seed, fed_steps):
+    """fed_steps 0: blocked multi-step FED kernel; 1: one step per launch."""
     A = gpu_lib
-    ak, orc, _ = run_both(A, W, H, seed)
+    ak, orc, _ = run_both(A, W, H, seed, fed_steps)
     assert ak.nlev == orc.nlev
     for lvl in range(ak.nlev):
         assert ak.read_kcontrast(lvl) == orc.kcontrast(lvl), lvl
@@ -88,3 +90,51 @@ def test_matcher_exact(gpu_lib):
         m = A.Matcher().match2nn(q, t)
         o = O.match2nn(q, t)
         assert m.tobytes() == o.tobytes(), (nq, nt)
+
+
+def test_graph_mode_replays(gpu_lib):
+    """The captured HIP graph gives the oracle's result on every replay,
+    with eager work and host synchronisation between replays."""
+    A = gpu_lib
+    W, H = 640, 480
+    img = A.to_float(A.synth_image(5, W, H))
+    orc = O.Pipeline(W, H)
+    okp, odesc = orc.run(img)
+    ak = A.AKAZE(A.AKAZEOptions(img_width=W, img_height=H))
+    ptr, pitch = ak.input_buffer()
+    padded = np.zeros((H, pitch // 4), np.float32)
+    padded[:, :W] = img
+    A.lib().akz_memcpy_h2d(ptr, padded.ctypes.data, padded.nbytes)
+    ak.set_graph_mode(True)
+    other = A.AKAZE(A.AKAZEOptions(img_width=W, img_height=H))
+    for rep in range(4):
+        ak.process_device_async(ptr, pitch)
+        ak.wait()
+        kps, desc, n = ak.read_results(len(okp) + 16)
+        assert n == len(okp), rep
+        assert kps.tobytes() == okp.tobytes(), rep
+        assert np.array_equal(desc, odesc), rep
+        other.run(A.to_float(A.synth_image(100 + rep, W, H)))  # eager work between replays
+
+
+@pytest.mark.parametrize("W,H,thr", [(1280, 720, 1e-4), (1920, 1080, 1e-6)])
+def test_many_candidates(gpu_lib, W, H, thr):
+    """Dense candidates: the suppression's global-memory path (> 8192
+    candidates) and its sort fallback (> 65536) against the oracle."""
+    A = gpu_lib
+    rng = np.random.default_rng(W)
+    img = rng.random((H, W)).astype(np.float32)
+    ak = A.AKAZE(A.AKAZEOptions(img_width=W, img_height=H, dthreshold=thr, min_dthreshold=thr * 0.1))
+    ak.Create_Nonlinear_Scale_Space(img)
+    n = ak.Feature_Detection()
+    orc = O.Pipeline(W, H, dthreshold=thr, min_dthreshold=thr * 0.1)
+    orc.scale_space(img)
+    on = orc.detect()
+    cand, ls, waves = ak.read_candidates()
+    assert len(cand) == orc.candidates().shape[0] and len(cand) > 8192
+    assert cand.tobytes() == orc.candidates().tobytes()
+    assert waves == orc.waves()
+    kps, desc = ak.Compute_Descriptors()
+    odesc = orc.describe()
+    assert n == on and kps.tobytes() == orc.keypoints().tobytes()
+    assert np.array_equal(desc, odesc)
