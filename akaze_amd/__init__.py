@@ -81,6 +81,15 @@ SIGNATURES = {
     "akz_get_timing": [_P, _P],
     "akz_profile_fed": [_P, _P, _SZ, _P, _P, _P],
     "akz_copy_kernel_run": [_P, _P, _SZ, _I, _P],
+    "akz_create_batch": [_P, _I, _P],
+    "akz_batch_capacity": [_P, _P],
+    "akz_batch_input_buffer": [_P, _P, _P, _P],
+    "akz_process_batch_device_async": [_P, _P, _SZ, _SZ, _I],
+    "akz_read_batch_counts": [_P, _I, _P, _P],
+    "akz_read_batch_results": [_P, _I, _P, _P, _I, _P],
+    "akz_batch_results_device": [_P, _I, _P, _P, _P],
+    "akz_read_batch_level": [_P, _I, _I, _I, _P],
+    "akz_profile_fed_batch": [_P, _P, _SZ, _SZ, _I, _P, _P, _P],
     "akz_matcher_create": [_P],
     "akz_matcher_destroy": [_P],
     "akz_match2nn": [_P, _P, _I, _P, _I, _SZ, _P],
@@ -342,6 +351,104 @@ class AKAZE:
             raise ValueError("image shape does not match the options")
         self.Feature_Detection()
         return self.Compute_Descriptors()
+
+
+class BatchAKAZE:
+    """Same-size image batches on one context: every stage is one launch over
+    all images of a call.  No counterpart in the reference, which processes
+    one image per AKAZE object; results equal AKAZE's per image."""
+
+    def __init__(self, options, batch):
+        self.options = options
+        p = ctypes.c_void_p()
+        _check(lib().akz_create_batch(ctypes.byref(options._c), int(batch), ctypes.byref(p)), "akz_create_batch")
+        self.ctx = p.value
+        self.batch = int(batch)
+        self.W, self.H = options.img_width, options.img_height
+        n = ctypes.c_int()
+        lib().akz_num_levels(self.ctx, ctypes.byref(n))
+        self.nlev = n.value
+
+    def close(self):
+        if getattr(self, "ctx", None) and _lib is not None:
+            _lib.akz_destroy(self.ctx)
+        self.ctx = None
+
+    def __del__(self):
+        self.close()
+
+    def input_buffer(self):
+        """(device pointer of image 0, row pitch bytes, image stride bytes)."""
+        p, pitch, zs = ctypes.c_void_p(), ctypes.c_size_t(), ctypes.c_size_t()
+        _check(lib().akz_batch_input_buffer(self.ctx, ctypes.byref(p), ctypes.byref(pitch), ctypes.byref(zs)),
+               "akz_batch_input_buffer")
+        return p.value, pitch.value, zs.value
+
+    def upload(self, imgs):
+        """Copies host float32 images (n x H x W) into the context's input buffers."""
+        ptr, pitch, zs = self.input_buffer()
+        n = len(imgs)
+        if n > self.batch:
+            raise ValueError("batch capacity %d < %d images" % (self.batch, n))
+        row = pitch // 4
+        for z, img in enumerate(imgs):
+            img = np.ascontiguousarray(img, np.float32)
+            if img.shape != (self.H, self.W):
+                raise ValueError("image shape does not match the options")
+            padded = np.zeros((self.H, row), np.float32)
+            padded[:, :self.W] = img
+            _check(lib().akz_memcpy_h2d(ptr + z * zs, padded.ctypes.data, padded.nbytes), "akz_memcpy_h2d")
+        return n
+
+    def set_graph_mode(self, on):
+        _check(lib().akz_set_graph_mode(self.ctx, int(bool(on))), "akz_set_graph_mode")
+
+    def process_async(self, n, d_imgs=None, pitch_bytes=0, image_stride_bytes=0):
+        """Runs images 0..n-1 (the context's input buffers unless d_imgs is given)."""
+        if d_imgs is None:
+            d_imgs, pitch_bytes, image_stride_bytes = self.input_buffer()
+        _check(lib().akz_process_batch_device_async(self.ctx, d_imgs, pitch_bytes, image_stride_bytes, int(n)),
+               "akz_process_batch_device_async")
+
+    def counts(self, n):
+        c = np.zeros(n, np.int32)
+        st = np.zeros(n, np.int32)
+        _check(lib().akz_read_batch_counts(self.ctx, int(n), c.ctypes.data, st.ctypes.data), "akz_read_batch_counts")
+        return c, st
+
+    def read_results(self, z, cap):
+        kps = np.zeros(max(cap, 1), KP_DTYPE)
+        desc = np.zeros((max(cap, 1), 61), np.uint8)
+        n = ctypes.c_int()
+        st = lib().akz_read_batch_results(self.ctx, int(z), kps.ctypes.data, desc.ctypes.data, cap, ctypes.byref(n))
+        _check(st, "akz_read_batch_results", allow=(AKZ_ERR_OVERFLOW,))
+        m = min(n.value, cap)
+        return kps[:m].copy(), desc[:m].copy(), n.value
+
+    def read_level(self, z, lvl, which):
+        w, h = AKAZE.level_size(self, lvl)[:2]
+        out = np.empty((h, w), np.float32)
+        _check(lib().akz_read_batch_level(self.ctx, int(z), lvl, which, out.ctypes.data), "akz_read_batch_level")
+        return out
+
+    level_size = AKAZE.level_size
+
+    def profile_fed(self, n):
+        d, pitch, zs = self.input_buffer()
+        ms = ctypes.c_float(); by = ctypes.c_double(); k = ctypes.c_int()
+        _check(lib().akz_profile_fed_batch(self.ctx, d, pitch, zs, int(n), ctypes.byref(ms), ctypes.byref(by),
+                                           ctypes.byref(k)), "akz_profile_fed_batch")
+        return ms.value, by.value, k.value
+
+    def wait(self):
+        _check(lib().akz_wait(self.ctx), "akz_wait")
+
+    def run(self, imgs):
+        """Scale space + detection + description of host images; list of (kps, desc)."""
+        n = self.upload(imgs)
+        self.process_async(n)
+        counts, _ = self.counts(n)
+        return [self.read_results(z, int(counts[z]))[:2] for z in range(n)]
 
 
 class Matcher:

@@ -102,25 +102,19 @@ struct akz_ctx {
   int32_t* hist = nullptr;
   uint32_t* blockmax = nullptr;
   float* kc = nullptr;
-  int32_t* seg_count = nullptr;
-  int32_t* seg_off = nullptr;
-  uint8_t* seg_slots = nullptr;
-  int32_t* scan_scratch = nullptr;
+  akz_det_bufs db{};  // detector arrays of image 0
   akz_kp* cand = nullptr;
   int cap = 0;
   int32_t* level_start = nullptr;
   int32_t* ncand = nullptr;
-  int32_t* flags = nullptr;
-  int32_t* nb_count = nullptr;
-  int32_t* nb_list = nullptr;
-  int32_t *nms_mem = nullptr, *nms_minnb = nullptr, *nms_code = nullptr, *nms_act = nullptr;
   akz_kp* kps = nullptr;
-  int32_t* nkps = nullptr;
-  int32_t* waves = nullptr;
-  int32_t* total = nullptr;
-  int32_t* status = nullptr;
+  int32_t* status = nullptr;  // AKZ_ST_* block of image 0
   uint8_t* desc = nullptr;
-  int32_t* h_small = nullptr;  // pinned copy of the status block
+  // batch: `batch` images of identical layout, image z at slab + z * zs
+  int batch = 1;
+  size_t zs = 0;
+  int last_nz = 1;            // images in the last processed batch
+  int32_t* h_small = nullptr;  // pinned copy of the status blocks (4 ints per image)
   uint8_t* h_stage = nullptr;  // pinned staging for keypoints + 64-byte descriptor rows
   size_t h_stage_bytes = 0;
   akz_det_params det{};
@@ -134,8 +128,9 @@ struct akz_ctx {
   int state = 0;  // 0 created, 1 scale space, 2 detected, 3 described
   bool graph_mode = false;
   bool capturing = false;
-  hipGraphExec_t graph_exec = nullptr;
-  hipGraph_t graph = nullptr;
+  // one captured graph per batch size (index n - 1), instantiated on first use
+  std::vector<hipGraphExec_t> graph_exec;
+  std::vector<hipGraph_t> graph;
   // optional per-level events around the FED launches (akz_profile_fed)
   bool fed_profile = false;
   int fed_block = 1;  // FED steps per launch
@@ -250,23 +245,30 @@ akz_status allocate(akz_ctx* c) {
     c->det32.tile_start[i] = t32;
     if (i < c->nlev) t32 += c->det.lev[i].nseg_x * ((c->lev[i].h + dr - 1) / dr);
   }
+  const int nblk_seg = akz_scan_blocks(seg_total);
   size_t o_segc = cv.take<int32_t>(seg_total + 4), o_sego = cv.take<int32_t>(seg_total + 4);
   size_t o_slots = cv.take<uint8_t>((size_t)seg_total * 32);
-  size_t o_scan = cv.take<int32_t>((seg_total + 4095) / 4096 + 2);
+  size_t o_scan = cv.take<int32_t>(nblk_seg + 4);
   // candidate capacity: every strict 3x3 maximum (at most one per 2x2 block)
   size_t capz = 0;
   for (int i = 0; i < c->nlev; i++) capz += (size_t)((c->lev[i].w + 1) / 2) * ((c->lev[i].h + 1) / 2);
   if (c->opts.maxkeypoints > 0) capz = std::min(capz, (size_t)c->opts.maxkeypoints);
   if (capz > (size_t)(1u << 22) - 1) capz = (1u << 22) - 1;
   c->cap = (int)std::max<size_t>(capz, 1);
+  const size_t nb_cap = (size_t)c->cap * AKZ_NB_PER_CAND;
+  const int nblk_nb = akz_scan_blocks(c->cap);
   size_t o_cand = cv.take<akz_kp>(c->cap), o_ls = cv.take<int32_t>(AKZ_MAX_LEVELS + 1);
   size_t o_nc = cv.take<int32_t>(4), o_status = cv.take<int32_t>(4);
-  size_t o_nbc = cv.take<int32_t>(c->cap), o_nbl = cv.take<int32_t>((size_t)c->cap * AKZ_NMS_MAX_NB);
+  size_t o_nbc = cv.take<int32_t>(c->cap + 4), o_nbo = cv.take<int32_t>(c->cap + 4);
+  size_t o_nbs = cv.take<int32_t>(nblk_nb + 4);
+  size_t o_nbl = cv.take<int32_t>(nb_cap);
   size_t o_mem = cv.take<int32_t>(c->cap), o_minnb = cv.take<int32_t>(c->cap), o_code = cv.take<int32_t>(c->cap);
   size_t o_act = cv.take<int32_t>((size_t)c->cap * 2);
   size_t o_kps = cv.take<akz_kp>(c->cap);
   size_t o_desc = cv.take<uint8_t>((size_t)c->cap * AKZ_DESC_STRIDE);
-  c->slab_bytes = cv.off;
+  c->zs = align_up(cv.off, ALIGN);
+  // after the batch's sub-slabs: the contiguous status blocks of all images
+  c->slab_bytes = c->zs * (size_t)c->batch + align_up(16 * (size_t)c->batch, ALIGN);
   AKZ_HIP_TRY(hipMalloc(&c->slab, c->slab_bytes));
   char* b = (char*)c->slab;
   for (int i = 0; i < c->nlev; i++) {
@@ -284,29 +286,39 @@ akz_status allocate(akz_ctx* c) {
   c->hist = (int32_t*)(b + o_hist);
   c->blockmax = (uint32_t*)(b + o_bmax);
   c->kc = (float*)(b + o_kc);
-  c->seg_count = (int32_t*)(b + o_segc);
-  c->seg_off = (int32_t*)(b + o_sego);
-  c->seg_slots = (uint8_t*)(b + o_slots);
-  c->scan_scratch = (int32_t*)(b + o_scan);
   c->cand = (akz_kp*)(b + o_cand);
   c->level_start = (int32_t*)(b + o_ls);
   c->ncand = (int32_t*)(b + o_nc);
-  // status block read back in one copy: kept count, overflow flags, total candidates, waves
   c->status = (int32_t*)(b + o_status);
-  c->nkps = c->status + 0;
-  c->flags = c->status + 1;
-  c->total = c->status + 2;
-  c->waves = c->status + 3;
-  c->nb_count = (int32_t*)(b + o_nbc);
-  c->nb_list = (int32_t*)(b + o_nbl);
-  c->nms_mem = (int32_t*)(b + o_mem);
-  c->nms_minnb = (int32_t*)(b + o_minnb);
-  c->nms_code = (int32_t*)(b + o_code);
-  c->nms_act = (int32_t*)(b + o_act);
   c->kps = (akz_kp*)(b + o_kps);
   c->desc = (uint8_t*)(b + o_desc);
+  akz_det_bufs& D = c->db;
+  D.seg_count = (int32_t*)(b + o_segc);
+  D.seg_off = (int32_t*)(b + o_sego);
+  D.seg_slots = (uint8_t*)(b + o_slots);
+  D.scan_bsum = (int32_t*)(b + o_scan);
+  D.scan_total = D.scan_bsum + nblk_seg + 1;
+  D.level_start = c->level_start;
+  D.ncand = c->ncand;
+  D.cand = c->cand;
+  D.nb_count = (int32_t*)(b + o_nbc);
+  D.nb_off = (int32_t*)(b + o_nbo);
+  D.nb_bsum = (int32_t*)(b + o_nbs);
+  D.nb_total = D.nb_bsum + nblk_nb + 1;
+  D.nb_list = (int32_t*)(b + o_nbl);
+  D.mem = (int32_t*)(b + o_mem);
+  D.minnb = (int32_t*)(b + o_minnb);
+  D.code = (int32_t*)(b + o_code);
+  D.act = (int32_t*)(b + o_act);
+  D.kps = c->kps;
+  D.status = c->status;
+  D.status_all = (int32_t*)(b + c->zs * (size_t)c->batch);
+  D.cap = c->cap;
+  D.nb_cap = (int)nb_cap;
+  D.max_waves = c->opts.nms_max_waves;
+  D.zs = c->zs;
   AKZ_HIP_TRY(hipMemsetAsync(c->slab, 0, c->slab_bytes, c->stream));
-  AKZ_HIP_TRY(hipHostMalloc((void**)&c->h_small, 64, hipHostMallocDefault));
+  AKZ_HIP_TRY(hipHostMalloc((void**)&c->h_small, 16 * (size_t)c->batch, hipHostMallocDefault));
   // kernel parameter blocks
   for (int i = 0; i < c->nlev; i++) {
     Level& L = c->lev[i];
@@ -322,6 +334,8 @@ akz_status allocate(akz_ctx* c) {
     c->dp.pitch[i] = L.pitch;
   }
   c->dp.patsize = c->opts.descriptor_pattern_size;
+  c->dp.zs = c->zs;
+  c->det.zs = c->zs;
   c->det.thr = std::max(c->opts.dthreshold, c->opts.min_dthreshold);
   float wgt = 10.0 / 3.0;  // HessianDeterminant, cuda_akaze.cu:706-708
   c->det.fac1 = 1.0 / (2.0 * (wgt + 2.0));
@@ -338,17 +352,19 @@ inline void record(akz_ctx* c, int e) {
   if (!c->capturing) hipEventRecord(c->ev[e], c->stream);
 }
 
-void enqueue_scale_space(akz_ctx* c, const float* img, int in_pitch) {
+// Images 0..nz-1 of the batch: input z at img + z * in_zs bytes.
+void enqueue_scale_space(akz_ctx* c, const float* img, int in_pitch, size_t in_zs, int nz) {
   hipStream_t s = c->stream;
+  const akz_batch bt{nz, c->zs};
   Level& L0 = c->lev[0];
   record(c, EV_START);
   // contrast factor of the input (ContrastPercentile, cuda_akaze.cu:604-649);
   // level-0 Lx / Ly serve as blur / gradient scratch until detection
-  akz_launch_gauss(s, img, in_pitch, L0.Lx, L0.pitch, L0.w, L0.h, 2, c->taps_c);
+  akz_launch_gauss(s, img, in_pitch, in_zs, L0.Lx, L0.pitch, L0.w, L0.h, 2, c->taps_c, bt);
   akz_launch_contrast(s, L0.Lx, L0.pitch, L0.Ly, L0.w, L0.h, std::min(c->opts.kcontrast_nbins, 512),
-                      c->opts.kcontrast_percentile, c->maxbits, c->blockmax, c->hist, c->kc, c->nlev, c->octs);
+                      c->opts.kcontrast_percentile, c->maxbits, c->blockmax, c->hist, c->kc, c->nlev, c->octs, bt);
   // level 0: Gaussian of soffset; Lsmooth aliases Lt (AKAZE.cpp:234-236)
-  akz_launch_gauss(s, img, in_pitch, L0.Lt, L0.pitch, L0.w, L0.h, c->R0, c->taps0);
+  akz_launch_gauss(s, img, in_pitch, in_zs, L0.Lt, L0.pitch, L0.w, L0.h, c->R0, c->taps0, bt);
   record(c, EV_CONTRAST);
   for (int i = 1; i < c->nlev; i++) {
     Level& L = c->lev[i];
@@ -363,13 +379,13 @@ void enqueue_scale_space(akz_ctx* c, const float* img, int in_pitch) {
     if (L.octave > P.octave) {
       float* hs = (nl % 2 == 1) ? c->fed_tmp : L.Lt;  // must differ from first_dst
       if (n == 0) hs = L.Lt;
-      akz_launch_halfsample(s, P.Lt, P.w, P.h, P.pitch, hs, L.w, L.h, L.pitch);
+      akz_launch_halfsample(s, P.Lt, P.w, P.h, P.pitch, hs, L.w, L.h, L.pitch, bt);
       base = hs;
     } else {
       base = P.Lt;  // Copy (AKAZE.cpp:258) is folded into the first step's read
-      if (n == 0) akz_launch_copy2d(s, P.Lt, P.pitch, L.Lt, L.pitch, L.w, L.h);
+      if (n == 0) akz_launch_copy2d(s, P.Lt, P.pitch, L.Lt, L.pitch, L.w, L.h, bt);
     }
-    akz_launch_smooth_flow(s, base, L.pitch, L.Lsm, c->g, L.w, L.h, c->opts.diffusivity, c->kc + i, c->taps1);
+    akz_launch_smooth_flow(s, base, L.pitch, L.Lsm, c->g, L.w, L.h, c->opts.diffusivity, c->kc + i, c->taps1, bt);
     const float* cur = base;
     float* dst = first_dst;
     if (c->fed_profile) hipEventRecord(c->fed_ev[2 * i], s);
@@ -378,9 +394,9 @@ void enqueue_scale_space(akz_ctx* c, const float* img, int in_pitch) {
       float ht[AKZ_MAX_TAU];
       for (int q = 0; q < k; q++) ht[q] = 0.5 * L.tau[j + q];
       if (bs == 1)
-        akz_launch_fed_step(s, cur, c->g, dst, L.pitch, L.w, L.h, ht[0]);
+        akz_launch_fed_step(s, cur, c->g, dst, L.pitch, L.w, L.h, ht[0], bt);
       else
-        akz_launch_fed_multi(s, cur, c->g, dst, L.pitch, L.w, L.h, ht, k);
+        akz_launch_fed_multi(s, cur, c->g, dst, L.pitch, L.w, L.h, ht, k, bt);
       cur = dst;
       dst = (dst == L.Lt) ? c->fed_tmp : L.Lt;
     }
@@ -390,33 +406,29 @@ void enqueue_scale_space(akz_ctx* c, const float* img, int in_pitch) {
 }
 
 // Feature_Detection (AKAZE.cpp:316-378) + FilterExtrema (cuda_akaze.cu:1318-1402).
-void enqueue_detect(akz_ctx* c) {
+void enqueue_detect(akz_ctx* c, int nz) {
   hipStream_t s = c->stream;
   int smax = 0;
   for (int i = 0; i < c->nlev; i++) smax = std::max(smax, c->lev[i].deriv_step);
   if (smax <= akz_detect_max_step()) {
-    akz_launch_detect_fused(s, c->det32, c->seg_count, c->seg_slots);
+    akz_launch_detect_fused(s, c->det32, c->db, nz);
     record(c, EV_DERIV);
   } else {
-    akz_launch_hessian(s, c->det);
+    akz_launch_hessian(s, c->det, nz);
     record(c, EV_DERIV);
-    akz_launch_extrema(s, c->det, c->seg_count, c->seg_slots);
+    akz_launch_extrema(s, c->det, c->db, nz);
   }
-  akz_launch_scan(s, c->det, c->seg_count, c->seg_off, c->level_start, c->ncand, c->cap, c->scan_scratch,
-                  c->total, c->flags);
-  akz_launch_scatter(s, c->det, c->seg_count, c->seg_off, c->seg_slots, c->cand, c->cap);
+  akz_launch_candidates(s, c->det, c->db, nz);
   record(c, EV_EXTREMA);
-  akz_launch_neighbors(s, c->cand, c->ncand, c->level_start, c->nb_count, c->nb_list, c->flags, c->cap);
-  akz_launch_nms(s, c->cand, c->ncand, c->nb_count, c->nb_list, c->nms_mem, c->nms_minnb, c->nms_code, c->nms_act,
-                 c->kps, c->nkps, c->waves, c->opts.nms_max_waves, c->cap);
+  akz_launch_suppression(s, c->db, nz);
   record(c, EV_NMS);
 }
 
 // Compute_Descriptors (AKAZE.cpp:405-449): orientation, then MLDB.
-void enqueue_describe(akz_ctx* c) {
+void enqueue_describe(akz_ctx* c, int nz) {
   hipStream_t s = c->stream;
-  akz_launch_orientation(s, c->dp, c->kps, c->nkps, c->cap);
-  akz_launch_mldb(s, c->dp, c->kps, c->nkps, c->desc, c->cap);
+  akz_launch_orientation(s, c->dp, c->kps, c->status + AKZ_ST_KEPT, c->cap, nz);
+  akz_launch_mldb(s, c->dp, c->kps, c->status + AKZ_ST_KEPT, c->desc, c->cap, nz);
   record(c, EV_DESC);
 }
 
@@ -429,14 +441,25 @@ float ev_ms(akz_ctx* c, int a, int b) {
   return ms;
 }
 
-// Waits for the stream and reads the small status block: kept count,
-// overflow flags, total candidates, waves.
-akz_status sync_status(akz_ctx* c, int* nk) {
-  AKZ_HIP_TRY(hipMemcpyAsync(c->h_small, c->status, 16, hipMemcpyDeviceToHost, c->stream));
+// Waits for the stream and reads the status blocks of images 0..nz-1 (one
+// copy): kept count, overflow flags, total candidates, waves.
+akz_status sync_status(akz_ctx* c, int nz) {
+  AKZ_HIP_TRY(hipMemcpyAsync(c->h_small, c->db.status_all, 16 * (size_t)nz, hipMemcpyDeviceToHost, c->stream));
   AKZ_HIP_TRY(hipStreamSynchronize(c->stream));
-  if (nk) *nk = c->h_small[0];
-  if (c->h_small[1] != 0 || c->h_small[2] > c->cap) return AKZ_ERR_OVERFLOW;
   return AKZ_OK;
+}
+
+// Result status of image z after sync_status.
+akz_status image_status(const akz_ctx* c, int z, int* nk) {
+  const int32_t* st = c->h_small + 4 * z;
+  if (nk) *nk = st[AKZ_ST_KEPT];
+  if (st[AKZ_ST_FLAGS] != 0 || st[AKZ_ST_TOTAL] > c->cap) return AKZ_ERR_OVERFLOW;
+  return AKZ_OK;
+}
+
+template <class T>
+T* zptr(const akz_ctx* c, T* p, int z) {
+  return reinterpret_cast<T*>(reinterpret_cast<uintptr_t>(p) + (size_t)z * c->zs);
 }
 
 }  // namespace
@@ -488,12 +511,13 @@ akz_status akz_device_count(int* n) {
   return AKZ_OK;
 }
 
-akz_status akz_create(const akz_options* opts, akz_ctx** out) {
-  if (!opts || !out) return AKZ_ERR_ARG;
+akz_status akz_create_batch(const akz_options* opts, int batch, akz_ctx** out) {
+  if (!opts || !out || batch < 1 || batch > AKZ_MAX_BATCH) return AKZ_ERR_ARG;
   *out = nullptr;
   akz_ctx* c = new (std::nothrow) akz_ctx();
   if (!c) return AKZ_ERR_OOM;
   c->opts = *opts;
+  c->batch = batch;
   akz_status st = compute_geometry(c);
   if (st != AKZ_OK) { delete c; return st; }
   int ndev = 0;
@@ -513,6 +537,8 @@ akz_status akz_create(const akz_options* opts, akz_ctx** out) {
   gauss_taps(1.0, 2, &c->taps1);
   gauss_taps(1.0f, 2, &c->taps_c);
   akz_describe_init_tables();
+  c->graph_exec.assign(batch, nullptr);
+  c->graph.assign(batch, nullptr);
   st = allocate(c);
   if (st != AKZ_OK) { akz_destroy(c); return st; }
   if (hipStreamSynchronize(c->stream) != hipSuccess) { akz_destroy(c); return AKZ_ERR_HIP; }
@@ -520,11 +546,15 @@ akz_status akz_create(const akz_options* opts, akz_ctx** out) {
   return AKZ_OK;
 }
 
+akz_status akz_create(const akz_options* opts, akz_ctx** out) { return akz_create_batch(opts, 1, out); }
+
 akz_status akz_destroy(akz_ctx* c) {
   if (!c) return AKZ_ERR_ARG;
   if (c->stream) hipStreamSynchronize(c->stream);
-  if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
-  if (c->graph) hipGraphDestroy(c->graph);
+  for (size_t i = 0; i < c->graph_exec.size(); i++)
+    if (c->graph_exec[i]) hipGraphExecDestroy(c->graph_exec[i]);
+  for (size_t i = 0; i < c->graph.size(); i++)
+    if (c->graph[i]) hipGraphDestroy(c->graph[i]);
   for (int i = 0; i < EV_COUNT; i++)
     if (c->ev[i]) hipEventDestroy(c->ev[i]);
   for (int i = 0; i < 2 * AKZ_MAX_LEVELS; i++)
@@ -534,6 +564,12 @@ akz_status akz_destroy(akz_ctx* c) {
   if (c->h_stage) hipHostFree(c->h_stage);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
+  return AKZ_OK;
+}
+
+akz_status akz_batch_capacity(const akz_ctx* c, int* batch) {
+  if (!c || !batch) return AKZ_ERR_ARG;
+  *batch = c->batch;
   return AKZ_OK;
 }
 
@@ -561,9 +597,10 @@ akz_status akz_build_scale_space(akz_ctx* c, const float* img, size_t pitch_byte
   if (sp < sizeof(float) * L0.w) return AKZ_ERR_ARG;
   AKZ_HIP_TRY(hipMemcpy2DAsync(c->img_in, sizeof(float) * L0.pitch, img, sp, sizeof(float) * L0.w, L0.h,
                                hipMemcpyHostToDevice, c->stream));
-  enqueue_scale_space(c, c->img_in, L0.pitch);
+  enqueue_scale_space(c, c->img_in, L0.pitch, 0, 1);
   AKZ_HIP_TRY(hipGetLastError());
   c->state = 1;
+  c->last_nz = 1;
   c->timing_valid = false;
   return AKZ_OK;
 }
@@ -573,9 +610,10 @@ akz_status akz_build_scale_space_device(akz_ctx* c, const float* d_img, size_t p
   const Level& L0 = c->lev[0];
   size_t sp = pitch_bytes ? pitch_bytes : sizeof(float) * L0.w;
   if (sp < sizeof(float) * L0.w || sp % sizeof(float)) return AKZ_ERR_ARG;
-  enqueue_scale_space(c, d_img, (int)(sp / sizeof(float)));
+  enqueue_scale_space(c, d_img, (int)(sp / sizeof(float)), 0, 1);
   AKZ_HIP_TRY(hipGetLastError());
   c->state = 1;
+  c->last_nz = 1;
   c->timing_valid = false;
   return AKZ_OK;
 }
@@ -583,10 +621,12 @@ akz_status akz_build_scale_space_device(akz_ctx* c, const float* d_img, size_t p
 akz_status akz_detect(akz_ctx* c, int* n_out) {
   if (!c) return AKZ_ERR_ARG;
   if (c->state < 1) return AKZ_ERR_STATE;
-  enqueue_detect(c);
+  enqueue_detect(c, 1);
   AKZ_HIP_TRY(hipGetLastError());
+  akz_status st = sync_status(c, 1);
+  if (st != AKZ_OK) return st;
   int nk = 0;
-  akz_status st = sync_status(c, &nk);
+  st = image_status(c, 0, &nk);
   if (n_out) *n_out = nk;
   c->state = 2;
   return st;
@@ -595,17 +635,19 @@ akz_status akz_detect(akz_ctx* c, int* n_out) {
 akz_status akz_describe(akz_ctx* c, akz_keypoint* kps, uint8_t* desc, int cap, int* n_out) {
   if (!c || cap < 0) return AKZ_ERR_ARG;
   if (c->state < 2) return AKZ_ERR_STATE;
-  enqueue_describe(c);
+  enqueue_describe(c, 1);
   AKZ_HIP_TRY(hipGetLastError());
   c->state = 3;
   return akz_read_results(c, kps, desc, cap, n_out);
 }
 
-akz_status akz_read_results(akz_ctx* c, akz_keypoint* kps, uint8_t* desc, int cap, int* n_out) {
-  if (!c || cap < 0) return AKZ_ERR_ARG;
+akz_status akz_read_batch_results(akz_ctx* c, int z, akz_keypoint* kps, uint8_t* desc, int cap, int* n_out) {
+  if (!c || cap < 0 || z < 0 || z >= c->batch) return AKZ_ERR_ARG;
+  if (z >= c->last_nz) return AKZ_ERR_STATE;  // image not part of the last run
+  akz_status st = sync_status(c, z + 1);
+  if (st != AKZ_OK) return st;
   int nk = 0;
-  akz_status st = sync_status(c, &nk);
-  if (st != AKZ_OK && st != AKZ_ERR_OVERFLOW) return st;
+  st = image_status(c, z, &nk);
   int m = std::min(nk, cap);
   if (m > 0 && (kps || desc)) {
     // one contiguous copy each into pinned staging, then repack 64-byte rows to 61
@@ -618,8 +660,9 @@ akz_status akz_read_results(akz_ctx* c, akz_keypoint* kps, uint8_t* desc, int ca
       AKZ_HIP_TRY(hipHostMalloc((void**)&c->h_stage, want, hipHostMallocDefault));
       c->h_stage_bytes = want;
     }
-    if (kps) AKZ_HIP_TRY(hipMemcpyAsync(c->h_stage, c->kps, kb, hipMemcpyDeviceToHost, c->stream));
-    if (desc) AKZ_HIP_TRY(hipMemcpyAsync(c->h_stage + kb, c->desc, db, hipMemcpyDeviceToHost, c->stream));
+    if (kps) AKZ_HIP_TRY(hipMemcpyAsync(c->h_stage, zptr(c, c->kps, z), kb, hipMemcpyDeviceToHost, c->stream));
+    if (desc)
+      AKZ_HIP_TRY(hipMemcpyAsync(c->h_stage + kb, zptr(c, c->desc, z), db, hipMemcpyDeviceToHost, c->stream));
     AKZ_HIP_TRY(hipStreamSynchronize(c->stream));
     if (kps) memcpy(kps, c->h_stage, kb);
     if (desc)
@@ -629,44 +672,79 @@ akz_status akz_read_results(akz_ctx* c, akz_keypoint* kps, uint8_t* desc, int ca
   return (nk > cap) ? AKZ_ERR_OVERFLOW : st;
 }
 
-akz_status akz_process_device_async(akz_ctx* c, const float* d_img, size_t pitch_bytes) {
-  if (!c || !d_img) return AKZ_ERR_ARG;
+akz_status akz_read_results(akz_ctx* c, akz_keypoint* kps, uint8_t* desc, int cap, int* n_out) {
+  return akz_read_batch_results(c, 0, kps, desc, cap, n_out);
+}
+
+akz_status akz_read_batch_counts(akz_ctx* c, int n, int* counts, int* statuses) {
+  if (!c || n < 1 || n > c->batch) return AKZ_ERR_ARG;
+  if (n > c->last_nz) return AKZ_ERR_STATE;
+  akz_status st = sync_status(c, n);
+  if (st != AKZ_OK) return st;
+  for (int z = 0; z < n; z++) {
+    int nk = 0;
+    akz_status s = image_status(c, z, &nk);
+    if (counts) counts[z] = nk;
+    if (statuses) statuses[z] = (int)s;
+  }
+  return AKZ_OK;
+}
+
+akz_status akz_process_batch_device_async(akz_ctx* c, const float* d_imgs, size_t pitch_bytes,
+                                          size_t image_stride_bytes, int n) {
+  if (!c || !d_imgs || n < 1 || n > c->batch) return AKZ_ERR_ARG;
   const Level& L0 = c->lev[0];
   size_t sp = pitch_bytes ? pitch_bytes : sizeof(float) * L0.w;
   if (sp < sizeof(float) * L0.w || sp % sizeof(float)) return AKZ_ERR_ARG;
+  size_t is = image_stride_bytes ? image_stride_bytes : sp * L0.h;
+  if (n > 1 && is < sp * L0.h) return AKZ_ERR_ARG;
   if (!c->graph_mode) {
-    enqueue_scale_space(c, d_img, (int)(sp / sizeof(float)));
-    enqueue_detect(c);
-    enqueue_describe(c);
+    enqueue_scale_space(c, d_imgs, (int)(sp / sizeof(float)), is, n);
+    enqueue_detect(c, n);
+    enqueue_describe(c, n);
     AKZ_HIP_TRY(hipGetLastError());
   } else {
-    // the captured graph reads the context's own input buffer
-    if (d_img != c->img_in || (int)(sp / sizeof(float)) != L0.pitch)
-      AKZ_HIP_TRY(hipMemcpy2DAsync(c->img_in, sizeof(float) * L0.pitch, d_img, sp, sizeof(float) * L0.w, L0.h,
-                                   hipMemcpyDeviceToDevice, c->stream));
-    if (!c->graph_exec) {
+    // the captured graphs read the context's own input buffers
+    if (d_imgs != c->img_in || (int)(sp / sizeof(float)) != L0.pitch || (n > 1 && is != c->zs)) {
+      for (int z = 0; z < n; z++)
+        AKZ_HIP_TRY(hipMemcpy2DAsync(zptr(c, c->img_in, z), sizeof(float) * L0.pitch,
+                                     reinterpret_cast<const char*>(d_imgs) + (size_t)z * is, sp, sizeof(float) * L0.w,
+                                     L0.h, hipMemcpyDeviceToDevice, c->stream));
+    }
+    hipGraphExec_t& ge = c->graph_exec[n - 1];
+    if (!ge) {
       AKZ_HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
       c->capturing = true;
-      enqueue_scale_space(c, c->img_in, L0.pitch);
-      enqueue_detect(c);
-      enqueue_describe(c);
+      enqueue_scale_space(c, c->img_in, L0.pitch, c->zs, n);
+      enqueue_detect(c, n);
+      enqueue_describe(c, n);
       c->capturing = false;
-      hipError_t e = hipStreamEndCapture(c->stream, &c->graph);
+      hipError_t e = hipStreamEndCapture(c->stream, &c->graph[n - 1]);
       if (e != hipSuccess) return akz_hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
-      AKZ_HIP_TRY(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
+      AKZ_HIP_TRY(hipGraphInstantiate(&ge, c->graph[n - 1], nullptr, nullptr, 0));
     }
-    AKZ_HIP_TRY(hipGraphLaunch(c->graph_exec, c->stream));
+    AKZ_HIP_TRY(hipGraphLaunch(ge, c->stream));
   }
   c->state = 3;
+  c->last_nz = n;
   c->timing_valid = false;
   return AKZ_OK;
 }
 
-akz_status akz_input_buffer(akz_ctx* c, float** d_ptr, size_t* pitch_bytes) {
+akz_status akz_process_device_async(akz_ctx* c, const float* d_img, size_t pitch_bytes) {
+  return akz_process_batch_device_async(c, d_img, pitch_bytes, 0, 1);
+}
+
+akz_status akz_batch_input_buffer(akz_ctx* c, float** d_ptr, size_t* pitch_bytes, size_t* image_stride_bytes) {
   if (!c || !d_ptr) return AKZ_ERR_ARG;
   *d_ptr = c->img_in;
   if (pitch_bytes) *pitch_bytes = sizeof(float) * c->lev[0].pitch;
+  if (image_stride_bytes) *image_stride_bytes = c->zs;
   return AKZ_OK;
+}
+
+akz_status akz_input_buffer(akz_ctx* c, float** d_ptr, size_t* pitch_bytes) {
+  return akz_batch_input_buffer(c, d_ptr, pitch_bytes, nullptr);
 }
 
 akz_status akz_set_graph_mode(akz_ctx* c, int enabled) {
@@ -681,23 +759,33 @@ akz_status akz_wait(akz_ctx* c) {
   return AKZ_OK;
 }
 
-akz_status akz_results_device(akz_ctx* c, const int32_t** d_count, const akz_keypoint** d_kps,
-                              const uint8_t** d_desc64) {
-  if (!c) return AKZ_ERR_ARG;
-  if (d_count) *d_count = c->nkps;
-  if (d_kps) *d_kps = reinterpret_cast<const akz_keypoint*>(c->kps);
-  if (d_desc64) *d_desc64 = c->desc;
+akz_status akz_batch_results_device(akz_ctx* c, int z, const int32_t** d_count, const akz_keypoint** d_kps,
+                                    const uint8_t** d_desc64) {
+  if (!c || z < 0 || z >= c->batch) return AKZ_ERR_ARG;
+  if (d_count) *d_count = zptr(c, c->status, z) + AKZ_ST_KEPT;
+  if (d_kps) *d_kps = reinterpret_cast<const akz_keypoint*>(zptr(c, c->kps, z));
+  if (d_desc64) *d_desc64 = zptr(c, c->desc, z);
   return AKZ_OK;
 }
 
+akz_status akz_results_device(akz_ctx* c, const int32_t** d_count, const akz_keypoint** d_kps,
+                              const uint8_t** d_desc64) {
+  return akz_batch_results_device(c, 0, d_count, d_kps, d_desc64);
+}
+
 akz_status akz_read_level(akz_ctx* c, int lvl, int which, float* dst) {
-  if (!c || !dst || lvl < 0 || lvl >= c->nlev || which < 0 || which > 4) return AKZ_ERR_ARG;
+  return akz_read_batch_level(c, 0, lvl, which, dst);
+}
+
+akz_status akz_read_batch_level(akz_ctx* c, int z, int lvl, int which, float* dst) {
+  if (!c || !dst || lvl < 0 || lvl >= c->nlev || which < 0 || which > 4 || z < 0 || z >= c->batch)
+    return AKZ_ERR_ARG;
   AKZ_HIP_TRY(hipStreamSynchronize(c->stream));
   const Level& L = c->lev[lvl];
   const float* src = which == 0 ? L.Lt : which == 1 ? (lvl == 0 ? L.Lt : L.Lsm) : which == 2 ? L.Lx
                      : which == 3 ? L.Ly : L.det;
-  AKZ_HIP_TRY(hipMemcpy2D(dst, sizeof(float) * L.w, src, sizeof(float) * L.pitch, sizeof(float) * L.w, L.h,
-                          hipMemcpyDeviceToHost));
+  AKZ_HIP_TRY(hipMemcpy2D(dst, sizeof(float) * L.w, zptr(c, src, z), sizeof(float) * L.pitch, sizeof(float) * L.w,
+                          L.h, hipMemcpyDeviceToHost));
   return AKZ_OK;
 }
 
@@ -715,7 +803,7 @@ akz_status akz_read_candidates(akz_ctx* c, akz_keypoint* cand, int cap, int* n_o
   AKZ_HIP_TRY(hipStreamSynchronize(c->stream));
   int n = 0, w = 0;
   AKZ_HIP_TRY(hipMemcpy(&n, c->ncand, 4, hipMemcpyDeviceToHost));
-  AKZ_HIP_TRY(hipMemcpy(&w, c->waves, 4, hipMemcpyDeviceToHost));
+  AKZ_HIP_TRY(hipMemcpy(&w, c->status + AKZ_ST_WAVES, 4, hipMemcpyDeviceToHost));
   int m = std::min(n, cap);
   if (cand && m > 0) AKZ_HIP_TRY(hipMemcpy(cand, c->cand, sizeof(akz_kp) * m, hipMemcpyDeviceToHost));
   if (level_start) AKZ_HIP_TRY(hipMemcpy(level_start, c->level_start, 4 * (c->nlev + 1), hipMemcpyDeviceToHost));
@@ -743,22 +831,24 @@ akz_status akz_get_timing(akz_ctx* c, akz_timing* t) {
   return AKZ_OK;
 }
 
-akz_status akz_profile_fed(akz_ctx* c, const float* d_img, size_t pitch_bytes, float* fed_ms, double* fed_bytes,
-                           int* fed_launches) {
-  if (!c || !d_img) return AKZ_ERR_ARG;
+akz_status akz_profile_fed_batch(akz_ctx* c, const float* d_imgs, size_t pitch_bytes, size_t image_stride_bytes,
+                                 int n, float* fed_ms, double* fed_bytes, int* fed_launches) {
+  if (!c || !d_imgs || n < 1 || n > c->batch) return AKZ_ERR_ARG;
   const Level& L0 = c->lev[0];
   size_t sp = pitch_bytes ? pitch_bytes : sizeof(float) * L0.w;
   if (sp < sizeof(float) * L0.w || sp % sizeof(float)) return AKZ_ERR_ARG;
+  size_t is = image_stride_bytes ? image_stride_bytes : sp * L0.h;
   for (int i = 0; i < 2 * c->nlev; i++)
     if (!c->fed_ev[i]) AKZ_HIP_TRY(hipEventCreate(&c->fed_ev[i]));
   c->fed_profile = true;
-  enqueue_scale_space(c, d_img, (int)(sp / sizeof(float)));
+  enqueue_scale_space(c, d_imgs, (int)(sp / sizeof(float)), is, n);
   c->fed_profile = false;
-  enqueue_detect(c);
-  enqueue_describe(c);
+  enqueue_detect(c, n);
+  enqueue_describe(c, n);
   AKZ_HIP_TRY(hipGetLastError());
   AKZ_HIP_TRY(hipStreamSynchronize(c->stream));
   c->state = 3;
+  c->last_nz = n;
   float total = 0.0f;
   double bytes = 0.0;
   int launches = 0;
@@ -767,13 +857,18 @@ akz_status akz_profile_fed(akz_ctx* c, const float* d_img, size_t pitch_bytes, f
     AKZ_HIP_TRY(hipEventElapsedTime(&ms, c->fed_ev[2 * i], c->fed_ev[2 * i + 1]));
     total += ms;
     // algorithmic traffic of one step: read L and g, write L (4 B each per pixel)
-    bytes += 12.0 * c->lev[i].w * c->lev[i].h * c->lev[i].nsteps;
+    bytes += 12.0 * c->lev[i].w * c->lev[i].h * c->lev[i].nsteps * n;
     launches += (c->lev[i].nsteps + c->fed_block - 1) / c->fed_block;
   }
   if (fed_ms) *fed_ms = total;
   if (fed_bytes) *fed_bytes = bytes;
   if (fed_launches) *fed_launches = launches;
   return AKZ_OK;
+}
+
+akz_status akz_profile_fed(akz_ctx* c, const float* d_img, size_t pitch_bytes, float* fed_ms, double* fed_bytes,
+                           int* fed_launches) {
+  return akz_profile_fed_batch(c, d_img, pitch_bytes, 0, 1, fed_ms, fed_bytes, fed_launches);
 }
 
 akz_status akz_copy_kernel_run(const float* d_src, float* d_dst, size_t nfloats, int iters, float* avg_ms) {
@@ -785,7 +880,7 @@ akz_status akz_copy_kernel_run(const float* d_src, float* d_dst, size_t nfloats,
   AKZ_HIP_TRY(hipEventCreate(&a));
   AKZ_HIP_TRY(hipEventCreate(&b));
   AKZ_HIP_TRY(hipEventRecord(a, nullptr));
-  for (int i = 0; i < iters; i++) akz_launch_copy2d(nullptr, d_src, w, d_dst, w, w, h);
+  for (int i = 0; i < iters; i++) akz_launch_copy2d(nullptr, d_src, w, d_dst, w, w, h, akz_batch{1, 0});
   AKZ_HIP_TRY(hipEventRecord(b, nullptr));
   AKZ_HIP_TRY(hipEventSynchronize(b));
   float ms = 0.0f;

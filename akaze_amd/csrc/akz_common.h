@@ -16,7 +16,7 @@
 
 #define AKZ_MAX_LEVELS 32
 #define AKZ_MAX_TAU 256
-#define AKZ_NMS_MAX_NB 64     // neighbour-list capacity per candidate
+#define AKZ_NB_PER_CAND 4     // CSR neighbour-list capacity: this many entries per candidate on average
 #define AKZ_DESC_STRIDE 64    // descriptor rows are 61 bytes padded to 64
 
 #define AKZ_HD __host__ __device__ __forceinline__
@@ -39,6 +39,24 @@ struct akz_level_desc {
   int seg_base;             // offset of this level's row-segment counts
   int nseg_x;               // row segments per row: ceil(w / 64)
 };
+
+// Status block of one image (read back with one copy): kept keypoints,
+// overflow flags, candidates before the capacity cut, suppression waves.
+enum { AKZ_ST_KEPT = 0, AKZ_ST_FLAGS = 1, AKZ_ST_TOTAL = 2, AKZ_ST_WAVES = 3 };
+
+// Batch of same-size images: image z's arrays sit z * zs bytes after image
+// 0's, and kernels launched with grid.z = nz pick their image by blockIdx.z.
+struct akz_batch {
+  int nz;
+  size_t zs;
+};
+
+#ifdef __HIPCC__
+template <class T>
+__device__ __forceinline__ T* akz_zoff(T* p, size_t zs) {
+  return reinterpret_cast<T*>(reinterpret_cast<uintptr_t>(p) + (size_t)blockIdx.z * zs);
+}
+#endif
 
 AKZ_HD int akz_imin(int a, int b) { return a < b ? a : b; }
 AKZ_HD int akz_imax(int a, int b) { return a > b ? a : b; }

@@ -3,6 +3,8 @@
 // Both kernels are persistent-style (grid-stride over the keypoint count read
 // from device memory), so they are launched with a fixed grid and need no host
 // knowledge of how many keypoints the detector kept.
+#include <algorithm>
+
 #include "akz_kernels.h"
 
 namespace {
@@ -48,7 +50,8 @@ __global__ __launch_bounds__(64 * OR_WAVES) void k_orientation(akz_desc_params p
   __shared__ float s_rx[OR_WAVES][42], s_ry[OR_WAVES][42];
   __shared__ float s_ex[OR_WAVES][42], s_ey[OR_WAVES][42];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int n = *nkps_p;
+  kps = akz_zoff(kps, p.zs);
+  const int n = *akz_zoff(nkps_p, p.zs);
   // the loop bound is uniform over the block so every barrier is reached by all waves
   for (int base = blockIdx.x * OR_WAVES; base < n; base += gridDim.x * OR_WAVES) {
     const int kp = base + wv;
@@ -56,8 +59,8 @@ __global__ __launch_bounds__(64 * OR_WAVES) void k_orientation(akz_desc_params p
     if (active) {
       akz_kp k = kps[kp];
       const int lev = k.class_id;
-      const float* dxd = p.Lx[lev];
-      const float* dyd = p.Ly[lev];
+      const float* dxd = akz_zoff(p.Lx[lev], p.zs);
+      const float* dyd = akz_zoff(p.Ly[lev], p.zs);
       const int pitch = p.pitch[lev], w = p.w[lev], h = p.h[lev];
       const int octave = k.octave;
       const int step = (int)(0.5f * k.size + 0.5f) >> octave;
@@ -143,7 +146,9 @@ __global__ __launch_bounds__(64 * MLDB_WAVES) void k_mldb(akz_desc_params p, con
   __shared__ float acc[87 * 64];
   __shared__ float vals[88];
   const int lane = threadIdx.x & 63;
-  const int n = *nkps_p;
+  kps = akz_zoff(kps, p.zs);
+  desc = akz_zoff(desc, p.zs);
+  const int n = *akz_zoff(nkps_p, p.zs);
   const int size2 = p.patsize;
   const int size3 = (int)ceilf(2.0f * p.patsize / 3.0f);
   const int size4 = (int)ceilf(0.5f * p.patsize);
@@ -151,9 +156,9 @@ __global__ __launch_bounds__(64 * MLDB_WAVES) void k_mldb(akz_desc_params p, con
   for (int kp = blockIdx.x; kp < n; kp += gridDim.x) {
     const akz_kp k = kps[kp];
     const int lev = k.class_id;
-    const float* imd = p.Lt[lev];
-    const float* dxd = p.Lx[lev];
-    const float* dyd = p.Ly[lev];
+    const float* imd = akz_zoff(p.Lt[lev], p.zs);
+    const float* dxd = akz_zoff(p.Lx[lev], p.zs);
+    const float* dyd = akz_zoff(p.Ly[lev], p.zs);
     const int pitch = p.pitch[lev], w = p.w[lev], h = p.h[lev];
     const float iratio = 1.0f / (1 << k.octave);
     const int scale = (int)(0.5f * k.size * iratio + 0.5f);
@@ -254,14 +259,23 @@ void akz_describe_init_tables() {
   g_weights_ready = true;
 }
 
-void akz_launch_orientation(hipStream_t s, const akz_desc_params& p, akz_kp* kps, const int32_t* nkps, int cap) {
+// Blocks per image shrink as the batch grows (the kernels stride over the
+// keypoints), so a batch launch stays near 4096-8192 resident blocks.
+static int per_image_blocks(int want, int nz, int floor_blocks) {
+  int b = want / (nz > 0 ? nz : 1);
+  if (b < floor_blocks) b = floor_blocks;
+  return b;
+}
+
+void akz_launch_orientation(hipStream_t s, const akz_desc_params& p, akz_kp* kps, const int32_t* nkps, int cap,
+                            int nz) {
   int blocks = (cap + OR_WAVES - 1) / OR_WAVES;
-  blocks = blocks < 2048 ? (blocks > 0 ? blocks : 1) : 2048;
-  hipLaunchKernelGGL(k_orientation, dim3(blocks), dim3(64 * OR_WAVES), 0, s, p, kps, nkps, g_weights);
+  blocks = std::min(std::max(blocks, 1), per_image_blocks(2048, nz, 256));
+  hipLaunchKernelGGL(k_orientation, dim3(blocks, 1, nz), dim3(64 * OR_WAVES), 0, s, p, kps, nkps, g_weights);
 }
 
 void akz_launch_mldb(hipStream_t s, const akz_desc_params& p, const akz_kp* kps, const int32_t* nkps, uint8_t* desc64,
-                     int cap) {
-  int blocks = cap < 4096 ? (cap > 0 ? cap : 1) : 4096;
-  hipLaunchKernelGGL(k_mldb, dim3(blocks), dim3(64 * MLDB_WAVES), 0, s, p, kps, nkps, desc64);
+                     int cap, int nz) {
+  int blocks = std::min(std::max(cap, 1), per_image_blocks(4096, nz, 512));
+  hipLaunchKernelGGL(k_mldb, dim3(blocks, 1, nz), dim3(64 * MLDB_WAVES), 0, s, p, kps, nkps, desc64);
 }

@@ -2,7 +2,8 @@
 // ordered compaction, neighbour search and wave-synchronous suppression.
 //
 // All levels are handled by one launch per stage: blocks map to (level, tile)
-// through the prefix table in akz_det_params.  Candidate order is
+// through the prefix table in akz_det_params, and blockIdx.z selects the
+// image of a batch (its arrays sit z * zs bytes after image 0's).  Candidate order is
 // (level, y, x), the order the reference establishes with its per-level
 // bitonic sort (cuda_akaze.cu:882-939), produced here without sorting: each
 // 64-pixel row segment is one wavefront whose ballot gives in-segment ranks,
@@ -47,9 +48,9 @@ __global__ __launch_bounds__(256) void k_deriv(akz_det_params p) {
   if (x >= L.w) return;
   for (int y = y0 + (threadIdx.x >> 6); y < y0 + 16 && y < L.h; y += 4) {
     float dx, dy;
-    deriv_xy(p.Lsm[l], L.pitch, L.w, L.h, x, y, L.deriv_step, p.fac1, p.fac2, &dx, &dy);
-    p.Lx[l][(size_t)y * L.pitch + x] = dx;
-    p.Ly[l][(size_t)y * L.pitch + x] = dy;
+    deriv_xy(akz_zoff(p.Lsm[l], p.zs), L.pitch, L.w, L.h, x, y, L.deriv_step, p.fac1, p.fac2, &dx, &dy);
+    akz_zoff(p.Lx[l], p.zs)[(size_t)y * L.pitch + x] = dx;
+    akz_zoff(p.Ly[l], p.zs)[(size_t)y * L.pitch + x] = dy;
   }
 }
 
@@ -65,9 +66,9 @@ __global__ __launch_bounds__(256) void k_hess(akz_det_params p) {
   if (x >= L.w) return;
   for (int y = y0 + (threadIdx.x >> 6); y < y0 + 16 && y < L.h; y += 4) {
     float lxx, lyx, lyy;
-    deriv_xy(p.Lx[l], L.pitch, L.w, L.h, x, y, L.deriv_step, p.fac1, p.fac2, &lxx, &lyx);
-    deriv_xy(p.Ly[l], L.pitch, L.w, L.h, x, y, L.deriv_step, p.fac1, p.fac2, nullptr, &lyy);
-    p.det[l][(size_t)y * L.pitch + x] = lxx * lyy - lyx * lyx;
+    deriv_xy(akz_zoff(p.Lx[l], p.zs), L.pitch, L.w, L.h, x, y, L.deriv_step, p.fac1, p.fac2, &lxx, &lyx);
+    deriv_xy(akz_zoff(p.Ly[l], p.zs), L.pitch, L.w, L.h, x, y, L.deriv_step, p.fac1, p.fac2, nullptr, &lyy);
+    akz_zoff(p.det[l], p.zs)[(size_t)y * L.pitch + x] = lxx * lyy - lyx * lyx;
   }
 }
 
@@ -91,6 +92,8 @@ __device__ __forceinline__ bool is_extremum(const float* imd, int pitch, int w, 
 // the segment's count and each candidate's rank within it.
 __global__ __launch_bounds__(256) void k_extrema(akz_det_params p, int32_t* __restrict__ seg_count,
                                                  uint8_t* __restrict__ seg_slots) {
+  seg_count = akz_zoff(seg_count, p.zs);
+  seg_slots = akz_zoff(seg_slots, p.zs);
   const int l = find_level(p, blockIdx.x);
   const akz_level_desc& L = p.lev[l];
   int t = blockIdx.x - p.tile_start[l];
@@ -99,7 +102,7 @@ __global__ __launch_bounds__(256) void k_extrema(akz_det_params p, int32_t* __re
   int x = tx * 64 + lane;
   int y0 = (t / L.nseg_x) * 16;
   for (int y = y0 + (threadIdx.x >> 6); y < y0 + 16 && y < L.h; y += 4) {
-    bool pred = is_extremum(p.det[l], L.pitch, L.w, L.h, x, y, L.border, p.thr);
+    bool pred = is_extremum(akz_zoff(p.det[l], p.zs), L.pitch, L.w, L.h, x, y, L.border, p.thr);
     uint64_t mask = __ballot(pred);
     int seg = L.seg_base + y * L.nseg_x + tx;
     if (lane == 0) seg_count[seg] = __popcll(mask);
@@ -126,6 +129,8 @@ constexpr int DF_R1 = DF_TH + 2 * DF_H1, DF_R2 = DF_TH + 2 * DF_H2, DF_RD = DF_T
 
 __global__ __launch_bounds__(512) void k_detect_fused(akz_det_params p, int32_t* __restrict__ seg_count,
                                                       uint8_t* __restrict__ seg_slots) {
+  seg_count = akz_zoff(seg_count, p.zs);
+  seg_slots = akz_zoff(seg_slots, p.zs);
   __shared__ float s_l[DF_R1 * DF_RW];
   __shared__ float s_x[DF_R2 * DF_RW];
   __shared__ float s_y[DF_R2 * DF_RW];
@@ -140,7 +145,7 @@ __global__ __launch_bounds__(512) void k_detect_fused(akz_det_params p, int32_t*
   const int W1 = 64 + 2 * H1, W2 = 64 + 2 * H2;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   constexpr int NW = 8;
-  const float* lsm = p.Lsm[l];
+  const float* lsm = akz_zoff(p.Lsm[l], p.zs);
   // Lsmooth rows [y0-H1, y0+TH+H1), columns [x0-H1, x0+64+H1): every load of
   // a thread is issued before any is written to LDS
   {
@@ -213,9 +218,9 @@ __global__ __launch_bounds__(512) void k_detect_fused(akz_det_params p, int32_t*
   for (int k = threadIdx.x; k < DF_RD * 2; k += 64 * NW) det_at(k >> 1, 64 + (k & 1));
   __syncthreads();
   const int x = x0 + lane;
-  float* Lx = p.Lx[l];
-  float* Ly = p.Ly[l];
-  float* D = p.det[l];
+  float* Lx = akz_zoff(p.Lx[l], p.zs);
+  float* Ly = akz_zoff(p.Ly[l], p.zs);
+  float* D = akz_zoff(p.det[l], p.zs);
   for (int r = wv; r < DF_TH; r += NW) {
     const int y = y0 + r;
     if (y >= h) break;
@@ -245,18 +250,20 @@ __global__ __launch_bounds__(512) void k_detect_fused(akz_det_params p, int32_t*
   }
 }
 
-// Exclusive scan of the segment counts in (level, y, segment) order: block
-// sums (k_scan_reduce), a scan of the block sums (k_scan_top), then each
-// block scans its 4096 counts with its carry (k_scan_down).  Each thread owns
-// four consecutive counts, read as one int4.
+// Exclusive scans over per-image count arrays: block sums (k_xscan_reduce),
+// a scan of the block sums per image (k_xscan_top), then each block scans its
+// 4096 counts with its carry (k_xscan_down).  Counts at indices >= nvalid
+// count as zero, where nvalid is *n_ptr when given and `len` otherwise;
+// offsets are written for indices 0..nvalid, so off[nvalid] is the total.
+// Each thread owns four consecutive counts, read as one int4.
 constexpr int SCAN_T = 1024, SCAN_PER = 4, SCAN_BLK = SCAN_T * SCAN_PER;
 
-__device__ __forceinline__ int4 load4(const int32_t* a, int i, int total) {
-  if (i + 3 < total) return *reinterpret_cast<const int4*>(a + i);
+__device__ __forceinline__ int4 load4(const int32_t* a, int i, int n) {
+  if (i + 3 < n) return *reinterpret_cast<const int4*>(a + i);
   int4 v = make_int4(0, 0, 0, 0);
-  if (i < total) v.x = a[i];
-  if (i + 1 < total) v.y = a[i + 1];
-  if (i + 2 < total) v.z = a[i + 2];
+  if (i < n) v.x = a[i];
+  if (i + 1 < n) v.y = a[i + 1];
+  if (i + 2 < n) v.z = a[i + 2];
   return v;
 }
 
@@ -282,19 +289,22 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* excl, int32_t* s
   return tot;
 }
 
-__global__ __launch_bounds__(SCAN_T) void k_scan_reduce(const int32_t* __restrict__ cnt, int total,
-                                                        int32_t* __restrict__ bsum) {
+__global__ __launch_bounds__(SCAN_T) void k_xscan_reduce(const int32_t* __restrict__ cnt, int len, const int32_t* n_ptr,
+                                                         int32_t* __restrict__ bsum, size_t zs) {
   __shared__ int32_t s_w[17];
-  int i = blockIdx.x * SCAN_BLK + threadIdx.x * SCAN_PER;
-  int4 v = load4(cnt, i, total);
+  cnt = akz_zoff(cnt, zs);
+  bsum = akz_zoff(bsum, zs);
+  const int nvalid = n_ptr ? *akz_zoff(n_ptr, zs) : len;
+  const int i = blockIdx.x * SCAN_BLK + threadIdx.x * SCAN_PER;
+  int4 v = load4(cnt, i, nvalid);
   int ex;
   int tot = block_exclusive_scan(v.x + v.y + v.z + v.w, &ex, s_w);
   if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 
-__global__ __launch_bounds__(SCAN_T) void k_scan_top(int32_t* bsum, int nblk, int32_t* ncand, int32_t* total_out,
-                                                     int cap, int32_t* flags) {
+__global__ __launch_bounds__(SCAN_T) void k_xscan_top(int32_t* bsum, int nblk, int32_t* total_out, size_t zs) {
   __shared__ int32_t s_w[17];
+  bsum = akz_zoff(bsum, zs);
   int carry = 0;
   for (int base = 0; base < nblk; base += SCAN_T) {
     int i = base + threadIdx.x;
@@ -304,64 +314,67 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_top(int32_t* bsum, int nblk, in
     if (i < nblk) bsum[i] = carry + ex;
     carry += tot;
   }
-  if (threadIdx.x == 0) {
-    *total_out = carry;
-    *ncand = akz_imin(carry, cap);
-    *flags = 0;  // overflow flags of this detection run (set by k_neighbors)
-  }
+  if (threadIdx.x == 0) *akz_zoff(total_out, zs) = carry;
 }
 
-__global__ __launch_bounds__(SCAN_T) void k_scan_down(const int32_t* __restrict__ cnt, int total,
-                                                      const int32_t* __restrict__ bsum, int32_t* __restrict__ off,
-                                                      akz_det_params p, int32_t* level_start,
-                                                      const int32_t* total_p) {
+__global__ __launch_bounds__(SCAN_T) void k_xscan_down(const int32_t* __restrict__ cnt, int len, const int32_t* n_ptr,
+                                                       const int32_t* __restrict__ bsum, int32_t* __restrict__ off,
+                                                       size_t zs) {
   __shared__ int32_t s_w[17];
-  int i = blockIdx.x * SCAN_BLK + threadIdx.x * SCAN_PER;
-  int4 v = load4(cnt, i, total);
+  cnt = akz_zoff(cnt, zs);
+  bsum = akz_zoff(bsum, zs);
+  off = akz_zoff(off, zs);
+  const int nvalid = n_ptr ? *akz_zoff(n_ptr, zs) : len;
+  const int i = blockIdx.x * SCAN_BLK + threadIdx.x * SCAN_PER;
+  int4 v = load4(cnt, i, nvalid);
   int ex;
   block_exclusive_scan(v.x + v.y + v.z + v.w, &ex, s_w);
-  int o0 = bsum[blockIdx.x] + ex;
-  int4 o = make_int4(o0, o0 + v.x, o0 + v.x + v.y, o0 + v.x + v.y + v.z);
-  if (i + 3 < total) {
-    *reinterpret_cast<int4*>(off + i) = o;
-  } else {
-    if (i < total) off[i] = o.x;
-    if (i + 1 < total) off[i + 1] = o.y;
-    if (i + 2 < total) off[i + 2] = o.z;
+  const int o0 = bsum[blockIdx.x] + ex;
+  const int o[4] = {o0, o0 + v.x, o0 + v.x + v.y, o0 + v.x + v.y + v.z};
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+    if (i + k <= nvalid) off[i + k] = o[k];
+}
+
+// After the segment scan: per-level start offsets, the candidate count (cut
+// at the capacity) and the cleared overflow flags of this detection run.
+__global__ void k_seg_finish(akz_det_params p, akz_det_bufs b) {
+  const int32_t* off = akz_zoff(b.seg_off, b.zs);
+  int32_t* ls = akz_zoff(b.level_start, b.zs);
+  int32_t* st = akz_zoff(b.status, b.zs);
+  const int t = threadIdx.x;
+  if (t < p.nlev) ls[t] = off[p.lev[t].seg_base];
+  if (t == 0) {
+    const int total = off[p.seg_total];
+    ls[p.nlev] = total;
+    st[AKZ_ST_TOTAL] = total;
+    st[AKZ_ST_FLAGS] = 0;
+    *akz_zoff(b.ncand, b.zs) = akz_imin(total, b.cap);
   }
-  // level start offsets: the block holding a level's first segment writes it
-  for (int l = 0; l < p.nlev; l++) {
-    int sb = p.lev[l].seg_base;
-    if (sb >= i && sb < i + SCAN_PER && sb < total) {
-      int k = sb - i;
-      level_start[l] = k == 0 ? o.x : (k == 1 ? o.y : (k == 2 ? o.z : o.w));
-    }
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0) level_start[p.nlev] = *total_p;
 }
 
 // Materialise candidates in keypoint layout (FindExtrema, cuda_akaze.cu:744-772).
-__global__ __launch_bounds__(256) void k_scatter(akz_det_params p, const int32_t* __restrict__ seg_count,
-                                                 const int32_t* __restrict__ seg_off,
-                                                 const uint8_t* __restrict__ seg_slots, akz_kp* __restrict__ cand,
-                                                 int cap) {
-  int seg = blockIdx.x * 256 + threadIdx.x;
+__global__ __launch_bounds__(256) void k_scatter(akz_det_params p, akz_det_bufs b) {
+  const int seg = blockIdx.x * 256 + threadIdx.x;
   if (seg >= p.seg_total) return;
-  int c = seg_count[seg];
+  const int32_t* seg_count = akz_zoff(b.seg_count, b.zs);
+  const int c = seg_count[seg];
   if (c == 0) return;
   int l = 0;
   while (l + 1 < p.nlev && seg >= p.lev[l + 1].seg_base) l++;
   const akz_level_desc& L = p.lev[l];
-  int ls = seg - L.seg_base;
-  int y = ls / L.nseg_x, tx = ls - y * L.nseg_x;
-  int base = seg_off[seg];
-  const float* imd = p.det[l];
+  const int ls = seg - L.seg_base;
+  const int y = ls / L.nseg_x, tx = ls - y * L.nseg_x;
+  const int base = akz_zoff(b.seg_off, b.zs)[seg];
+  const uint8_t* slots = akz_zoff(b.seg_slots, b.zs);
+  akz_kp* cand = akz_zoff(b.cand, b.zs);
+  const float* imd = akz_zoff(p.det[l], p.zs);
   const size_t pitch = L.pitch;
   for (int r = 0; r < c; r++) {
-    int idx = base + r;
-    if (idx >= cap) return;
-    int x = tx * 64 + seg_slots[(size_t)seg * 32 + r];
-    size_t q = (size_t)y * pitch + x;
+    const int idx = base + r;
+    if (idx >= b.cap) return;
+    const int x = tx * 64 + slots[(size_t)seg * 32 + r];
+    const size_t q = (size_t)y * pitch + x;
     float v = imd[q];
     float dx = 0.5f * (imd[q + 1] - imd[q - 1]);
     float dy = 0.5f * (imd[q + pitch] - imd[q - pitch]);
@@ -392,11 +405,18 @@ __global__ __launch_bounds__(256) void k_scatter(akz_det_params p, const int32_t
 // within half the candidate's size.  Weak candidates (negative size) get
 // none, as the reference's early break gives.  The previous level is entered
 // at the last candidate whose y is within reach (binary search) instead of
-// being walked from its end.
-__global__ __launch_bounds__(256) void k_neighbors(const akz_kp* __restrict__ cand, const int32_t* ncand_p,
-                                                   const int32_t* __restrict__ level_start, int32_t* nb_count,
-                                                   int32_t* nb_list, int32_t* flags) {
-  const int n = *ncand_p;
+// being walked from its end.  FILL = false counts, FILL = true writes the
+// CSR list at the scanned offsets.  Entries past nb_cap are dropped and flag
+// an overflow; the suppression reads the lists cut at nb_cap, so every entry
+// it reads was written by this run.
+template <bool FILL>
+__global__ __launch_bounds__(256) void k_neighbors(akz_det_bufs b) {
+  const akz_kp* __restrict__ cand = akz_zoff(b.cand, b.zs);
+  const int32_t* __restrict__ level_start = akz_zoff(b.level_start, b.zs);
+  int32_t* nb_count = akz_zoff(b.nb_count, b.zs);
+  const int32_t* nb_off = akz_zoff(b.nb_off, b.zs);
+  int32_t* nb_list = akz_zoff(b.nb_list, b.zs);
+  const int n = *akz_zoff(b.ncand, b.zs);
   for (int c = blockIdx.x * 256 + threadIdx.x; c < n; c += gridDim.x * 256) {
     const akz_kp k = cand[c];
     const int lvl = k.class_id;
@@ -405,39 +425,36 @@ __global__ __launch_bounds__(256) void k_neighbors(const akz_kp* __restrict__ ca
     const float half = size * .5f;
     const double r2 = (double)(size * size) * 0.25;
     int cnt = 0;
-    int32_t* out = nb_list + (size_t)c * AKZ_NMS_MAX_NB;
+    const int o = FILL ? nb_off[c] : 0;
     for (int i = c - 1; i >= start; i--) {
       const akz_kp q = cand[i];
       if (k.y - q.y > half) break;
       float dist = (k.x - q.x) * (k.x - q.x) + (k.y - q.y) * (k.y - q.y);
       if ((double)dist < r2) {
-        if (cnt < AKZ_NMS_MAX_NB) out[cnt] = i;
+        if (FILL && o + cnt < b.nb_cap) nb_list[o + cnt] = i;
         cnt++;
       }
     }
     if (lvl > 0) {
-      int lo = level_start[lvl - 1], hi = start;
+      const int lo = level_start[lvl - 1], hi = start;
       // first index in [lo, hi) with q.y - k.y > half (monotone in sorted order)
-      int a = lo, b = hi;
-      while (a < b) {
-        int m = (a + b) >> 1;
-        if (cand[m].y - k.y > half) b = m; else a = m + 1;
+      int a = lo, e = hi;
+      while (a < e) {
+        int m = (a + e) >> 1;
+        if (cand[m].y - k.y > half) e = m; else a = m + 1;
       }
       for (int i = a - 1; i >= lo; i--) {
         const akz_kp q = cand[i];
         if (k.y - q.y > half) break;
         float dist = (k.x - q.x) * (k.x - q.x) + (k.y - q.y) * (k.y - q.y);
         if ((double)dist < r2) {
-          if (cnt < AKZ_NMS_MAX_NB) out[cnt] = i;
+          if (FILL && o + cnt < b.nb_cap) nb_list[o + cnt] = i;
           cnt++;
         }
       }
     }
-    if (cnt > AKZ_NMS_MAX_NB) {
-      atomicOr(flags, 1);
-      cnt = AKZ_NMS_MAX_NB;
-    }
-    nb_count[c] = cnt;
+    if (!FILL) nb_count[c] = cnt;
+    else if (o + cnt > b.nb_cap) atomicOr(akz_zoff(b.status, b.zs) + AKZ_ST_FLAGS, 1);
   }
 }
 
@@ -488,22 +505,17 @@ __device__ __forceinline__ akz_kp emit(const akz_kp& ok) {
 }
 
 // Wave-synchronous greedy suppression of FilterExtrema_kernel
-// (cuda_akaze.cu:1072-1224) in one workgroup.  Per wave: a candidate whose
-// neighbours are all processed is "ready"; a ready candidate with no kept
-// neighbour is kept; otherwise it contends for its kept neighbour with the
-// smallest member value, the lowest contending index wins (atomicMin here,
-// a racy min in the reference), and the winner replaces that neighbour if its
-// response is higher, else is dropped.  Member values live in LDS when the
-// candidate count allows, in global memory otherwise.  Only unprocessed
+// (cuda_akaze.cu:1072-1224), one workgroup per image.  Per wave: a candidate
+// whose neighbours are all processed is "ready"; a ready candidate with no
+// kept neighbour is kept; otherwise it contends for its kept neighbour with
+// the smallest member value, the lowest contending index wins (atomicMin
+// here, a racy min in the reference), and the winner replaces that neighbour
+// if its response is higher, else is dropped.  Member values live in LDS when
+// the candidate count allows, in global memory otherwise.  Only unprocessed
 // candidates are revisited (compacted active list).  Then the output of
 // sortFiltered_kernel (:1227-1316): per 2048-slot chunk, the kept member
 // values whose point is strong, ascending.
-__global__ __launch_bounds__(NMS_THREADS) void k_nms(const akz_kp* __restrict__ cand, const int32_t* ncand_p,
-                                                     const int32_t* __restrict__ nb_count,
-                                                     const int32_t* __restrict__ nb_list, int32_t* g_mem,
-                                                     int32_t* g_minnb, int32_t* g_code, int32_t* g_act,
-                                                     akz_kp* __restrict__ out, int32_t* nout, int32_t* waves_out,
-                                                     int max_waves, int cap) {
+__global__ __launch_bounds__(NMS_THREADS) void k_nms(akz_det_bufs b) {
   __shared__ int32_t s_mem[NMS_LDS_N];
   __shared__ int32_t s_minnb[NMS_LDS_N];
   __shared__ int32_t s_sort[2048];
@@ -512,12 +524,19 @@ __global__ __launch_bounds__(NMS_THREADS) void k_nms(const akz_kp* __restrict__ 
   __shared__ int32_t s_w[17];
   __shared__ int32_t s_nact[2];
   __shared__ int32_t s_run;
+  const akz_kp* __restrict__ cand = akz_zoff(b.cand, b.zs);
+  const int32_t* __restrict__ nb_off = akz_zoff(b.nb_off, b.zs);
+  const int32_t* __restrict__ nb_list = akz_zoff(b.nb_list, b.zs);
+  int32_t* g_code = akz_zoff(b.code, b.zs);
+  akz_kp* __restrict__ out = akz_zoff(b.kps, b.zs);
+  int32_t* st = akz_zoff(b.status, b.zs);
+  const int cap = b.cap;
   const int t = threadIdx.x;
-  const int n = *ncand_p;
+  const int n = *akz_zoff(b.ncand, b.zs);
   const bool lds = n <= NMS_LDS_N;
-  int32_t* mem = lds ? s_mem : g_mem;
-  int32_t* minnb = lds ? s_minnb : g_minnb;
-  int32_t* act[2] = {g_act, g_act + cap};
+  int32_t* mem = lds ? s_mem : akz_zoff(b.mem, b.zs);
+  int32_t* minnb = lds ? s_minnb : akz_zoff(b.minnb, b.zs);
+  int32_t* act[2] = {akz_zoff(b.act, b.zs), akz_zoff(b.act, b.zs) + cap};
   for (int i = t; i < n; i += NMS_THREADS) {
     mem[i] = -1;
     minnb[i] = INT32_MAX;
@@ -526,20 +545,19 @@ __global__ __launch_bounds__(NMS_THREADS) void k_nms(const akz_kp* __restrict__ 
   if (t == 0) { s_nact[0] = n; s_nact[1] = 0; s_run = 0; }
   __syncthreads();
   int cur = 0, wave = 0;
-  while (wave < max_waves) {
+  while (wave < b.max_waves) {
     const int na = s_nact[cur];
     if (na == 0) break;
     const int32_t* A = act[cur];
     // phase 1: classify ready candidates and arbitrate contention
     for (int j = t; j < na; j += NMS_THREADS) {
-      int c = A[j];
-      int e = nb_count[c];
-      const int32_t* nb = nb_list + (size_t)c * AKZ_NMS_MAX_NB;
+      const int c = A[j];
+      const int e0 = akz_imin(nb_off[c], b.nb_cap), e1 = akz_imin(nb_off[c + 1], b.nb_cap);
       bool ready = true, anykept = false;
       int best = -1, bestv = INT32_MAX;
-      for (int k = 0; k < e; k++) {
-        int q = nb[k];
-        int m = mem[q];
+      for (int k = e0; k < e1; k++) {
+        const int q = nb_list[k];
+        const int m = mem[q];
         if (m == -1) { ready = false; break; }
         if (m >= 0) {
           anykept = true;
@@ -556,8 +574,8 @@ __global__ __launch_bounds__(NMS_THREADS) void k_nms(const akz_kp* __restrict__ 
     __syncthreads();
     // phase 2: additions and replacements
     for (int j = t; j < na; j += NMS_THREADS) {
-      int c = A[j];
-      int code = g_code[c];
+      const int c = A[j];
+      const int code = g_code[c];
       if (code == CODE_ADD) {
         mem[c] = c;
       } else if (code >= 0 && minnb[code] == c) {
@@ -573,8 +591,8 @@ __global__ __launch_bounds__(NMS_THREADS) void k_nms(const akz_kp* __restrict__ 
     // phase 3: reset arbitration slots, compact the still-unprocessed list
     const int nxt = cur ^ 1;
     for (int j = t; j < na; j += NMS_THREADS) {
-      int c = A[j];
-      int code = g_code[c];
+      const int c = A[j];
+      const int code = g_code[c];
       if (code >= 0) minnb[code] = INT32_MAX;
       if (mem[c] == -1) {
         int pos = atomicAdd(&s_nact[nxt], 1);
@@ -587,7 +605,7 @@ __global__ __launch_bounds__(NMS_THREADS) void k_nms(const akz_kp* __restrict__ 
     wave++;
     __syncthreads();
   }
-  if (t == 0) *waves_out = wave;
+  if (t == 0) st[AKZ_ST_WAVES] = wave;
   // output chunks: the member values of a chunk are distinct candidate
   // indices, so a bitmap over [0, n) and a prefix popcount of its words give
   // every value its rank in ascending order (the order of the reference's
@@ -598,25 +616,25 @@ __global__ __launch_bounds__(NMS_THREADS) void k_nms(const akz_kp* __restrict__ 
       for (int i = t; i < nwords; i += NMS_THREADS) s_bits[i] = 0u;
       __syncthreads();
       for (int e = t; e < 2048; e += NMS_THREADS) {
-        int i = base + e;
-        int v = (i < n) ? mem[i] : -1;
+        const int i = base + e;
+        const int v = (i < n) ? mem[i] : -1;
         if (v >= 0 && !(cand[v].size < 0)) atomicOr(&s_bits[v >> 5], 1u << (v & 31));
       }
       __syncthreads();
       // exclusive prefix of word popcounts, two words per thread
-      int c0 = (2 * t < nwords) ? __popc(s_bits[2 * t]) : 0;
-      int c1 = (2 * t + 1 < nwords) ? __popc(s_bits[2 * t + 1]) : 0;
+      const int c0 = (2 * t < nwords) ? __popc(s_bits[2 * t]) : 0;
+      const int c1 = (2 * t + 1 < nwords) ? __popc(s_bits[2 * t + 1]) : 0;
       int ex;
-      int tot = nms_block_scan(c0 + c1, &ex, s_w);
+      const int tot = nms_block_scan(c0 + c1, &ex, s_w);
       if (2 * t < nwords) s_pre[2 * t] = ex;
       if (2 * t + 1 < nwords) s_pre[2 * t + 1] = ex + c0;
       __syncthreads();
       const int run = s_run;
       for (int e = t; e < 2048; e += NMS_THREADS) {
-        int i = base + e;
-        int v = (i < n) ? mem[i] : -1;
+        const int i = base + e;
+        const int v = (i < n) ? mem[i] : -1;
         if (v >= 0 && !(cand[v].size < 0)) {
-          int rank = s_pre[v >> 5] + __popc(s_bits[v >> 5] & ((1u << (v & 31)) - 1u));
+          const int rank = s_pre[v >> 5] + __popc(s_bits[v >> 5] & ((1u << (v & 31)) - 1u));
           if (run + rank < cap) out[run + rank] = emit(cand[v]);
         }
       }
@@ -625,27 +643,27 @@ __global__ __launch_bounds__(NMS_THREADS) void k_nms(const akz_kp* __restrict__ 
       __syncthreads();
     } else {
       for (int e = t; e < 2048; e += NMS_THREADS) {
-        int i = base + e;
-        int v = (i < n) ? mem[i] : -1;
-        bool ok = v >= 0 && !(cand[v].size < 0);
+        const int i = base + e;
+        const int v = (i < n) ? mem[i] : -1;
+        const bool ok = v >= 0 && !(cand[v].size < 0);
         s_sort[e] = ok ? v : INT32_MAX;
       }
       __syncthreads();
       for (int k = 2; k <= 2048; k <<= 1) {
         for (int j = k >> 1; j > 0; j >>= 1) {
-          int a = ((t & ~(j - 1)) << 1) | (t & (j - 1));
-          int b = a | j;
-          int va = s_sort[a], vb = s_sort[b];
-          bool asc = (a & k) == 0;
-          if ((va > vb) == asc) { s_sort[a] = vb; s_sort[b] = va; }
+          const int a = ((t & ~(j - 1)) << 1) | (t & (j - 1));
+          const int bb = a | j;
+          const int va = s_sort[a], vb = s_sort[bb];
+          const bool asc = (a & k) == 0;
+          if ((va > vb) == asc) { s_sort[a] = vb; s_sort[bb] = va; }
           __syncthreads();
         }
       }
-      int cnt = __syncthreads_count((s_sort[t] != INT32_MAX) ? 1 : 0) +
-                __syncthreads_count((s_sort[t + 1024] != INT32_MAX) ? 1 : 0);
+      const int cnt = __syncthreads_count((s_sort[t] != INT32_MAX) ? 1 : 0) +
+                      __syncthreads_count((s_sort[t + 1024] != INT32_MAX) ? 1 : 0);
       const int run = s_run;
       for (int e = t; e < 2048; e += NMS_THREADS) {
-        int v = s_sort[e];
+        const int v = s_sort[e];
         if (v != INT32_MAX && run + e < cap) out[run + e] = emit(cand[v]);
       }
       __syncthreads();
@@ -653,7 +671,13 @@ __global__ __launch_bounds__(NMS_THREADS) void k_nms(const akz_kp* __restrict__ 
       __syncthreads();
     }
   }
-  if (t == 0) *nout = akz_imin(s_run, cap);
+  if (t == 0) {
+    st[AKZ_ST_KEPT] = akz_imin(s_run, cap);
+    // the whole block (flags and total come from earlier kernels) into the
+    // contiguous per-batch array the host reads with one copy
+    int4 blk = make_int4(akz_imin(s_run, cap), st[AKZ_ST_FLAGS], st[AKZ_ST_TOTAL], wave);
+    reinterpret_cast<int4*>(b.status_all)[blockIdx.z] = blk;
+  }
 }
 
 inline int grid_stride_blocks(int cap) {
@@ -661,53 +685,48 @@ inline int grid_stride_blocks(int cap) {
   return b < 2048 ? (b > 0 ? b : 1) : 2048;
 }
 
+void xscan(hipStream_t s, const int32_t* cnt, int len, const int32_t* n_ptr, int32_t* bsum, int32_t* off,
+           int32_t* total, int nz, size_t zs) {
+  // grid covers indices 0..len (len + 1 offsets)
+  const int nblk = (len + 1 + SCAN_BLK - 1) / SCAN_BLK;
+  hipLaunchKernelGGL(k_xscan_reduce, dim3(nblk, 1, nz), dim3(SCAN_T), 0, s, cnt, len, n_ptr, bsum, zs);
+  hipLaunchKernelGGL(k_xscan_top, dim3(1, 1, nz), dim3(SCAN_T), 0, s, bsum, nblk, total, zs);
+  hipLaunchKernelGGL(k_xscan_down, dim3(nblk, 1, nz), dim3(SCAN_T), 0, s, cnt, len, n_ptr, bsum, off, zs);
+}
+
 }  // namespace
 
-void akz_launch_hessian(hipStream_t s, const akz_det_params& p) {
-  int nb = p.tile_start[p.nlev];
-  hipLaunchKernelGGL(k_deriv, dim3(nb), dim3(256), 0, s, p);
-  hipLaunchKernelGGL(k_hess, dim3(nb), dim3(256), 0, s, p);
+int akz_scan_blocks(int len) { return (len + 1 + SCAN_BLK - 1) / SCAN_BLK; }
+
+void akz_launch_hessian(hipStream_t s, const akz_det_params& p, int nz) {
+  const int nb = p.tile_start[p.nlev];
+  hipLaunchKernelGGL(k_deriv, dim3(nb, 1, nz), dim3(256), 0, s, p);
+  hipLaunchKernelGGL(k_hess, dim3(nb, 1, nz), dim3(256), 0, s, p);
 }
 
 int akz_detect_max_step() { return DF_SMAX; }
 
 int akz_detect_tile_rows() { return DF_TH; }
 
-void akz_launch_detect_fused(hipStream_t s, const akz_det_params& p32, int32_t* seg_count, uint8_t* seg_slots) {
-  hipLaunchKernelGGL(k_detect_fused, dim3(p32.tile_start[p32.nlev]), dim3(512), 0, s, p32, seg_count, seg_slots);
+void akz_launch_detect_fused(hipStream_t s, const akz_det_params& p32, const akz_det_bufs& b, int nz) {
+  hipLaunchKernelGGL(k_detect_fused, dim3(p32.tile_start[p32.nlev], 1, nz), dim3(512), 0, s, p32, b.seg_count,
+                     b.seg_slots);
 }
 
-void akz_launch_extrema(hipStream_t s, const akz_det_params& p, int32_t* seg_count, uint8_t* seg_slots) {
-  hipLaunchKernelGGL(k_extrema, dim3(p.tile_start[p.nlev]), dim3(256), 0, s, p, seg_count, seg_slots);
+void akz_launch_extrema(hipStream_t s, const akz_det_params& p, const akz_det_bufs& b, int nz) {
+  hipLaunchKernelGGL(k_extrema, dim3(p.tile_start[p.nlev], 1, nz), dim3(256), 0, s, p, b.seg_count, b.seg_slots);
 }
 
-void akz_launch_scan(hipStream_t s, const akz_det_params& p, const int32_t* seg_count, int32_t* seg_off,
-                     int32_t* level_start, int32_t* ncand, int cap, int32_t* scratch, int32_t* total,
-                     int32_t* flags) {
-  // scratch: block sums (ceil(seg_total/4096) ints)
-  int nblk = (p.seg_total + SCAN_BLK - 1) / SCAN_BLK;
-  int32_t* bsum = scratch;
-  hipLaunchKernelGGL(k_scan_reduce, dim3(nblk), dim3(SCAN_T), 0, s, seg_count, p.seg_total, bsum);
-  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_T), 0, s, bsum, nblk, ncand, total, cap, flags);
-  hipLaunchKernelGGL(k_scan_down, dim3(nblk), dim3(SCAN_T), 0, s, seg_count, p.seg_total, bsum, seg_off, p,
-                     level_start, total);
+void akz_launch_candidates(hipStream_t s, const akz_det_params& p, const akz_det_bufs& b, int nz) {
+  xscan(s, b.seg_count, p.seg_total, nullptr, b.scan_bsum, b.seg_off, b.scan_total, nz, b.zs);
+  hipLaunchKernelGGL(k_seg_finish, dim3(1, 1, nz), dim3(64), 0, s, p, b);
+  hipLaunchKernelGGL(k_scatter, dim3((p.seg_total + 255) / 256, 1, nz), dim3(256), 0, s, p, b);
 }
 
-void akz_launch_scatter(hipStream_t s, const akz_det_params& p, const int32_t* seg_count, const int32_t* seg_off,
-                        const uint8_t* seg_slots, akz_kp* cand, int cap) {
-  hipLaunchKernelGGL(k_scatter, dim3((p.seg_total + 255) / 256), dim3(256), 0, s, p, seg_count, seg_off, seg_slots,
-                     cand, cap);
-}
-
-void akz_launch_neighbors(hipStream_t s, const akz_kp* cand, const int32_t* ncand, const int32_t* level_start,
-                          int32_t* nb_count, int32_t* nb_list, int32_t* flags, int cap) {
-  hipLaunchKernelGGL(k_neighbors, dim3(grid_stride_blocks(cap)), dim3(256), 0, s, cand, ncand, level_start, nb_count,
-                     nb_list, flags);
-}
-
-void akz_launch_nms(hipStream_t s, const akz_kp* cand, const int32_t* ncand, const int32_t* nb_count,
-                    const int32_t* nb_list, int32_t* mem, int32_t* minnb, int32_t* target, int32_t* act,
-                    akz_kp* out, int32_t* nout, int32_t* waves, int max_waves, int cap) {
-  hipLaunchKernelGGL(k_nms, dim3(1), dim3(NMS_THREADS), 0, s, cand, ncand, nb_count, nb_list, mem, minnb, target,
-                     act, out, nout, waves, max_waves, cap);
+void akz_launch_suppression(hipStream_t s, const akz_det_bufs& b, int nz) {
+  const int g = grid_stride_blocks(b.cap);
+  hipLaunchKernelGGL(k_neighbors<false>, dim3(g, 1, nz), dim3(256), 0, s, b);
+  xscan(s, b.nb_count, b.cap, b.ncand, b.nb_bsum, b.nb_off, b.nb_total, nz, b.zs);
+  hipLaunchKernelGGL(k_neighbors<true>, dim3(g, 1, nz), dim3(256), 0, s, b);
+  hipLaunchKernelGGL(k_nms, dim3(1, 1, nz), dim3(NMS_THREADS), 0, s, b);
 }

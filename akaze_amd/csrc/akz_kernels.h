@@ -6,34 +6,38 @@
 struct akz_taps { float k[11]; };
 
 // ---- scale space (akz_scale_space.hip) ----
-void akz_launch_gauss(hipStream_t s, const float* in, int in_pitch, float* out, int out_pitch, int w, int h, int R,
-                      const akz_taps& taps);
+// Every launcher takes the batch (akz_batch): pointers name image 0's arrays,
+// image z's sit z * bt.zs bytes further; the input of akz_launch_gauss has
+// its own image stride in_zs.
+void akz_launch_gauss(hipStream_t s, const float* in, int in_pitch, size_t in_zs, float* out, int out_pitch, int w,
+                      int h, int R, const akz_taps& taps, akz_batch bt);
 // Contrast factor of ContrastPercentile on the device: gradient of `blur`
 // (written to `grad`), its maximum, the histogram and the percentile, leaving
 // the per-level factor in kc[0..nlev).  blockmax: 2048 entries; hist_part:
 // 512 x 512 + 512 ints.
 void akz_launch_contrast(hipStream_t s, const float* blur, int pitch, float* grad, int w, int h, int nbins,
                          float perc, uint32_t* maxbits, uint32_t* blockmax, int32_t* hist_part, float* kc, int nlev,
-                         const int* octaves_host);
+                         const int* octaves_host, akz_batch bt);
 // FED kernel variant chosen for a level size
 int akz_fed_variant_for(int w, int h);
 // LowPass(sigma=1) + Flow of one level in one kernel; taps: 5 taps of sigma 1
 void akz_launch_smooth_flow(hipStream_t s, const float* in, int pitch, float* lsm, float* g, int w, int h, int type,
-                            const float* kc, const akz_taps& taps);
+                            const float* kc, const akz_taps& taps, akz_batch bt);
 void akz_launch_fed_step(hipStream_t s, const float* Lin, const float* g, float* Lout, int pitch, int w, int h,
-                         float half_tau);
+                         float half_tau, akz_batch bt);
 // up to akz_fed_block_steps() FED steps in one launch (n <= that many)
 int akz_fed_block_steps();
 void akz_launch_fed_multi(hipStream_t s, const float* Lin, const float* g, float* Lout, int pitch, int w, int h,
-                          const float* half_tau, int n);
+                          const float* half_tau, int n, akz_batch bt);
 // n FED steps with one kernel variant (0 single step, 1 single step LDS-tiled,
-// 2..4 blocked); variants 0 and 1 need Lin != Lout and n == 1 for a
+// 2..8 blocked); variants 0 and 1 need Lin != Lout and n == 1 for a
 // ping-pong-free call, the blocked ones accept n <= akz_fed_block_steps()
 void akz_launch_fed_variant(hipStream_t s, int variant, const float* Lin, const float* g, float* Lout, int pitch,
-                            int w, int h, const float* half_tau, int n);
+                            int w, int h, const float* half_tau, int n, akz_batch bt);
 void akz_launch_halfsample(hipStream_t s, const float* in, int iw, int ih, int ipitch, float* out, int ow, int oh,
-                           int opitch);
-void akz_launch_copy2d(hipStream_t s, const float* in, int ipitch, float* out, int opitch, int w, int h);
+                           int opitch, akz_batch bt);
+void akz_launch_copy2d(hipStream_t s, const float* in, int ipitch, float* out, int opitch, int w, int h,
+                       akz_batch bt);
 
 // ---- detection (akz_detect.hip) ----
 struct akz_det_params {
@@ -47,28 +51,44 @@ struct akz_det_params {
   int seg_total;                       // row segments over all levels
   float thr;
   float fac1, fac2;
+  size_t zs;                           // image stride of the batch in bytes
 };
-void akz_launch_hessian(hipStream_t s, const akz_det_params& p);
-void akz_launch_extrema(hipStream_t s, const akz_det_params& p, int32_t* seg_count, uint8_t* seg_slots);
+// Detector work arrays of image 0 (image z's sit z * zs bytes further).
+struct akz_det_bufs {
+  int32_t* seg_count;    // seg_total (+4) in-segment candidate counts
+  uint8_t* seg_slots;    // 32 x-offsets per segment
+  int32_t* seg_off;      // seg_total + 1 offsets
+  int32_t* scan_bsum;    // akz_scan_blocks(seg_total) block sums
+  int32_t* scan_total;
+  int32_t* level_start;  // nlev + 1
+  int32_t* ncand;        // candidates after the capacity cut
+  akz_kp* cand;          // cap
+  int32_t* nb_count;     // cap
+  int32_t* nb_off;       // cap + 1 CSR offsets
+  int32_t* nb_bsum;      // akz_scan_blocks(cap)
+  int32_t* nb_total;
+  int32_t* nb_list;      // nb_cap
+  int32_t *mem, *minnb, *code;  // cap each (suppression state beyond LDS)
+  int32_t* act;          // 2 x cap
+  akz_kp* kps;           // cap output keypoints
+  int32_t* status;       // AKZ_ST_* block
+  int32_t* status_all;   // contiguous copies of every image's block (4 ints per image, not z-strided)
+  int cap, nb_cap, max_waves;
+  size_t zs;
+};
+int akz_scan_blocks(int len);
+void akz_launch_hessian(hipStream_t s, const akz_det_params& p, int nz);
+void akz_launch_extrema(hipStream_t s, const akz_det_params& p, const akz_det_bufs& b, int nz);
 // Hessian response + extrema in one kernel (derivative steps up to
-// akz_detect_max_step())
-// akz_detect_tile_rows()); `p32.tile_start` must count 64 x akz_detect_tile_rows()
-// tiles per level
+// akz_detect_max_step()); `p32.tile_start` must count 64 x
+// akz_detect_tile_rows() tiles per level
 int akz_detect_max_step();
 int akz_detect_tile_rows();
-void akz_launch_detect_fused(hipStream_t s, const akz_det_params& p32, int32_t* seg_count, uint8_t* seg_slots);
-// scratch holds ceil(seg_total/4096) ints; total receives the candidate count
-// before the capacity cut; flags is cleared for the detection run
-void akz_launch_scan(hipStream_t s, const akz_det_params& p, const int32_t* seg_count, int32_t* seg_off,
-                     int32_t* level_start, int32_t* ncand, int cap, int32_t* scratch, int32_t* total,
-                     int32_t* flags);
-void akz_launch_scatter(hipStream_t s, const akz_det_params& p, const int32_t* seg_count, const int32_t* seg_off,
-                        const uint8_t* seg_slots, akz_kp* cand, int cap);
-void akz_launch_neighbors(hipStream_t s, const akz_kp* cand, const int32_t* ncand, const int32_t* level_start,
-                          int32_t* nb_count, int32_t* nb_list, int32_t* flags, int cap);
-void akz_launch_nms(hipStream_t s, const akz_kp* cand, const int32_t* ncand, const int32_t* nb_count,
-                    const int32_t* nb_list, int32_t* mem, int32_t* minnb, int32_t* target, int32_t* act,
-                    akz_kp* out, int32_t* nout, int32_t* waves, int max_waves, int cap);
+void akz_launch_detect_fused(hipStream_t s, const akz_det_params& p32, const akz_det_bufs& b, int nz);
+// segment scan, level starts, status reset and candidate materialisation
+void akz_launch_candidates(hipStream_t s, const akz_det_params& p, const akz_det_bufs& b, int nz);
+// CSR neighbour lists and the wave-synchronous suppression with its output
+void akz_launch_suppression(hipStream_t s, const akz_det_bufs& b, int nz);
 
 // ---- description (akz_describe.hip) ----
 struct akz_desc_params {
@@ -77,11 +97,13 @@ struct akz_desc_params {
   const float* Ly[AKZ_MAX_LEVELS];
   int w[AKZ_MAX_LEVELS], h[AKZ_MAX_LEVELS], pitch[AKZ_MAX_LEVELS];
   int patsize;
+  size_t zs;
 };
 void akz_describe_init_tables();
-void akz_launch_orientation(hipStream_t s, const akz_desc_params& p, akz_kp* kps, const int32_t* nkps, int cap);
+void akz_launch_orientation(hipStream_t s, const akz_desc_params& p, akz_kp* kps, const int32_t* nkps, int cap,
+                            int nz);
 void akz_launch_mldb(hipStream_t s, const akz_desc_params& p, const akz_kp* kps, const int32_t* nkps,
-                     uint8_t* desc64, int cap);
+                     uint8_t* desc64, int cap, int nz);
 
 // ---- matcher (akz_match.hip) ----
 int akz_match_splits(int nq, int nt);

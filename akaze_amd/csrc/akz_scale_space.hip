@@ -43,8 +43,10 @@ constexpr int GT_THREADS = 256;
 template <int R>
 __global__ __launch_bounds__(GT_THREADS) void k_gauss(const float* __restrict__ in, int in_pitch,
                                                       float* __restrict__ out, int out_pitch, int w, int h,
-                                                      akz_taps taps) {
+                                                      akz_taps taps, size_t in_zs, size_t out_zs) {
   constexpr int AW = GT_X + 2 * R, AH = GT_Y + 2 * R;
+  in = akz_zoff(in, in_zs);
+  out = akz_zoff(out, out_zs);
   __shared__ float A[AH][AW];
   __shared__ float B[AH][GT_X];
   const int x0 = blockIdx.x * GT_X, y0 = blockIdx.y * GT_Y;
@@ -83,8 +85,11 @@ constexpr int CT_HIST_THREADS = 512;
 // atomics on one address serialise across the whole chip.
 __global__ __launch_bounds__(256) void k_contrast_grad(const float* __restrict__ blur, int pitch,
                                                        float* __restrict__ grad, int w, int h,
-                                                       uint32_t* __restrict__ blockmax) {
+                                                       uint32_t* __restrict__ blockmax, size_t zs) {
   __shared__ uint32_t s_w[4];
+  blur = akz_zoff(blur, zs);
+  grad = akz_zoff(grad, zs);
+  blockmax = akz_zoff(blockmax, zs);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int tiles_x = (w - 2 + 63) / 64, tiles_y = (h - 2 + 3) / 4;
   uint32_t bits = 0u;
@@ -116,8 +121,10 @@ __global__ __launch_bounds__(256) void k_contrast_grad(const float* __restrict__
 }
 
 __global__ __launch_bounds__(1024) void k_contrast_max(const uint32_t* __restrict__ blockmax, int nb,
-                                                       uint32_t* maxbits) {
+                                                       uint32_t* maxbits, size_t zs) {
   __shared__ uint32_t s_w[16];
+  blockmax = akz_zoff(blockmax, zs);
+  maxbits = akz_zoff(maxbits, zs);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   uint32_t m = 0u;
   for (int i = threadIdx.x; i < nb; i += 1024) m = m > blockmax[i] ? m : blockmax[i];
@@ -141,9 +148,12 @@ struct akz_octaves { int nlev; int oct[AKZ_MAX_LEVELS]; };
 // collide on a bin; each block stores its bin totals in hist_part.
 __global__ __launch_bounds__(CT_HIST_THREADS) void k_contrast_hist(const float* __restrict__ grad, int pitch,
                                                                    int w, int h, const uint32_t* maxbits, int nbins,
-                                                                   int32_t* __restrict__ hist_part) {
+                                                                   int32_t* __restrict__ hist_part, size_t zs) {
   __shared__ int32_t lh[300 * 64];  // bins >= 300 (nbins <= 512) go to `tail`
   __shared__ int32_t tail[212];
+  grad = akz_zoff(grad, zs);
+  maxbits = akz_zoff(maxbits, zs);
+  hist_part = akz_zoff(hist_part, zs);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   constexpr int NW = CT_HIST_THREADS / 64;
   for (int i = threadIdx.x; i < 300 * 64; i += CT_HIST_THREADS) lh[i] = 0;
@@ -182,8 +192,10 @@ __global__ __launch_bounds__(CT_HIST_THREADS) void k_contrast_hist(const float* 
 // Bin totals over the block histograms: one wavefront per bin, lanes read
 // consecutive block partials, a single round of loads per lane.
 __global__ __launch_bounds__(64) void k_contrast_bins(const int32_t* __restrict__ hist_part, int nb,
-                                                      int32_t* __restrict__ tot) {
+                                                      int32_t* __restrict__ tot, size_t zs) {
   const int lane = threadIdx.x, i = blockIdx.x;
+  hist_part = akz_zoff(hist_part, zs);
+  tot = akz_zoff(tot, zs);
   const int32_t* row = hist_part + (size_t)i * CT_HIST_BLOCKS;
   int sum = 0;
 #pragma unroll
@@ -197,8 +209,11 @@ __global__ __launch_bounds__(64) void k_contrast_bins(const int32_t* __restrict_
 
 __global__ __launch_bounds__(1024) void k_contrast_finish(const int32_t* __restrict__ bin_tot, int nbins,
                                                           const uint32_t* maxbits, int w, int h, float perc, float* kc,
-                                                          akz_octaves octs) {
+                                                          akz_octaves octs, size_t zs) {
   __shared__ int32_t tot[512];
+  bin_tot = akz_zoff(bin_tot, zs);
+  maxbits = akz_zoff(maxbits, zs);
+  kc = akz_zoff(kc, zs);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   for (int i = threadIdx.x; i < nbins; i += 1024) tot[i] = bin_tot[i];
   __syncthreads();
@@ -243,7 +258,7 @@ __global__ __launch_bounds__(1024) void k_contrast_finish(const int32_t* __restr
 // reflected image coordinates, which always fall inside that region.
 __global__ __launch_bounds__(256) void k_smooth_flow(const float* __restrict__ in, int pitch,
                                                      float* __restrict__ lsm, float* __restrict__ g, int w, int h,
-                                                     int type, const float* kcp, akz_taps taps) {
+                                                     int type, const float* kcp, akz_taps taps, size_t zs) {
   constexpr int TW = 64, TH = 16, R = 2;
   constexpr int AW = TW + 2 + 2 * R, AH = TH + 2 + 2 * R;  // input region
   constexpr int CW = TW + 2, CH = TH + 2;                  // Lsmooth region
@@ -252,6 +267,10 @@ __global__ __launch_bounds__(256) void k_smooth_flow(const float* __restrict__ i
   __shared__ float C[CH][CW];
   const int tid = threadIdx.x;
   const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
+  in = akz_zoff(in, zs);
+  lsm = akz_zoff(lsm, zs);
+  g = akz_zoff(g, zs);
+  kcp = akz_zoff(kcp, zs);
   load_window<AH, AW, 256>(&A[0][0], in, pitch, x0 - 1 - R, y0 - 1 - R, w, h);
   __syncthreads();
   for (int idx = tid; idx < AH * CW; idx += 256) {
@@ -304,10 +323,13 @@ __global__ __launch_bounds__(256) void k_smooth_flow(const float* __restrict__ i
 // L_out = L_in + (tau/2) * div(g grad L_in), neighbours clamped at the border.
 __global__ __launch_bounds__(256) void k_fed_step(const float* __restrict__ L, const float* __restrict__ g,
                                                   float* __restrict__ out, int pitch, int w, int h,
-                                                  float stepsize) {
+                                                  float stepsize, size_t zs) {
   int x = blockIdx.x * 64 + (threadIdx.x & 63);
   int y = blockIdx.y * 4 + (threadIdx.x >> 6);
   if (x >= w || y >= h) return;
+  L = akz_zoff(L, zs);
+  g = akz_zoff(g, zs);
+  out = akz_zoff(out, zs);
   int ym = akz_imax(y - 1, 0), yp = akz_imin(y + 1, h - 1);
   int xm = akz_imax(x - 1, 0), xp = akz_imin(x + 1, w - 1);
   size_t c = (size_t)y * pitch + x;
@@ -336,8 +358,11 @@ struct akz_fed_steps { float half_tau[FB_K]; int n; };
 template <int RW, int RH, int NT>
 __global__ __launch_bounds__(NT) void k_fed_multi(const float* __restrict__ L, const float* __restrict__ g,
                                                   float* __restrict__ out, int pitch, int w, int h,
-                                                  akz_fed_steps st) {
+                                                  akz_fed_steps st, size_t zs) {
   constexpr int K = FB_K, TW = RW - 2 * K, TH = RH - 2 * K;
+  L = akz_zoff(L, zs);
+  g = akz_zoff(g, zs);
+  out = akz_zoff(out, zs);
   constexpr int LW = (RW == 64) ? 6 : (RW == 32 ? 5 : 4);
   static_assert((1 << LW) == RW, "power-of-two region width");
   __shared__ float sg[RH * RW];
@@ -385,8 +410,11 @@ __global__ __launch_bounds__(NT) void k_fed_multi(const float* __restrict__ L, c
 // g is read once with coalesced rows; arithmetic as k_fed_step.
 __global__ __launch_bounds__(256) void k_fed_tile(const float* __restrict__ L, const float* __restrict__ g,
                                                   float* __restrict__ out, int pitch, int w, int h,
-                                                  float stepsize) {
+                                                  float stepsize, size_t zs) {
   constexpr int TW = 64, TH = 16, SW = TW + 2, SH = TH + 2;
+  L = akz_zoff(L, zs);
+  g = akz_zoff(g, zs);
+  out = akz_zoff(out, zs);
   __shared__ float sl[SH * SW], sg[SH * SW];
   const int tid = threadIdx.x;
   const int x0 = blockIdx.x * TW - 1, y0 = blockIdx.y * TH - 1;
@@ -413,10 +441,12 @@ __global__ __launch_bounds__(256) void k_fed_tile(const float* __restrict__ L, c
 
 // 2x2 box mean for exact halving (HalfSample2, cuda_akaze.cu:432-440).
 __global__ __launch_bounds__(256) void k_half2(const float* __restrict__ in, int ipitch, float* __restrict__ out,
-                                               int ow, int oh, int opitch) {
+                                               int ow, int oh, int opitch, size_t zs) {
   int x = blockIdx.x * 64 + (threadIdx.x & 63);
   int y = blockIdx.y * 4 + (threadIdx.x >> 6);
   if (x >= ow || y >= oh) return;
+  in = akz_zoff(in, zs);
+  out = akz_zoff(out, zs);
   const float* p = in + (size_t)(2 * y) * ipitch + 2 * x;
   out[(size_t)y * opitch + x] = 0.25f * (p[0] + p[1] + p[ipitch] + p[ipitch + 1]);
 }
@@ -432,10 +462,13 @@ __device__ __forceinline__ float hs_row(const float* in, int ipitch, int iw, int
 }
 
 __global__ __launch_bounds__(256) void k_half_general(const float* __restrict__ in, int iw, int ih, int ipitch,
-                                                      float* __restrict__ out, int ow, int oh, int opitch) {
+                                                      float* __restrict__ out, int ow, int oh, int opitch,
+                                                      size_t zs) {
   int x = blockIdx.x * 64 + (threadIdx.x & 63);
   int y = blockIdx.y * 4 + (threadIdx.x >> 6);
   if (x >= ow || y >= oh) return;
+  in = akz_zoff(in, zs);
+  out = akz_zoff(out, zs);
   bool oddw = (2 * ow != iw), oddh = (2 * oh != ih);
   float b0 = hs_row(in, ipitch, iw, 2 * y, x, ow, false);
   float b16 = hs_row(in, ipitch, iw, 2 * y + 1, x, ow, false);
@@ -451,57 +484,62 @@ __global__ __launch_bounds__(256) void k_half_general(const float* __restrict__ 
 }
 
 __global__ __launch_bounds__(256) void k_copy2d(const float* __restrict__ in, int ipitch, float* __restrict__ out,
-                                                int opitch, int w, int h) {
+                                                int opitch, int w, int h, size_t zs) {
   int x = blockIdx.x * 64 + (threadIdx.x & 63);
   int y = blockIdx.y * 4 + (threadIdx.x >> 6);
   if (x >= w || y >= h) return;
+  in = akz_zoff(in, zs);
+  out = akz_zoff(out, zs);
   out[(size_t)y * opitch + x] = in[(size_t)y * ipitch + x];
 }
 
-inline dim3 grid64x4(int w, int h) { return dim3((w + 63) / 64, (h + 3) / 4); }
+inline dim3 grid64x4(int w, int h, int nz) { return dim3((w + 63) / 64, (h + 3) / 4, nz); }
 
 }  // namespace
 
-void akz_launch_gauss(hipStream_t s, const float* in, int in_pitch, float* out, int out_pitch, int w, int h, int R,
-                      const akz_taps& taps) {
-  dim3 grid((w + GT_X - 1) / GT_X, (h + GT_Y - 1) / GT_Y);
+void akz_launch_gauss(hipStream_t s, const float* in, int in_pitch, size_t in_zs, float* out, int out_pitch, int w,
+                      int h, int R, const akz_taps& taps, akz_batch bt) {
+  dim3 grid((w + GT_X - 1) / GT_X, (h + GT_Y - 1) / GT_Y, bt.nz);
+  const size_t zs = bt.zs;
   switch (R) {
-    case 2: hipLaunchKernelGGL(k_gauss<2>, grid, dim3(GT_THREADS), 0, s, in, in_pitch, out, out_pitch, w, h, taps); break;
-    case 3: hipLaunchKernelGGL(k_gauss<3>, grid, dim3(GT_THREADS), 0, s, in, in_pitch, out, out_pitch, w, h, taps); break;
-    case 4: hipLaunchKernelGGL(k_gauss<4>, grid, dim3(GT_THREADS), 0, s, in, in_pitch, out, out_pitch, w, h, taps); break;
-    default: hipLaunchKernelGGL(k_gauss<5>, grid, dim3(GT_THREADS), 0, s, in, in_pitch, out, out_pitch, w, h, taps); break;
+    case 2: hipLaunchKernelGGL(k_gauss<2>, grid, dim3(GT_THREADS), 0, s, in, in_pitch, out, out_pitch, w, h, taps, in_zs, zs); break;
+    case 3: hipLaunchKernelGGL(k_gauss<3>, grid, dim3(GT_THREADS), 0, s, in, in_pitch, out, out_pitch, w, h, taps, in_zs, zs); break;
+    case 4: hipLaunchKernelGGL(k_gauss<4>, grid, dim3(GT_THREADS), 0, s, in, in_pitch, out, out_pitch, w, h, taps, in_zs, zs); break;
+    default: hipLaunchKernelGGL(k_gauss<5>, grid, dim3(GT_THREADS), 0, s, in, in_pitch, out, out_pitch, w, h, taps, in_zs, zs); break;
   }
 }
 
 void akz_launch_contrast(hipStream_t s, const float* blur, int pitch, float* grad, int w, int h, int nbins,
                          float perc, uint32_t* maxbits, uint32_t* blockmax, int32_t* hist_part, float* kc, int nlev,
-                         const int* octaves_host) {
+                         const int* octaves_host, akz_batch bt) {
   akz_octaves o;
   o.nlev = nlev;
   for (int i = 0; i < nlev; i++) o.oct[i] = octaves_host[i];
+  const int nz = bt.nz;
+  const size_t zs = bt.zs;
   int gtiles = ((w - 2 + 63) / 64) * ((h - 2 + 3) / 4);
   int gb = gtiles < CT_GRAD_BLOCKS ? gtiles : CT_GRAD_BLOCKS;
-  hipLaunchKernelGGL(k_contrast_grad, dim3(gb), dim3(256), 0, s, blur, pitch, grad, w, h, blockmax);
-  hipLaunchKernelGGL(k_contrast_max, dim3(1), dim3(1024), 0, s, blockmax, gb, maxbits);
+  hipLaunchKernelGGL(k_contrast_grad, dim3(gb, 1, nz), dim3(256), 0, s, blur, pitch, grad, w, h, blockmax, zs);
+  hipLaunchKernelGGL(k_contrast_max, dim3(1, 1, nz), dim3(1024), 0, s, blockmax, gb, maxbits, zs);
   int htiles = ((w - 2 + 63) / 64) * ((h - 2 + CT_HIST_THREADS / 64 - 1) / (CT_HIST_THREADS / 64));
   int hb = htiles < CT_HIST_BLOCKS ? htiles : CT_HIST_BLOCKS;
-  hipLaunchKernelGGL(k_contrast_hist, dim3(hb), dim3(CT_HIST_THREADS), 0, s, grad, pitch, w, h, maxbits, nbins,
-                     hist_part);
+  hipLaunchKernelGGL(k_contrast_hist, dim3(hb, 1, nz), dim3(CT_HIST_THREADS), 0, s, grad, pitch, w, h, maxbits,
+                     nbins, hist_part, zs);
   int32_t* bin_tot = hist_part + (size_t)512 * CT_HIST_BLOCKS;
-  hipLaunchKernelGGL(k_contrast_bins, dim3(nbins), dim3(64), 0, s, hist_part, hb, bin_tot);
-  hipLaunchKernelGGL(k_contrast_finish, dim3(1), dim3(1024), 0, s, bin_tot, nbins, maxbits, w, h, perc, kc, o);
+  hipLaunchKernelGGL(k_contrast_bins, dim3(nbins, 1, nz), dim3(64), 0, s, hist_part, hb, bin_tot, zs);
+  hipLaunchKernelGGL(k_contrast_finish, dim3(1, 1, nz), dim3(1024), 0, s, bin_tot, nbins, maxbits, w, h, perc, kc, o,
+                     zs);
 }
 
 void akz_launch_smooth_flow(hipStream_t s, const float* in, int pitch, float* lsm, float* g, int w, int h, int type,
-                            const float* kc, const akz_taps& taps) {
-  hipLaunchKernelGGL(k_smooth_flow, dim3((w + 63) / 64, (h + 15) / 16), dim3(256), 0, s, in, pitch, lsm, g, w, h,
-                     type, kc, taps);
+                            const float* kc, const akz_taps& taps, akz_batch bt) {
+  hipLaunchKernelGGL(k_smooth_flow, dim3((w + 63) / 64, (h + 15) / 16, bt.nz), dim3(256), 0, s, in, pitch, lsm, g, w,
+                     h, type, kc, taps, bt.zs);
 }
 
-
 void akz_launch_fed_step(hipStream_t s, const float* Lin, const float* g, float* Lout, int pitch, int w, int h,
-                         float half_tau) {
-  hipLaunchKernelGGL(k_fed_step, grid64x4(w, h), dim3(256), 0, s, Lin, g, Lout, pitch, w, h, half_tau);
+                         float half_tau, akz_batch bt) {
+  hipLaunchKernelGGL(k_fed_step, grid64x4(w, h, bt.nz), dim3(256), 0, s, Lin, g, Lout, pitch, w, h, half_tau, bt.zs);
 }
 
 int akz_fed_block_steps() { return FB_K; }
@@ -510,16 +548,18 @@ int akz_fed_block_steps() { return FB_K; }
 // (tile = region minus the 8-pixel halo): 2 64x64/512 threads, 3 32x32/256,
 // 4 64x32/256, 5 32x32/1024, 6 64x64/1024, 7 64x48/512, 8 32x48/512.
 void akz_launch_fed_variant(hipStream_t s, int variant, const float* Lin, const float* g, float* Lout, int pitch,
-                            int w, int h, const float* half_tau, int n) {
+                            int w, int h, const float* half_tau, int n, akz_batch bt) {
+  const int nz = bt.nz;
+  const size_t zs = bt.zs;
   if (variant <= 1) {
     for (int j = 0; j < n; j++) {
       const float* src = (j == 0) ? Lin : Lout;
       if (variant == 0) {
-        hipLaunchKernelGGL(k_fed_step, dim3((w + 63) / 64, (h + 3) / 4), dim3(256), 0, s, src, g, Lout, pitch, w, h,
-                           half_tau[j]);
+        hipLaunchKernelGGL(k_fed_step, dim3((w + 63) / 64, (h + 3) / 4, nz), dim3(256), 0, s, src, g, Lout, pitch, w,
+                           h, half_tau[j], zs);
       } else {
-        hipLaunchKernelGGL(k_fed_tile, dim3((w + 63) / 64, (h + 15) / 16), dim3(256), 0, s, src, g, Lout, pitch, w,
-                           h, half_tau[j]);
+        hipLaunchKernelGGL(k_fed_tile, dim3((w + 63) / 64, (h + 15) / 16, nz), dim3(256), 0, s, src, g, Lout, pitch,
+                           w, h, half_tau[j], zs);
       }
     }
     return;
@@ -528,26 +568,26 @@ void akz_launch_fed_variant(hipStream_t s, int variant, const float* Lin, const 
   st.n = n;
   for (int i = 0; i < FB_K; i++) st.half_tau[i] = i < n ? half_tau[i] : 0.0f;
   if (variant == 2) {
-    dim3 grid((w + 47) / 48, (h + 47) / 48);
-    hipLaunchKernelGGL((k_fed_multi<64, 64, 512>), grid, dim3(512), 0, s, Lin, g, Lout, pitch, w, h, st);
+    dim3 grid((w + 47) / 48, (h + 47) / 48, nz);
+    hipLaunchKernelGGL((k_fed_multi<64, 64, 512>), grid, dim3(512), 0, s, Lin, g, Lout, pitch, w, h, st, zs);
   } else if (variant == 3) {
-    dim3 grid((w + 15) / 16, (h + 15) / 16);
-    hipLaunchKernelGGL((k_fed_multi<32, 32, 256>), grid, dim3(256), 0, s, Lin, g, Lout, pitch, w, h, st);
+    dim3 grid((w + 15) / 16, (h + 15) / 16, nz);
+    hipLaunchKernelGGL((k_fed_multi<32, 32, 256>), grid, dim3(256), 0, s, Lin, g, Lout, pitch, w, h, st, zs);
   } else if (variant == 5) {
-    dim3 grid((w + 15) / 16, (h + 15) / 16);
-    hipLaunchKernelGGL((k_fed_multi<32, 32, 1024>), grid, dim3(1024), 0, s, Lin, g, Lout, pitch, w, h, st);
+    dim3 grid((w + 15) / 16, (h + 15) / 16, nz);
+    hipLaunchKernelGGL((k_fed_multi<32, 32, 1024>), grid, dim3(1024), 0, s, Lin, g, Lout, pitch, w, h, st, zs);
   } else if (variant == 6) {
-    dim3 grid((w + 47) / 48, (h + 47) / 48);
-    hipLaunchKernelGGL((k_fed_multi<64, 64, 1024>), grid, dim3(1024), 0, s, Lin, g, Lout, pitch, w, h, st);
+    dim3 grid((w + 47) / 48, (h + 47) / 48, nz);
+    hipLaunchKernelGGL((k_fed_multi<64, 64, 1024>), grid, dim3(1024), 0, s, Lin, g, Lout, pitch, w, h, st, zs);
   } else if (variant == 7) {
-    dim3 grid((w + 47) / 48, (h + 31) / 32);
-    hipLaunchKernelGGL((k_fed_multi<64, 48, 512>), grid, dim3(512), 0, s, Lin, g, Lout, pitch, w, h, st);
+    dim3 grid((w + 47) / 48, (h + 31) / 32, nz);
+    hipLaunchKernelGGL((k_fed_multi<64, 48, 512>), grid, dim3(512), 0, s, Lin, g, Lout, pitch, w, h, st, zs);
   } else if (variant == 8) {
-    dim3 grid((w + 15) / 16, (h + 31) / 32);
-    hipLaunchKernelGGL((k_fed_multi<32, 48, 512>), grid, dim3(512), 0, s, Lin, g, Lout, pitch, w, h, st);
+    dim3 grid((w + 15) / 16, (h + 31) / 32, nz);
+    hipLaunchKernelGGL((k_fed_multi<32, 48, 512>), grid, dim3(512), 0, s, Lin, g, Lout, pitch, w, h, st, zs);
   } else {
-    dim3 grid((w + 47) / 48, (h + 15) / 16);
-    hipLaunchKernelGGL((k_fed_multi<64, 32, 256>), grid, dim3(256), 0, s, Lin, g, Lout, pitch, w, h, st);
+    dim3 grid((w + 47) / 48, (h + 15) / 16, nz);
+    hipLaunchKernelGGL((k_fed_multi<64, 32, 256>), grid, dim3(256), 0, s, Lin, g, Lout, pitch, w, h, st, zs);
   }
 }
 
@@ -563,18 +603,20 @@ int akz_fed_variant_for(int w, int h) {
 }
 
 void akz_launch_fed_multi(hipStream_t s, const float* Lin, const float* g, float* Lout, int pitch, int w, int h,
-                          const float* half_tau, int n) {
-  akz_launch_fed_variant(s, akz_fed_variant_for(w, h), Lin, g, Lout, pitch, w, h, half_tau, n);
+                          const float* half_tau, int n, akz_batch bt) {
+  akz_launch_fed_variant(s, akz_fed_variant_for(w, h), Lin, g, Lout, pitch, w, h, half_tau, n, bt);
 }
 
 void akz_launch_halfsample(hipStream_t s, const float* in, int iw, int ih, int ipitch, float* out, int ow, int oh,
-                           int opitch) {
+                           int opitch, akz_batch bt) {
   if (iw == 2 * ow && ih == 2 * oh)
-    hipLaunchKernelGGL(k_half2, grid64x4(ow, oh), dim3(256), 0, s, in, ipitch, out, ow, oh, opitch);
+    hipLaunchKernelGGL(k_half2, grid64x4(ow, oh, bt.nz), dim3(256), 0, s, in, ipitch, out, ow, oh, opitch, bt.zs);
   else
-    hipLaunchKernelGGL(k_half_general, grid64x4(ow, oh), dim3(256), 0, s, in, iw, ih, ipitch, out, ow, oh, opitch);
+    hipLaunchKernelGGL(k_half_general, grid64x4(ow, oh, bt.nz), dim3(256), 0, s, in, iw, ih, ipitch, out, ow, oh,
+                       opitch, bt.zs);
 }
 
-void akz_launch_copy2d(hipStream_t s, const float* in, int ipitch, float* out, int opitch, int w, int h) {
-  hipLaunchKernelGGL(k_copy2d, grid64x4(w, h), dim3(256), 0, s, in, ipitch, out, opitch, w, h);
+void akz_launch_copy2d(hipStream_t s, const float* in, int ipitch, float* out, int opitch, int w, int h,
+                       akz_batch bt) {
+  hipLaunchKernelGGL(k_copy2d, grid64x4(w, h, bt.nz), dim3(256), 0, s, in, ipitch, out, opitch, w, h, bt.zs);
 }

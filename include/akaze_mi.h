@@ -129,6 +129,33 @@ akz_status akz_input_buffer(akz_ctx* ctx, float** d_ptr, size_t* pitch_bytes);
  * run it as eager launches (0). */
 akz_status akz_set_graph_mode(akz_ctx* ctx, int enabled);
 
+/* ---- Batches of same-size images.  The reference processes one image per
+ * AKAZE object and call (AKAZE.cpp:211-449); these entry points have no
+ * counterpart there and serve callers with many images of one size.  A batch
+ * context holds `batch` images with identical device layouts; every stage
+ * runs as one launch over all images of a call (grid z = image), and results
+ * equal those of one single-image context per image. ---- */
+#define AKZ_MAX_BATCH 64
+akz_status akz_create_batch(const akz_options* opts, int batch, akz_ctx** out);
+akz_status akz_batch_capacity(const akz_ctx* ctx, int* batch);
+/* Image 0's input buffer; image z's starts `image_stride_bytes` * z later. */
+akz_status akz_batch_input_buffer(akz_ctx* ctx, float** d_ptr, size_t* pitch_bytes, size_t* image_stride_bytes);
+/* Images 0..n-1 (n <= batch) at d_imgs + z * image_stride_bytes (0 = dense
+ * images), scale space + detection + description, asynchronous.  In graph
+ * mode one graph per batch size is captured on first use. */
+akz_status akz_process_batch_device_async(akz_ctx* ctx, const float* d_imgs, size_t pitch_bytes,
+                                          size_t image_stride_bytes, int n);
+/* Waits; per-image kept counts and statuses (AKZ_OK or AKZ_ERR_OVERFLOW). */
+akz_status akz_read_batch_counts(akz_ctx* ctx, int n, int* counts, int* statuses);
+/* Host copy of image z's results (as akz_read_results); waits. */
+akz_status akz_read_batch_results(akz_ctx* ctx, int z, akz_keypoint* kps, uint8_t* desc, int cap, int* n_out);
+akz_status akz_batch_results_device(akz_ctx* ctx, int z, const int32_t** d_count, const akz_keypoint** d_kps,
+                                    const uint8_t** d_desc64);
+akz_status akz_read_batch_level(akz_ctx* ctx, int z, int lvl, int which, float* dst);
+/* akz_profile_fed over a batch of n images (bytes count all n images). */
+akz_status akz_profile_fed_batch(akz_ctx* ctx, const float* d_imgs, size_t pitch_bytes, size_t image_stride_bytes,
+                                 int n, float* fed_ms, double* fed_bytes, int* fed_launches);
+
 /* Debug readback of a level image (AKAZE::Save_Scale_Space reads Lt back,
  * AKAZE.cpp:453-463).  which: 0 Lt, 1 Lsmooth, 2 Lx, 3 Ly, 4 Ldet.
  * dst is dense w*h floats. */

@@ -1,0 +1,89 @@
+"""GPU parity of same-size image batches (grid z = image) against the CPU
+oracle, image by image: keypoints and descriptors must be identical to the
+oracle's for each image, whatever the other images of the batch hold."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_run(W, H, img, **kw):
+    orc = O.Pipeline(W, H, **kw)
+    return orc.run(img)
+
+
+def check_image(bk, z, okp, odesc, where):
+    kps, desc, n = bk.read_results(z, len(okp) + 16)
+    assert n == len(okp), (where, z, n, len(okp))
+    assert kps.tobytes() == okp.tobytes(), (where, z)
+    assert np.array_equal(desc, odesc), (where, z)
+
+
+def test_batch_eager_mixed_density(gpu_lib):
+    """Four 640x480 images in one batch, one of them noise with tens of
+    thousands of candidates (global-memory suppression path) beside sparse
+    ones (LDS path); then a smaller batch reuses the context."""
+    A = gpu_lib
+    W, H, thr = 640, 480, 2e-4
+    rng = np.random.default_rng(3)
+    imgs = [A.to_float(A.synth_image(10 + i, W, H)) for i in range(3)]
+    imgs.insert(2, rng.random((H, W)).astype(np.float32))
+    ref = [oracle_run(W, H, im, dthreshold=thr) for im in imgs]
+    bk = A.BatchAKAZE(A.AKAZEOptions(img_width=W, img_height=H, dthreshold=thr), 4)
+    bk.upload(imgs)
+    bk.process_async(4)
+    counts, st = bk.counts(4)
+    assert list(counts) == [len(r[0]) for r in ref]
+    assert all(s == 0 for s in st)
+    for z in range(4):
+        check_image(bk, z, ref[z][0], ref[z][1], "batch4")
+    # level readback of image 2 (the noise image)
+    orc = O.Pipeline(W, H, dthreshold=thr)
+    orc.scale_space(imgs[2])
+    for lvl in (0, 5, bk.nlev - 1):
+        assert np.array_equal(bk.read_level(2, lvl, 0), orc.level(lvl, 0)), lvl
+    imgs2 = [A.to_float(A.synth_image(40 + i, W, H)) for i in range(2)]
+    ref2 = [oracle_run(W, H, im, dthreshold=thr) for im in imgs2]
+    bk.upload(imgs2)
+    bk.process_async(2)
+    for z in range(2):
+        check_image(bk, z, ref2[z][0], ref2[z][1], "batch2")
+
+
+def test_batch_graph_sizes(gpu_lib):
+    """Graph mode captures one graph per batch size; replays of sizes 3, 1,
+    3 with eager work in between each give the oracle's results."""
+    A = gpu_lib
+    W, H = 320, 240
+    imgs = [A.to_float(A.synth_image(60 + i, W, H)) for i in range(3)]
+    ref = [oracle_run(W, H, im) for im in imgs]
+    bk = A.BatchAKAZE(A.AKAZEOptions(img_width=W, img_height=H), 3)
+    bk.set_graph_mode(True)
+    bk.upload(imgs)
+    other = A.AKAZE(A.AKAZEOptions(img_width=W, img_height=H))
+    for rep, n in enumerate([3, 1, 3, 1]):
+        bk.process_async(n)
+        bk.wait()
+        for z in range(n):
+            check_image(bk, z, ref[z][0], ref[z][1], "graph rep %d" % rep)
+        other.run(A.to_float(A.synth_image(200 + rep, W, H)))
+
+
+def test_batch_1080p_equals_single(gpu_lib):
+    """Two 1920x1080 images: the batch result of each equals a single-image
+    context's result and the oracle's."""
+    A = gpu_lib
+    W, H = 1920, 1080
+    imgs = [A.to_float(A.synth_image(70 + i, W, H)) for i in range(2)]
+    bk = A.BatchAKAZE(A.AKAZEOptions(img_width=W, img_height=H), 2)
+    bk.upload(imgs)
+    bk.process_async(2)
+    bk.wait()
+    for z in range(2):
+        single = A.AKAZE(A.AKAZEOptions(img_width=W, img_height=H))
+        skp, sdesc = single.run(imgs[z])
+        okp, odesc = oracle_run(W, H, imgs[z])
+        assert skp.tobytes() == okp.tobytes() and np.array_equal(sdesc, odesc)
+        check_image(bk, z, okp, odesc, "1080p")
