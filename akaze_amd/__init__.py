@@ -90,6 +90,18 @@ SIGNATURES = {
     "akz_batch_results_device": [_P, _I, _P, _P, _P],
     "akz_read_batch_level": [_P, _I, _I, _I, _P],
     "akz_profile_fed_batch": [_P, _P, _SZ, _SZ, _I, _P, _P, _P],
+    "akz_prop_num_params": [_P],
+    "akz_prop_create": [_U64, _P],
+    "akz_prop_destroy": [_P],
+    "akz_prop_set_params": [_P, _P],
+    "akz_prop_get_params": [_P, _P],
+    "akz_prop_positions": [_P, _P],
+    "akz_prop_extract": [_P, _I, _P, _P],
+    "akz_prop_forward": [_P, _P, _P, _P, _I, _P],
+    "akz_prop_train_step": [_P, _P, _P, _P, _P, _I, _F, _F, _P, _P],
+    "akz_set_proposal": [_P, _P, _F],
+    "akz_prop_read_probs": [_P, _I, _P],
+    "akz_synth_device_batch": [_U64, _I, _I, _I, _P, _SZ, _SZ],
     "akz_matcher_create": [_P],
     "akz_matcher_destroy": [_P],
     "akz_match2nn": [_P, _P, _I, _P, _I, _SZ, _P],
@@ -442,6 +454,30 @@ class BatchAKAZE:
 
     def wait(self):
         _check(lib().akz_wait(self.ctx), "akz_wait")
+
+    def set_proposal(self, net, threshold=0.5):
+        """Detect with the learned tile proposal `net` (None: full scan)."""
+        _check(lib().akz_set_proposal(self.ctx, net.net if net is not None else None, float(threshold)),
+               "akz_set_proposal")
+
+    def proposal_positions(self):
+        n = ctypes.c_int()
+        _check(lib().akz_prop_positions(self.ctx, ctypes.byref(n)), "akz_prop_positions")
+        return n.value
+
+    def read_probs(self, z):
+        out = np.zeros(self.proposal_positions(), np.float32)
+        _check(lib().akz_prop_read_probs(self.ctx, int(z), out.ctypes.data), "akz_prop_read_probs")
+        return out
+
+    def extract(self, n, d_feat, d_labels=None):
+        """Proposal features (and labels) of images 0..n-1 of the last run into device arrays."""
+        _check(lib().akz_prop_extract(self.ctx, int(n), d_feat, d_labels), "akz_prop_extract")
+
+    def synth(self, seed0, n):
+        """Renders n device-generated synthetic images into the input buffers."""
+        ptr, pitch, zs = self.input_buffer()
+        _check(lib().akz_synth_device_batch(seed0, int(n), self.W, self.H, ptr, pitch, zs), "akz_synth_device_batch")
 
     def run(self, imgs):
         """Scale space + detection + description of host images; list of (kps, desc)."""

@@ -27,6 +27,8 @@ SOURCES = [
     "akz_detect.hip",
     "akz_describe.hip",
     "akz_match.hip",
+    "akz_proposal.hip",
+    "akz_synth_dev.hip",
     "akz_capi.hip",
     "akz_synth.cpp",
     "AKAZE.cpp",

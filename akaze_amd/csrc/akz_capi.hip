@@ -132,6 +132,12 @@ struct akz_ctx {
   std::vector<hipGraphExec_t> graph_exec;
   std::vector<hipGraph_t> graph;
   // optional per-level events around the FED launches (akz_profile_fed)
+  // learned tile proposal
+  akz_prop_src psrc{};
+  akz_prop_act pact{};  // inference arrays of `batch` images, allocated on first use
+  void* prop_mem = nullptr;
+  akz_prop_net* prop_net = nullptr;
+  float prop_thr = 0.5f;
   bool fed_profile = false;
   int fed_block = 1;  // FED steps per launch
   hipEvent_t fed_ev[2 * AKZ_MAX_LEVELS] = {};
@@ -264,7 +270,7 @@ akz_status allocate(akz_ctx* c) {
   size_t o_nbl = cv.take<int32_t>(nb_cap);
   size_t o_mem = cv.take<int32_t>(c->cap), o_minnb = cv.take<int32_t>(c->cap), o_code = cv.take<int32_t>(c->cap);
   size_t o_act = cv.take<int32_t>((size_t)c->cap * 2);
-  size_t o_kps = cv.take<akz_kp>(c->cap);
+  size_t o_kps = cv.take<akz_kp>(c->cap), o_kidx = cv.take<int32_t>(c->cap);
   size_t o_desc = cv.take<uint8_t>((size_t)c->cap * AKZ_DESC_STRIDE);
   c->zs = align_up(cv.off, ALIGN);
   // after the batch's sub-slabs: the contiguous status blocks of all images
@@ -311,6 +317,7 @@ akz_status allocate(akz_ctx* c) {
   D.code = (int32_t*)(b + o_code);
   D.act = (int32_t*)(b + o_act);
   D.kps = c->kps;
+  D.kidx = (int32_t*)(b + o_kidx);
   D.status = c->status;
   D.status_all = (int32_t*)(b + c->zs * (size_t)c->batch);
   D.cap = c->cap;
@@ -332,6 +339,25 @@ akz_status allocate(akz_ctx* c) {
     c->dp.w[i] = L.w;
     c->dp.h[i] = L.h;
     c->dp.pitch[i] = L.pitch;
+  }
+  // proposal source view: level geometry, determinant and Lt arrays
+  {
+    int w[AKZ_MAX_LEVELS], h[AKZ_MAX_LEVELS];
+    for (int i = 0; i < c->nlev; i++) { w[i] = c->lev[i].w; h[i] = c->lev[i].h; }
+    c->psrc.g = akz_prop_make_geom(c->nlev, w, h, c->octs);
+    int ts = 0;
+    for (int i = 0; i < c->nlev; i++) {
+      c->psrc.det[i] = c->lev[i].det;
+      c->psrc.Lt[i] = c->lev[i].Lt;
+      c->psrc.w[i] = w[i];
+      c->psrc.h[i] = h[i];
+      c->psrc.pitch[i] = c->lev[i].pitch;
+      c->psrc.tile_start[i] = ts;
+      ts += ((w[i] + 63) / 64) * c->psrc.g.th[i];
+    }
+    c->psrc.tile_start[c->nlev] = ts;
+    c->psrc.thr = std::max(c->opts.dthreshold, c->opts.min_dthreshold);
+    c->psrc.zs = c->zs;
   }
   c->dp.patsize = c->opts.descriptor_pattern_size;
   c->dp.zs = c->zs;
@@ -418,6 +444,11 @@ void enqueue_detect(akz_ctx* c, int nz) {
     record(c, EV_DERIV);
     akz_launch_extrema(s, c->det, c->db, nz);
   }
+  if (c->prop_net) {
+    akz_prop_launch_features(s, c->psrc, c->pact, nz);
+    akz_prop_launch_infer(s, akz_prop_net_weights(c->prop_net), c->psrc.g, c->pact, nz, c->prop_thr);
+    akz_prop_launch_mask_segments(s, c->det, c->db, c->psrc.g, c->pact.mask, nz);
+  }
   akz_launch_candidates(s, c->det, c->db, nz);
   record(c, EV_EXTREMA);
   akz_launch_suppression(s, c->db, nz);
@@ -455,6 +486,36 @@ akz_status image_status(const akz_ctx* c, int z, int* nk) {
   if (nk) *nk = st[AKZ_ST_KEPT];
   if (st[AKZ_ST_FLAGS] != 0 || st[AKZ_ST_TOTAL] > c->cap) return AKZ_ERR_OVERFLOW;
   return AKZ_OK;
+}
+
+// Inference arrays of the proposal for the context's batch.
+akz_status ensure_prop_act(akz_ctx* c) {
+  if (c->prop_mem) return AKZ_OK;
+  const size_t P = (size_t)c->psrc.g.P * c->batch;
+  Carver cv;
+  size_t o_x0 = cv.take<uint16_t>(P * 4), o_h1 = cv.take<uint16_t>(P * 16), o_h2 = cv.take<uint16_t>(P * 32);
+  size_t o_h3 = cv.take<uint16_t>(P * 32), o_tm = cv.take<float>(P), o_ts = cv.take<float>(P);
+  size_t o_pr = cv.take<float>(P), o_mk = cv.take<uint8_t>(P);
+  AKZ_HIP_TRY(hipMalloc(&c->prop_mem, cv.off));
+  char* b = (char*)c->prop_mem;
+  c->pact.x0 = (uint16_t*)(b + o_x0);
+  c->pact.h1 = (uint16_t*)(b + o_h1);
+  c->pact.h2 = (uint16_t*)(b + o_h2);
+  c->pact.h3 = (uint16_t*)(b + o_h3);
+  c->pact.tmax = (float*)(b + o_tm);
+  c->pact.tmean = (float*)(b + o_ts);
+  c->pact.prob = (float*)(b + o_pr);
+  c->pact.mask = (uint8_t*)(b + o_mk);
+  return AKZ_OK;
+}
+
+void drop_graphs(akz_ctx* c) {
+  for (size_t i = 0; i < c->graph_exec.size(); i++) {
+    if (c->graph_exec[i]) (void)hipGraphExecDestroy(c->graph_exec[i]);
+    if (c->graph[i]) (void)hipGraphDestroy(c->graph[i]);
+    c->graph_exec[i] = nullptr;
+    c->graph[i] = nullptr;
+  }
 }
 
 template <class T>
@@ -560,6 +621,7 @@ akz_status akz_destroy(akz_ctx* c) {
   for (int i = 0; i < 2 * AKZ_MAX_LEVELS; i++)
     if (c->fed_ev[i]) hipEventDestroy(c->fed_ev[i]);
   if (c->slab) hipFree(c->slab);
+  if (c->prop_mem) hipFree(c->prop_mem);
   if (c->h_small) hipHostFree(c->h_small);
   if (c->h_stage) hipHostFree(c->h_stage);
   if (c->stream) hipStreamDestroy(c->stream);
@@ -869,6 +931,61 @@ akz_status akz_profile_fed_batch(akz_ctx* c, const float* d_imgs, size_t pitch_b
 akz_status akz_profile_fed(akz_ctx* c, const float* d_img, size_t pitch_bytes, float* fed_ms, double* fed_bytes,
                            int* fed_launches) {
   return akz_profile_fed_batch(c, d_img, pitch_bytes, 0, 1, fed_ms, fed_bytes, fed_launches);
+}
+
+akz_status akz_prop_positions(const akz_ctx* c, int* per_image) {
+  if (!c || !per_image) return AKZ_ERR_ARG;
+  *per_image = c->psrc.g.P;
+  return AKZ_OK;
+}
+
+akz_status akz_prop_extract(akz_ctx* c, int n, uint16_t* d_feat, uint8_t* d_labels) {
+  if (!c || !d_feat || n < 1 || n > c->batch) return AKZ_ERR_ARG;
+  if (c->state < 2 || n > c->last_nz) return AKZ_ERR_STATE;
+  akz_status st = ensure_prop_act(c);
+  if (st != AKZ_OK) return st;
+  akz_prop_act act = c->pact;
+  act.x0 = d_feat;
+  akz_prop_launch_features(c->stream, c->psrc, act, n);
+  if (d_labels) akz_prop_launch_labels(c->stream, c->db, c->psrc.g, d_labels, n);
+  AKZ_HIP_TRY(hipGetLastError());
+  AKZ_HIP_TRY(hipStreamSynchronize(c->stream));
+  return AKZ_OK;
+}
+
+akz_status akz_prop_forward(akz_prop_net* net, const akz_ctx* c, const uint16_t* d_feat, const int* idx, int nimg,
+                            float* probs) {
+  if (!net || !c || !d_feat || nimg < 1) return AKZ_ERR_ARG;
+  return akz_prop_forward_geom(net, c->psrc.g, d_feat, idx, nimg, probs);
+}
+
+akz_status akz_prop_train_step(akz_prop_net* net, const akz_ctx* c, const uint16_t* d_feat,
+                               const uint8_t* d_labels, const int* idx, int nimg, float pos_weight, float lr,
+                               float* grad, float* loss) {
+  if (!net || !c || !d_feat || !d_labels || nimg < 1 || !(pos_weight > 0.0f) || lr < 0.0f) return AKZ_ERR_ARG;
+  return akz_prop_grad_geom(net, c->psrc.g, d_feat, d_labels, idx, nimg, pos_weight, lr, grad, loss);
+}
+
+akz_status akz_set_proposal(akz_ctx* c, akz_prop_net* net, float threshold) {
+  if (!c || !(threshold >= 0.0f && threshold <= 1.0f)) return AKZ_ERR_ARG;
+  if (net) {
+    akz_status st = ensure_prop_act(c);
+    if (st != AKZ_OK) return st;
+  }
+  AKZ_HIP_TRY(hipStreamSynchronize(c->stream));
+  c->prop_net = net;
+  c->prop_thr = threshold;
+  drop_graphs(c);
+  return AKZ_OK;
+}
+
+akz_status akz_prop_read_probs(akz_ctx* c, int z, float* probs) {
+  if (!c || !probs || z < 0 || z >= c->batch) return AKZ_ERR_ARG;
+  if (!c->prop_net || z >= c->last_nz) return AKZ_ERR_STATE;
+  AKZ_HIP_TRY(hipStreamSynchronize(c->stream));
+  AKZ_HIP_TRY(hipMemcpy(probs, c->pact.prob + (size_t)z * c->psrc.g.P, sizeof(float) * c->psrc.g.P,
+                        hipMemcpyDeviceToHost));
+  return AKZ_OK;
 }
 
 akz_status akz_copy_kernel_run(const float* d_src, float* d_dst, size_t nfloats, int iters, float* avg_ms) {

@@ -117,7 +117,9 @@ __global__ __launch_bounds__(256) void k_extrema(akz_det_params p, int32_t* __re
 // per 64x32 tile of every level in one launch.  Lsmooth is loaded with a
 // (2s+1)-pixel halo, Lx and Ly are computed on an (s+1)-pixel halo and the
 // determinant on a one-pixel ring, all in LDS (rows padded to 96 floats, so a
-// wavefront's lanes walk one row and nothing is divided).  Every stencil
+// wavefront's lanes walk one row and nothing is divided).  The determinant
+// reuses the Lsmooth buffer, dead once the first derivatives exist, which
+// keeps a block at 51.5 KB of LDS: three blocks per CU.  Every stencil
 // addresses its reflect-101 neighbours through image coordinates, which stay
 // inside the loaded regions.  Lx, Ly and det are written for the tile's own
 // pixels and each 64-pixel row is one wavefront whose ballot gives the
@@ -126,6 +128,7 @@ constexpr int DF_SMAX = 4, DF_TH = 32;
 constexpr int DF_H1 = 2 * DF_SMAX + 1, DF_H2 = DF_SMAX + 1;
 constexpr int DF_RW = 96;                       // LDS row stride, >= 64 + 2*DF_H1
 constexpr int DF_R1 = DF_TH + 2 * DF_H1, DF_R2 = DF_TH + 2 * DF_H2, DF_RD = DF_TH + 2;
+static_assert(DF_RD <= DF_R1, "the determinant fits the Lsmooth buffer");
 
 __global__ __launch_bounds__(512) void k_detect_fused(akz_det_params p, int32_t* __restrict__ seg_count,
                                                       uint8_t* __restrict__ seg_slots) {
@@ -134,7 +137,7 @@ __global__ __launch_bounds__(512) void k_detect_fused(akz_det_params p, int32_t*
   __shared__ float s_l[DF_R1 * DF_RW];
   __shared__ float s_x[DF_R2 * DF_RW];
   __shared__ float s_y[DF_R2 * DF_RW];
-  __shared__ float s_d[DF_RD * DF_RW];
+  float* const s_d = s_l;  // DF_RD <= DF_R1 rows
   const int l = find_level(p, blockIdx.x);
   const akz_level_desc& L = p.lev[l];
   const int t = blockIdx.x - p.tile_start[l];
@@ -529,6 +532,7 @@ __global__ __launch_bounds__(NMS_THREADS) void k_nms(akz_det_bufs b) {
   const int32_t* __restrict__ nb_list = akz_zoff(b.nb_list, b.zs);
   int32_t* g_code = akz_zoff(b.code, b.zs);
   akz_kp* __restrict__ out = akz_zoff(b.kps, b.zs);
+  int32_t* __restrict__ kidx = akz_zoff(b.kidx, b.zs);
   int32_t* st = akz_zoff(b.status, b.zs);
   const int cap = b.cap;
   const int t = threadIdx.x;
@@ -635,7 +639,10 @@ __global__ __launch_bounds__(NMS_THREADS) void k_nms(akz_det_bufs b) {
         const int v = (i < n) ? mem[i] : -1;
         if (v >= 0 && !(cand[v].size < 0)) {
           const int rank = s_pre[v >> 5] + __popc(s_bits[v >> 5] & ((1u << (v & 31)) - 1u));
-          if (run + rank < cap) out[run + rank] = emit(cand[v]);
+          if (run + rank < cap) {
+            out[run + rank] = emit(cand[v]);
+            kidx[run + rank] = v;
+          }
         }
       }
       __syncthreads();
@@ -664,7 +671,10 @@ __global__ __launch_bounds__(NMS_THREADS) void k_nms(akz_det_bufs b) {
       const int run = s_run;
       for (int e = t; e < 2048; e += NMS_THREADS) {
         const int v = s_sort[e];
-        if (v != INT32_MAX && run + e < cap) out[run + e] = emit(cand[v]);
+        if (v != INT32_MAX && run + e < cap) {
+          out[run + e] = emit(cand[v]);
+          kidx[run + e] = v;
+        }
       }
       __syncthreads();
       if (t == 0) s_run = run + cnt;

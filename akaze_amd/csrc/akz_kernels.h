@@ -71,6 +71,7 @@ struct akz_det_bufs {
   int32_t *mem, *minnb, *code;  // cap each (suppression state beyond LDS)
   int32_t* act;          // 2 x cap
   akz_kp* kps;           // cap output keypoints
+  int32_t* kidx;         // cap: candidate index of each output keypoint
   int32_t* status;       // AKZ_ST_* block
   int32_t* status_all;   // contiguous copies of every image's block (4 ints per image, not z-strided)
   int cap, nb_cap, max_waves;
@@ -89,6 +90,56 @@ void akz_launch_detect_fused(hipStream_t s, const akz_det_params& p32, const akz
 void akz_launch_candidates(hipStream_t s, const akz_det_params& p, const akz_det_bufs& b, int nz);
 // CSR neighbour lists and the wave-synchronous suppression with its output
 void akz_launch_suppression(hipStream_t s, const akz_det_bufs& b, int nz);
+
+// ---- learned tile proposal (akz_proposal.hip) ----
+// Tile maps: every level is cut into 16x16-pixel tiles; the tiles of an
+// image are numbered level-major, row-major (base[l] is level l's first
+// tile, P the tiles per image); image z's tiles follow at z * P.
+constexpr int AKZ_PROP_TILE = 16;
+struct akz_prop_geom {
+  int nlev, P;
+  int tw[AKZ_MAX_LEVELS], th[AKZ_MAX_LEVELS], oct[AKZ_MAX_LEVELS];
+  int base[AKZ_MAX_LEVELS + 1];
+};
+// What the proposal reads from a context's batch.
+struct akz_prop_src {
+  akz_prop_geom g;
+  const float* det[AKZ_MAX_LEVELS];
+  const float* Lt[AKZ_MAX_LEVELS];
+  int w[AKZ_MAX_LEVELS], h[AKZ_MAX_LEVELS], pitch[AKZ_MAX_LEVELS];
+  int tile_start[AKZ_MAX_LEVELS + 1];  // prefix of 64x16-pixel tile groups per level
+  float thr;                           // detector threshold (feature scale)
+  size_t zs;
+};
+akz_prop_geom akz_prop_make_geom(int nlev, const int* w, const int* h, const int* oct);
+// per-image device work arrays of the proposal inference (all images of a batch)
+struct akz_prop_act {
+  uint16_t *x0, *h1, *h2, *h3;  // bf16 activations, P x {4, 16, 32, 32} per image
+  float *tmax, *tmean;          // tile statistics, P per image
+  float* prob;                  // P per image
+  uint8_t* mask;                // P per image
+};
+struct akz_prop_weights;        // device weights of a network (opaque here)
+// features of images 0..nz-1 into act.x0 (tile statistics in act.tmax / tmean)
+void akz_prop_launch_features(hipStream_t s, const akz_prop_src& src, const akz_prop_act& act, int nz);
+// inference on nz images' features: probabilities and the tile mask (prob >= thr)
+void akz_prop_launch_infer(hipStream_t s, const akz_prop_weights* w, const akz_prop_geom& g, const akz_prop_act& act,
+                           int nz, float thr);
+// drops detector candidates outside masked-in tiles (segment slots, in order)
+void akz_prop_launch_mask_segments(hipStream_t s, const akz_det_params& p, const akz_det_bufs& b,
+                                   const akz_prop_geom& g, const uint8_t* mask, int nz);
+// labels: 1 for the tiles holding a kept keypoint of the last detection
+void akz_prop_launch_labels(hipStream_t s, const akz_det_bufs& b, const akz_prop_geom& g, uint8_t* labels, int nz);
+
+struct akz_prop_net;
+const akz_prop_weights* akz_prop_net_weights(const akz_prop_net* n);
+// forward pass on images idx[0..nimg) (all if idx is null) of a feature set
+akz_status akz_prop_forward_geom(akz_prop_net* n, const akz_prop_geom& g, const uint16_t* d_feat, const int* idx,
+                                 int nimg, float* h_prob);
+// loss gradient (and an Adam step when lr > 0) on images idx[0..nimg)
+akz_status akz_prop_grad_geom(akz_prop_net* n, const akz_prop_geom& g, const uint16_t* d_feat,
+                              const uint8_t* d_labels, const int* idx, int nimg, float wpos, float lr,
+                              float* h_grad, float* h_loss);
 
 // ---- description (akz_describe.hip) ----
 struct akz_desc_params {

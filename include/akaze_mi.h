@@ -180,6 +180,50 @@ akz_status akz_copy_kernel_run(const float* d_src, float* d_dst, size_t nfloats,
 /* AKAZE::Get_Computation_Times (AKAZE.h:139-141). */
 akz_status akz_get_timing(akz_ctx* ctx, akz_timing* t);
 
+/* ---- Learned tile proposal.  No counterpart in the reference.  A small
+ * convolutional network over 16x16-pixel tile maps of every Hessian-response
+ * level predicts the tiles that can hold kept keypoints.  With a network set
+ * on a context, detector candidates outside predicted tiles are dropped
+ * before the suppression; the full scan (no network) stays the fallback.
+ * Parameters: 14513 FP32 values in the flat order W1[9][4][16], b1[16],
+ * W2[9][16][32], b2[32], W3[9][32][32], b3[32], w4[32], b4, with
+ * W[tap][in][out] and tap = 3 (dy + 1) + (dx + 1).  Features are 4 bf16 per
+ * tile, tiles numbered level-major, row-major per image. ---- */
+typedef struct akz_prop_net akz_prop_net;
+akz_status akz_prop_num_params(int* n);
+/* He-normal weights from `seed`, zero biases, Adam state cleared. */
+akz_status akz_prop_create(uint64_t seed, akz_prop_net** out);
+akz_status akz_prop_destroy(akz_prop_net* net);
+akz_status akz_prop_set_params(akz_prop_net* net, const float* params);
+akz_status akz_prop_get_params(akz_prop_net* net, float* params);
+/* Tiles per image of the context's level geometry. */
+akz_status akz_prop_positions(const akz_ctx* ctx, int* per_image);
+/* Features of images 0..n-1 of the context's last run into d_feat
+ * (n x P x 4 bf16) and, if d_labels is not null, the labels (n x P bytes:
+ * 1 where a kept keypoint's detector pixel lies in the tile).  Waits. */
+akz_status akz_prop_extract(akz_ctx* ctx, int n, uint16_t* d_feat, uint8_t* d_labels);
+/* Probabilities (host, nimg x P) for images idx[0..nimg) of a feature set
+ * (idx null: images 0..nimg-1).  Waits. */
+akz_status akz_prop_forward(akz_prop_net* net, const akz_ctx* ctx, const uint16_t* d_feat, const int* idx, int nimg,
+                            float* probs);
+/* Weighted binary cross-entropy (weight pos_weight on positive tiles, mean
+ * over tiles) of images idx[0..nimg): gradient (host, optional), loss
+ * (optional) and, when lr > 0, one Adam step (beta 0.9 / 0.999, eps 1e-8).
+ * Waits. */
+akz_status akz_prop_train_step(akz_prop_net* net, const akz_ctx* ctx, const uint16_t* d_feat,
+                               const uint8_t* d_labels, const int* idx, int nimg, float pos_weight, float lr,
+                               float* grad, float* loss);
+/* Use `net` (null: full scan) with probability threshold `threshold` in the
+ * context's detection.  Captured graphs are rebuilt on the next call. */
+akz_status akz_set_proposal(akz_ctx* ctx, akz_prop_net* net, float threshold);
+/* Tile probabilities of image z from the last detection with a network. */
+akz_status akz_prop_read_probs(akz_ctx* ctx, int z, float* probs);
+/* Synthetic images rendered on the device (seeds seed0 .. seed0+n-1), float
+ * in [0,1], same image model as akz_synth_image with a hashed noise lattice.
+ * Waits. */
+akz_status akz_synth_device_batch(uint64_t seed0, int n, int w, int h, float* d_out, size_t pitch_bytes,
+                                  size_t image_stride_bytes);
+
 /* ---- Brute-force Hamming 2-NN matcher: Matcher (AKAZE.h:29-55, AKAZE.cpp:
  * 36-121) and MatchDescriptors (cuda_akaze.cu:1842-2031). ---- */
 akz_status akz_matcher_create(akz_matcher** out);

@@ -463,6 +463,11 @@ int orc_find_extrema(const float* imd, int w, int h, float border, float thr, in
 /* sortFiltered_kernel (:1227-1316).                                         */
 /* ------------------------------------------------------------------------ */
 int orc_nms(const orc_kp* cand, int n, const int* level_start, int nlev, int max_waves, orc_kp* out, int* waves_out) {
+  return orc_nms_ex(cand, n, level_start, nlev, max_waves, out, waves_out, NULL);
+}
+
+int orc_nms_ex(const orc_kp* cand, int n, const int* level_start, int nlev, int max_waves, orc_kp* out,
+               int* waves_out, int* kidx_out) {
   /* neighbour lists (CSR) */
   int* nb_off = (int*)malloc(sizeof(int) * (size_t)(n + 1));
   int nb_cap = n * 4 + 16, nb_n = 0;
@@ -567,6 +572,7 @@ int orc_nms(const orc_kp* cand, int n, const int* level_start, int nlev, int max
       int octave = (int)octsub;
       float subp = (octf < 0 ? -1 : 1) * (octsub - octave);
       float ratio = 1 << octave;
+      if (kidx_out) kidx_out[nout] = vals[a];
       orc_kp* t = &out[nout++];
       t->y = ratio * ((int)(0.5f + ok->y / ratio) + ok->angle);
       t->x = ratio * ((int)(0.5f + ok->x / ratio) + subp);
@@ -825,6 +831,7 @@ struct orc_ctx {
   int ncand, cand_cap;
   int lstart[ORC_MAX_LEVELS + 1];
   orc_kp* kps;
+  int* kidx;
   int nkps;
   int waves;
   uint8_t* desc;
@@ -847,6 +854,7 @@ void orc_destroy(orc_ctx* c) {
     for (int k = 0; k < 6; k++) free(c->lev[i][k]);
   free(c->cand);
   free(c->kps);
+  free(c->kidx);
   free(c->desc);
   free(c);
 }
@@ -914,7 +922,9 @@ int orc_detect(orc_ctx* c) {
   c->ncand = n;
   free(c->kps);
   c->kps = (orc_kp*)malloc(sizeof(orc_kp) * (size_t)(n + 1));
-  c->nkps = orc_nms(c->cand, n, c->lstart, g->nlev, c->o.nms_max_waves, c->kps, &c->waves);
+  free(c->kidx);
+  c->kidx = (int*)malloc(sizeof(int) * (size_t)(n + 1));
+  c->nkps = orc_nms_ex(c->cand, n, c->lstart, g->nlev, c->o.nms_max_waves, c->kps, &c->waves, c->kidx);
   return c->nkps;
 }
 
@@ -923,6 +933,7 @@ const orc_kp* orc_candidates(const orc_ctx* c) { return c->cand; }
 const int* orc_level_start(const orc_ctx* c) { return c->lstart; }
 int orc_nms_waves(const orc_ctx* c) { return c->waves; }
 int orc_num_keypoints(const orc_ctx* c) { return c->nkps; }
+const int* orc_kept_candidates(const orc_ctx* c) { return c->kidx; }
 const orc_kp* orc_keypoints(const orc_ctx* c) { return c->kps; }
 const uint8_t* orc_descriptors(const orc_ctx* c) { return c->desc; }
 

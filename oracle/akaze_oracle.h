@@ -95,6 +95,9 @@ int orc_find_extrema(const float* det, int w, int h, float border, float thr, in
                      float size, orc_kp* out, int cap);
 int orc_nms(const orc_kp* cand, int n, const int* level_start, int nlev, int max_waves, orc_kp* out,
             int* waves_out);
+/* orc_nms, also returning the candidate index of every output keypoint. */
+int orc_nms_ex(const orc_kp* cand, int n, const int* level_start, int nlev, int max_waves, orc_kp* out,
+               int* waves_out, int* kidx_out);
 void orc_orientation(orc_kp* kp, const float* Lx, const float* Ly, int w, int h);
 void orc_mldb(const orc_kp* kp, const float* Lt, const float* Lx, const float* Ly, int w, int h,
               int patsize, uint8_t* desc61, float* vals87);
@@ -122,6 +125,34 @@ const int* orc_level_start(const orc_ctx* c);
 int orc_nms_waves(const orc_ctx* c);
 int orc_num_keypoints(const orc_ctx* c);
 const orc_kp* orc_keypoints(const orc_ctx* c);
+/* candidate index of every kept keypoint (orc_candidates order) */
+const int* orc_kept_candidates(const orc_ctx* c);
+
+/* ---- Learned tile proposal (test oracle of akaze_amd/csrc/akz_proposal.hip;
+ * no reference counterpart).  Tile maps of 16x16 pixels per level, numbered
+ * level-major and row-major; features x0 (4 bf16 per tile), a 3x3 conv net
+ * 4 -> 16 -> 32 -> 32 with relu and a 1x1 sigmoid head.  Parameters are the
+ * flat FP32 vector documented in include/akaze_mi.h (14513 values). ---- */
+typedef struct {
+  int nlev, P;
+  int tw[ORC_MAX_LEVELS], th[ORC_MAX_LEVELS], oct[ORC_MAX_LEVELS];
+  int base[ORC_MAX_LEVELS + 1];
+} orc_prop_geom;
+void orc_prop_make_geom(const orc_ctx* c, orc_prop_geom* g);
+/* features of the context's levels (after orc_detect): P x 4 bf16 */
+void orc_prop_features(const orc_ctx* c, float thr, uint16_t* x0);
+/* labels (after orc_detect): 1 where a kept keypoint's candidate pixel lies */
+void orc_prop_labels(const orc_ctx* c, uint8_t* lab);
+/* Weighted BCE loss over nimg images' tiles and, if grad is not null, its
+ * gradient (flat parameter order); prob (optional) receives the tile
+ * probabilities.  emulate_bf16 = 1 rounds weights, activations and
+ * back-propagated gradients to bf16 where the device feeds MFMAs; 0 computes
+ * everything in double (the reference for finite differences). */
+double orc_prop_loss_grad(const orc_prop_geom* g, int nimg, const float* params, const uint16_t* x0,
+                          const uint8_t* lab, float wpos, int emulate_bf16, double* grad, float* prob);
+/* Adam (Kingma & Ba 2015), beta 0.9 / 0.999, eps 1e-8, bias-corrected; `step`
+ * counts from 1. */
+void orc_prop_adam(float* p, float* m, float* v, const double* g, int n, float lr, int step);
 int orc_describe(orc_ctx* c);
 const uint8_t* orc_descriptors(const orc_ctx* c);
 int orc_set_threads(int n);

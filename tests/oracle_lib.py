@@ -42,6 +42,11 @@ class Geom(ctypes.Structure):
                 ("det_size", ctypes.c_float * MAXL)]
 
 
+class PropGeom(ctypes.Structure):
+    _fields_ = [("nlev", ctypes.c_int), ("P", ctypes.c_int), ("tw", ctypes.c_int * MAXL), ("th", ctypes.c_int * MAXL),
+                ("oct", ctypes.c_int * MAXL), ("base", ctypes.c_int * (MAXL + 1))]
+
+
 _lib = None
 
 
@@ -103,6 +108,17 @@ def lib():
         L.orc_descriptors.restype = ctypes.c_void_p
         L.orc_descriptors.argtypes = [ctypes.c_void_p]
         L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_kept_candidates.restype = c_int_p
+        L.orc_kept_candidates.argtypes = [ctypes.c_void_p]
+        L.orc_prop_make_geom.argtypes = [ctypes.c_void_p, ctypes.POINTER(PropGeom)]
+        L.orc_prop_features.argtypes = [ctypes.c_void_p, ctypes.c_float, ctypes.c_void_p]
+        L.orc_prop_labels.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_prop_loss_grad.restype = ctypes.c_double
+        L.orc_prop_loss_grad.argtypes = [ctypes.POINTER(PropGeom), ctypes.c_int, c_float_p, ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.c_float, ctypes.c_int, ctypes.c_void_p,
+                                         ctypes.c_void_p]
+        L.orc_prop_adam.argtypes = [c_float_p, c_float_p, c_float_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_float,
+                                    ctypes.c_int]
         _lib = L
     return _lib
 
@@ -193,6 +209,22 @@ def nms(cand, level_start, max_waves=10000):
     n = lib().orc_nms(cand.ctypes.data, len(cand), ls.ctypes.data_as(c_int_p), len(ls) - 1, max_waves,
                       out.ctypes.data, ctypes.byref(waves))
     return out[:n], waves.value
+
+
+def orient_describe(pipe, kps, patsize=10):
+    """Orientation and MLDB (oracle) of keypoints on a Pipeline's levels."""
+    kps = np.ascontiguousarray(kps, KP_DTYPE).copy()
+    desc = np.zeros((len(kps), 61), np.uint8)
+    L = lib()
+    g = pipe.geom
+    for i in range(len(kps)):
+        l = int(kps[i]["class_id"])
+        w, h = g.w[l], g.h[l]
+        kp = ctypes.c_void_p(kps.ctypes.data + 28 * i)
+        lx, ly, lt = L.orc_level(pipe.ctx, l, 2), L.orc_level(pipe.ctx, l, 3), L.orc_level(pipe.ctx, l, 0)
+        L.orc_orientation(kp, lx, ly, w, h)
+        L.orc_mldb(kp, lt, lx, ly, w, h, patsize, desc[i].ctypes.data_as(c_u8_p), None)
+    return kps, desc
 
 
 def match2nn(q, t):
@@ -291,8 +323,78 @@ class Pipeline:
         buf = (ctypes.c_char * (n * 61)).from_address(L.orc_descriptors(self.ctx))
         return np.frombuffer(bytes(buf), np.uint8).reshape(n, 61).copy()
 
+    def prop_geom(self):
+        g = PropGeom()
+        lib().orc_prop_make_geom(self.ctx, ctypes.byref(g))
+        return g
+
+    def prop_features(self):
+        g = self.prop_geom()
+        out = np.zeros(g.P * 4, np.uint16)
+        lib().orc_prop_features(self.ctx, ctypes.c_float(max(self.opts.dthreshold, self.opts.min_dthreshold)),
+                                out.ctypes.data)
+        return out
+
+    def prop_labels(self):
+        g = self.prop_geom()
+        out = np.zeros(g.P, np.uint8)
+        lib().orc_prop_labels(self.ctx, out.ctypes.data)
+        return out
+
     def run(self, img):
         self.scale_space(img)
         self.detect()
         desc = self.describe()
         return self.keypoints(), desc
+
+
+PROP_NPARAM = 14513
+# flat parameter layout of the proposal network (offsets into the vector)
+PROP_OFF = {"W1": 0, "b1": 576, "W2": 592, "b2": 5200, "W3": 5232, "b3": 14448, "w4": 14480, "b4": 14512}
+
+
+def make_prop_geom(levels):
+    """PropGeom from [(tw, th, octave), ...] tile maps."""
+    g = PropGeom()
+    g.nlev = len(levels)
+    P = 0
+    for i, (tw, th, o) in enumerate(levels):
+        g.tw[i], g.th[i], g.oct[i], g.base[i] = tw, th, o, P
+        P += tw * th
+    g.base[len(levels)] = P
+    g.P = P
+    return g
+
+
+def prop_loss_grad(g, nimg, params, x0, labels, wpos=1.0, emulate=True, want_grad=True):
+    """(loss, grad float64 or None, prob float32) of the proposal oracle."""
+    params = np.ascontiguousarray(params, np.float32)
+    x0 = np.ascontiguousarray(x0, np.uint16)
+    lab = None if labels is None else np.ascontiguousarray(labels, np.uint8)
+    grad = np.zeros(PROP_NPARAM, np.float64) if want_grad else None
+    prob = np.zeros(g.P * nimg, np.float32)
+    loss = lib().orc_prop_loss_grad(ctypes.byref(g), nimg, fptr(params), x0.ctypes.data,
+                                    lab.ctypes.data if lab is not None else None, ctypes.c_float(wpos),
+                                    1 if emulate else 0, grad.ctypes.data if grad is not None else None,
+                                    prob.ctypes.data)
+    return loss, grad, prob
+
+
+def prop_adam(params, m, v, grad, lr, step):
+    p = np.ascontiguousarray(params, np.float32).copy()
+    m = np.ascontiguousarray(m, np.float32).copy()
+    v = np.ascontiguousarray(v, np.float32).copy()
+    g = np.ascontiguousarray(grad, np.float64)
+    lib().orc_prop_adam(fptr(p), fptr(m), fptr(v), g.ctypes.data, len(p), ctypes.c_float(lr), step)
+    return p, m, v
+
+
+def f32_to_bf16(a):
+    """Round-to-nearest-even bfloat16 bits of float32 values."""
+    u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return u.astype(np.uint16)
+
+
+def bf16_to_f32(b):
+    return (np.ascontiguousarray(b, np.uint16).astype(np.uint32) << 16).view(np.float32)
