@@ -13,7 +13,8 @@ import time
 
 import numpy as np
 
-from . import BatchAKAZE, AKAZEOptions, DeviceBuffer, lib, _check, AKZ_ERR_OVERFLOW
+import akaze_amd as _pkg
+from . import BatchAKAZE, AKAZEOptions, DeviceBuffer, lib, _check
 
 NPARAM = 14513
 WEIGHTS_DIR = os.path.dirname(os.path.abspath(__file__))
@@ -33,9 +34,9 @@ class ProposalNet:
 
     def close(self):
         if getattr(self, "net", None):
-            from . import _lib
-            if _lib is not None:
-                _lib.akz_prop_destroy(self.net)
+            L = getattr(_pkg, "_lib", None)
+            if L is not None:
+                L.akz_prop_destroy(self.net)
         self.net = None
 
     def __del__(self):

@@ -8,12 +8,16 @@ Two contexts alternate so the host readback of one image overlaps the next
 image's kernels.  The timed region starts with the images already in HBM.
 
 Prints one JSON line (the last line on stdout).  Extra objects:
-  roofline      FED step kernel (the dominant kernel): algorithmic bytes
-                (12 B per pixel and step) over its duration, measured with HIP
-                events around the FED launches on the context stream;
+  roofline      FED kernel (the dominant kernel, up to 8 steps per launch):
+                algorithmic bytes (12 B per pixel and step) over its duration,
+                measured with HIP events around the FED launches on the
+                context stream;
   cpu_baseline  the CPU oracle (the C restatement of the reference
                 semantics, all host threads) on a bounded sample of the same
-                workload.
+                workload;
+  batched       16-image batches (one launch per stage, HIP graph) with the
+                full scan and with the committed learned tile proposal, and
+                the proposal's keypoint recall against the full scan.
 """
 import argparse
 import json
@@ -46,24 +50,61 @@ def cpu_baseline(img, W, H, budget_s):
         P.run(img)
         n += 1
         el = time.perf_counter() - t0
-        if el >= budget_s or n >= 50:
+        if el >= budget_s:
             break
     return {"value": n / el, "unit": "images/s", "cores": threads, "kind": "port",
             "sample": "%d x %dx%d synthetic image, full detect+MLDB on the CPU oracle (%.1f s)" % (n, W, H, el)}
 
 
+def batched_comparison(A, W, H, B=16, reps=20):
+    """Images/s of 16-image batches (graph mode, resident device-rendered
+    images) with the full scan and with the committed learned proposal, and
+    the proposal's keypoint recall on those images."""
+    from akaze_amd.proposal import ProposalNet, keypoint_recall
+    here = os.path.join(REPO, "akaze_amd")
+    rec = json.load(open(os.path.join(here, "proposal_1080p.json")))
+    net = ProposalNet()
+    net.load(os.path.join(here, "proposal_1080p.f32"))
+    bk = A.BatchAKAZE(A.AKAZEOptions(img_width=W, img_height=H), B)
+    bk.synth(20_000_000, B)
+    out = {"images_per_call": B, "threshold": rec["threshold"]}
+    kps = {}
+    for name, nn in (("full_scan", None), ("learned_proposal", net)):
+        bk.set_proposal(nn, rec["threshold"])
+        bk.process_async(B)
+        cnt, _ = bk.counts(B)
+        kps[name] = [bk.read_results(z, int(cnt[z]))[0] for z in range(B)]
+        bk.set_graph_mode(True)
+        for _ in range(3):
+            bk.process_async(B)
+        bk.wait()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            bk.process_async(B)
+        bk.wait()
+        out[name + "_images_per_s"] = round(reps * B / (time.perf_counter() - t0), 1)
+        bk.set_graph_mode(False)
+    full, prop = kps["full_scan"], kps["learned_proposal"]
+    total = sum(len(f) for f in full)
+    out["proposal_kp_recall"] = round(sum(len(f) * keypoint_recall(f, p) for f, p in zip(full, prop)) / total, 4)
+    out["heldout_kp_recall_2000_images"] = round(rec["heldout_kp_recall"], 4)
+    out["proposal_tiles_kept"] = round(rec["tiles_kept"], 4)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--graph", type=int, default=1, help="capture each context's pipeline in a HIP graph")
     ap.add_argument("--depth", type=int, default=2, help="contexts in flight")
     ap.add_argument("--batch", type=int, default=1, help="images per context call (one launch per stage)")
-    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched / learned-proposal comparison")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,
@@ -135,8 +176,13 @@ def main():
         (fed_launches, fed_ms, B, avg_ms * 1e3, achieved))
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_fed_step", "avg_launch_us": round(avg_ms * 1e3, 3),
+                "kernel": "k_fed_multi (up to 8 FED steps per launch)", "avg_launch_us": round(avg_ms * 1e3, 3),
                 "alg_bytes_per_launch": round(fed_bytes / fed_launches)}
+
+    batched = None
+    if not args.no_batched and B == 1:
+        batched = batched_comparison(A, W, H)
+        log("batched: %s" % batched)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -151,7 +197,7 @@ def main():
                       "octaves": 4, "sublevels": 4, "diffusivity": "PM_G2", "descriptor": "MLDB-486",
                       "graph": bool(args.graph), "contexts_in_flight": args.depth, "images_per_call": B,
                       "keypoints_per_image": int(np.median(counts)) if counts else 0},
-           "roofline": roofline, "cpu_baseline": cpu}
+           "roofline": roofline, "cpu_baseline": cpu, "batched": batched}
     print(json.dumps(out), flush=True)
 
 

@@ -178,3 +178,60 @@ def test_short_training_learns_the_labels(gpu_lib):
     thr = threshold_for_recall(ptr, tr.labels, 0.99)
     rec, kept, _ = tile_recall(net, te, thr)
     assert rec > 0.95 and kept < 0.9, (rec, kept, thr)
+
+
+def test_trained_network_recall_and_gain(gpu_lib):
+    """The committed 1080p network on 32 held-out device-rendered images:
+    keypoint recall against the full scan at the committed threshold, tiles
+    kept against the candidate-tile rule (a tile is kept when its maximum
+    response passes the detector threshold, which keeps every keypoint), and
+    the images/s of both batched pipelines (printed)."""
+    import json
+    import os
+    import time
+    A = gpu_lib
+    from akaze_amd.proposal import ProposalNet, keypoint_recall
+    here = os.path.dirname(A.__file__)
+    rec = json.load(open(os.path.join(here, "proposal_1080p.json")))
+    net = ProposalNet()
+    net.load(os.path.join(here, "proposal_1080p.f32"))
+    W, H, B, n = 1920, 1080, 16, 32
+    seed0 = 10_000_000 + 50_000
+    bk = A.BatchAKAZE(A.AKAZEOptions(img_width=W, img_height=H), B)
+    P = bk.proposal_positions()
+    feat = A.DeviceBuffer(B * P * 8)
+    full, prop, rule_kept, net_kept = [], [], [], []
+    for s in range(0, n, B):
+        bk.synth(seed0 + s, B)
+        bk.set_proposal(None)
+        bk.process_async(B)
+        cnt, _ = bk.counts(B)
+        full += [bk.read_results(z, int(cnt[z]))[0] for z in range(B)]
+        bk.extract(B, feat.ptr)
+        x = np.zeros(B * P * 4, np.uint16)
+        feat.download(x)
+        rule_kept.append(np.mean(O.bf16_to_f32(x).reshape(-1, 4)[:, 1] > 0.125))
+        bk.set_proposal(net, rec["threshold"])
+        bk.process_async(B)
+        cnt, _ = bk.counts(B)
+        prop += [bk.read_results(z, int(cnt[z]))[0] for z in range(B)]
+        net_kept += [np.mean(bk.read_probs(z) >= rec["threshold"]) for z in range(B)]
+    total = sum(len(f) for f in full)
+    recall = sum(len(f) * keypoint_recall(f, p) for f, p in zip(full, prop)) / total
+    speeds = {}
+    for name, nn in (("full_scan", None), ("proposal", net)):
+        bk.set_proposal(nn, rec["threshold"])
+        bk.set_graph_mode(True)
+        for _ in range(3):
+            bk.process_async(B)
+        bk.wait()
+        t0 = time.perf_counter()
+        for _ in range(10):
+            bk.process_async(B)
+        bk.wait()
+        speeds[name] = 10 * B / (time.perf_counter() - t0)
+        bk.set_graph_mode(False)
+    print("held-out keypoint recall %.4f, tiles kept: network %.3f vs candidate-tile rule %.3f, images/s %s"
+          % (recall, np.mean(net_kept), np.mean(rule_kept), {k: round(v, 1) for k, v in speeds.items()}))
+    assert recall >= 0.98, recall
+    assert np.mean(net_kept) < np.mean(rule_kept), (np.mean(net_kept), np.mean(rule_kept))
