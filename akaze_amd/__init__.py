@@ -102,6 +102,17 @@ SIGNATURES = {
     "akz_set_proposal": [_P, _P, _F],
     "akz_prop_read_probs": [_P, _I, _P],
     "akz_synth_device_batch": [_U64, _I, _I, _I, _P, _SZ, _SZ],
+    "akz_autotune": [_P, _I, _I],
+    "akz_get_fed_variants": [_P, _P],
+    "akz_set_fed_variants": [_P, _P],
+    "akz_ragged_create": [_P, _I, _P, _P, _P, _P],
+    "akz_ragged_destroy": [_P],
+    "akz_ragged_context": [_P, _I, _P],
+    "akz_ragged_set_graph_mode": [_P, _I],
+    "akz_ragged_autotune": [_P, _I],
+    "akz_ragged_process": [_P, _I, _P, _P, _P, _P, _P],
+    "akz_ragged_read": [_P, _I, _P, _P, _I, _P],
+    "akz_ragged_results_device": [_P, _I, _P, _P, _P],
     "akz_matcher_create": [_P],
     "akz_matcher_destroy": [_P],
     "akz_match2nn": [_P, _P, _I, _P, _I, _SZ, _P],
@@ -455,6 +466,21 @@ class BatchAKAZE:
     def wait(self):
         _check(lib().akz_wait(self.ctx), "akz_wait")
 
+    def autotune(self, nz=None, reps=3):
+        """Times the FED kernel variants per level size on this context's
+        buffers (batches of nz images) and keeps the fastest."""
+        _check(lib().akz_autotune(self.ctx, int(nz or self.batch), int(reps)), "akz_autotune")
+        return self.fed_variants()
+
+    def fed_variants(self):
+        v = np.zeros(self.nlev, np.int32)
+        _check(lib().akz_get_fed_variants(self.ctx, v.ctypes.data), "akz_get_fed_variants")
+        return v
+
+    def set_fed_variants(self, v):
+        v = np.ascontiguousarray(v, np.int32)
+        _check(lib().akz_set_fed_variants(self.ctx, v.ctypes.data), "akz_set_fed_variants")
+
     def set_proposal(self, net, threshold=0.5):
         """Detect with the learned tile proposal `net` (None: full scan)."""
         _check(lib().akz_set_proposal(self.ctx, net.net if net is not None else None, float(threshold)),
@@ -485,6 +511,80 @@ class BatchAKAZE:
         self.process_async(n)
         counts, _ = self.counts(n)
         return [self.read_results(z, int(counts[z]))[:2] for z in range(n)]
+
+
+class Ragged:
+    """Ragged batches: one batch context per size class, the classes on
+    concurrent streams.  No counterpart in the reference."""
+
+    def __init__(self, options, sizes, capacity):
+        n = len(sizes)
+        w = np.array([s[0] for s in sizes], np.int32)
+        h = np.array([s[1] for s in sizes], np.int32)
+        cap = np.array(capacity if np.iterable(capacity) else [capacity] * n, np.int32)
+        p = ctypes.c_void_p()
+        _check(lib().akz_ragged_create(ctypes.byref(options._c), n, w.ctypes.data, h.ctypes.data, cap.ctypes.data,
+                                       ctypes.byref(p)), "akz_ragged_create")
+        self.r = p.value
+        self.sizes = [tuple(s) for s in sizes]
+        self._bufs = []
+
+    def close(self):
+        if getattr(self, "r", None) and _lib is not None:
+            _lib.akz_ragged_destroy(self.r)
+        self.r = None
+
+    def __del__(self):
+        self.close()
+
+    def set_graph_mode(self, on):
+        _check(lib().akz_ragged_set_graph_mode(self.r, int(bool(on))), "akz_ragged_set_graph_mode")
+
+    def autotune(self, reps=3):
+        _check(lib().akz_ragged_autotune(self.r, int(reps)), "akz_ragged_autotune")
+
+    def process_device(self, ptrs, pitches, sizes):
+        """Device images (pointer, pitch bytes, (w, h)); waits; kept counts."""
+        n = len(ptrs)
+        P = (ctypes.c_void_p * n)(*ptrs)
+        pit = np.array(pitches, np.uint64)
+        w = np.array([s[0] for s in sizes], np.int32)
+        h = np.array([s[1] for s in sizes], np.int32)
+        counts = np.zeros(n, np.int32)
+        _check(lib().akz_ragged_process(self.r, n, P, pit.ctypes.data, w.ctypes.data, h.ctypes.data,
+                                        counts.ctypes.data), "akz_ragged_process")
+        return counts
+
+    def process(self, imgs):
+        """Host float32 images of the class sizes; list of (kps, desc)."""
+        self._bufs = []
+        ptrs, pitches, sizes = [], [], []
+        for img in imgs:
+            img = np.ascontiguousarray(img, np.float32)
+            b = DeviceBuffer(img.nbytes)
+            b.upload(img)
+            self._bufs.append(b)
+            ptrs.append(b.ptr)
+            pitches.append(img.strides[0])
+            sizes.append((img.shape[1], img.shape[0]))
+        counts = self.process_device(ptrs, pitches, sizes)
+        return [self.read(i, int(c))[:2] for i, c in enumerate(counts)]
+
+    def read(self, i, cap):
+        kps = np.zeros(max(cap, 1), KP_DTYPE)
+        desc = np.zeros((max(cap, 1), 61), np.uint8)
+        n = ctypes.c_int()
+        st = lib().akz_ragged_read(self.r, int(i), kps.ctypes.data, desc.ctypes.data, cap, ctypes.byref(n))
+        _check(st, "akz_ragged_read", allow=(AKZ_ERR_OVERFLOW,))
+        m = min(n.value, cap)
+        return kps[:m].copy(), desc[:m].copy(), n.value
+
+    def results_device(self, i):
+        """(device count pointer, keypoints pointer, 64-byte descriptor rows pointer) of image i."""
+        c, k, d = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        _check(lib().akz_ragged_results_device(self.r, int(i), ctypes.byref(c), ctypes.byref(k), ctypes.byref(d)),
+               "akz_ragged_results_device")
+        return c.value, k.value, d.value
 
 
 class Matcher:

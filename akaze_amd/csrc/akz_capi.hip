@@ -140,7 +140,14 @@ struct akz_ctx {
   float prop_thr = 0.5f;
   bool fed_profile = false;
   int fed_block = 1;  // FED steps per launch
+  int fed_variant[AKZ_MAX_LEVELS] = {};  // blocked FED kernel variant per level (akz_autotune)
   hipEvent_t fed_ev[2 * AKZ_MAX_LEVELS] = {};
+};
+
+struct akz_ragged {
+  int ncls = 0;
+  akz_ctx* cls[AKZ_MAX_CLASSES] = {};
+  std::vector<int> img_cls, img_slot;  // class and batch slot of each image of the last call
 };
 
 struct akz_matcher {
@@ -422,7 +429,7 @@ void enqueue_scale_space(akz_ctx* c, const float* img, int in_pitch, size_t in_z
       if (bs == 1)
         akz_launch_fed_step(s, cur, c->g, dst, L.pitch, L.w, L.h, ht[0], bt);
       else
-        akz_launch_fed_multi(s, cur, c->g, dst, L.pitch, L.w, L.h, ht, k, bt);
+        akz_launch_fed_variant(s, c->fed_variant[i], cur, c->g, dst, L.pitch, L.w, L.h, ht, k, bt);
       cur = dst;
       dst = (dst == L.Lt) ? c->fed_tmp : L.Lt;
     }
@@ -598,6 +605,7 @@ akz_status akz_create_batch(const akz_options* opts, int batch, akz_ctx** out) {
   gauss_taps(1.0, 2, &c->taps1);
   gauss_taps(1.0f, 2, &c->taps_c);
   akz_describe_init_tables();
+  for (int i = 0; i < c->nlev; i++) c->fed_variant[i] = akz_fed_variant_for(c->lev[i].w, c->lev[i].h);
   c->graph_exec.assign(batch, nullptr);
   c->graph.assign(batch, nullptr);
   st = allocate(c);
@@ -985,6 +993,181 @@ akz_status akz_prop_read_probs(akz_ctx* c, int z, float* probs) {
   AKZ_HIP_TRY(hipStreamSynchronize(c->stream));
   AKZ_HIP_TRY(hipMemcpy(probs, c->pact.prob + (size_t)z * c->psrc.g.P, sizeof(float) * c->psrc.g.P,
                         hipMemcpyDeviceToHost));
+  return AKZ_OK;
+}
+
+// ---- autotuner ----
+akz_status akz_autotune(akz_ctx* c, int nz, int reps) {
+  if (!c || nz < 1 || nz > c->batch || reps < 1) return AKZ_ERR_ARG;
+  static const int kVariants[] = {2, 3, 4, 5, 6, 7, 8};
+  hipEvent_t a, b;
+  AKZ_HIP_TRY(hipEventCreate(&a));
+  AKZ_HIP_TRY(hipEventCreate(&b));
+  const akz_batch bt{nz, c->zs};
+  akz_status result = AKZ_OK;
+  for (int i = 1; i < c->nlev && result == AKZ_OK; i++) {
+    Level& L = c->lev[i];
+    if (L.nsteps == 0 || c->fed_block < 2) continue;
+    // same-size levels share the choice of the first of them
+    bool done = false;
+    for (int j = 1; j < i; j++)
+      if (c->lev[j].w == L.w && c->lev[j].h == L.h && c->lev[j].nsteps > 0) {
+        c->fed_variant[i] = c->fed_variant[j];
+        done = true;
+        break;
+      }
+    if (done) continue;
+    const int k = std::min(c->fed_block, L.nsteps);
+    float ht[AKZ_MAX_TAU];
+    for (int q = 0; q < k; q++) ht[q] = 0.5f * L.tau[q];
+    float best = 1e30f;
+    int bestv = c->fed_variant[i];
+    for (int v : kVariants) {
+      // warm-up launch, then `reps` timed launches ping-ponging Lt and the scratch
+      akz_launch_fed_variant(c->stream, v, L.Lt, c->g, c->fed_tmp, L.pitch, L.w, L.h, ht, k, bt);
+      if (hipEventRecord(a, c->stream) != hipSuccess) { result = AKZ_ERR_HIP; break; }
+      for (int r = 0; r < reps; r++) {
+        const bool even = (r & 1) == 0;
+        akz_launch_fed_variant(c->stream, v, even ? c->fed_tmp : L.Lt, c->g, even ? L.Lt : c->fed_tmp, L.pitch, L.w,
+                               L.h, ht, k, bt);
+      }
+      if (hipEventRecord(b, c->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess) {
+        result = AKZ_ERR_HIP;
+        break;
+      }
+      float ms = 0.0f;
+      (void)hipEventElapsedTime(&ms, a, b);
+      if (ms < best) { best = ms; bestv = v; }
+    }
+    c->fed_variant[i] = bestv;
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  if (result != AKZ_OK) { (void)hipGetLastError(); return result; }
+  AKZ_HIP_TRY(hipGetLastError());
+  drop_graphs(c);
+  c->state = 0;  // level contents were overwritten by the timing launches
+  return AKZ_OK;
+}
+
+akz_status akz_get_fed_variants(const akz_ctx* c, int* variants) {
+  if (!c || !variants) return AKZ_ERR_ARG;
+  for (int i = 0; i < c->nlev; i++) variants[i] = c->fed_variant[i];
+  return AKZ_OK;
+}
+
+akz_status akz_set_fed_variants(akz_ctx* c, const int* variants) {
+  if (!c || !variants) return AKZ_ERR_ARG;
+  for (int i = 0; i < c->nlev; i++)
+    if (variants[i] < 2 || variants[i] > 8) return AKZ_ERR_ARG;
+  AKZ_HIP_TRY(hipStreamSynchronize(c->stream));
+  for (int i = 0; i < c->nlev; i++) c->fed_variant[i] = variants[i];
+  drop_graphs(c);
+  return AKZ_OK;
+}
+
+// ---- ragged batches: one batch context (and stream) per size class ----
+akz_status akz_ragged_create(const akz_options* base, int nclass, const int* widths, const int* heights,
+                             const int* capacity, akz_ragged** out) {
+  if (!base || !out || nclass < 1 || nclass > AKZ_MAX_CLASSES || !widths || !heights || !capacity)
+    return AKZ_ERR_ARG;
+  *out = nullptr;
+  for (int k = 0; k < nclass; k++)
+    for (int j = 0; j < k; j++)
+      if (widths[j] == widths[k] && heights[j] == heights[k]) return AKZ_ERR_ARG;
+  akz_ragged* r = new (std::nothrow) akz_ragged();
+  if (!r) return AKZ_ERR_OOM;
+  r->ncls = nclass;
+  for (int k = 0; k < nclass; k++) {
+    akz_options o = *base;
+    o.img_width = widths[k];
+    o.img_height = heights[k];
+    akz_status st = akz_create_batch(&o, capacity[k], &r->cls[k]);
+    if (st != AKZ_OK) { akz_ragged_destroy(r); return st; }
+  }
+  *out = r;
+  return AKZ_OK;
+}
+
+akz_status akz_ragged_destroy(akz_ragged* r) {
+  if (!r) return AKZ_ERR_ARG;
+  for (int k = 0; k < r->ncls; k++)
+    if (r->cls[k]) akz_destroy(r->cls[k]);
+  delete r;
+  return AKZ_OK;
+}
+
+akz_status akz_ragged_context(akz_ragged* r, int cls, akz_ctx** ctx) {
+  if (!r || !ctx || cls < 0 || cls >= r->ncls) return AKZ_ERR_ARG;
+  *ctx = r->cls[cls];
+  return AKZ_OK;
+}
+
+akz_status akz_ragged_process(akz_ragged* r, int n, const float* const* d_imgs, const size_t* pitch_bytes,
+                              const int* widths, const int* heights, int* counts) {
+  if (!r || n < 1 || !d_imgs || !widths || !heights) return AKZ_ERR_ARG;
+  std::vector<int> cls(n), slot(n);
+  int cnt[AKZ_MAX_CLASSES] = {};
+  for (int i = 0; i < n; i++) {
+    int k = 0;
+    while (k < r->ncls && !(r->cls[k]->lev[0].w == widths[i] && r->cls[k]->lev[0].h == heights[i])) k++;
+    if (k == r->ncls || !d_imgs[i]) return AKZ_ERR_ARG;
+    const size_t sp = pitch_bytes && pitch_bytes[i] ? pitch_bytes[i] : sizeof(float) * widths[i];
+    if (sp < sizeof(float) * widths[i]) return AKZ_ERR_ARG;
+    cls[i] = k;
+    slot[i] = cnt[k]++;
+    if (cnt[k] > r->cls[k]->batch) return AKZ_ERR_ARG;
+  }
+  // stage every image into its class context's input slot, then one batch
+  // call per class; the classes run concurrently on their own streams
+  for (int i = 0; i < n; i++) {
+    akz_ctx* c = r->cls[cls[i]];
+    const Level& L0 = c->lev[0];
+    const size_t sp = pitch_bytes && pitch_bytes[i] ? pitch_bytes[i] : sizeof(float) * widths[i];
+    AKZ_HIP_TRY(hipMemcpy2DAsync(zptr(c, c->img_in, slot[i]), sizeof(float) * L0.pitch, d_imgs[i], sp,
+                                 sizeof(float) * L0.w, L0.h, hipMemcpyDeviceToDevice, c->stream));
+  }
+  for (int k = 0; k < r->ncls; k++) {
+    if (cnt[k] == 0) continue;
+    akz_ctx* c = r->cls[k];
+    akz_status st = akz_process_batch_device_async(c, c->img_in, sizeof(float) * c->lev[0].pitch, c->zs, cnt[k]);
+    if (st != AKZ_OK) return st;
+  }
+  r->img_cls = cls;
+  r->img_slot = slot;
+  for (int k = 0; k < r->ncls; k++) {
+    if (cnt[k] == 0) continue;
+    akz_status st = sync_status(r->cls[k], cnt[k]);
+    if (st != AKZ_OK) return st;
+  }
+  if (counts)
+    for (int i = 0; i < n; i++) image_status(r->cls[cls[i]], slot[i], &counts[i]);
+  return AKZ_OK;
+}
+
+akz_status akz_ragged_read(akz_ragged* r, int i, akz_keypoint* kps, uint8_t* desc, int cap, int* n_out) {
+  if (!r || i < 0 || i >= (int)r->img_cls.size()) return AKZ_ERR_ARG;
+  return akz_read_batch_results(r->cls[r->img_cls[i]], r->img_slot[i], kps, desc, cap, n_out);
+}
+
+akz_status akz_ragged_results_device(akz_ragged* r, int i, const int32_t** d_count, const akz_keypoint** d_kps,
+                                     const uint8_t** d_desc64) {
+  if (!r || i < 0 || i >= (int)r->img_cls.size()) return AKZ_ERR_ARG;
+  return akz_batch_results_device(r->cls[r->img_cls[i]], r->img_slot[i], d_count, d_kps, d_desc64);
+}
+
+akz_status akz_ragged_set_graph_mode(akz_ragged* r, int enabled) {
+  if (!r) return AKZ_ERR_ARG;
+  for (int k = 0; k < r->ncls; k++) akz_set_graph_mode(r->cls[k], enabled);
+  return AKZ_OK;
+}
+
+akz_status akz_ragged_autotune(akz_ragged* r, int reps) {
+  if (!r) return AKZ_ERR_ARG;
+  for (int k = 0; k < r->ncls; k++) {
+    akz_status st = akz_autotune(r->cls[k], r->cls[k]->batch, reps);
+    if (st != AKZ_OK) return st;
+  }
   return AKZ_OK;
 }
 

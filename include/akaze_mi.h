@@ -152,6 +152,37 @@ akz_status akz_read_batch_results(akz_ctx* ctx, int z, akz_keypoint* kps, uint8_
 akz_status akz_batch_results_device(akz_ctx* ctx, int z, const int32_t** d_count, const akz_keypoint** d_kps,
                                     const uint8_t** d_desc64);
 akz_status akz_read_batch_level(akz_ctx* ctx, int z, int lvl, int which, float* dst);
+/* Autotuner (no reference counterpart): times every blocked FED kernel
+ * variant (LDS region shape and threads per block, 8-pixel halo) on each
+ * level's own buffers with batches of nz images and keeps the fastest per
+ * level size.  Overwrites the level contents; captured graphs are rebuilt on
+ * the next call.  Variants are readable and settable per level (2..8). */
+akz_status akz_autotune(akz_ctx* ctx, int nz, int reps);
+akz_status akz_get_fed_variants(const akz_ctx* ctx, int* variants);
+akz_status akz_set_fed_variants(akz_ctx* ctx, const int* variants);
+
+/* Ragged batches (no reference counterpart): images of several sizes go to
+ * one batch context per size class; each class runs as one batch call on
+ * its own stream, concurrently with the others.  At most `capacity[k]`
+ * images of class k per call. */
+#define AKZ_MAX_CLASSES 16
+typedef struct akz_ragged akz_ragged;
+akz_status akz_ragged_create(const akz_options* base, int nclass, const int* widths, const int* heights,
+                             const int* capacity, akz_ragged** out);
+akz_status akz_ragged_destroy(akz_ragged* r);
+/* The batch context of class `cls` (owned by r). */
+akz_status akz_ragged_context(akz_ragged* r, int cls, akz_ctx** ctx);
+akz_status akz_ragged_set_graph_mode(akz_ragged* r, int enabled);
+akz_status akz_ragged_autotune(akz_ragged* r, int reps);
+/* Processes n device images (sizes widths[i] x heights[i], each one of the
+ * classes; pitch 0 = dense) and waits; counts[i] receives image i's kept
+ * keypoint count. */
+akz_status akz_ragged_process(akz_ragged* r, int n, const float* const* d_imgs, const size_t* pitch_bytes,
+                              const int* widths, const int* heights, int* counts);
+akz_status akz_ragged_read(akz_ragged* r, int i, akz_keypoint* kps, uint8_t* desc, int cap, int* n_out);
+akz_status akz_ragged_results_device(akz_ragged* r, int i, const int32_t** d_count, const akz_keypoint** d_kps,
+                                     const uint8_t** d_desc64);
+
 /* akz_profile_fed over a batch of n images (bytes count all n images). */
 akz_status akz_profile_fed_batch(akz_ctx* ctx, const float* d_imgs, size_t pitch_bytes, size_t image_stride_bytes,
                                  int n, float* fed_ms, double* fed_bytes, int* fed_launches);

@@ -87,3 +87,45 @@ def test_batch_1080p_equals_single(gpu_lib):
         okp, odesc = oracle_run(W, H, imgs[z])
         assert skp.tobytes() == okp.tobytes() and np.array_equal(sdesc, odesc)
         check_image(bk, z, okp, odesc, "1080p")
+
+
+def test_ragged_batches_match_oracle(gpu_lib):
+    """Images of three sizes in one ragged call (one batch per size class on
+    concurrent streams), eager and graph mode: each image's result is the
+    oracle's."""
+    A = gpu_lib
+    sizes = [(320, 240), (640, 480), (500, 380)]
+    order = [0, 1, 2, 1, 0, 2, 1]
+    imgs = [A.to_float(A.synth_image(80 + i, *sizes[k])) for i, k in enumerate(order)]
+    ref = [oracle_run(im.shape[1], im.shape[0], im) for im in imgs]
+    rg = A.Ragged(A.AKAZEOptions(), sizes, [3, 3, 2])
+    for graph in (False, True):
+        rg.set_graph_mode(graph)
+        for rep in range(2 if graph else 1):
+            res = rg.process(imgs)
+            for i, (kps, desc) in enumerate(res):
+                assert kps.tobytes() == ref[i][0].tobytes(), (graph, rep, i)
+                assert np.array_equal(desc, ref[i][1]), (graph, rep, i)
+
+
+def test_every_fed_variant_and_the_autotuner(gpu_lib):
+    """Each blocked FED kernel variant (region shapes 2..8) on every level,
+    and the autotuner's choice, give the oracle's levels bit for bit."""
+    A = gpu_lib
+    W, H = 900, 675
+    img = A.to_float(A.synth_image(91, W, H))
+    orc = O.Pipeline(W, H)
+    okp, odesc = orc.run(img)
+    bk = A.BatchAKAZE(A.AKAZEOptions(img_width=W, img_height=H), 2)
+    for v in range(2, 9):
+        bk.set_fed_variants([v] * bk.nlev)
+        bk.upload([img, img])
+        bk.process_async(2)
+        for z in range(2):
+            check_image(bk, z, okp, odesc, "variant %d" % v)
+        assert np.array_equal(bk.read_level(1, bk.nlev - 1, 0), orc.level(bk.nlev - 1, 0)), v
+    chosen = bk.autotune(2, reps=2)
+    assert all(2 <= v <= 8 for v in chosen[1:])
+    bk.upload([img, img])
+    bk.process_async(2)
+    check_image(bk, 1, okp, odesc, "autotuned %s" % list(chosen))
