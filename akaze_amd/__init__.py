@@ -102,6 +102,7 @@ SIGNATURES = {
     "akz_set_proposal": [_P, _P, _F],
     "akz_prop_read_probs": [_P, _I, _P],
     "akz_synth_device_batch": [_U64, _I, _I, _I, _P, _SZ, _SZ],
+    "akz_synth_device_warp": [_P, _I, _I, _SZ, _P, _F, _U64, _P, _SZ],
     "akz_autotune": [_P, _I, _I],
     "akz_get_fed_variants": [_P, _P],
     "akz_set_fed_variants": [_P, _P],
@@ -123,6 +124,8 @@ SIGNATURES = {
     "akz_dev_free": [_P],
     "akz_memcpy_h2d": [_P, _P, _SZ],
     "akz_memcpy_d2h": [_P, _P, _SZ],
+    "akz_memcpy_d2d": [_P, _P, _SZ],
+    "akz_memcpy_2d": [_P, _SZ, _P, _SZ, _SZ, _SZ],
     "akz_device_synchronize": [],
     "akz_synth_image": [_U64, _I, _I, _P],
     "akz_synth_homography": [_U64, _I, _I, _D, _D, _D, _D, _P],
@@ -495,6 +498,13 @@ class BatchAKAZE:
         out = np.zeros(self.proposal_positions(), np.float32)
         _check(lib().akz_prop_read_probs(self.ctx, int(z), out.ctypes.data), "akz_prop_read_probs")
         return out
+
+    def results_device(self, z):
+        """(device count pointer, keypoints pointer, 64-byte descriptor rows pointer) of image z."""
+        c, k, d = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        _check(lib().akz_batch_results_device(self.ctx, int(z), ctypes.byref(c), ctypes.byref(k), ctypes.byref(d)),
+               "akz_batch_results_device")
+        return c.value, k.value, d.value
 
     def extract(self, n, d_feat, d_labels=None):
         """Proposal features (and labels) of images 0..n-1 of the last run into device arrays."""

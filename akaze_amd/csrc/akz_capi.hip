@@ -1299,6 +1299,19 @@ akz_status akz_memcpy_d2h(void* dst, const void* src, size_t bytes) {
   AKZ_HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
   return AKZ_OK;
 }
+akz_status akz_memcpy_d2d(void* dst, const void* src, size_t bytes) {
+  if ((!dst || !src) && bytes) return AKZ_ERR_ARG;
+  if (bytes) AKZ_HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+  return AKZ_OK;
+}
+
+akz_status akz_memcpy_2d(void* dst, size_t dpitch, const void* src, size_t spitch, size_t width_bytes,
+                         size_t height) {
+  if (!dst || !src || dpitch < width_bytes || spitch < width_bytes) return AKZ_ERR_ARG;
+  AKZ_HIP_TRY(hipMemcpy2D(dst, dpitch, src, spitch, width_bytes, height, hipMemcpyDefault));
+  return AKZ_OK;
+}
+
 akz_status akz_device_synchronize(void) {
   AKZ_HIP_TRY(hipDeviceSynchronize());
   return AKZ_OK;

@@ -154,7 +154,59 @@ __global__ __launch_bounds__(256) void k_synth_compose(const float* __restrict__
   out[(size_t)y * pitch + x] = rintf(v) * (1.0f / 255.0f);
 }
 
+// Homography warp of a float image in [0,1]: destination pixel centre u
+// maps to H^-1 u in the source, bilinear inside the source, 0 outside; plus
+// Gaussian noise (hashed Box-Muller), clipped to [0,1] and quantised to
+// 1/255 steps like an 8-bit frame.
+struct Hinv { float m[9]; };
+
+__global__ __launch_bounds__(256) void k_warp(const float* __restrict__ src, int w, int h, int spitch, Hinv H,
+                                              float noise_std, uint64_t seed, float* __restrict__ dst, int dpitch) {
+  const int x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (x >= w || y >= h) return;
+  const float u = x + 0.5f, v = y + 0.5f;
+  const float zz = H.m[6] * u + H.m[7] * v + H.m[8];
+  const float sx = (H.m[0] * u + H.m[1] * v + H.m[2]) / zz - 0.5f;
+  const float sy = (H.m[3] * u + H.m[4] * v + H.m[5]) / zz - 0.5f;
+  float val = 0.0f;
+  if (sx >= 0.0f && sy >= 0.0f && sx <= w - 1 && sy <= h - 1) {
+    const int ix = (int)sx, iy = (int)sy;
+    const int ix1 = akz_imin(ix + 1, w - 1), iy1 = akz_imin(iy + 1, h - 1);
+    const float fx = sx - ix, fy = sy - iy;
+    const float* r0 = src + (size_t)iy * spitch;
+    const float* r1 = src + (size_t)iy1 * spitch;
+    val = (r0[ix] * (1 - fx) + r0[ix1] * fx) * (1 - fy) + (r1[ix] * (1 - fx) + r1[ix1] * fx) * fy;
+  }
+  if (noise_std > 0.0f) val += noise_std * lattice(seed ^ ((uint64_t)y << 32 | (uint32_t)x));
+  val = fminf(fmaxf(val, 0.0f), 1.0f);
+  dst[(size_t)y * dpitch + x] = rintf(val * 255.0f) * (1.0f / 255.0f);
+}
+
 }  // namespace
+
+extern "C" akz_status akz_synth_device_warp(const float* d_src, int w, int h, size_t src_pitch_bytes,
+                                            const double* H9, float noise_std, uint64_t noise_seed, float* d_dst,
+                                            size_t dst_pitch_bytes) {
+  if (!d_src || !d_dst || !H9 || w < 1 || h < 1 || src_pitch_bytes < sizeof(float) * w ||
+      dst_pitch_bytes < sizeof(float) * w || src_pitch_bytes % 4 || dst_pitch_bytes % 4 || d_src == d_dst)
+    return AKZ_ERR_ARG;
+  const double* m = H9;
+  const double a = m[0], b = m[1], c = m[2], d = m[3], e = m[4], f = m[5], g = m[6], hh = m[7], i = m[8];
+  const double A = e * i - f * hh, B = -(d * i - f * g), C = d * hh - e * g;
+  const double det = a * A + b * B + c * C;
+  if (std::fabs(det) < 1e-300) return AKZ_ERR_ARG;
+  const double id = 1.0 / det;
+  const double r[9] = {A * id, -(b * i - c * hh) * id, (b * f - c * e) * id,
+                       B * id, (a * i - c * g) * id,   -(a * f - c * d) * id,
+                       C * id, -(a * hh - b * g) * id, (a * e - b * d) * id};
+  Hinv Hi;
+  for (int k = 0; k < 9; k++) Hi.m[k] = (float)r[k];
+  hipLaunchKernelGGL(k_warp, dim3((w + 63) / 64, (h + 3) / 4), dim3(256), 0, nullptr, d_src, w, h,
+                     (int)(src_pitch_bytes / 4), Hi, noise_std, noise_seed, d_dst, (int)(dst_pitch_bytes / 4));
+  AKZ_HIP_TRY(hipGetLastError());
+  AKZ_HIP_TRY(hipDeviceSynchronize());
+  return AKZ_OK;
+}
 
 extern "C" akz_status akz_synth_device_batch(uint64_t seed0, int n, int w, int h, float* d_out, size_t pitch_bytes,
                                              size_t image_stride_bytes) {

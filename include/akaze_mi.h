@@ -254,6 +254,11 @@ akz_status akz_prop_read_probs(akz_ctx* ctx, int z, float* probs);
  * Waits. */
 akz_status akz_synth_device_batch(uint64_t seed0, int n, int w, int h, float* d_out, size_t pitch_bytes,
                                   size_t image_stride_bytes);
+/* Device homography warp of a float [0,1] image (destination pixel centres
+ * mapped through H^-1, bilinear, 0 outside) plus Gaussian noise of
+ * `noise_std`, clipped and quantised to 1/255.  Waits. */
+akz_status akz_synth_device_warp(const float* d_src, int w, int h, size_t src_pitch_bytes, const double* H9,
+                                 float noise_std, uint64_t noise_seed, float* d_dst, size_t dst_pitch_bytes);
 
 /* ---- Brute-force Hamming 2-NN matcher: Matcher (AKAZE.h:29-55, AKAZE.cpp:
  * 36-121) and MatchDescriptors (cuda_akaze.cu:1842-2031). ---- */
@@ -277,6 +282,10 @@ akz_status akz_dev_alloc(size_t bytes, void** ptr);
 akz_status akz_dev_free(void* ptr);
 akz_status akz_memcpy_h2d(void* dst, const void* src, size_t bytes);
 akz_status akz_memcpy_d2h(void* dst, const void* src, size_t bytes);
+akz_status akz_memcpy_d2d(void* dst, const void* src, size_t bytes);
+/* 2-D copy between any host or device memory (pitches and width in bytes). */
+akz_status akz_memcpy_2d(void* dst, size_t dpitch, const void* src, size_t spitch, size_t width_bytes,
+                         size_t height);
 akz_status akz_device_synchronize(void);
 
 /* ---- Seeded synthetic inputs (no reference counterpart: the benchmark
