@@ -130,6 +130,43 @@ constexpr int DF_RW = 96;                       // LDS row stride, >= 64 + 2*DF_
 constexpr int DF_R1 = DF_TH + 2 * DF_H1, DF_R2 = DF_TH + 2 * DF_H2, DF_RD = DF_TH + 2;
 static_assert(DF_RD <= DF_R1, "the determinant fits the Lsmooth buffer");
 
+// Derivative and determinant phases of a tile whose stencils never reach
+// the image border (every reflect-101 index is the plain one): with the step
+// S known at compile time, position (r, c) of each grid reads rows r, r+S,
+// r+2S and columns c, c+S, c+2S of the grid below it, at constant LDS
+// offsets.  Same operands in the same order as the general path.
+template <int S>
+__device__ __forceinline__ void detect_interior(const float* s_l, float* s_x, float* s_y, float* s_d, float fac1,
+                                                float fac2, int wv, int lane) {
+  constexpr int NW = 8, H2 = S + 1, R2n = DF_TH + 2 * H2, E2 = 2 * H2, R = DF_RW;
+  auto deriv = [&](int r, int c) {
+    const float* b = s_l + r * R + c;
+    const float ul = b[0], ur = b[2 * S], ll = b[2 * S * R], lr = b[2 * S * R + 2 * S];
+    const float cl = b[S * R], cr = b[S * R + 2 * S];
+    const float uc = b[S], lc = b[2 * S * R + S];
+    s_x[r * R + c] = fac1 * (ur + lr - ul - ll) + fac2 * (cr - cl);
+    s_y[r * R + c] = fac1 * (lr + ll - ur - ul) + fac2 * (lc - uc);
+  };
+  for (int r = wv; r < R2n; r += NW) deriv(r, lane);
+  for (int k = threadIdx.x; k < R2n * E2; k += 64 * NW) deriv(k / E2, 64 + k % E2);
+  __syncthreads();
+  auto det = [&](int r, int c) {
+    const float* bx = s_x + r * R + c;
+    const float* by = s_y + r * R + c;
+    float ul = bx[0], ur = bx[2 * S], ll = bx[2 * S * R], lr = bx[2 * S * R + 2 * S];
+    const float cl = bx[S * R], cr = bx[S * R + 2 * S];
+    const float lxx = fac1 * (ur + lr - ul - ll) + fac2 * (cr - cl);
+    float uc = bx[S], lc = bx[2 * S * R + S];
+    const float lyx = fac1 * (lr + ll - ur - ul) + fac2 * (lc - uc);
+    ul = by[0]; ur = by[2 * S]; ll = by[2 * S * R]; lr = by[2 * S * R + 2 * S];
+    uc = by[S]; lc = by[2 * S * R + S];
+    const float lyy = fac1 * (lr + ll - ur - ul) + fac2 * (lc - uc);
+    s_d[r * R + c] = lxx * lyy - lyx * lyx;
+  };
+  for (int r = wv; r < DF_RD; r += NW) det(r, lane);
+  for (int k = threadIdx.x; k < DF_RD * 2; k += 64 * NW) det(k >> 1, 64 + (k & 1));
+}
+
 __global__ __launch_bounds__(512) void k_detect_fused(akz_det_params p, int32_t* __restrict__ seg_count,
                                                       uint8_t* __restrict__ seg_slots) {
   seg_count = akz_zoff(seg_count, p.zs);
@@ -176,49 +213,56 @@ __global__ __launch_bounds__(512) void k_detect_fused(akz_det_params p, int32_t*
   }
   __syncthreads();
   const float fac1 = p.fac1, fac2 = p.fac2;
-  // first derivatives on rows [y0-H2, y0+TH+H2), columns [x0-H2, x0+64+H2):
-  // 64 columns per row-pass, then the 2*H2 leftover columns as one flat list
-  auto deriv_at = [&](int r, int c) {
-    const int y = y0 - H2 + r, x = x0 - H2 + c;
-    if (x < 0 || x >= w || y < 0 || y >= h) return;
-    const int yl = ((y < s ? s - y : y - s) - (y0 - H1)) * DF_RW;
-    const int yh = ((y >= h - s ? 2 * h - y - s - 2 : y + s) - (y0 - H1)) * DF_RW;
-    const int yc = (y - (y0 - H1)) * DF_RW;
-    const int xl = (x < s ? s - x : x - s) - (x0 - H1);
-    const int xh = (x >= w - s ? 2 * w - x - s - 2 : x + s) - (x0 - H1);
-    const int xc = x - (x0 - H1);
-    float ul = s_l[yl + xl], ur = s_l[yl + xh], ll = s_l[yh + xl], lr = s_l[yh + xh];
-    float cl = s_l[yc + xl], cr = s_l[yc + xh];
-    float uc = s_l[yl + xc], lc = s_l[yh + xc];
-    s_x[r * DF_RW + c] = fac1 * (ur + lr - ul - ll) + fac2 * (cr - cl);
-    s_y[r * DF_RW + c] = fac1 * (lr + ll - ur - ul) + fac2 * (lc - uc);
-  };
-  const int R2n = DF_TH + 2 * H2, E2 = 2 * H2;
-  for (int r = wv; r < R2n; r += NW) deriv_at(r, lane);
-  for (int k = threadIdx.x; k < R2n * E2; k += 64 * NW) deriv_at(k / E2, 64 + k % E2);
-  __syncthreads();
-  // determinant on rows [y0-1, y0+TH+1), columns [x0-1, x0+65)
-  auto det_at = [&](int r, int c) {
-    const int y = y0 - 1 + r, x = x0 - 1 + c;
-    if (x < 0 || x >= w || y < 0 || y >= h) return;
-    const int yl = ((y < s ? s - y : y - s) - (y0 - H2)) * DF_RW;
-    const int yh = ((y >= h - s ? 2 * h - y - s - 2 : y + s) - (y0 - H2)) * DF_RW;
-    const int yc = (y - (y0 - H2)) * DF_RW;
-    const int xl = (x < s ? s - x : x - s) - (x0 - H2);
-    const int xh = (x >= w - s ? 2 * w - x - s - 2 : x + s) - (x0 - H2);
-    const int xc = x - (x0 - H2);
-    float ul = s_x[yl + xl], ur = s_x[yl + xh], ll = s_x[yh + xl], lr = s_x[yh + xh];
-    float cl = s_x[yc + xl], cr = s_x[yc + xh];
-    float lxx = fac1 * (ur + lr - ul - ll) + fac2 * (cr - cl);
-    float uc = s_x[yl + xc], lc = s_x[yh + xc];
-    float lyx = fac1 * (lr + ll - ur - ul) + fac2 * (lc - uc);
-    ul = s_y[yl + xl]; ur = s_y[yl + xh]; ll = s_y[yh + xl]; lr = s_y[yh + xh];
-    uc = s_y[yl + xc]; lc = s_y[yh + xc];
-    float lyy = fac1 * (lr + ll - ur - ul) + fac2 * (lc - uc);
-    s_d[r * DF_RW + c] = lxx * lyy - lyx * lyx;
-  };
-  for (int r = wv; r < DF_RD; r += NW) det_at(r, lane);
-  for (int k = threadIdx.x; k < DF_RD * 2; k += 64 * NW) det_at(k >> 1, 64 + (k & 1));
+  const bool interior = x0 >= H1 && x0 + 64 + H1 <= w && y0 >= H1 && y0 + DF_TH + H1 <= h;
+  if (interior && s >= 2 && s <= 4) {
+    if (s == 2) detect_interior<2>(s_l, s_x, s_y, s_d, fac1, fac2, wv, lane);
+    else if (s == 3) detect_interior<3>(s_l, s_x, s_y, s_d, fac1, fac2, wv, lane);
+    else detect_interior<4>(s_l, s_x, s_y, s_d, fac1, fac2, wv, lane);
+  } else {
+    // first derivatives on rows [y0-H2, y0+TH+H2), columns [x0-H2, x0+64+H2):
+    // 64 columns per row-pass, then the 2*H2 leftover columns as one flat list
+    auto deriv_at = [&](int r, int c) {
+      const int y = y0 - H2 + r, x = x0 - H2 + c;
+      if (x < 0 || x >= w || y < 0 || y >= h) return;
+      const int yl = ((y < s ? s - y : y - s) - (y0 - H1)) * DF_RW;
+      const int yh = ((y >= h - s ? 2 * h - y - s - 2 : y + s) - (y0 - H1)) * DF_RW;
+      const int yc = (y - (y0 - H1)) * DF_RW;
+      const int xl = (x < s ? s - x : x - s) - (x0 - H1);
+      const int xh = (x >= w - s ? 2 * w - x - s - 2 : x + s) - (x0 - H1);
+      const int xc = x - (x0 - H1);
+      float ul = s_l[yl + xl], ur = s_l[yl + xh], ll = s_l[yh + xl], lr = s_l[yh + xh];
+      float cl = s_l[yc + xl], cr = s_l[yc + xh];
+      float uc = s_l[yl + xc], lc = s_l[yh + xc];
+      s_x[r * DF_RW + c] = fac1 * (ur + lr - ul - ll) + fac2 * (cr - cl);
+      s_y[r * DF_RW + c] = fac1 * (lr + ll - ur - ul) + fac2 * (lc - uc);
+    };
+    const int R2n = DF_TH + 2 * H2, E2 = 2 * H2;
+    for (int r = wv; r < R2n; r += NW) deriv_at(r, lane);
+    for (int k = threadIdx.x; k < R2n * E2; k += 64 * NW) deriv_at(k / E2, 64 + k % E2);
+    __syncthreads();
+    // determinant on rows [y0-1, y0+TH+1), columns [x0-1, x0+65)
+    auto det_at = [&](int r, int c) {
+      const int y = y0 - 1 + r, x = x0 - 1 + c;
+      if (x < 0 || x >= w || y < 0 || y >= h) return;
+      const int yl = ((y < s ? s - y : y - s) - (y0 - H2)) * DF_RW;
+      const int yh = ((y >= h - s ? 2 * h - y - s - 2 : y + s) - (y0 - H2)) * DF_RW;
+      const int yc = (y - (y0 - H2)) * DF_RW;
+      const int xl = (x < s ? s - x : x - s) - (x0 - H2);
+      const int xh = (x >= w - s ? 2 * w - x - s - 2 : x + s) - (x0 - H2);
+      const int xc = x - (x0 - H2);
+      float ul = s_x[yl + xl], ur = s_x[yl + xh], ll = s_x[yh + xl], lr = s_x[yh + xh];
+      float cl = s_x[yc + xl], cr = s_x[yc + xh];
+      float lxx = fac1 * (ur + lr - ul - ll) + fac2 * (cr - cl);
+      float uc = s_x[yl + xc], lc = s_x[yh + xc];
+      float lyx = fac1 * (lr + ll - ur - ul) + fac2 * (lc - uc);
+      ul = s_y[yl + xl]; ur = s_y[yl + xh]; ll = s_y[yh + xl]; lr = s_y[yh + xh];
+      uc = s_y[yl + xc]; lc = s_y[yh + xc];
+      float lyy = fac1 * (lr + ll - ur - ul) + fac2 * (lc - uc);
+      s_d[r * DF_RW + c] = lxx * lyy - lyx * lyx;
+    };
+    for (int r = wv; r < DF_RD; r += NW) det_at(r, lane);
+    for (int k = threadIdx.x; k < DF_RD * 2; k += 64 * NW) det_at(k >> 1, 64 + (k & 1));
+  }
   __syncthreads();
   const int x = x0 + lane;
   float* Lx = akz_zoff(p.Lx[l], p.zs);
@@ -462,7 +506,8 @@ __global__ __launch_bounds__(256) void k_neighbors(akz_det_bufs b) {
 }
 
 constexpr int NMS_THREADS = 1024;
-constexpr int NMS_LDS_N = 8192;
+constexpr int NMS_LDS_N = 8192;     // candidates whose suppression state fits LDS
+constexpr int NMS_LDS_NB = 16384;   // neighbour entries that fit LDS beside it
 constexpr int CODE_WAIT = -1, CODE_ADD = -2;
 constexpr int NMS_BITMAP_WORDS = 2048;  // bitmap ranking covers up to 65536 candidates
 
@@ -507,113 +552,19 @@ __device__ __forceinline__ akz_kp emit(const akz_kp& ok) {
   return t;
 }
 
-// Wave-synchronous greedy suppression of FilterExtrema_kernel
-// (cuda_akaze.cu:1072-1224), one workgroup per image.  Per wave: a candidate
-// whose neighbours are all processed is "ready"; a ready candidate with no
-// kept neighbour is kept; otherwise it contends for its kept neighbour with
-// the smallest member value, the lowest contending index wins (atomicMin
-// here, a racy min in the reference), and the winner replaces that neighbour
-// if its response is higher, else is dropped.  Member values live in LDS when
-// the candidate count allows, in global memory otherwise.  Only unprocessed
-// candidates are revisited (compacted active list).  Then the output of
-// sortFiltered_kernel (:1227-1316): per 2048-slot chunk, the kept member
-// values whose point is strong, ascending.
-__global__ __launch_bounds__(NMS_THREADS) void k_nms(akz_det_bufs b) {
-  __shared__ int32_t s_mem[NMS_LDS_N];
-  __shared__ int32_t s_minnb[NMS_LDS_N];
-  __shared__ int32_t s_sort[2048];
-  __shared__ uint32_t s_bits[NMS_BITMAP_WORDS];
-  __shared__ int32_t s_pre[NMS_BITMAP_WORDS];
-  __shared__ int32_t s_w[17];
-  __shared__ int32_t s_nact[2];
-  __shared__ int32_t s_run;
-  const akz_kp* __restrict__ cand = akz_zoff(b.cand, b.zs);
-  const int32_t* __restrict__ nb_off = akz_zoff(b.nb_off, b.zs);
-  const int32_t* __restrict__ nb_list = akz_zoff(b.nb_list, b.zs);
-  int32_t* g_code = akz_zoff(b.code, b.zs);
-  akz_kp* __restrict__ out = akz_zoff(b.kps, b.zs);
-  int32_t* __restrict__ kidx = akz_zoff(b.kidx, b.zs);
-  int32_t* st = akz_zoff(b.status, b.zs);
-  const int cap = b.cap;
+// Output of sortFiltered_kernel (cuda_akaze.cu:1227-1316): per 2048-slot
+// chunk, the kept member values whose point is strong, ascending.  The
+// member values of a chunk are distinct candidate indices, so a bitmap over
+// [0, n) and a prefix popcount of its words give every value its rank;
+// candidate counts beyond the bitmap use a bitonic sort.  Returns the number
+// of keypoints (thread 0's value).
+template <class MemAt>
+__device__ int nms_output(const akz_kp* __restrict__ cand, int n, MemAt mem_at, akz_kp* __restrict__ out,
+                          int32_t* __restrict__ kidx, int cap, int32_t* s_sort, uint32_t* s_bits, int32_t* s_pre,
+                          int32_t* s_w, int32_t* s_run) {
   const int t = threadIdx.x;
-  const int n = *akz_zoff(b.ncand, b.zs);
-  const bool lds = n <= NMS_LDS_N;
-  int32_t* mem = lds ? s_mem : akz_zoff(b.mem, b.zs);
-  int32_t* minnb = lds ? s_minnb : akz_zoff(b.minnb, b.zs);
-  int32_t* act[2] = {akz_zoff(b.act, b.zs), akz_zoff(b.act, b.zs) + cap};
-  for (int i = t; i < n; i += NMS_THREADS) {
-    mem[i] = -1;
-    minnb[i] = INT32_MAX;
-    act[0][i] = i;
-  }
-  if (t == 0) { s_nact[0] = n; s_nact[1] = 0; s_run = 0; }
+  if (t == 0) *s_run = 0;
   __syncthreads();
-  int cur = 0, wave = 0;
-  while (wave < b.max_waves) {
-    const int na = s_nact[cur];
-    if (na == 0) break;
-    const int32_t* A = act[cur];
-    // phase 1: classify ready candidates and arbitrate contention
-    for (int j = t; j < na; j += NMS_THREADS) {
-      const int c = A[j];
-      const int e0 = akz_imin(nb_off[c], b.nb_cap), e1 = akz_imin(nb_off[c + 1], b.nb_cap);
-      bool ready = true, anykept = false;
-      int best = -1, bestv = INT32_MAX;
-      for (int k = e0; k < e1; k++) {
-        const int q = nb_list[k];
-        const int m = mem[q];
-        if (m == -1) { ready = false; break; }
-        if (m >= 0) {
-          anykept = true;
-          if (m < bestv) { bestv = m; best = q; }
-        }
-      }
-      int code = CODE_WAIT;
-      if (ready) {
-        if (!anykept) code = CODE_ADD;
-        else { code = best; atomicMin(&minnb[best], c); }
-      }
-      g_code[c] = code;
-    }
-    __syncthreads();
-    // phase 2: additions and replacements
-    for (int j = t; j < na; j += NMS_THREADS) {
-      const int c = A[j];
-      const int code = g_code[c];
-      if (code == CODE_ADD) {
-        mem[c] = c;
-      } else if (code >= 0 && minnb[code] == c) {
-        if (cand[c].response > cand[code].response) {
-          mem[c] = code;
-          mem[code] = -2;
-        } else {
-          mem[c] = -2;
-        }
-      }
-    }
-    __syncthreads();
-    // phase 3: reset arbitration slots, compact the still-unprocessed list
-    const int nxt = cur ^ 1;
-    for (int j = t; j < na; j += NMS_THREADS) {
-      const int c = A[j];
-      const int code = g_code[c];
-      if (code >= 0) minnb[code] = INT32_MAX;
-      if (mem[c] == -1) {
-        int pos = atomicAdd(&s_nact[nxt], 1);
-        act[nxt][pos] = c;
-      }
-    }
-    __syncthreads();
-    if (t == 0) s_nact[cur] = 0;
-    cur = nxt;
-    wave++;
-    __syncthreads();
-  }
-  if (t == 0) st[AKZ_ST_WAVES] = wave;
-  // output chunks: the member values of a chunk are distinct candidate
-  // indices, so a bitmap over [0, n) and a prefix popcount of its words give
-  // every value its rank in ascending order (the order of the reference's
-  // per-chunk sort); candidate counts beyond the bitmap use a bitonic sort
   const int nwords = (n + 31) >> 5;
   for (int base = 0; base < n; base += 2048) {
     if (nwords <= NMS_BITMAP_WORDS) {
@@ -621,7 +572,7 @@ __global__ __launch_bounds__(NMS_THREADS) void k_nms(akz_det_bufs b) {
       __syncthreads();
       for (int e = t; e < 2048; e += NMS_THREADS) {
         const int i = base + e;
-        const int v = (i < n) ? mem[i] : -1;
+        const int v = (i < n) ? mem_at(i) : -1;
         if (v >= 0 && !(cand[v].size < 0)) atomicOr(&s_bits[v >> 5], 1u << (v & 31));
       }
       __syncthreads();
@@ -633,10 +584,10 @@ __global__ __launch_bounds__(NMS_THREADS) void k_nms(akz_det_bufs b) {
       if (2 * t < nwords) s_pre[2 * t] = ex;
       if (2 * t + 1 < nwords) s_pre[2 * t + 1] = ex + c0;
       __syncthreads();
-      const int run = s_run;
+      const int run = *s_run;
       for (int e = t; e < 2048; e += NMS_THREADS) {
         const int i = base + e;
-        const int v = (i < n) ? mem[i] : -1;
+        const int v = (i < n) ? mem_at(i) : -1;
         if (v >= 0 && !(cand[v].size < 0)) {
           const int rank = s_pre[v >> 5] + __popc(s_bits[v >> 5] & ((1u << (v & 31)) - 1u));
           if (run + rank < cap) {
@@ -646,12 +597,12 @@ __global__ __launch_bounds__(NMS_THREADS) void k_nms(akz_det_bufs b) {
         }
       }
       __syncthreads();
-      if (t == 0) s_run = run + tot;
+      if (t == 0) *s_run = run + tot;
       __syncthreads();
     } else {
       for (int e = t; e < 2048; e += NMS_THREADS) {
         const int i = base + e;
-        const int v = (i < n) ? mem[i] : -1;
+        const int v = (i < n) ? mem_at(i) : -1;
         const bool ok = v >= 0 && !(cand[v].size < 0);
         s_sort[e] = ok ? v : INT32_MAX;
       }
@@ -668,7 +619,7 @@ __global__ __launch_bounds__(NMS_THREADS) void k_nms(akz_det_bufs b) {
       }
       const int cnt = __syncthreads_count((s_sort[t] != INT32_MAX) ? 1 : 0) +
                       __syncthreads_count((s_sort[t + 1024] != INT32_MAX) ? 1 : 0);
-      const int run = s_run;
+      const int run = *s_run;
       for (int e = t; e < 2048; e += NMS_THREADS) {
         const int v = s_sort[e];
         if (v != INT32_MAX && run + e < cap) {
@@ -677,15 +628,196 @@ __global__ __launch_bounds__(NMS_THREADS) void k_nms(akz_det_bufs b) {
         }
       }
       __syncthreads();
-      if (t == 0) s_run = run + cnt;
+      if (t == 0) *s_run = run + cnt;
       __syncthreads();
     }
   }
+  return *s_run;
+}
+
+// Wave-synchronous greedy suppression of FilterExtrema_kernel
+// (cuda_akaze.cu:1072-1224), one workgroup per image.  Per wave: a candidate
+// whose neighbours are all processed is "ready"; a ready candidate with no
+// kept neighbour is kept; otherwise it contends for its kept neighbour with
+// the smallest member value, the lowest contending index wins (atomicMin
+// here, a racy min in the reference), and the winner replaces that neighbour
+// if its response is higher, else is dropped.  No step of a wave depends on
+// the order in which its candidates are visited.
+//
+// Up to NMS_LDS_N candidates with at most NMS_LDS_NB neighbour entries, the
+// whole state lives in LDS (16-bit member values, codes and offsets, the CSR
+// lists and the responses) and every wave visits all candidates, skipping
+// the processed ones: a wave costs three barriers and LDS reads only.  Beyond
+// that the state is in global memory and each wave visits a compacted list
+// of the unprocessed candidates.
+__global__ __launch_bounds__(NMS_THREADS) void k_nms(akz_det_bufs b) {
+  __shared__ int16_t s_mem[NMS_LDS_N];
+  __shared__ int16_t s_code[NMS_LDS_N];
+  __shared__ int32_t s_minnb[NMS_LDS_N];
+  __shared__ uint16_t s_off[NMS_LDS_N + 2];
+  __shared__ float s_resp[NMS_LDS_N];
+  __shared__ __align__(16) int16_t s_nb[NMS_LDS_NB];  // the output phase reuses it (24 KB)
+  __shared__ int32_t s_w[17];
+  __shared__ int32_t s_nact[2];
+  __shared__ int32_t s_run;
+  int32_t* s_sort = reinterpret_cast<int32_t*>(s_nb);
+  uint32_t* s_bits = reinterpret_cast<uint32_t*>(s_nb) + 2048;
+  int32_t* s_pre = reinterpret_cast<int32_t*>(s_nb) + 2048 + NMS_BITMAP_WORDS;
+  const akz_kp* __restrict__ cand = akz_zoff(b.cand, b.zs);
+  const int32_t* __restrict__ nb_off = akz_zoff(b.nb_off, b.zs);
+  const int32_t* __restrict__ nb_list = akz_zoff(b.nb_list, b.zs);
+  akz_kp* __restrict__ out = akz_zoff(b.kps, b.zs);
+  int32_t* __restrict__ kidx = akz_zoff(b.kidx, b.zs);
+  int32_t* st = akz_zoff(b.status, b.zs);
+  const int cap = b.cap;
+  const int t = threadIdx.x;
+  const int n = *akz_zoff(b.ncand, b.zs);
+  const int ntot = n > 0 ? akz_imin(nb_off[n], b.nb_cap) : 0;
+  const bool lds = n <= NMS_LDS_N && ntot <= NMS_LDS_NB;
+  int wave = 0;
+  int kept;
+  if (lds) {
+    for (int i = t; i < n; i += NMS_THREADS) {
+      s_mem[i] = -1;
+      s_code[i] = CODE_WAIT;
+      s_minnb[i] = INT32_MAX;
+      s_resp[i] = cand[i].response;
+    }
+    for (int i = t; i <= n; i += NMS_THREADS) s_off[i] = (uint16_t)akz_imin(nb_off[i], b.nb_cap);
+    for (int k = t; k < ntot; k += NMS_THREADS) s_nb[k] = (int16_t)nb_list[k];
+    __syncthreads();
+    bool any = n > 0;
+    while (wave < b.max_waves && any) {
+      // phase 1: classify ready candidates and arbitrate contention
+      for (int c = t; c < n; c += NMS_THREADS) {
+        if (s_mem[c] != -1) continue;
+        const int e0 = s_off[c], e1 = s_off[c + 1];
+        bool ready = true, anykept = false;
+        int best = -1, bestv = INT32_MAX;
+        for (int k = e0; k < e1; k++) {
+          const int q = s_nb[k];
+          const int m = s_mem[q];
+          if (m == -1) { ready = false; break; }
+          if (m >= 0) {
+            anykept = true;
+            if (m < bestv) { bestv = m; best = q; }
+          }
+        }
+        int code = CODE_WAIT;
+        if (ready) {
+          if (!anykept) code = CODE_ADD;
+          else { code = best; atomicMin(&s_minnb[best], c); }
+        }
+        s_code[c] = (int16_t)code;
+      }
+      __syncthreads();
+      // phase 2: additions and replacements (targets are kept, never active)
+      for (int c = t; c < n; c += NMS_THREADS) {
+        if (s_mem[c] != -1) continue;
+        const int code = s_code[c];
+        if (code == CODE_ADD) {
+          s_mem[c] = (int16_t)c;
+        } else if (code >= 0 && s_minnb[code] == c) {
+          if (s_resp[c] > s_resp[code]) {
+            s_mem[c] = (int16_t)code;
+            s_mem[code] = -2;
+          } else {
+            s_mem[c] = -2;
+          }
+        }
+      }
+      __syncthreads();
+      // phase 3: reset the arbitration slots and codes, look for work left
+      bool left = false;
+      for (int c = t; c < n; c += NMS_THREADS) {
+        const int code = s_code[c];
+        if (code >= 0) s_minnb[code] = INT32_MAX;
+        if (code != CODE_WAIT) s_code[c] = CODE_WAIT;
+        left |= s_mem[c] == -1;
+      }
+      wave++;
+      any = __syncthreads_or(left);
+    }
+    kept = nms_output(cand, n, [&](int i) { return (int)s_mem[i]; }, out, kidx, cap, s_sort, s_bits, s_pre, s_w,
+                      &s_run);
+  } else {
+    int32_t* mem = akz_zoff(b.mem, b.zs);
+    int32_t* minnb = akz_zoff(b.minnb, b.zs);
+    int32_t* g_code = akz_zoff(b.code, b.zs);
+    int32_t* act[2] = {akz_zoff(b.act, b.zs), akz_zoff(b.act, b.zs) + cap};
+    for (int i = t; i < n; i += NMS_THREADS) {
+      mem[i] = -1;
+      minnb[i] = INT32_MAX;
+      act[0][i] = i;
+    }
+    if (t == 0) { s_nact[0] = n; s_nact[1] = 0; }
+    __syncthreads();
+    int cur = 0;
+    while (wave < b.max_waves) {
+      const int na = s_nact[cur];
+      if (na == 0) break;
+      const int32_t* A = act[cur];
+      for (int j = t; j < na; j += NMS_THREADS) {
+        const int c = A[j];
+        const int e0 = akz_imin(nb_off[c], b.nb_cap), e1 = akz_imin(nb_off[c + 1], b.nb_cap);
+        bool ready = true, anykept = false;
+        int best = -1, bestv = INT32_MAX;
+        for (int k = e0; k < e1; k++) {
+          const int q = nb_list[k];
+          const int m = mem[q];
+          if (m == -1) { ready = false; break; }
+          if (m >= 0) {
+            anykept = true;
+            if (m < bestv) { bestv = m; best = q; }
+          }
+        }
+        int code = CODE_WAIT;
+        if (ready) {
+          if (!anykept) code = CODE_ADD;
+          else { code = best; atomicMin(&minnb[best], c); }
+        }
+        g_code[c] = code;
+      }
+      __syncthreads();
+      for (int j = t; j < na; j += NMS_THREADS) {
+        const int c = A[j];
+        const int code = g_code[c];
+        if (code == CODE_ADD) {
+          mem[c] = c;
+        } else if (code >= 0 && minnb[code] == c) {
+          if (cand[c].response > cand[code].response) {
+            mem[c] = code;
+            mem[code] = -2;
+          } else {
+            mem[c] = -2;
+          }
+        }
+      }
+      __syncthreads();
+      const int nxt = cur ^ 1;
+      for (int j = t; j < na; j += NMS_THREADS) {
+        const int c = A[j];
+        const int code = g_code[c];
+        if (code >= 0) minnb[code] = INT32_MAX;
+        if (mem[c] == -1) {
+          int pos = atomicAdd(&s_nact[nxt], 1);
+          act[nxt][pos] = c;
+        }
+      }
+      __syncthreads();
+      if (t == 0) s_nact[cur] = 0;
+      cur = nxt;
+      wave++;
+      __syncthreads();
+    }
+    kept = nms_output(cand, n, [&](int i) { return mem[i]; }, out, kidx, cap, s_sort, s_bits, s_pre, s_w, &s_run);
+  }
   if (t == 0) {
-    st[AKZ_ST_KEPT] = akz_imin(s_run, cap);
+    st[AKZ_ST_WAVES] = wave;
+    st[AKZ_ST_KEPT] = akz_imin(kept, cap);
     // the whole block (flags and total come from earlier kernels) into the
     // contiguous per-batch array the host reads with one copy
-    int4 blk = make_int4(akz_imin(s_run, cap), st[AKZ_ST_FLAGS], st[AKZ_ST_TOTAL], wave);
+    int4 blk = make_int4(akz_imin(kept, cap), st[AKZ_ST_FLAGS], st[AKZ_ST_TOTAL], wave);
     reinterpret_cast<int4*>(b.status_all)[blockIdx.z] = blk;
   }
 }

@@ -363,7 +363,7 @@ __global__ __launch_bounds__(NT) void k_fed_multi(const float* __restrict__ L, c
   L = akz_zoff(L, zs);
   g = akz_zoff(g, zs);
   out = akz_zoff(out, zs);
-  constexpr int LW = (RW == 64) ? 6 : (RW == 32 ? 5 : 4);
+  constexpr int LW = (RW == 128) ? 7 : (RW == 64) ? 6 : (RW == 32 ? 5 : 4);
   static_assert((1 << LW) == RW, "power-of-two region width");
   __shared__ float sg[RH * RW];
   __shared__ float sa[RH * RW];
@@ -546,7 +546,8 @@ int akz_fed_block_steps() { return FB_K; }
 
 // Kernel variant per FED launch: 0 k_fed_step, 1 k_fed_tile; blocked regions
 // (tile = region minus the 8-pixel halo): 2 64x64/512 threads, 3 32x32/256,
-// 4 64x32/256, 5 32x32/1024, 6 64x64/1024, 7 64x48/512, 8 32x48/512.
+// 4 64x32/256, 5 32x32/1024, 6 64x64/1024, 7 64x48/512, 8 32x48/512,
+// 9 128x64/1024, 10 64x128/1024, 11 128x32/512.
 void akz_launch_fed_variant(hipStream_t s, int variant, const float* Lin, const float* g, float* Lout, int pitch,
                             int w, int h, const float* half_tau, int n, akz_batch bt) {
   const int nz = bt.nz;
@@ -585,6 +586,15 @@ void akz_launch_fed_variant(hipStream_t s, int variant, const float* Lin, const 
   } else if (variant == 8) {
     dim3 grid((w + 15) / 16, (h + 31) / 32, nz);
     hipLaunchKernelGGL((k_fed_multi<32, 48, 512>), grid, dim3(512), 0, s, Lin, g, Lout, pitch, w, h, st, zs);
+  } else if (variant == 9) {
+    dim3 grid((w + 111) / 112, (h + 47) / 48, nz);
+    hipLaunchKernelGGL((k_fed_multi<128, 64, 1024>), grid, dim3(1024), 0, s, Lin, g, Lout, pitch, w, h, st, zs);
+  } else if (variant == 10) {
+    dim3 grid((w + 47) / 48, (h + 111) / 112, nz);
+    hipLaunchKernelGGL((k_fed_multi<64, 128, 1024>), grid, dim3(1024), 0, s, Lin, g, Lout, pitch, w, h, st, zs);
+  } else if (variant == 11) {
+    dim3 grid((w + 111) / 112, (h + 15) / 16, nz);
+    hipLaunchKernelGGL((k_fed_multi<128, 32, 512>), grid, dim3(512), 0, s, Lin, g, Lout, pitch, w, h, st, zs);
   } else {
     dim3 grid((w + 47) / 48, (h + 15) / 16, nz);
     hipLaunchKernelGGL((k_fed_multi<64, 32, 256>), grid, dim3(256), 0, s, Lin, g, Lout, pitch, w, h, st, zs);

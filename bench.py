@@ -80,8 +80,9 @@ def batched_comparison(A, W, H, B=16, reps=20):
     net = ProposalNet()
     net.load(os.path.join(here, "proposal_1080p.f32"))
     bk = A.BatchAKAZE(A.AKAZEOptions(img_width=W, img_height=H), B)
+    variants = bk.autotune(B)
     bk.synth(20_000_000, B)
-    out = {"images_per_call": B, "threshold": rec["threshold"]}
+    out = {"images_per_call": B, "threshold": rec["threshold"], "autotuned_fed_variants": [int(v) for v in variants]}
     kps = {}
     for name, nn in (("full_scan", None), ("learned_proposal", net)):
         bk.set_proposal(nn, rec["threshold"])
