@@ -605,7 +605,7 @@ akz_status akz_create_batch(const akz_options* opts, int batch, akz_ctx** out) {
   gauss_taps(1.0, 2, &c->taps1);
   gauss_taps(1.0f, 2, &c->taps_c);
   akz_describe_init_tables();
-  for (int i = 0; i < c->nlev; i++) c->fed_variant[i] = akz_fed_variant_for(c->lev[i].w, c->lev[i].h);
+  for (int i = 0; i < c->nlev; i++) c->fed_variant[i] = akz_fed_variant_for(c->lev[i].w, c->lev[i].h, batch);
   c->graph_exec.assign(batch, nullptr);
   c->graph.assign(batch, nullptr);
   st = allocate(c);
@@ -999,7 +999,7 @@ akz_status akz_prop_read_probs(akz_ctx* c, int z, float* probs) {
 // ---- autotuner ----
 akz_status akz_autotune(akz_ctx* c, int nz, int reps) {
   if (!c || nz < 1 || nz > c->batch || reps < 1) return AKZ_ERR_ARG;
-  static const int kVariants[] = {2, 3, 4, 5, 6, 7, 8, 9, 10, 11};
+  static const int kVariants[] = {2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13};
   hipEvent_t a, b;
   AKZ_HIP_TRY(hipEventCreate(&a));
   AKZ_HIP_TRY(hipEventCreate(&b));
@@ -1059,7 +1059,7 @@ akz_status akz_get_fed_variants(const akz_ctx* c, int* variants) {
 akz_status akz_set_fed_variants(akz_ctx* c, const int* variants) {
   if (!c || !variants) return AKZ_ERR_ARG;
   for (int i = 0; i < c->nlev; i++)
-    if (variants[i] < 2 || variants[i] > 11) return AKZ_ERR_ARG;
+    if (variants[i] < 2 || variants[i] > 13) return AKZ_ERR_ARG;
   AKZ_HIP_TRY(hipStreamSynchronize(c->stream));
   for (int i = 0; i < c->nlev; i++) c->fed_variant[i] = variants[i];
   drop_graphs(c);

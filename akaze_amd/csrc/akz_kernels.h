@@ -18,8 +18,8 @@ void akz_launch_gauss(hipStream_t s, const float* in, int in_pitch, size_t in_zs
 void akz_launch_contrast(hipStream_t s, const float* blur, int pitch, float* grad, int w, int h, int nbins,
                          float perc, uint32_t* maxbits, uint32_t* blockmax, int32_t* hist_part, float* kc, int nlev,
                          const int* octaves_host, akz_batch bt);
-// FED kernel variant chosen for a level size
-int akz_fed_variant_for(int w, int h);
+// FED kernel variant chosen for a level size and the images per launch
+int akz_fed_variant_for(int w, int h, int nz);
 // LowPass(sigma=1) + Flow of one level in one kernel; taps: 5 taps of sigma 1
 void akz_launch_smooth_flow(hipStream_t s, const float* in, int pitch, float* lsm, float* g, int w, int h, int type,
                             const float* kc, const akz_taps& taps, akz_batch bt);

@@ -406,6 +406,90 @@ __global__ __launch_bounds__(NT) void k_fed_multi(const float* __restrict__ L, c
   }
 }
 
+// FED steps with temporal blocking in registers: one wavefront per region of
+// 64 columns (one per lane) x R rows (R registers per lane), 8-pixel halo.
+// The conductance sums of each pixel pair are formed once per launch: Wh[r]
+// = g + g(east) per lane, Wv[r] = g(row r) + g(row r+1); the west and north
+// sums are the neighbours' east and south sums (IEEE addition commutes).
+// Vertical neighbours are registers, horizontal ones arrive by DPP
+// wavefront shifts; no LDS and no barriers.  Per pixel and step the
+// arithmetic is k_fed_step's in the same order.  Rows 1..R-2 are updated
+// every step; the rows and lanes near the region edge go stale one pixel
+// per step, never reaching the output tile.  At image borders the outward
+// neighbour is the pixel itself (NLDStep's clamping), which zeroes that
+// flux whatever the sum next to it.
+__device__ __forceinline__ float from_right(float v) {  // lane i receives lane i+1 (lane 63 keeps its own)
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float from_left(float v) {  // lane i receives lane i-1 (lane 0 keeps its own)
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x138, 0xf, 0xf, false));
+}
+
+template <int R, bool BORDER>
+__device__ __forceinline__ void fed_reg_body(const float* __restrict__ L, const float* __restrict__ g,
+                                             float* __restrict__ out, int pitch, int w, int h,
+                                             const akz_fed_steps& st, int x0, int y0) {
+  constexpr int K = FB_K;
+  const int lane = threadIdx.x;
+  const int gx = x0 + lane;
+  const int cx = BORDER ? akz_iclamp(gx, 0, w - 1) : gx;
+  float Lr[R], Wv[R], Wh[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int gy = BORDER ? akz_iclamp(y0 + r, 0, h - 1) : y0 + r;
+    Lr[r] = L[(size_t)gy * pitch + cx];
+    Wv[r] = g[(size_t)gy * pitch + cx];  // g for now
+  }
+#pragma unroll
+  for (int r = 0; r < R; r++) Wh[r] = Wv[r] + from_right(Wv[r]);
+#pragma unroll
+  for (int r = 0; r < R - 1; r++) Wv[r] = Wv[r] + Wv[r + 1];
+  const bool east_self = BORDER && gx >= w - 1, west_self = BORDER && gx <= 0;
+  for (int s = 0; s < st.n; s++) {
+    const float stepsize = st.half_tau[s];
+    float up = Lr[0];
+#pragma unroll
+    for (int r = 1; r < R - 1; r++) {
+      const float i0 = Lr[r];
+      float le = from_right(i0), lw = from_left(i0);
+      float ln = up, ls = Lr[r + 1];
+      if (BORDER) {
+        if (east_self) le = i0;
+        if (west_self) lw = i0;
+        if (y0 + r <= 0) ln = i0;
+        if (y0 + r >= h - 1) ls = i0;
+      }
+      const float ww = from_left(Wh[r]);
+      const float xpos = Wh[r] * (le - i0);
+      const float xneg = ww * (i0 - lw);
+      const float ypos = Wv[r] * (ls - i0);
+      const float yneg = Wv[r - 1] * (i0 - ln);
+      up = i0;
+      Lr[r] = i0 + stepsize * (xpos - xneg + ypos - yneg);
+    }
+  }
+  if (lane < K || lane >= 64 - K || gx >= w) return;
+#pragma unroll
+  for (int r = K; r < R - K; r++) {
+    const int gy = y0 + r;
+    if (gy < h) out[(size_t)gy * pitch + gx] = Lr[r];
+  }
+}
+
+template <int R>
+__global__ __launch_bounds__(64) void k_fed_reg(const float* __restrict__ L, const float* __restrict__ g,
+                                                float* __restrict__ out, int pitch, int w, int h, akz_fed_steps st,
+                                                size_t zs) {
+  L = akz_zoff(L, zs);
+  g = akz_zoff(g, zs);
+  out = akz_zoff(out, zs);
+  const int x0 = blockIdx.x * (64 - 2 * FB_K) - FB_K, y0 = blockIdx.y * (R - 2 * FB_K) - FB_K;
+  // no pixel of the region is on or beyond an image border
+  const bool interior = x0 >= 1 && x0 + 64 <= w - 1 && y0 >= 1 && y0 + R <= h - 1;
+  if (interior) fed_reg_body<R, false>(L, g, out, pitch, w, h, st, x0, y0);
+  else fed_reg_body<R, true>(L, g, out, pitch, w, h, st, x0, y0);
+}
+
 // One FED step, LDS-tiled: a 64x16 output tile plus a one-pixel halo of L and
 // g is read once with coalesced rows; arithmetic as k_fed_step.
 __global__ __launch_bounds__(256) void k_fed_tile(const float* __restrict__ L, const float* __restrict__ g,
@@ -547,7 +631,8 @@ int akz_fed_block_steps() { return FB_K; }
 // Kernel variant per FED launch: 0 k_fed_step, 1 k_fed_tile; blocked regions
 // (tile = region minus the 8-pixel halo): 2 64x64/512 threads, 3 32x32/256,
 // 4 64x32/256, 5 32x32/1024, 6 64x64/1024, 7 64x48/512, 8 32x48/512,
-// 9 128x64/1024, 10 64x128/1024, 11 128x32/512.
+// 9 128x64/1024, 10 64x128/1024, 11 128x32/512; registers (one wavefront per
+// 64-column region): 12 64 rows, 13 48 rows.
 void akz_launch_fed_variant(hipStream_t s, int variant, const float* Lin, const float* g, float* Lout, int pitch,
                             int w, int h, const float* half_tau, int n, akz_batch bt) {
   const int nz = bt.nz;
@@ -592,6 +677,12 @@ void akz_launch_fed_variant(hipStream_t s, int variant, const float* Lin, const 
   } else if (variant == 10) {
     dim3 grid((w + 47) / 48, (h + 111) / 112, nz);
     hipLaunchKernelGGL((k_fed_multi<64, 128, 1024>), grid, dim3(1024), 0, s, Lin, g, Lout, pitch, w, h, st, zs);
+  } else if (variant == 12) {
+    dim3 grid((w + 47) / 48, (h + 47) / 48, nz);
+    hipLaunchKernelGGL(k_fed_reg<64>, grid, dim3(64), 0, s, Lin, g, Lout, pitch, w, h, st, zs);
+  } else if (variant == 13) {
+    dim3 grid((w + 47) / 48, (h + 31) / 32, nz);
+    hipLaunchKernelGGL(k_fed_reg<48>, grid, dim3(64), 0, s, Lin, g, Lout, pitch, w, h, st, zs);
   } else if (variant == 11) {
     dim3 grid((w + 111) / 112, (h + 15) / 16, nz);
     hipLaunchKernelGGL((k_fed_multi<128, 32, 512>), grid, dim3(512), 0, s, Lin, g, Lout, pitch, w, h, st, zs);
@@ -601,12 +692,22 @@ void akz_launch_fed_variant(hipStream_t s, int variant, const float* Lin, const 
   }
 }
 
-// Region shape by level size, from per-size timings of the variants on
-// MI355X (8 steps per launch, us per step):
+// Region shape by level size and images per launch, from timings of the
+// variants on MI355X (8 steps per launch).  One image, us per step:
 //   3840x2160: 64x48/512t 13.2, 64x64/512t 14.1;  1920x1080: 64x64/1024t 4.34;
 //   960x540: 64x64/1024t 1.70;  480x270: 32x32/1024t 0.97;  240x135: 0.73.
-int akz_fed_variant_for(int w, int h) {
-  size_t px = (size_t)w * h;
+// Batches of 16 1080p images, whole FED per call: register variant 48 rows
+// 1.27 ms, 64 rows 1.45 ms, 64x64/1024t 1.48 ms; per level the autotuner
+// picks registers down to 480x270 and 64x64/1024t at 240x135.
+int akz_fed_variant_for(int w, int h, int nz) {
+  const size_t px = (size_t)w * h;
+  if (nz > 1) {
+    const size_t tot = px * (size_t)nz;
+    if (tot >= 4000000) return 13;
+    if (tot >= 1000000) return 12;
+    if (tot >= 300000) return 6;
+    return 5;
+  }
   if (px >= 4000000) return 7;
   if (px >= 300000) return 6;
   return 5;
@@ -614,7 +715,7 @@ int akz_fed_variant_for(int w, int h) {
 
 void akz_launch_fed_multi(hipStream_t s, const float* Lin, const float* g, float* Lout, int pitch, int w, int h,
                           const float* half_tau, int n, akz_batch bt) {
-  akz_launch_fed_variant(s, akz_fed_variant_for(w, h), Lin, g, Lout, pitch, w, h, half_tau, n, bt);
+  akz_launch_fed_variant(s, akz_fed_variant_for(w, h, bt.nz), Lin, g, Lout, pitch, w, h, half_tau, n, bt);
 }
 
 void akz_launch_halfsample(hipStream_t s, const float* in, int iw, int ih, int ipitch, float* out, int ow, int oh,

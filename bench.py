@@ -99,6 +99,13 @@ def batched_comparison(A, W, H, B=16, reps=20):
         bk.wait()
         out[name + "_images_per_s"] = round(reps * B / (time.perf_counter() - t0), 1)
         bk.set_graph_mode(False)
+    # FED roofline of the batched launches (live HIP events, as for the headline)
+    bk.set_proposal(None)
+    fms, fby, fl = min(bk.profile_fed(B), bk.profile_fed(B))
+    ach = (fby / fl) / (fms / fl * 1e-3) / 1e9
+    out["fed_roofline"] = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": round(ach / HBM_PEAK_GBS, 4), "avg_launch_us": round(fms / fl * 1e3, 3),
+                           "alg_bytes_per_launch": round(fby / fl)}
     full, prop = kps["full_scan"], kps["learned_proposal"]
     total = sum(len(f) for f in full)
     out["proposal_kp_recall"] = round(sum(len(f) * keypoint_recall(f, p) for f, p in zip(full, prop)) / total, 4)
