@@ -39,13 +39,13 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def fed_traffic():
+def fed_traffic(profile="k_fed_multi_1080p_pmc.md", key="traffic_bytes_per_launch"):
     """HBM bytes per FED launch from the committed FETCH_SIZE / WRITE_SIZE
     counter runs (scaled by the copy-kernel calibration), or None."""
-    path = os.path.join(REPO, "profiles", "k_fed_multi_1080p_pmc.md")
+    path = os.path.join(REPO, "profiles", profile)
     try:
         for line in open(path):
-            if line.startswith("traffic_bytes_per_launch:"):
+            if line.startswith(key + ":"):
                 return int(line.split(":")[1])
     except OSError:
         pass
@@ -105,7 +105,9 @@ def batched_comparison(A, W, H, B=16, reps=20):
     ach = (fby / fl) / (fms / fl * 1e-3) / 1e9
     out["fed_roofline"] = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": round(ach / HBM_PEAK_GBS, 4), "avg_launch_us": round(fms / fl * 1e3, 3),
-                           "alg_bytes_per_launch": round(fby / fl)}
+                           "alg_bytes_per_launch": round(fby / fl),
+                           "traffic": fed_traffic("akaze_batched16_1080p_kernel_stats.md",
+                                                  "traffic_bytes_per_launch_batched16")}
     full, prop = kps["full_scan"], kps["learned_proposal"]
     total = sum(len(f) for f in full)
     out["proposal_kp_recall"] = round(sum(len(f) * keypoint_recall(f, p) for f, p in zip(full, prop)) / total, 4)

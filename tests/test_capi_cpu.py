@@ -89,3 +89,42 @@ def test_synthetic_generator_is_seeded():
     Hr = A.synth_homography(4, 640, 480, max_persp=0.0)
     c0 = Hr @ np.array([320.0, 240.0, 1.0])
     assert np.allclose(c0[:2] / c0[2], [320, 240])
+
+
+def test_batch_ragged_and_proposal_entry_points_validate_arguments():
+    """Argument checks that run before any device work."""
+    L = A.lib()
+    p = ctypes.c_void_p()
+    o = A.AKAZEOptions(img_width=640, img_height=480)
+    assert L.akz_create_batch(ctypes.byref(o._c), 0, ctypes.byref(p)) == 1
+    assert L.akz_create_batch(ctypes.byref(o._c), 65, ctypes.byref(p)) == 1
+    w = (ctypes.c_int * 2)(640, 640)
+    h = (ctypes.c_int * 2)(480, 480)
+    cap = (ctypes.c_int * 2)(1, 1)
+    # duplicate size classes are rejected
+    assert L.akz_ragged_create(ctypes.byref(o._c), 2, w, h, cap, ctypes.byref(p)) == 1
+    assert L.akz_ragged_create(ctypes.byref(o._c), 0, w, h, cap, ctypes.byref(p)) == 1
+    n = ctypes.c_int()
+    assert L.akz_prop_num_params(ctypes.byref(n)) == 0 and n.value == 14513
+    assert L.akz_prop_create(ctypes.c_uint64(1), None) == 1
+    assert L.akz_set_proposal(None, None, ctypes.c_float(0.5)) == 1
+    assert L.akz_synth_device_batch(ctypes.c_uint64(1), 0, 640, 480, None, 2560, 0) == 1
+    assert L.akz_autotune(None, 1, 1) == 1
+
+
+def test_bench_refuses_more_than_one_gpu():
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(REPO, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
+
+
+def test_committed_proposal_weights_are_well_formed():
+    import json
+    here = os.path.join(REPO, "akaze_amd")
+    w = np.fromfile(os.path.join(here, "proposal_1080p.f32"), "<f4")
+    assert w.shape == (14513,) and np.all(np.isfinite(w))
+    rec = json.load(open(os.path.join(here, "proposal_1080p.json")))
+    assert 0.0 < rec["threshold"] < 1.0
+    assert rec["heldout_kp_recall"] >= 0.98 and rec["heldout_images"] >= 2000 and rec["train_images"] >= 20000
