@@ -114,6 +114,7 @@ struct akz_ctx {
   int batch = 1;
   size_t zs = 0;
   int last_nz = 1;            // images in the last processed batch
+  int status_synced = 0;      // images whose status blocks are in h_small and current
   int32_t* h_small = nullptr;  // pinned copy of the status blocks (4 ints per image)
   uint8_t* h_stage = nullptr;  // pinned staging for keypoints + 64-byte descriptor rows
   size_t h_stage_bytes = 0;
@@ -380,6 +381,11 @@ akz_status allocate(akz_ctx* c) {
   return AKZ_OK;
 }
 
+// FED steps per launch at level i: the variant's, capped by the option
+int level_block(const akz_ctx* c, int i) {
+  return c->fed_block == 1 ? 1 : std::min(c->fed_block, akz_fed_variant_steps(c->fed_variant[i]));
+}
+
 // Create_Nonlinear_Scale_Space (AKAZE.cpp:211-276) on the context stream.
 inline void record(akz_ctx* c, int e) {
   if (!c->capturing) hipEventRecord(c->ev[e], c->stream);
@@ -405,7 +411,7 @@ void enqueue_scale_space(akz_ctx* c, const float* img, int in_pitch, size_t in_z
     const int n = L.nsteps;
     // FED launches alternate between Lt and the scratch so that the last one
     // lands in Lt; each launch runs up to `bs` steps (bs = 1: one per launch)
-    const int bs = c->fed_block;
+    const int bs = level_block(c, i);
     const int nl = (n + bs - 1) / bs;
     float* first_dst = (nl % 2 == 1) ? L.Lt : c->fed_tmp;
     const float* base;
@@ -441,6 +447,7 @@ void enqueue_scale_space(akz_ctx* c, const float* img, int in_pitch, size_t in_z
 // Feature_Detection (AKAZE.cpp:316-378) + FilterExtrema (cuda_akaze.cu:1318-1402).
 void enqueue_detect(akz_ctx* c, int nz) {
   hipStream_t s = c->stream;
+  c->status_synced = 0;
   int smax = 0;
   for (int i = 0; i < c->nlev; i++) smax = std::max(smax, c->lev[i].deriv_step);
   if (smax <= akz_detect_max_step()) {
@@ -484,6 +491,7 @@ float ev_ms(akz_ctx* c, int a, int b) {
 akz_status sync_status(akz_ctx* c, int nz) {
   AKZ_HIP_TRY(hipMemcpyAsync(c->h_small, c->db.status_all, 16 * (size_t)nz, hipMemcpyDeviceToHost, c->stream));
   AKZ_HIP_TRY(hipStreamSynchronize(c->stream));
+  c->status_synced = nz;
   return AKZ_OK;
 }
 
@@ -714,7 +722,9 @@ akz_status akz_describe(akz_ctx* c, akz_keypoint* kps, uint8_t* desc, int cap, i
 akz_status akz_read_batch_results(akz_ctx* c, int z, akz_keypoint* kps, uint8_t* desc, int cap, int* n_out) {
   if (!c || cap < 0 || z < 0 || z >= c->batch) return AKZ_ERR_ARG;
   if (z >= c->last_nz) return AKZ_ERR_STATE;  // image not part of the last run
-  akz_status st = sync_status(c, z + 1);
+  akz_status st = AKZ_OK;
+  // the status block is current if it was read back after the last detection
+  if (c->status_synced <= z) st = sync_status(c, z + 1);
   if (st != AKZ_OK) return st;
   int nk = 0;
   st = image_status(c, z, &nk);
@@ -768,6 +778,7 @@ akz_status akz_process_batch_device_async(akz_ctx* c, const float* d_imgs, size_
   if (sp < sizeof(float) * L0.w || sp % sizeof(float)) return AKZ_ERR_ARG;
   size_t is = image_stride_bytes ? image_stride_bytes : sp * L0.h;
   if (n > 1 && is < sp * L0.h) return AKZ_ERR_ARG;
+  c->status_synced = 0;  // a graph replay rewrites the status blocks too
   if (!c->graph_mode) {
     enqueue_scale_space(c, d_imgs, (int)(sp / sizeof(float)), is, n);
     enqueue_detect(c, n);
@@ -928,7 +939,7 @@ akz_status akz_profile_fed_batch(akz_ctx* c, const float* d_imgs, size_t pitch_b
     total += ms;
     // algorithmic traffic of one step: read L and g, write L (4 B each per pixel)
     bytes += 12.0 * c->lev[i].w * c->lev[i].h * c->lev[i].nsteps * n;
-    launches += (c->lev[i].nsteps + c->fed_block - 1) / c->fed_block;
+    launches += (c->lev[i].nsteps + level_block(c, i) - 1) / level_block(c, i);
   }
   if (fed_ms) *fed_ms = total;
   if (fed_bytes) *fed_bytes = bytes;
@@ -999,7 +1010,7 @@ akz_status akz_prop_read_probs(akz_ctx* c, int z, float* probs) {
 // ---- autotuner ----
 akz_status akz_autotune(akz_ctx* c, int nz, int reps) {
   if (!c || nz < 1 || nz > c->batch || reps < 1) return AKZ_ERR_ARG;
-  static const int kVariants[] = {2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13};
+  static const int kVariants[] = {2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15};
   hipEvent_t a, b;
   AKZ_HIP_TRY(hipEventCreate(&a));
   AKZ_HIP_TRY(hipEventCreate(&b));
@@ -1017,20 +1028,25 @@ akz_status akz_autotune(akz_ctx* c, int nz, int reps) {
         break;
       }
     if (done) continue;
-    const int k = std::min(c->fed_block, L.nsteps);
     float ht[AKZ_MAX_TAU];
-    for (int q = 0; q < k; q++) ht[q] = 0.5f * L.tau[q];
+    for (int q = 0; q < L.nsteps; q++) ht[q] = 0.5f * L.tau[q];
     float best = 1e30f;
     int bestv = c->fed_variant[i];
     for (int v : kVariants) {
-      // warm-up launch, then `reps` timed launches ping-ponging Lt and the scratch
-      akz_launch_fed_variant(c->stream, v, L.Lt, c->g, c->fed_tmp, L.pitch, L.w, L.h, ht, k, bt);
+      // the level's whole FED run with this variant (its steps per launch),
+      // once to warm up, then `reps` timed times, ping-ponging Lt and the scratch
+      const int bs = std::min(c->fed_block, akz_fed_variant_steps(v));
+      auto run_level = [&](int parity) {
+        int launch = parity;
+        for (int j = 0; j < L.nsteps; j += bs, launch++) {
+          const bool even = (launch & 1) == 0;
+          akz_launch_fed_variant(c->stream, v, even ? L.Lt : c->fed_tmp, c->g, even ? c->fed_tmp : L.Lt, L.pitch,
+                                 L.w, L.h, ht + j, std::min(bs, L.nsteps - j), bt);
+        }
+      };
+      run_level(0);
       if (hipEventRecord(a, c->stream) != hipSuccess) { result = AKZ_ERR_HIP; break; }
-      for (int r = 0; r < reps; r++) {
-        const bool even = (r & 1) == 0;
-        akz_launch_fed_variant(c->stream, v, even ? c->fed_tmp : L.Lt, c->g, even ? L.Lt : c->fed_tmp, L.pitch, L.w,
-                               L.h, ht, k, bt);
-      }
+      for (int r = 0; r < reps; r++) run_level(r);
       if (hipEventRecord(b, c->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess) {
         result = AKZ_ERR_HIP;
         break;
@@ -1059,7 +1075,7 @@ akz_status akz_get_fed_variants(const akz_ctx* c, int* variants) {
 akz_status akz_set_fed_variants(akz_ctx* c, const int* variants) {
   if (!c || !variants) return AKZ_ERR_ARG;
   for (int i = 0; i < c->nlev; i++)
-    if (variants[i] < 2 || variants[i] > 13) return AKZ_ERR_ARG;
+    if (variants[i] < 2 || variants[i] > 15) return AKZ_ERR_ARG;
   AKZ_HIP_TRY(hipStreamSynchronize(c->stream));
   for (int i = 0; i < c->nlev; i++) c->fed_variant[i] = variants[i];
   drop_graphs(c);

@@ -25,8 +25,10 @@ void akz_launch_smooth_flow(hipStream_t s, const float* in, int pitch, float* ls
                             const float* kc, const akz_taps& taps, akz_batch bt);
 void akz_launch_fed_step(hipStream_t s, const float* Lin, const float* g, float* Lout, int pitch, int w, int h,
                          float half_tau, akz_batch bt);
-// up to akz_fed_block_steps() FED steps in one launch (n <= that many)
+// most FED steps any variant runs in one launch
 int akz_fed_block_steps();
+// FED steps per launch of a variant (its halo width; 1 for variants 0 and 1)
+int akz_fed_variant_steps(int variant);
 void akz_launch_fed_multi(hipStream_t s, const float* Lin, const float* g, float* Lout, int pitch, int w, int h,
                           const float* half_tau, int n, akz_batch bt);
 // n FED steps with one kernel variant (0 single step, 1 single step LDS-tiled,

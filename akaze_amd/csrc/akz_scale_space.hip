@@ -352,14 +352,15 @@ __global__ __launch_bounds__(256) void k_fed_step(const float* __restrict__ L, c
 // launch per step bit for bit.  RW and RH are powers of two, so the flattened
 // region index splits into row and column with shifts.  Region shapes per
 // level size are chosen from measurements (fed_config below).
-constexpr int FB_K = 8;  // most steps per launch (the halo of every variant)
-struct akz_fed_steps { float half_tau[FB_K]; int n; };
+constexpr int FB_K = 8;      // steps per launch (and halo) of most variants
+constexpr int FB_KMAX = 16;  // the deep-halo variants
+struct akz_fed_steps { float half_tau[FB_KMAX]; int n; };
 
-template <int RW, int RH, int NT>
+template <int RW, int RH, int NT, int K = FB_K>
 __global__ __launch_bounds__(NT) void k_fed_multi(const float* __restrict__ L, const float* __restrict__ g,
                                                   float* __restrict__ out, int pitch, int w, int h,
                                                   akz_fed_steps st, size_t zs) {
-  constexpr int K = FB_K, TW = RW - 2 * K, TH = RH - 2 * K;
+  constexpr int TW = RW - 2 * K, TH = RH - 2 * K;
   L = akz_zoff(L, zs);
   g = akz_zoff(g, zs);
   out = akz_zoff(out, zs);
@@ -626,13 +627,20 @@ void akz_launch_fed_step(hipStream_t s, const float* Lin, const float* g, float*
   hipLaunchKernelGGL(k_fed_step, grid64x4(w, h, bt.nz), dim3(256), 0, s, Lin, g, Lout, pitch, w, h, half_tau, bt.zs);
 }
 
-int akz_fed_block_steps() { return FB_K; }
+int akz_fed_block_steps() { return FB_KMAX; }
+
+int akz_fed_variant_steps(int variant) {
+  if (variant <= 1) return 1;
+  return (variant == 14 || variant == 15) ? FB_KMAX : FB_K;
+}
 
 // Kernel variant per FED launch: 0 k_fed_step, 1 k_fed_tile; blocked regions
 // (tile = region minus the 8-pixel halo): 2 64x64/512 threads, 3 32x32/256,
 // 4 64x32/256, 5 32x32/1024, 6 64x64/1024, 7 64x48/512, 8 32x48/512,
 // 9 128x64/1024, 10 64x128/1024, 11 128x32/512; registers (one wavefront per
-// 64-column region): 12 64 rows, 13 48 rows.
+// 64-column region): 12 64 rows, 13 48 rows; 16 steps per launch with a
+// 16-pixel halo (half the launches of the small, latency-bound levels):
+// 14 64x64/1024, 15 128x64/1024.
 void akz_launch_fed_variant(hipStream_t s, int variant, const float* Lin, const float* g, float* Lout, int pitch,
                             int w, int h, const float* half_tau, int n, akz_batch bt) {
   const int nz = bt.nz;
@@ -651,8 +659,10 @@ void akz_launch_fed_variant(hipStream_t s, int variant, const float* Lin, const 
     return;
   }
   akz_fed_steps st;
+  // callers split a level's steps into runs of at most akz_fed_variant_steps()
+  if (n > akz_fed_variant_steps(variant)) n = akz_fed_variant_steps(variant);
   st.n = n;
-  for (int i = 0; i < FB_K; i++) st.half_tau[i] = i < n ? half_tau[i] : 0.0f;
+  for (int i = 0; i < FB_KMAX; i++) st.half_tau[i] = i < n ? half_tau[i] : 0.0f;
   if (variant == 2) {
     dim3 grid((w + 47) / 48, (h + 47) / 48, nz);
     hipLaunchKernelGGL((k_fed_multi<64, 64, 512>), grid, dim3(512), 0, s, Lin, g, Lout, pitch, w, h, st, zs);
@@ -677,6 +687,12 @@ void akz_launch_fed_variant(hipStream_t s, int variant, const float* Lin, const 
   } else if (variant == 10) {
     dim3 grid((w + 47) / 48, (h + 111) / 112, nz);
     hipLaunchKernelGGL((k_fed_multi<64, 128, 1024>), grid, dim3(1024), 0, s, Lin, g, Lout, pitch, w, h, st, zs);
+  } else if (variant == 14) {
+    dim3 grid((w + 31) / 32, (h + 31) / 32, nz);
+    hipLaunchKernelGGL((k_fed_multi<64, 64, 1024, 16>), grid, dim3(1024), 0, s, Lin, g, Lout, pitch, w, h, st, zs);
+  } else if (variant == 15) {
+    dim3 grid((w + 95) / 96, (h + 31) / 32, nz);
+    hipLaunchKernelGGL((k_fed_multi<128, 64, 1024, 16>), grid, dim3(1024), 0, s, Lin, g, Lout, pitch, w, h, st, zs);
   } else if (variant == 12) {
     dim3 grid((w + 47) / 48, (h + 47) / 48, nz);
     hipLaunchKernelGGL(k_fed_reg<64>, grid, dim3(64), 0, s, Lin, g, Lout, pitch, w, h, st, zs);
