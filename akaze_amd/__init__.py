@@ -44,6 +44,9 @@ class COptions(ctypes.Structure):
                 ("verbosity", ctypes.c_int), ("fed_steps_per_launch", ctypes.c_int)]
 
 
+_OPTION_FIELDS = frozenset(f[0] for f in COptions._fields_)
+
+
 class CTiming(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in
                 ("kcontrast", "scale", "derivatives", "detector", "extrema", "subpixel", "descriptor")]
@@ -237,8 +240,10 @@ class AKAZEOptions:
     def __setattr__(self, k, v):
         if k == "_c":
             object.__setattr__(self, k, v)
-        else:
+        elif k in _OPTION_FIELDS:
             setattr(self._c, k, v)
+        else:
+            raise AttributeError("unknown AKAZE option %r" % k)
 
     def setWidth(self, w):
         self._c.img_width = w

@@ -128,3 +128,10 @@ def test_committed_proposal_weights_are_well_formed():
     rec = json.load(open(os.path.join(here, "proposal_1080p.json")))
     assert 0.0 < rec["threshold"] < 1.0
     assert rec["heldout_kp_recall"] >= 0.98 and rec["heldout_images"] >= 2000 and rec["train_images"] >= 20000
+
+
+def test_unknown_option_names_are_rejected():
+    with pytest.raises(AttributeError):
+        A.AKAZEOptions(max_keypoints=10)
+    o = A.AKAZEOptions(maxkeypoints=10)
+    assert o.maxkeypoints == 10
