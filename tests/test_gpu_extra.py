@@ -88,3 +88,58 @@ def test_cpp_api_runs_on_the_gpu(gpu_lib, tmp_path):
     img = A.to_float(A.synth_image(3, 640, 480))
     okp, _ = O.Pipeline(640, 480).run(img)
     assert tag == "ok" and int(nk) == len(okp) and int(nm) == len(okp)
+
+
+@pytest.mark.parametrize("factor", [2.5, 3.0])
+def test_large_derivative_steps_use_the_unfused_detector(gpu_lib, factor):
+    """derivative_factor 2.5 / 3.0 give Hessian steps of 4..8 pixels, beyond
+    the fused detector's LDS halo: the separate derivative, determinant and
+    extrema kernels run, and match the oracle level by level."""
+    A = gpu_lib
+    W, H = 640, 480
+    img = A.to_float(A.synth_image(12, W, H))
+    ak = A.AKAZE(A.AKAZEOptions(img_width=W, img_height=H, derivative_factor=factor))
+    ak.Create_Nonlinear_Scale_Space(img)
+    n = ak.Feature_Detection()
+    orc = O.Pipeline(W, H, derivative_factor=factor)
+    orc.scale_space(img)
+    on = orc.detect()
+    for lvl in range(ak.nlev):
+        for which in (2, 3, 4):
+            assert np.array_equal(ak.read_level(lvl, which), orc.level(lvl, which)), (lvl, which)
+    cand, ls, waves = ak.read_candidates()
+    assert cand.tobytes() == orc.candidates().tobytes() and waves == orc.waves()
+    kps, desc = ak.Compute_Descriptors()
+    okp = orc.keypoints()
+    odesc = orc.describe()
+    assert n == on and kps.tobytes() == orc.keypoints().tobytes() and np.array_equal(desc, odesc)
+
+
+def test_proposal_in_graph_batches_equals_eager(gpu_lib):
+    """A learned-proposal batch context replayed as a HIP graph gives, image
+    by image, the eager single-image result with the same network."""
+    A = gpu_lib
+    from akaze_amd.proposal import ProposalNet
+    here = os.path.join(REPO, "akaze_amd")
+    net = ProposalNet()
+    net.load(os.path.join(here, "proposal_1080p.f32"))
+    W, H, thr = 640, 480, 0.3
+    imgs = [A.to_float(A.synth_image(40 + i, W, H)) for i in range(3)]
+    ref = []
+    one = A.BatchAKAZE(A.AKAZEOptions(img_width=W, img_height=H), 1)
+    one.set_proposal(net, thr)
+    for im in imgs:
+        one.upload([im])
+        one.process_async(1)
+        c, _ = one.counts(1)
+        ref.append(one.read_results(0, int(c[0]))[:2])
+    bk = A.BatchAKAZE(A.AKAZEOptions(img_width=W, img_height=H), 3)
+    bk.set_proposal(net, thr)
+    bk.set_graph_mode(True)
+    bk.upload(imgs)
+    for rep in range(3):
+        bk.process_async(3)
+        c, _ = bk.counts(3)
+        for z in range(3):
+            kps, desc, n = bk.read_results(z, int(c[z]))
+            assert kps.tobytes() == ref[z][0].tobytes() and np.array_equal(desc, ref[z][1]), (rep, z)
