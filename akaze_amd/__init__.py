@@ -78,6 +78,7 @@ SIGNATURES = {
     "akz_read_results": [_P, _P, _P, _I, _P],
     "akz_input_buffer": [_P, _P, _P],
     "akz_set_graph_mode": [_P, _I],
+    "akz_set_keep_determinant": [_P, _I],
     "akz_read_level": [_P, _I, _I, _P],
     "akz_read_kcontrast": [_P, _I, _P],
     "akz_read_candidates": [_P, _P, _I, _P, _P, _P],
@@ -488,6 +489,11 @@ class BatchAKAZE:
     def set_fed_variants(self, v):
         v = np.ascontiguousarray(v, np.int32)
         _check(lib().akz_set_fed_variants(self.ctx, v.ctypes.data), "akz_set_fed_variants")
+
+    def set_keep_determinant(self, keep):
+        """Write full determinant images in process_async runs (for read_level(.., 4)
+        and extract); by default only the candidates' neighbourhoods are written."""
+        _check(lib().akz_set_keep_determinant(self.ctx, int(bool(keep))), "akz_set_keep_determinant")
 
     def set_proposal(self, net, threshold=0.5):
         """Detect with the learned tile proposal `net` (None: full scan)."""

@@ -115,6 +115,8 @@ struct akz_ctx {
   size_t zs = 0;
   int last_nz = 1;            // images in the last processed batch
   int status_synced = 0;      // images whose status blocks are in h_small and current
+  bool keep_det = false;      // throughput path: write full determinant images
+  bool det_full_last = true;  // the last detection wrote full determinant images
   int32_t* h_small = nullptr;  // pinned copy of the status blocks (4 ints per image)
   uint8_t* h_stage = nullptr;  // pinned staging for keypoints + 64-byte descriptor rows
   size_t h_stage_bytes = 0;
@@ -445,11 +447,16 @@ void enqueue_scale_space(akz_ctx* c, const float* img, int in_pitch, size_t in_z
 }
 
 // Feature_Detection (AKAZE.cpp:316-378) + FilterExtrema (cuda_akaze.cu:1318-1402).
-void enqueue_detect(akz_ctx* c, int nz) {
+// full_det: write every determinant pixel (parity readback, tile features);
+// otherwise only the candidates' 3x3 neighbourhoods the subpixel fit reads.
+void enqueue_detect(akz_ctx* c, int nz, bool full_det) {
   hipStream_t s = c->stream;
   c->status_synced = 0;
+  full_det = full_det || c->prop_net != nullptr;
+  c->det32.det_full = full_det ? 1 : 0;
   int smax = 0;
   for (int i = 0; i < c->nlev; i++) smax = std::max(smax, c->lev[i].deriv_step);
+  c->det_full_last = full_det || smax > akz_detect_max_step();  // the unfused kernels write all of it
   if (smax <= akz_detect_max_step()) {
     akz_launch_detect_fused(s, c->det32, c->db, nz);
     record(c, EV_DERIV);
@@ -699,7 +706,7 @@ akz_status akz_build_scale_space_device(akz_ctx* c, const float* d_img, size_t p
 akz_status akz_detect(akz_ctx* c, int* n_out) {
   if (!c) return AKZ_ERR_ARG;
   if (c->state < 1) return AKZ_ERR_STATE;
-  enqueue_detect(c, 1);
+  enqueue_detect(c, 1, true);  // the reference API keeps every level image readable
   AKZ_HIP_TRY(hipGetLastError());
   akz_status st = sync_status(c, 1);
   if (st != AKZ_OK) return st;
@@ -781,7 +788,7 @@ akz_status akz_process_batch_device_async(akz_ctx* c, const float* d_imgs, size_
   c->status_synced = 0;  // a graph replay rewrites the status blocks too
   if (!c->graph_mode) {
     enqueue_scale_space(c, d_imgs, (int)(sp / sizeof(float)), is, n);
-    enqueue_detect(c, n);
+    enqueue_detect(c, n, c->keep_det);
     enqueue_describe(c, n);
     AKZ_HIP_TRY(hipGetLastError());
   } else {
@@ -797,7 +804,7 @@ akz_status akz_process_batch_device_async(akz_ctx* c, const float* d_imgs, size_
       AKZ_HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
       c->capturing = true;
       enqueue_scale_space(c, c->img_in, L0.pitch, c->zs, n);
-      enqueue_detect(c, n);
+      enqueue_detect(c, n, c->keep_det);
       enqueue_describe(c, n);
       c->capturing = false;
       hipError_t e = hipStreamEndCapture(c->stream, &c->graph[n - 1]);
@@ -826,6 +833,14 @@ akz_status akz_batch_input_buffer(akz_ctx* c, float** d_ptr, size_t* pitch_bytes
 
 akz_status akz_input_buffer(akz_ctx* c, float** d_ptr, size_t* pitch_bytes) {
   return akz_batch_input_buffer(c, d_ptr, pitch_bytes, nullptr);
+}
+
+akz_status akz_set_keep_determinant(akz_ctx* c, int keep) {
+  if (!c) return AKZ_ERR_ARG;
+  AKZ_HIP_TRY(hipStreamSynchronize(c->stream));
+  c->keep_det = keep != 0;
+  drop_graphs(c);
+  return AKZ_OK;
 }
 
 akz_status akz_set_graph_mode(akz_ctx* c, int enabled) {
@@ -861,6 +876,7 @@ akz_status akz_read_level(akz_ctx* c, int lvl, int which, float* dst) {
 akz_status akz_read_batch_level(akz_ctx* c, int z, int lvl, int which, float* dst) {
   if (!c || !dst || lvl < 0 || lvl >= c->nlev || which < 0 || which > 4 || z < 0 || z >= c->batch)
     return AKZ_ERR_ARG;
+  if (which == 4 && !c->det_full_last) return AKZ_ERR_STATE;  // see akz_set_keep_determinant
   AKZ_HIP_TRY(hipStreamSynchronize(c->stream));
   const Level& L = c->lev[lvl];
   const float* src = which == 0 ? L.Lt : which == 1 ? (lvl == 0 ? L.Lt : L.Lsm) : which == 2 ? L.Lx
@@ -924,7 +940,7 @@ akz_status akz_profile_fed_batch(akz_ctx* c, const float* d_imgs, size_t pitch_b
   c->fed_profile = true;
   enqueue_scale_space(c, d_imgs, (int)(sp / sizeof(float)), is, n);
   c->fed_profile = false;
-  enqueue_detect(c, n);
+  enqueue_detect(c, n, c->keep_det);
   enqueue_describe(c, n);
   AKZ_HIP_TRY(hipGetLastError());
   AKZ_HIP_TRY(hipStreamSynchronize(c->stream));
@@ -960,7 +976,7 @@ akz_status akz_prop_positions(const akz_ctx* c, int* per_image) {
 
 akz_status akz_prop_extract(akz_ctx* c, int n, uint16_t* d_feat, uint8_t* d_labels) {
   if (!c || !d_feat || n < 1 || n > c->batch) return AKZ_ERR_ARG;
-  if (c->state < 2 || n > c->last_nz) return AKZ_ERR_STATE;
+  if (c->state < 2 || n > c->last_nz || !c->det_full_last) return AKZ_ERR_STATE;
   akz_status st = ensure_prop_act(c);
   if (st != AKZ_OK) return st;
   akz_prop_act act = c->pact;

@@ -272,14 +272,14 @@ __global__ __launch_bounds__(512) void k_detect_fused(akz_det_params p, int32_t*
     const int y = y0 + r;
     if (y >= h) break;
     bool pred = false;
+    const int id = (r + 1) * DF_RW + lane + 1;
+    const size_t o = (size_t)y * pitch + x;
     if (x < w) {
       const int i2 = (r + H2) * DF_RW + lane + H2;
-      const int id = (r + 1) * DF_RW + lane + 1;
-      const size_t o = (size_t)y * pitch + x;
       Lx[o] = s_x[i2];
       Ly[o] = s_y[i2];
       const float v = s_d[id];
-      D[o] = v;
+      if (p.det_full) D[o] = v;
       int left_x = (int)(x - L.border + 0.5f) - 1;
       int right_x = (int)(x + L.border + 0.5f) + 1;
       int up_y = (int)(y - L.border + 0.5f) - 1;
@@ -289,6 +289,15 @@ __global__ __launch_bounds__(512) void k_detect_fused(akz_det_params p, int32_t*
         pred = v > p.thr && v > s_d[id - R - 1] && v > s_d[id + R + 1] && v > s_d[id + R - 1] &&
                v > s_d[id - R + 1] && v > s_d[id - 1] && v > s_d[id + 1] && v > s_d[id + R] && v > s_d[id - R];
       }
+    }
+    // without the full determinant image, the subpixel fit of k_scatter
+    // still finds the 3x3 neighbourhood of every candidate (all inside the
+    // image: candidates keep the detector border)
+    if (pred && !p.det_full) {
+#pragma unroll
+      for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+        for (int dx = -1; dx <= 1; dx++) D[o + (ptrdiff_t)dy * pitch + dx] = s_d[id + dy * DF_RW + dx];
     }
     uint64_t mask = __ballot(pred);
     int seg = L.seg_base + y * L.nseg_x + tx;

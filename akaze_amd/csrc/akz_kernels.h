@@ -53,6 +53,7 @@ struct akz_det_params {
   int seg_total;                       // row segments over all levels
   float thr;
   float fac1, fac2;
+  int det_full;                        // 1: write every determinant; 0: only candidates' 3x3
   size_t zs;                           // image stride of the batch in bytes
 };
 // Detector work arrays of image 0 (image z's sit z * zs bytes further).

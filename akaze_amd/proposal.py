@@ -87,6 +87,7 @@ class Dataset:
     def __init__(self, W, H, n, seed0, batch=16, log=None):
         self.W, self.H, self.n, self.seed0 = W, H, n, seed0
         self.ctx = BatchAKAZE(AKAZEOptions(img_width=W, img_height=H), batch)
+        self.ctx.set_keep_determinant(True)  # the features read the determinant images
         self.P = self.ctx.proposal_positions()
         self.feat = DeviceBuffer(n * self.P * 8)
         self.lab = DeviceBuffer(n * self.P)

@@ -128,6 +128,13 @@ akz_status akz_input_buffer(akz_ctx* ctx, float** d_ptr, size_t* pitch_bytes);
 /* Capture the asynchronous pipeline into a HIP graph on the next call (1) or
  * run it as eager launches (0). */
 akz_status akz_set_graph_mode(akz_ctx* ctx, int enabled);
+/* The asynchronous pipeline (akz_process_*) writes only the determinant
+ * pixels its subpixel fit reads (the 3x3 neighbourhood of every candidate);
+ * keep = 1 makes it write the whole determinant image of every level, as
+ * akz_detect always does, for akz_read_level(..., 4, ...) and
+ * akz_prop_extract.  Both return AKZ_ERR_STATE after a run without it.  A
+ * context with a proposal network always writes them. */
+akz_status akz_set_keep_determinant(akz_ctx* ctx, int keep);
 
 /* ---- Batches of same-size images.  The reference processes one image per
  * AKAZE object and call (AKAZE.cpp:211-449); these entry points have no

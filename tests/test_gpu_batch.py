@@ -129,3 +129,28 @@ def test_every_fed_variant_and_the_autotuner(gpu_lib):
     bk.upload([img, img])
     bk.process_async(2)
     check_image(bk, 1, okp, odesc, "autotuned %s" % list(chosen))
+
+
+def test_determinant_images_on_request(gpu_lib):
+    """The asynchronous path writes only the candidates' determinant
+    neighbourhoods unless asked: the keypoints are the same either way, the
+    full images come back when kept, and a readback that would see partial
+    images is refused."""
+    A = gpu_lib
+    W, H = 640, 480
+    img = A.to_float(A.synth_image(95, W, H))
+    orc = O.Pipeline(W, H)
+    okp, odesc = orc.run(img)
+    bk = A.BatchAKAZE(A.AKAZEOptions(img_width=W, img_height=H), 1)
+    bk.upload([img])
+    for keep in (False, True, False):
+        bk.set_keep_determinant(keep)
+        bk.process_async(1)
+        check_image(bk, 0, okp, odesc, "keep %s" % keep)
+        if keep:
+            for lvl in range(bk.nlev):
+                assert np.array_equal(bk.read_level(0, lvl, 4), orc.level(lvl, 4)), lvl
+        else:
+            with pytest.raises(A.AkazeError) as e:
+                bk.read_level(0, 0, 4)
+            assert e.value.status == 5

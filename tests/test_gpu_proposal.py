@@ -29,6 +29,7 @@ def test_features_and_labels_match_oracle(gpu_lib):
     W, H = 640, 480
     imgs = [A.to_float(A.synth_image(21 + i, W, H)) for i in range(2)]
     bk = A.BatchAKAZE(A.AKAZEOptions(img_width=W, img_height=H), 2)
+    bk.set_keep_determinant(True)
     bk.upload(imgs)
     bk.process_async(2)
     P = bk.proposal_positions()
@@ -198,6 +199,7 @@ def test_trained_network_recall_and_gain(gpu_lib):
     W, H, B, n = 1920, 1080, 16, 32
     seed0 = 10_000_000 + 50_000
     bk = A.BatchAKAZE(A.AKAZEOptions(img_width=W, img_height=H), B)
+    bk.set_keep_determinant(True)
     P = bk.proposal_positions()
     feat = A.DeviceBuffer(B * P * 8)
     full, prop, rule_kept, net_kept = [], [], [], []
