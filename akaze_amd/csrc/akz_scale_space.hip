@@ -253,13 +253,15 @@ __global__ __launch_bounds__(1024) void k_contrast_finish(const int32_t* __restr
 // Lsmooth and conductance of one level in one pass: the sigma=1 Gaussian of
 // LowPass (5 taps, replicate border, row pass then column pass) fused with
 // Flow (unnormalised Scharr of Lsmooth, reflect-101, then the diffusivity),
-// AKAZE.cpp:260-261.  The tile keeps Lsmooth for its 64x16 pixels plus a
+// AKAZE.cpp:260-261.  The tile keeps Lsmooth for its 64x32 pixels plus a
 // one-pixel ring in LDS; flow neighbours are addressed through their
 // reflected image coordinates, which always fall inside that region.
-__global__ __launch_bounds__(256) void k_smooth_flow(const float* __restrict__ in, int pitch,
-                                                     float* __restrict__ lsm, float* __restrict__ g, int w, int h,
-                                                     int type, const float* kcp, akz_taps taps, size_t zs) {
-  constexpr int TW = 64, TH = 16, R = 2;
+constexpr int SF_TH = 32, SF_NT = 512;  // smooth+flow tile rows and threads (64 columns)
+
+__global__ __launch_bounds__(SF_NT) void k_smooth_flow(const float* __restrict__ in, int pitch,
+                                                       float* __restrict__ lsm, float* __restrict__ g, int w, int h,
+                                                       int type, const float* kcp, akz_taps taps, size_t zs) {
+  constexpr int TW = 64, TH = SF_TH, R = 2, NT = SF_NT;
   constexpr int AW = TW + 2 + 2 * R, AH = TH + 2 + 2 * R;  // input region
   constexpr int CW = TW + 2, CH = TH + 2;                  // Lsmooth region
   __shared__ float A[AH][AW];
@@ -271,9 +273,9 @@ __global__ __launch_bounds__(256) void k_smooth_flow(const float* __restrict__ i
   lsm = akz_zoff(lsm, zs);
   g = akz_zoff(g, zs);
   kcp = akz_zoff(kcp, zs);
-  load_window<AH, AW, 256>(&A[0][0], in, pitch, x0 - 1 - R, y0 - 1 - R, w, h);
+  load_window<AH, AW, NT>(&A[0][0], in, pitch, x0 - 1 - R, y0 - 1 - R, w, h);
   __syncthreads();
-  for (int idx = tid; idx < AH * CW; idx += 256) {
+  for (int idx = tid; idx < AH * CW; idx += NT) {
     int r = idx / CW, c = idx - r * CW;
     float sum = 0.0f;
 #pragma unroll
@@ -281,7 +283,7 @@ __global__ __launch_bounds__(256) void k_smooth_flow(const float* __restrict__ i
     B[r][c] = sum;
   }
   __syncthreads();
-  for (int idx = tid; idx < CH * CW; idx += 256) {
+  for (int idx = tid; idx < CH * CW; idx += NT) {
     int r = idx / CW, c = idx - r * CW;
     float sum = 0.0f;
 #pragma unroll
@@ -295,7 +297,7 @@ __global__ __launch_bounds__(256) void k_smooth_flow(const float* __restrict__ i
   if (x >= w) return;
   const int cx = x - x0 + 1;
   const int cxm = akz_reflect101(x - 1, w) - x0 + 1, cxp = akz_reflect101(x + 1, w) - x0 + 1;
-  for (int r = tid >> 6; r < TH; r += 4) {
+  for (int r = tid >> 6; r < TH; r += NT / 64) {
     const int y = y0 + r;
     if (y >= h) break;
     const int cy = r + 1;
@@ -618,8 +620,8 @@ void akz_launch_contrast(hipStream_t s, const float* blur, int pitch, float* gra
 
 void akz_launch_smooth_flow(hipStream_t s, const float* in, int pitch, float* lsm, float* g, int w, int h, int type,
                             const float* kc, const akz_taps& taps, akz_batch bt) {
-  hipLaunchKernelGGL(k_smooth_flow, dim3((w + 63) / 64, (h + 15) / 16, bt.nz), dim3(256), 0, s, in, pitch, lsm, g, w,
-                     h, type, kc, taps, bt.zs);
+  hipLaunchKernelGGL(k_smooth_flow, dim3((w + 63) / 64, (h + SF_TH - 1) / SF_TH, bt.nz), dim3(SF_NT), 0, s, in,
+                     pitch, lsm, g, w, h, type, kc, taps, bt.zs);
 }
 
 void akz_launch_fed_step(hipStream_t s, const float* Lin, const float* g, float* Lout, int pitch, int w, int h,
