@@ -116,17 +116,21 @@ __global__ __launch_bounds__(256) void k_extrema(akz_det_params p, int32_t* __re
 // Derivate + HessianDeterminant + FindExtrema (cuda_akaze.cu:651-774) fused
 // per 64x32 tile of every level in one launch.  Lsmooth is loaded with a
 // (2s+1)-pixel halo, Lx and Ly are computed on an (s+1)-pixel halo and the
-// determinant on a one-pixel ring, all in LDS (rows padded to 96 floats, so a
-// wavefront's lanes walk one row and nothing is divided).  The determinant
-// reuses the Lsmooth buffer, dead once the first derivatives exist, which
-// keeps a block at 51.5 KB of LDS: three blocks per CU.  Every stencil
+// determinant on a one-pixel ring, all in LDS (rows padded to 97 floats: a
+// wavefront's lanes walk one row, the few columns past 64 are walked down).
+// The determinant reuses the Lsmooth buffer, dead once the first derivatives
+// exist, which keeps a block at 52 KB of LDS: three blocks per CU.  Tiles
+// whose halo stays inside the image load without clamps or bounds tests.  Every stencil
 // addresses its reflect-101 neighbours through image coordinates, which stay
 // inside the loaded regions.  Lx, Ly and det are written for the tile's own
 // pixels and each 64-pixel row is one wavefront whose ballot gives the
 // candidate count and ranks, as in k_extrema.
 constexpr int DF_SMAX = 4, DF_TH = 32;
 constexpr int DF_H1 = 2 * DF_SMAX + 1, DF_H2 = DF_SMAX + 1;
-constexpr int DF_RW = 96;                       // LDS row stride, >= 64 + 2*DF_H1
+// LDS row stride, >= 64 + 2*DF_H1 and odd: the halo columns beyond the
+// first 64 are walked down their columns, 32 consecutive rows of one column
+// then fall into 32 different banks
+constexpr int DF_RW = 97;
 constexpr int DF_R1 = DF_TH + 2 * DF_H1, DF_R2 = DF_TH + 2 * DF_H2, DF_RD = DF_TH + 2;
 static_assert(DF_RD <= DF_R1, "the determinant fits the Lsmooth buffer");
 
@@ -148,7 +152,7 @@ __device__ __forceinline__ void detect_interior(const float* s_l, float* s_x, fl
     s_y[r * R + c] = fac1 * (lr + ll - ur - ul) + fac2 * (lc - uc);
   };
   for (int r = wv; r < R2n; r += NW) deriv(r, lane);
-  for (int k = threadIdx.x; k < R2n * E2; k += 64 * NW) deriv(k / E2, 64 + k % E2);
+  for (int k = threadIdx.x; k < R2n * E2; k += 64 * NW) deriv(k % R2n, 64 + k / R2n);
   __syncthreads();
   auto det = [&](int r, int c) {
     const float* bx = s_x + r * R + c;
@@ -164,7 +168,7 @@ __device__ __forceinline__ void detect_interior(const float* s_l, float* s_x, fl
     s_d[r * R + c] = lxx * lyy - lyx * lyx;
   };
   for (int r = wv; r < DF_RD; r += NW) det(r, lane);
-  for (int k = threadIdx.x; k < DF_RD * 2; k += 64 * NW) det(k >> 1, 64 + (k & 1));
+  for (int k = threadIdx.x; k < DF_RD * 2; k += 64 * NW) det(k % DF_RD, 64 + k / DF_RD);
 }
 
 __global__ __launch_bounds__(512) void k_detect_fused(akz_det_params p, int32_t* __restrict__ seg_count,
@@ -186,9 +190,31 @@ __global__ __launch_bounds__(512) void k_detect_fused(akz_det_params p, int32_t*
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   constexpr int NW = 8;
   const float* lsm = akz_zoff(p.Lsm[l], p.zs);
+  const bool interior = x0 >= H1 && x0 + 64 + H1 <= w && y0 >= H1 && y0 + DF_TH + H1 <= h;
   // Lsmooth rows [y0-H1, y0+TH+H1), columns [x0-H1, x0+64+H1): every load of
   // a thread is issued before any is written to LDS
-  {
+  if (interior) {
+    constexpr int ROWS = (DF_TH + 2 * DF_H1 + NW - 1) / NW;
+    const int R1n = DF_TH + 2 * H1;
+    const float* base = lsm + (size_t)(y0 - H1) * pitch + (x0 - H1);
+    float v[ROWS][2];
+#pragma unroll
+    for (int k = 0; k < ROWS; k++) {
+      const int r = wv + k * NW;
+      if (r < R1n) {
+        v[k][0] = base[r * pitch + lane];
+        if (lane < W1 - 64) v[k][1] = base[r * pitch + 64 + lane];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < ROWS; k++) {
+      const int r = wv + k * NW;
+      if (r < R1n) {
+        s_l[r * DF_RW + lane] = v[k][0];
+        if (lane < W1 - 64) s_l[r * DF_RW + 64 + lane] = v[k][1];
+      }
+    }
+  } else {
     constexpr int ROWS = (DF_TH + 2 * DF_H1 + NW - 1) / NW, COLS = 2;
     float v[ROWS][COLS];
 #pragma unroll
@@ -213,7 +239,6 @@ __global__ __launch_bounds__(512) void k_detect_fused(akz_det_params p, int32_t*
   }
   __syncthreads();
   const float fac1 = p.fac1, fac2 = p.fac2;
-  const bool interior = x0 >= H1 && x0 + 64 + H1 <= w && y0 >= H1 && y0 + DF_TH + H1 <= h;
   if (interior && s >= 2 && s <= 4) {
     if (s == 2) detect_interior<2>(s_l, s_x, s_y, s_d, fac1, fac2, wv, lane);
     else if (s == 3) detect_interior<3>(s_l, s_x, s_y, s_d, fac1, fac2, wv, lane);
@@ -238,7 +263,7 @@ __global__ __launch_bounds__(512) void k_detect_fused(akz_det_params p, int32_t*
     };
     const int R2n = DF_TH + 2 * H2, E2 = 2 * H2;
     for (int r = wv; r < R2n; r += NW) deriv_at(r, lane);
-    for (int k = threadIdx.x; k < R2n * E2; k += 64 * NW) deriv_at(k / E2, 64 + k % E2);
+    for (int k = threadIdx.x; k < R2n * E2; k += 64 * NW) deriv_at(k % R2n, 64 + k / R2n);
     __syncthreads();
     // determinant on rows [y0-1, y0+TH+1), columns [x0-1, x0+65)
     auto det_at = [&](int r, int c) {
@@ -261,30 +286,29 @@ __global__ __launch_bounds__(512) void k_detect_fused(akz_det_params p, int32_t*
       s_d[r * DF_RW + c] = lxx * lyy - lyx * lyx;
     };
     for (int r = wv; r < DF_RD; r += NW) det_at(r, lane);
-    for (int k = threadIdx.x; k < DF_RD * 2; k += 64 * NW) det_at(k >> 1, 64 + (k & 1));
+    for (int k = threadIdx.x; k < DF_RD * 2; k += 64 * NW) det_at(k % DF_RD, 64 + k / DF_RD);
   }
   __syncthreads();
   const int x = x0 + lane;
   float* Lx = akz_zoff(p.Lx[l], p.zs);
   float* Ly = akz_zoff(p.Ly[l], p.zs);
   float* D = akz_zoff(p.det[l], p.zs);
+  // the detector border test splits into a column part and a row part
+  const bool col_in = (int)(x - L.border + 0.5f) - 1 >= 0 && (int)(x + L.border + 0.5f) + 1 < w;
   for (int r = wv; r < DF_TH; r += NW) {
     const int y = y0 + r;
     if (y >= h) break;
     bool pred = false;
     const int id = (r + 1) * DF_RW + lane + 1;
     const size_t o = (size_t)y * pitch + x;
+    const bool row_in = (int)(y - L.border + 0.5f) - 1 >= 0 && (int)(y + L.border + 0.5f) + 1 < h;
     if (x < w) {
       const int i2 = (r + H2) * DF_RW + lane + H2;
       Lx[o] = s_x[i2];
       Ly[o] = s_y[i2];
       const float v = s_d[id];
       if (p.det_full) D[o] = v;
-      int left_x = (int)(x - L.border + 0.5f) - 1;
-      int right_x = (int)(x + L.border + 0.5f) + 1;
-      int up_y = (int)(y - L.border + 0.5f) - 1;
-      int down_y = (int)(y + L.border + 0.5f) + 1;
-      if (!(left_x < 0 || right_x >= w || up_y < 0 || down_y >= h)) {
+      if (col_in && row_in) {
         const int R = DF_RW;
         pred = v > p.thr && v > s_d[id - R - 1] && v > s_d[id + R + 1] && v > s_d[id + R - 1] &&
                v > s_d[id - R + 1] && v > s_d[id - 1] && v > s_d[id + 1] && v > s_d[id + R] && v > s_d[id - R];

@@ -250,77 +250,6 @@ __global__ __launch_bounds__(1024) void k_contrast_finish(const int32_t* __restr
   }
 }
 
-// Lsmooth and conductance of one level in one pass: the sigma=1 Gaussian of
-// LowPass (5 taps, replicate border, row pass then column pass) fused with
-// Flow (unnormalised Scharr of Lsmooth, reflect-101, then the diffusivity),
-// AKAZE.cpp:260-261.  The tile keeps Lsmooth for its 64x32 pixels plus a
-// one-pixel ring in LDS; flow neighbours are addressed through their
-// reflected image coordinates, which always fall inside that region.
-constexpr int SF_TH = 32, SF_NT = 512;  // smooth+flow tile rows and threads (64 columns)
-
-__global__ __launch_bounds__(SF_NT) void k_smooth_flow(const float* __restrict__ in, int pitch,
-                                                       float* __restrict__ lsm, float* __restrict__ g, int w, int h,
-                                                       int type, const float* kcp, akz_taps taps, size_t zs) {
-  constexpr int TW = 64, TH = SF_TH, R = 2, NT = SF_NT;
-  constexpr int AW = TW + 2 + 2 * R, AH = TH + 2 + 2 * R;  // input region
-  constexpr int CW = TW + 2, CH = TH + 2;                  // Lsmooth region
-  __shared__ float A[AH][AW];
-  __shared__ float B[AH][CW];
-  __shared__ float C[CH][CW];
-  const int tid = threadIdx.x;
-  const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
-  in = akz_zoff(in, zs);
-  lsm = akz_zoff(lsm, zs);
-  g = akz_zoff(g, zs);
-  kcp = akz_zoff(kcp, zs);
-  load_window<AH, AW, NT>(&A[0][0], in, pitch, x0 - 1 - R, y0 - 1 - R, w, h);
-  __syncthreads();
-  for (int idx = tid; idx < AH * CW; idx += NT) {
-    int r = idx / CW, c = idx - r * CW;
-    float sum = 0.0f;
-#pragma unroll
-    for (int i = 0; i <= 2 * R; i++) sum += A[r][c + i] * taps.k[i];
-    B[r][c] = sum;
-  }
-  __syncthreads();
-  for (int idx = tid; idx < CH * CW; idx += NT) {
-    int r = idx / CW, c = idx - r * CW;
-    float sum = 0.0f;
-#pragma unroll
-    for (int i = 0; i <= 2 * R; i++) sum += B[r + i][c] * taps.k[i];
-    C[r][c] = sum;
-  }
-  __syncthreads();
-  const float kcontrast = *kcp;
-  const float invk = 1.0f / (kcontrast * kcontrast);
-  const int x = x0 + (tid & 63);
-  if (x >= w) return;
-  const int cx = x - x0 + 1;
-  const int cxm = akz_reflect101(x - 1, w) - x0 + 1, cxp = akz_reflect101(x + 1, w) - x0 + 1;
-  for (int r = tid >> 6; r < TH; r += NT / 64) {
-    const int y = y0 + r;
-    if (y >= h) break;
-    const int cy = r + 1;
-    const int cym = akz_reflect101(y - 1, h) - y0 + 1, cyp = akz_reflect101(y + 1, h) - y0 + 1;
-    const float v = C[cy][cx];
-    lsm[(size_t)y * pitch + x] = v;
-    float ul = C[cym][cxm], ur = C[cym][cxp], ll = C[cyp][cxm], lr = C[cyp][cxp];
-    float lx = 3.0f * (lr - ll + ur - ul) + 10.0f * (C[cy][cxp] - C[cy][cxm]);
-    float ly = 3.0f * (lr + ll - ur - ul) + 10.0f * (C[cyp][cx] - C[cym][cx]);
-    float dif2 = invk * (lx * lx + ly * ly);
-    float res;
-    if (type == 0)
-      res = expf(-dif2);
-    else if (type == 1)
-      res = 1.0f / (1.0f + dif2);
-    else if (type == 2)
-      res = 1.0f - (float)exp(-3.315 / (dif2 * dif2 * dif2 * dif2));
-    else
-      res = 1.0f / sqrtf(1.0f + dif2);
-    g[(size_t)y * pitch + x] = res;
-  }
-}
-
 // One FED step (NLDStep + NLDUpdate, cuda_akaze.cu:289-345), out of place:
 // L_out = L_in + (tau/2) * div(g grad L_in), neighbours clamped at the border.
 __global__ __launch_bounds__(256) void k_fed_step(const float* __restrict__ L, const float* __restrict__ g,
@@ -421,11 +350,14 @@ __global__ __launch_bounds__(NT) void k_fed_multi(const float* __restrict__ L, c
 // per step, never reaching the output tile.  At image borders the outward
 // neighbour is the pixel itself (NLDStep's clamping), which zeroes that
 // flux whatever the sum next to it.
-__device__ __forceinline__ float from_right(float v) {  // lane i receives lane i+1 (lane 63 keeps its own)
-  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x130, 0xf, 0xf, false));
+// The shifts fill the lane with no source with zero (one instruction, no
+// copy of an old value); both kernels that use them read that lane's result
+// only in their halo, which never reaches an output.
+__device__ __forceinline__ float from_right(float v) {  // lane i receives lane i+1 (lane 63 receives 0)
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xf, 0xf, true));
 }
-__device__ __forceinline__ float from_left(float v) {  // lane i receives lane i-1 (lane 0 keeps its own)
-  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x138, 0xf, 0xf, false));
+__device__ __forceinline__ float from_left(float v) {  // lane i receives lane i-1 (lane 0 receives 0)
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xf, 0xf, true));
 }
 
 template <int R, bool BORDER>
@@ -450,6 +382,11 @@ __device__ __forceinline__ void fed_reg_body(const float* __restrict__ L, const 
   const bool east_self = BORDER && gx >= w - 1, west_self = BORDER && gx <= 0;
   for (int s = 0; s < st.n; s++) {
     const float stepsize = st.half_tau[s];
+    // the empty asm makes ytop opaque in every step, so the per-row border
+    // tests stay scalar compares inside the loop instead of R hoisted
+    // 64-bit masks (which spill)
+    int ytop = y0;
+    if (BORDER) asm volatile("" : "+s"(ytop));
     float up = Lr[0];
 #pragma unroll
     for (int r = 1; r < R - 1; r++) {
@@ -459,8 +396,8 @@ __device__ __forceinline__ void fed_reg_body(const float* __restrict__ L, const 
       if (BORDER) {
         if (east_self) le = i0;
         if (west_self) lw = i0;
-        if (y0 + r <= 0) ln = i0;
-        if (y0 + r >= h - 1) ls = i0;
+        if (ytop + r <= 0) ln = i0;
+        if (ytop + r >= h - 1) ls = i0;
       }
       const float ww = from_left(Wh[r]);
       const float xpos = Wh[r] * (le - i0);
@@ -491,6 +428,112 @@ __global__ __launch_bounds__(64) void k_fed_reg(const float* __restrict__ L, con
   const bool interior = x0 >= 1 && x0 + 64 <= w - 1 && y0 >= 1 && y0 + R <= h - 1;
   if (interior) fed_reg_body<R, false>(L, g, out, pitch, w, h, st, x0, y0);
   else fed_reg_body<R, true>(L, g, out, pitch, w, h, st, x0, y0);
+}
+
+// Lsmooth and conductance of one level in one pass (AKAZE.cpp:260-261): the
+// sigma=1 Gaussian of LowPass (5 taps, replicate border, row pass then
+// column pass, sums in tap order from 0.0f) fused with Flow (unnormalised
+// Scharr of Lsmooth with reflect-101 neighbours, then the diffusivity of
+// TYPE), cuda_akaze.cu:56-128, 240-287.  One wavefront per region of 64
+// columns, one per lane, whose lanes 3..60 are output columns, and TH output
+// rows.  The TH + 6 input rows are loaded at once (coordinates clamped: the
+// replicate border), the row pass takes its horizontal taps by DPP
+// wavefront shifts, the column pass and the Scharr rows are registers.  At
+// the image's first and last row and column the outward Scharr neighbour is
+// the one on the other side (reflect-101).
+template <int TH, int TYPE, bool BORDER>
+__device__ __forceinline__ void smooth_flow_reg_body(const float* __restrict__ in, int pitch, float* __restrict__ lsm,
+                                                     float* __restrict__ g, int w, int h, float invk,
+                                                     const akz_taps& taps, int x0, int y0) {
+  constexpr int NA = TH + 6, NC = TH + 2;
+  const int lane = threadIdx.x;
+  const int gx = x0 + lane;
+  const int cx = BORDER ? akz_iclamp(gx, 0, w - 1) : gx;
+  float A[NA];
+#pragma unroll
+  for (int i = 0; i < NA; i++) {
+    const int gy = BORDER ? akz_iclamp(y0 - 3 + i, 0, h - 1) : y0 - 3 + i;
+    A[i] = in[(size_t)gy * pitch + cx];
+  }
+  const float k0 = taps.k[0], k1 = taps.k[1], k2 = taps.k[2], k3 = taps.k[3], k4 = taps.k[4];
+  // row pass: B[i] is row y0 - 3 + i, valid on lanes 2..61
+  float B[NA];
+#pragma unroll
+  for (int i = 0; i < NA; i++) {
+    const float m1 = from_left(A[i]), p1 = from_right(A[i]);
+    const float m2 = from_left(m1), p2 = from_right(p1);
+    float sum = 0.0f;
+    sum += m2 * k0;
+    sum += m1 * k1;
+    sum += A[i] * k2;
+    sum += p1 * k3;
+    sum += p2 * k4;
+    B[i] = sum;
+  }
+  // column pass: C[j] is row y0 - 1 + j; CL / CR hold its Scharr column
+  // neighbours x-1 / x+1 (reflected at the image's first and last column)
+  float C[NC], CL[NC], CR[NC];
+  const bool first_col = BORDER && gx == 0, last_col = BORDER && gx == w - 1;
+#pragma unroll
+  for (int j = 0; j < NC; j++) {
+    float sum = 0.0f;
+    sum += B[j] * k0;
+    sum += B[j + 1] * k1;
+    sum += B[j + 2] * k2;
+    sum += B[j + 3] * k3;
+    sum += B[j + 4] * k4;
+    C[j] = sum;
+    const float fl = from_left(sum), fr = from_right(sum);
+    CL[j] = first_col ? fr : fl;
+    CR[j] = last_col ? fl : fr;
+  }
+  const bool out_col = lane >= 3 && lane < 61 && gx < w;
+#pragma unroll
+  for (int r = 0; r < TH; r++) {
+    const int y = y0 + r;
+    if (BORDER && y >= h) break;
+    // rows y-1 and y+1, reflected at the image's first and last row
+    const bool top = BORDER && y == 0, bot = BORDER && y == h - 1;
+    const int im = top ? r + 2 : r, ip = bot ? r : r + 2;
+    const float ul = CL[im], ur = CR[im], ll = CL[ip], lr = CR[ip];
+    const float lx = 3.0f * (lr - ll + ur - ul) + 10.0f * (CR[r + 1] - CL[r + 1]);
+    const float ly = 3.0f * (lr + ll - ur - ul) + 10.0f * (C[ip] - C[im]);
+    const float dif2 = invk * (lx * lx + ly * ly);
+    float res;
+    if (TYPE == 0)
+      res = expf(-dif2);
+    else if (TYPE == 1)
+      res = 1.0f / (1.0f + dif2);
+    else if (TYPE == 2)
+      res = 1.0f - (float)exp(-3.315 / (dif2 * dif2 * dif2 * dif2));
+    else
+      res = 1.0f / sqrtf(1.0f + dif2);
+    if (out_col) {
+      const size_t o = (size_t)y * pitch + gx;
+      lsm[o] = C[r + 1];
+      g[o] = res;
+    }
+  }
+}
+
+template <int TH, int TYPE>
+__global__ __launch_bounds__(64) void k_smooth_flow_reg(const float* __restrict__ in, int pitch,
+                                                        float* __restrict__ lsm, float* __restrict__ g, int w, int h,
+                                                        const float* kcp, akz_taps taps, size_t zs) {
+  in = akz_zoff(in, zs);
+  lsm = akz_zoff(lsm, zs);
+  g = akz_zoff(g, zs);
+  kcp = akz_zoff(kcp, zs);
+  const float kcontrast = *kcp;
+  const float invk = 1.0f / (kcontrast * kcontrast);
+  const int x0 = blockIdx.x * 58 - 3, y0 = blockIdx.y * TH;
+  // every input row and column in the image, no output on its first or
+  // last row or column
+  const bool interior = x0 >= 0 && x0 + 64 <= w && y0 >= 3 && y0 + TH + 3 <= h;
+  if (interior)
+    smooth_flow_reg_body<TH, TYPE, false>(in, pitch, lsm, g, w, h, invk, taps, x0, y0);
+  else
+    smooth_flow_reg_body<TH, TYPE, true>(in, pitch, lsm, g, w, h, invk, taps, x0, y0);
 }
 
 // One FED step, LDS-tiled: a 64x16 output tile plus a one-pixel halo of L and
@@ -618,10 +661,28 @@ void akz_launch_contrast(hipStream_t s, const float* blur, int pitch, float* gra
                      zs);
 }
 
+template <int TH>
+static void launch_smooth_flow_reg(hipStream_t s, const float* in, int pitch, float* lsm, float* g, int w, int h, int type,
+                            const float* kc, const akz_taps& taps, akz_batch bt) {
+  const dim3 grid((w + 57) / 58, (h + TH - 1) / TH, bt.nz);
+  if (type == 0)
+    hipLaunchKernelGGL((k_smooth_flow_reg<TH, 0>), grid, dim3(64), 0, s, in, pitch, lsm, g, w, h, kc, taps, bt.zs);
+  else if (type == 1)
+    hipLaunchKernelGGL((k_smooth_flow_reg<TH, 1>), grid, dim3(64), 0, s, in, pitch, lsm, g, w, h, kc, taps, bt.zs);
+  else if (type == 2)
+    hipLaunchKernelGGL((k_smooth_flow_reg<TH, 2>), grid, dim3(64), 0, s, in, pitch, lsm, g, w, h, kc, taps, bt.zs);
+  else
+    hipLaunchKernelGGL((k_smooth_flow_reg<TH, 3>), grid, dim3(64), 0, s, in, pitch, lsm, g, w, h, kc, taps, bt.zs);
+}
+
 void akz_launch_smooth_flow(hipStream_t s, const float* in, int pitch, float* lsm, float* g, int w, int h, int type,
                             const float* kc, const akz_taps& taps, akz_batch bt) {
-  hipLaunchKernelGGL(k_smooth_flow, dim3((w + 63) / 64, (h + SF_TH - 1) / SF_TH, bt.nz), dim3(SF_NT), 0, s, in,
-                     pitch, lsm, g, w, h, type, kc, taps, bt.zs);
+  // 32-row regions where they still give a few wavefronts per SIMD
+  const long waves32 = (long)((w + 57) / 58) * ((h + 31) / 32) * bt.nz;
+  if (waves32 >= 4096)
+    launch_smooth_flow_reg<32>(s, in, pitch, lsm, g, w, h, type, kc, taps, bt);
+  else
+    launch_smooth_flow_reg<16>(s, in, pitch, lsm, g, w, h, type, kc, taps, bt);
 }
 
 void akz_launch_fed_step(hipStream_t s, const float* Lin, const float* g, float* Lout, int pitch, int w, int h,
