@@ -1026,7 +1026,7 @@ akz_status akz_prop_read_probs(akz_ctx* c, int z, float* probs) {
 // ---- autotuner ----
 akz_status akz_autotune(akz_ctx* c, int nz, int reps) {
   if (!c || nz < 1 || nz > c->batch || reps < 1) return AKZ_ERR_ARG;
-  static const int kVariants[] = {2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15};
+  static const int kVariants[] = {2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16};
   hipEvent_t a, b;
   AKZ_HIP_TRY(hipEventCreate(&a));
   AKZ_HIP_TRY(hipEventCreate(&b));
@@ -1091,7 +1091,7 @@ akz_status akz_get_fed_variants(const akz_ctx* c, int* variants) {
 akz_status akz_set_fed_variants(akz_ctx* c, const int* variants) {
   if (!c || !variants) return AKZ_ERR_ARG;
   for (int i = 0; i < c->nlev; i++)
-    if (variants[i] < 2 || variants[i] > 15) return AKZ_ERR_ARG;
+    if (variants[i] < 2 || variants[i] > AKZ_FED_VARIANT_MAX) return AKZ_ERR_ARG;
   AKZ_HIP_TRY(hipStreamSynchronize(c->stream));
   for (int i = 0; i < c->nlev; i++) c->fed_variant[i] = variants[i];
   drop_graphs(c);

@@ -53,8 +53,32 @@ struct akz_batch {
 
 #ifdef __HIPCC__
 template <class T>
+__device__ __forceinline__ T* akz_zoff(T* p, size_t zs, int z) {
+  return reinterpret_cast<T*>(reinterpret_cast<uintptr_t>(p) + (size_t)z * zs);
+}
+template <class T>
 __device__ __forceinline__ T* akz_zoff(T* p, size_t zs) {
-  return reinterpret_cast<T*>(reinterpret_cast<uintptr_t>(p) + (size_t)blockIdx.z * zs);
+  return akz_zoff(p, zs, (int)blockIdx.z);
+}
+
+// Workgroups of a launch are dispatched round-robin over MI355X's 8 XCDs,
+// each with its own L2: workgroup i (x fastest, then y, then z) runs on XCD
+// i % 8.  Renumbering them so that XCD k takes the k-th eighth of the tiles
+// in that raster order keeps neighbouring tiles, which read each other's
+// halo and share the cache lines at their edges, in one L2.  A bijection for
+// any grid; only the placement changes.
+struct akz_tile3 { int x, y, z; };
+__device__ __forceinline__ akz_tile3 akz_xcd_tile() {
+  const uint32_t gx = gridDim.x, gy = gridDim.y;
+  const uint32_t nb = gx * gy * gridDim.z;
+  const uint32_t id = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  const uint32_t q = nb / 8, r = nb % 8, xcd = id % 8, k = id / 8;
+  const uint32_t t = xcd < r ? xcd * (q + 1) + k : r * (q + 1) + (xcd - r) * q + k;
+  akz_tile3 o;
+  o.x = (int)(t % gx);
+  o.y = (int)((t / gx) % gy);
+  o.z = (int)(t / (gx * gy));
+  return o;
 }
 #endif
 

@@ -161,12 +161,12 @@ __global__ __launch_bounds__(CT_HIST_THREADS) void k_contrast_hist(const float* 
   __syncthreads();
   const float maxv = __uint_as_float(*maxbits);
   const float imaxval = 1.0f / maxv;
-  const int tiles_x = (w - 2 + 63) / 64, tiles_y = (h - 2 + NW - 1) / NW;
-  for (int t = blockIdx.x; t < tiles_x * tiles_y; t += gridDim.x) {
-    int x = 1 + (t % tiles_x) * 64 + lane;
-    int y = 1 + (t / tiles_x) * NW + wv;
-    if (x < w - 1 && y < h - 1) {
-      float v = nbins * grad[(size_t)y * pitch + x] * imaxval;
+  // each wavefront walks whole interior rows
+  const int nwaves = gridDim.x * NW;
+  for (int y = 1 + blockIdx.x * NW + wv; y < h - 1; y += nwaves) {
+    const float* row = grad + (size_t)y * pitch;
+    for (int x = 1 + lane; x < w - 1; x += 64) {
+      float v = nbins * row[x] * imaxval;
       int idx = (v != v) ? 0 : (int)v;
       idx = akz_imin(idx, nbins - 1);
       if (idx < 300) atomicAdd(&lh[idx * 64 + lane], 1);
@@ -275,15 +275,16 @@ __global__ __launch_bounds__(256) void k_fed_step(const float* __restrict__ L, c
 }
 
 // FED steps with temporal blocking.  A block holds an RW x RH region of L
-// and g in LDS: its output tile plus a K-pixel halo.  Step s updates the part
-// of the region still exact after s steps (it shrinks by one pixel per side
-// and step, except along image borders, where neighbours are clamped to the
-// pixel itself as in NLDStep), ping-ponging between two LDS buffers.  Per
-// pixel and step the arithmetic is k_fed_step's, so the result equals one
-// launch per step bit for bit.  RW and RH are powers of two, so the flattened
-// region index splits into row and column with shifts.  Region shapes per
-// level size are chosen from measurements (fed_config below).
-constexpr int FB_K = 8;      // steps per launch (and halo) of most variants
+// and g in LDS: its output tile plus a halo as wide as the launch's step
+// count n (at most K).  Step s updates the part of the region still exact
+// after s steps (it shrinks by one pixel per side and step, except along
+// image borders, where neighbours are clamped to the pixel itself as in
+// NLDStep), ping-ponging between two LDS buffers.  Per pixel and step the
+// arithmetic is k_fed_step's, so the result equals one launch per step bit
+// for bit.  RW and RH are powers of two, so the flattened region index
+// splits into row and column with shifts.  Region shapes per level size are
+// chosen from measurements (akz_fed_variant_for below).
+constexpr int FB_K = 8;      // steps per launch of most variants
 constexpr int FB_KMAX = 16;  // the deep-halo variants
 struct akz_fed_steps { float half_tau[FB_KMAX]; int n; };
 
@@ -291,17 +292,19 @@ template <int RW, int RH, int NT, int K = FB_K>
 __global__ __launch_bounds__(NT) void k_fed_multi(const float* __restrict__ L, const float* __restrict__ g,
                                                   float* __restrict__ out, int pitch, int w, int h,
                                                   akz_fed_steps st, size_t zs) {
-  constexpr int TW = RW - 2 * K, TH = RH - 2 * K;
-  L = akz_zoff(L, zs);
-  g = akz_zoff(g, zs);
-  out = akz_zoff(out, zs);
+  static_assert(RW > 2 * K && RH > 2 * K, "a tile remains inside the halo");
+  const akz_tile3 tl = akz_xcd_tile();
+  L = akz_zoff(L, zs, tl.z);
+  g = akz_zoff(g, zs, tl.z);
+  out = akz_zoff(out, zs, tl.z);
   constexpr int LW = (RW == 128) ? 7 : (RW == 64) ? 6 : (RW == 32 ? 5 : 4);
   static_assert((1 << LW) == RW, "power-of-two region width");
   __shared__ float sg[RH * RW];
   __shared__ float sa[RH * RW];
   __shared__ float sb[RH * RW];
   const int tid = threadIdx.x;
-  const int x0 = blockIdx.x * TW - K, y0 = blockIdx.y * TH - K;
+  const int hn = st.n;  // halo: one pixel per step
+  const int x0 = tl.x * (RW - 2 * hn) - hn, y0 = tl.y * (RH - 2 * hn) - hn;
   load_window<RH, RW, NT>(sa, L, pitch, x0, y0, w, h);
   load_window<RH, RW, NT>(sg, g, pitch, x0, y0, w, h);
   __syncthreads();
@@ -331,15 +334,16 @@ __global__ __launch_bounds__(NT) void k_fed_multi(const float* __restrict__ L, c
     src = dst;
     dst = t;
   }
-  for (int idx = tid; idx < TH * TW; idx += NT) {
-    const int r = K + idx / TW, c = K + idx % TW;
+  for (int idx = hn * RW + tid; idx < (RH - hn) * RW; idx += NT) {
+    const int r = idx >> LW, c = idx & (RW - 1);
     const int gx = x0 + c, gy = y0 + r;
-    if (gx < w && gy < h) out[(size_t)gy * pitch + gx] = src[r * RW + c];
+    if (c >= hn && c < RW - hn && gx < w && gy < h) out[(size_t)gy * pitch + gx] = src[idx];
   }
 }
 
 // FED steps with temporal blocking in registers: one wavefront per region of
-// 64 columns (one per lane) x R rows (R registers per lane), 8-pixel halo.
+// 64 columns (one per lane) x R rows (R registers per lane), with a halo as
+// wide as the launch's step count.
 // The conductance sums of each pixel pair are formed once per launch: Wh[r]
 // = g + g(east) per lane, Wv[r] = g(row r) + g(row r+1); the west and north
 // sums are the neighbours' east and south sums (IEEE addition commutes).
@@ -364,7 +368,6 @@ template <int R, bool BORDER>
 __device__ __forceinline__ void fed_reg_body(const float* __restrict__ L, const float* __restrict__ g,
                                              float* __restrict__ out, int pitch, int w, int h,
                                              const akz_fed_steps& st, int x0, int y0) {
-  constexpr int K = FB_K;
   const int lane = threadIdx.x;
   const int gx = x0 + lane;
   const int cx = BORDER ? akz_iclamp(gx, 0, w - 1) : gx;
@@ -408,22 +411,29 @@ __device__ __forceinline__ void fed_reg_body(const float* __restrict__ L, const 
       Lr[r] = i0 + stepsize * (xpos - xneg + ypos - yneg);
     }
   }
-  if (lane < K || lane >= 64 - K || gx >= w) return;
+  const int hn = st.n;
+  if (lane < hn || lane >= 64 - hn || gx >= w) return;
 #pragma unroll
-  for (int r = K; r < R - K; r++) {
+  for (int r = 1; r < R - 1; r++) {
     const int gy = y0 + r;
-    if (gy < h) out[(size_t)gy * pitch + gx] = Lr[r];
+    if (r >= hn && r < R - hn && gy < h) out[(size_t)gy * pitch + gx] = Lr[r];
   }
 }
 
+// Resident wavefronts per SIMD asked of the register allocator for 48 rows:
+// three (150 VGPRs without spills; left alone it takes 169, two waves).
+// 32 rows fit four by themselves, 64 rows only one.
 template <int R>
-__global__ __launch_bounds__(64) void k_fed_reg(const float* __restrict__ L, const float* __restrict__ g,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(R == 48 ? 3 : 1)))
+void k_fed_reg(const float* __restrict__ L, const float* __restrict__ g,
                                                 float* __restrict__ out, int pitch, int w, int h, akz_fed_steps st,
                                                 size_t zs) {
-  L = akz_zoff(L, zs);
-  g = akz_zoff(g, zs);
-  out = akz_zoff(out, zs);
-  const int x0 = blockIdx.x * (64 - 2 * FB_K) - FB_K, y0 = blockIdx.y * (R - 2 * FB_K) - FB_K;
+  const akz_tile3 tl = akz_xcd_tile();
+  L = akz_zoff(L, zs, tl.z);
+  g = akz_zoff(g, zs, tl.z);
+  out = akz_zoff(out, zs, tl.z);
+  const int hn = st.n;  // halo: one pixel per step
+  const int x0 = tl.x * (64 - 2 * hn) - hn, y0 = tl.y * (R - 2 * hn) - hn;
   // no pixel of the region is on or beyond an image border
   const bool interior = x0 >= 1 && x0 + 64 <= w - 1 && y0 >= 1 && y0 + R <= h - 1;
   if (interior) fed_reg_body<R, false>(L, g, out, pitch, w, h, st, x0, y0);
@@ -520,13 +530,14 @@ template <int TH, int TYPE>
 __global__ __launch_bounds__(64) void k_smooth_flow_reg(const float* __restrict__ in, int pitch,
                                                         float* __restrict__ lsm, float* __restrict__ g, int w, int h,
                                                         const float* kcp, akz_taps taps, size_t zs) {
-  in = akz_zoff(in, zs);
-  lsm = akz_zoff(lsm, zs);
-  g = akz_zoff(g, zs);
-  kcp = akz_zoff(kcp, zs);
+  const akz_tile3 tl = akz_xcd_tile();
+  in = akz_zoff(in, zs, tl.z);
+  lsm = akz_zoff(lsm, zs, tl.z);
+  g = akz_zoff(g, zs, tl.z);
+  kcp = akz_zoff(kcp, zs, tl.z);
   const float kcontrast = *kcp;
   const float invk = 1.0f / (kcontrast * kcontrast);
-  const int x0 = blockIdx.x * 58 - 3, y0 = blockIdx.y * TH;
+  const int x0 = tl.x * 58 - 3, y0 = tl.y * TH;
   // every input row and column in the image, no output on its first or
   // last row or column
   const bool interior = x0 >= 0 && x0 + 64 <= w && y0 >= 3 && y0 + TH + 3 <= h;
@@ -647,12 +658,15 @@ void akz_launch_contrast(hipStream_t s, const float* blur, int pitch, float* gra
   for (int i = 0; i < nlev; i++) o.oct[i] = octaves_host[i];
   const int nz = bt.nz;
   const size_t zs = bt.zs;
+  // the histogram's blocks, which each clear and reduce 77 KB of LDS, are
+  // spread over the images of a batch; each wavefront gets a row at least
+  const int per = nz > 0 ? nz : 1;
   int gtiles = ((w - 2 + 63) / 64) * ((h - 2 + 3) / 4);
   int gb = gtiles < CT_GRAD_BLOCKS ? gtiles : CT_GRAD_BLOCKS;
   hipLaunchKernelGGL(k_contrast_grad, dim3(gb, 1, nz), dim3(256), 0, s, blur, pitch, grad, w, h, blockmax, zs);
   hipLaunchKernelGGL(k_contrast_max, dim3(1, 1, nz), dim3(1024), 0, s, blockmax, gb, maxbits, zs);
-  int htiles = ((w - 2 + 63) / 64) * ((h - 2 + CT_HIST_THREADS / 64 - 1) / (CT_HIST_THREADS / 64));
-  int hb = htiles < CT_HIST_BLOCKS ? htiles : CT_HIST_BLOCKS;
+  int hb = akz_imin((h - 2 + CT_HIST_THREADS / 64 - 1) / (CT_HIST_THREADS / 64),
+                    akz_imax(CT_HIST_BLOCKS / per, 32));
   hipLaunchKernelGGL(k_contrast_hist, dim3(hb, 1, nz), dim3(CT_HIST_THREADS), 0, s, grad, pitch, w, h, maxbits,
                      nbins, hist_part, zs);
   int32_t* bin_tot = hist_part + (size_t)512 * CT_HIST_BLOCKS;
@@ -698,12 +712,12 @@ int akz_fed_variant_steps(int variant) {
 }
 
 // Kernel variant per FED launch: 0 k_fed_step, 1 k_fed_tile; blocked regions
-// (tile = region minus the 8-pixel halo): 2 64x64/512 threads, 3 32x32/256,
-// 4 64x32/256, 5 32x32/1024, 6 64x64/1024, 7 64x48/512, 8 32x48/512,
-// 9 128x64/1024, 10 64x128/1024, 11 128x32/512; registers (one wavefront per
-// 64-column region): 12 64 rows, 13 48 rows; 16 steps per launch with a
-// 16-pixel halo (half the launches of the small, latency-bound levels):
-// 14 64x64/1024, 15 128x64/1024.
+// of up to 8 steps (tile = region minus a halo of one pixel per step of the
+// launch): 2 64x64/512 threads, 3 32x32/256, 4 64x32/256, 5 32x32/1024,
+// 6 64x64/1024, 7 64x48/512, 8 32x48/512, 9 128x64/1024, 10 64x128/1024,
+// 11 128x32/512; registers (one wavefront per 64-column region): 12 64 rows,
+// 13 48 rows, 16 32 rows; up to 16 steps per launch (half the launches of
+// the small, latency-bound levels): 14 64x64/1024, 15 128x64/1024.
 void akz_launch_fed_variant(hipStream_t s, int variant, const float* Lin, const float* g, float* Lout, int pitch,
                             int w, int h, const float* half_tau, int n, akz_batch bt) {
   const int nz = bt.nz;
@@ -726,68 +740,67 @@ void akz_launch_fed_variant(hipStream_t s, int variant, const float* Lin, const 
   if (n > akz_fed_variant_steps(variant)) n = akz_fed_variant_steps(variant);
   st.n = n;
   for (int i = 0; i < FB_KMAX; i++) st.half_tau[i] = i < n ? half_tau[i] : 0.0f;
+  // region (rw x rh) minus an n-pixel halo on each side is a block's tile
+  auto grid = [&](int rw, int rh) {
+    const int tw = rw - 2 * n, th = rh - 2 * n;
+    return dim3((w + tw - 1) / tw, (h + th - 1) / th, nz);
+  };
   if (variant == 2) {
-    dim3 grid((w + 47) / 48, (h + 47) / 48, nz);
-    hipLaunchKernelGGL((k_fed_multi<64, 64, 512>), grid, dim3(512), 0, s, Lin, g, Lout, pitch, w, h, st, zs);
+    hipLaunchKernelGGL((k_fed_multi<64, 64, 512>), grid(64, 64), dim3(512), 0, s, Lin, g, Lout, pitch, w, h, st, zs);
   } else if (variant == 3) {
-    dim3 grid((w + 15) / 16, (h + 15) / 16, nz);
-    hipLaunchKernelGGL((k_fed_multi<32, 32, 256>), grid, dim3(256), 0, s, Lin, g, Lout, pitch, w, h, st, zs);
+    hipLaunchKernelGGL((k_fed_multi<32, 32, 256>), grid(32, 32), dim3(256), 0, s, Lin, g, Lout, pitch, w, h, st, zs);
   } else if (variant == 5) {
-    dim3 grid((w + 15) / 16, (h + 15) / 16, nz);
-    hipLaunchKernelGGL((k_fed_multi<32, 32, 1024>), grid, dim3(1024), 0, s, Lin, g, Lout, pitch, w, h, st, zs);
+    hipLaunchKernelGGL((k_fed_multi<32, 32, 1024>), grid(32, 32), dim3(1024), 0, s, Lin, g, Lout, pitch, w, h, st,
+                       zs);
   } else if (variant == 6) {
-    dim3 grid((w + 47) / 48, (h + 47) / 48, nz);
-    hipLaunchKernelGGL((k_fed_multi<64, 64, 1024>), grid, dim3(1024), 0, s, Lin, g, Lout, pitch, w, h, st, zs);
+    hipLaunchKernelGGL((k_fed_multi<64, 64, 1024>), grid(64, 64), dim3(1024), 0, s, Lin, g, Lout, pitch, w, h, st,
+                       zs);
   } else if (variant == 7) {
-    dim3 grid((w + 47) / 48, (h + 31) / 32, nz);
-    hipLaunchKernelGGL((k_fed_multi<64, 48, 512>), grid, dim3(512), 0, s, Lin, g, Lout, pitch, w, h, st, zs);
+    hipLaunchKernelGGL((k_fed_multi<64, 48, 512>), grid(64, 48), dim3(512), 0, s, Lin, g, Lout, pitch, w, h, st, zs);
   } else if (variant == 8) {
-    dim3 grid((w + 15) / 16, (h + 31) / 32, nz);
-    hipLaunchKernelGGL((k_fed_multi<32, 48, 512>), grid, dim3(512), 0, s, Lin, g, Lout, pitch, w, h, st, zs);
+    hipLaunchKernelGGL((k_fed_multi<32, 48, 512>), grid(32, 48), dim3(512), 0, s, Lin, g, Lout, pitch, w, h, st, zs);
   } else if (variant == 9) {
-    dim3 grid((w + 111) / 112, (h + 47) / 48, nz);
-    hipLaunchKernelGGL((k_fed_multi<128, 64, 1024>), grid, dim3(1024), 0, s, Lin, g, Lout, pitch, w, h, st, zs);
+    hipLaunchKernelGGL((k_fed_multi<128, 64, 1024>), grid(128, 64), dim3(1024), 0, s, Lin, g, Lout, pitch, w, h, st,
+                       zs);
   } else if (variant == 10) {
-    dim3 grid((w + 47) / 48, (h + 111) / 112, nz);
-    hipLaunchKernelGGL((k_fed_multi<64, 128, 1024>), grid, dim3(1024), 0, s, Lin, g, Lout, pitch, w, h, st, zs);
+    hipLaunchKernelGGL((k_fed_multi<64, 128, 1024>), grid(64, 128), dim3(1024), 0, s, Lin, g, Lout, pitch, w, h, st,
+                       zs);
   } else if (variant == 14) {
-    dim3 grid((w + 31) / 32, (h + 31) / 32, nz);
-    hipLaunchKernelGGL((k_fed_multi<64, 64, 1024, 16>), grid, dim3(1024), 0, s, Lin, g, Lout, pitch, w, h, st, zs);
+    hipLaunchKernelGGL((k_fed_multi<64, 64, 1024, 16>), grid(64, 64), dim3(1024), 0, s, Lin, g, Lout, pitch, w, h, st,
+                       zs);
   } else if (variant == 15) {
-    dim3 grid((w + 95) / 96, (h + 31) / 32, nz);
-    hipLaunchKernelGGL((k_fed_multi<128, 64, 1024, 16>), grid, dim3(1024), 0, s, Lin, g, Lout, pitch, w, h, st, zs);
+    hipLaunchKernelGGL((k_fed_multi<128, 64, 1024, 16>), grid(128, 64), dim3(1024), 0, s, Lin, g, Lout, pitch, w, h,
+                       st, zs);
   } else if (variant == 12) {
-    dim3 grid((w + 47) / 48, (h + 47) / 48, nz);
-    hipLaunchKernelGGL(k_fed_reg<64>, grid, dim3(64), 0, s, Lin, g, Lout, pitch, w, h, st, zs);
+    hipLaunchKernelGGL(k_fed_reg<64>, grid(64, 64), dim3(64), 0, s, Lin, g, Lout, pitch, w, h, st, zs);
   } else if (variant == 13) {
-    dim3 grid((w + 47) / 48, (h + 31) / 32, nz);
-    hipLaunchKernelGGL(k_fed_reg<48>, grid, dim3(64), 0, s, Lin, g, Lout, pitch, w, h, st, zs);
+    hipLaunchKernelGGL(k_fed_reg<48>, grid(64, 48), dim3(64), 0, s, Lin, g, Lout, pitch, w, h, st, zs);
+  } else if (variant == 16) {
+    hipLaunchKernelGGL(k_fed_reg<32>, grid(64, 32), dim3(64), 0, s, Lin, g, Lout, pitch, w, h, st, zs);
   } else if (variant == 11) {
-    dim3 grid((w + 111) / 112, (h + 15) / 16, nz);
-    hipLaunchKernelGGL((k_fed_multi<128, 32, 512>), grid, dim3(512), 0, s, Lin, g, Lout, pitch, w, h, st, zs);
+    hipLaunchKernelGGL((k_fed_multi<128, 32, 512>), grid(128, 32), dim3(512), 0, s, Lin, g, Lout, pitch, w, h, st,
+                       zs);
   } else {
-    dim3 grid((w + 47) / 48, (h + 15) / 16, nz);
-    hipLaunchKernelGGL((k_fed_multi<64, 32, 256>), grid, dim3(256), 0, s, Lin, g, Lout, pitch, w, h, st, zs);
+    hipLaunchKernelGGL((k_fed_multi<64, 32, 256>), grid(64, 32), dim3(256), 0, s, Lin, g, Lout, pitch, w, h, st, zs);
   }
 }
 
-// Region shape by level size and images per launch, from timings of the
-// variants on MI355X (8 steps per launch).  One image, us per step:
-//   3840x2160: 64x48/512t 13.2, 64x64/512t 14.1;  1920x1080: 64x64/1024t 4.34;
-//   960x540: 64x64/1024t 1.70;  480x270: 32x32/1024t 0.97;  240x135: 0.73.
-// Batches of 16 1080p images, whole FED per call: register variant 48 rows
-// 1.27 ms, 64 rows 1.45 ms, 64x64/1024t 1.48 ms; per level the autotuner
-// picks registers down to 480x270 and 64x64/1024t at 240x135.
+// Region shape by level size and images per launch, from the autotuner's
+// choices on MI355X (halo = steps of the launch, XCD-aware tile placement).
+// One 1920x1080 image: 32-row registers at 1920x1080, 64x64/1024t at
+// 960x540, 32x32/1024t below.  Batches of 16: 48-row registers at
+// 1920x1080, 32-row registers at 960x540 and 480x270, 64x64/1024t at
+// 240x135.
 int akz_fed_variant_for(int w, int h, int nz) {
   const size_t px = (size_t)w * h;
   if (nz > 1) {
     const size_t tot = px * (size_t)nz;
-    if (tot >= 4000000) return 13;
-    if (tot >= 1000000) return 12;
+    if (tot >= 16000000) return 13;
+    if (tot >= 1000000) return 16;
     if (tot >= 300000) return 6;
     return 5;
   }
-  if (px >= 4000000) return 7;
+  if (px >= 1500000) return 16;
   if (px >= 300000) return 6;
   return 5;
 }

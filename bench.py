@@ -39,7 +39,25 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def fed_traffic(profile="k_fed_multi_1080p_pmc.md", key="traffic_bytes_per_launch"):
+def fed_roofline(achieved, avg_ms, alg_bytes, traffic):
+    """Roofline object of the FED launches.  `achieved` counts the algorithmic
+    bytes (12 per pixel and step: read L and g, write L, as if every step were
+    its own pass) over the measured average launch; temporal blocking moves
+    fewer, so `frac` can pass 1.  `traffic` is the HBM bytes per launch from
+    the committed counter runs, and `traffic_GBs` the rate they imply at the
+    live launch time."""
+    r = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+         "kernel": "FED launches: k_fed_reg / k_fed_multi, up to 8 steps each, halo = steps",
+         "avg_launch_us": round(avg_ms * 1e3, 3), "alg_bytes_per_launch": round(alg_bytes)}
+    if traffic:
+        rate = traffic / (avg_ms * 1e-3) / 1e9
+        r["traffic_GBs"] = round(rate, 1)
+        r["traffic_frac"] = round(rate / HBM_PEAK_GBS, 4)
+    return r
+
+
+def fed_traffic(profile="fed_plain_1080p_pmc.md", key="traffic_bytes_per_launch"):
     """HBM bytes per FED launch from the committed FETCH_SIZE / WRITE_SIZE
     counter runs (scaled by the copy-kernel calibration), or None."""
     path = os.path.join(REPO, "profiles", profile)
@@ -103,12 +121,10 @@ def batched_comparison(A, W, H, B=16, reps=20):
     bk.set_proposal(None)
     fms, fby, fl = min(bk.profile_fed(B), bk.profile_fed(B))
     ach = (fby / fl) / (fms / fl * 1e-3) / 1e9
-    out["fed_roofline"] = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": round(ach / HBM_PEAK_GBS, 4), "avg_launch_us": round(fms / fl * 1e3, 3),
-                           "alg_bytes_per_launch": round(fby / fl),
-                           "traffic": fed_traffic("akaze_batched16_1080p_kernel_stats.md",
-                                                  "traffic_bytes_per_launch_batched16")
-                           if (W, H, B) == (1920, 1080, 16) else None}
+    out["fed_roofline"] = fed_roofline(ach, fms / fl, fby / fl,
+                                       fed_traffic("akaze_batched16_1080p_kernel_stats.md",
+                                                   "traffic_bytes_per_launch_batched16")
+                                       if (W, H, B) == (1920, 1080, 16) else None)
     full, prop = kps["full_scan"], kps["learned_proposal"]
     total = sum(len(f) for f in full)
     out["proposal_kp_recall"] = round(sum(len(f) * keypoint_recall(f, p) for f, p in zip(full, prop)) / total, 4)
@@ -340,6 +356,30 @@ def match_workload(A, args):
                          "kernel": "k_match", "ops_per_pair": 37}}
 
 
+def hbm_probe(nbytes=1 << 30, reps=5):
+    """Practical HBM rates of the box: best of `reps` device-to-device copies,
+    float sums (read only) and fills (write only) of a 1 GiB buffer, timed
+    with events on torch's stream.  Context for the roofline fractions, which
+    are taken against the 8 TB/s published peak."""
+    import torch
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    out = {"probe": "hbm", "bytes": nbytes}
+    for name, fn, moved in (("copy", lambda: b.copy_(a), 2 * nbytes), ("read", lambda: a.sum(), nbytes),
+                            ("write", lambda: b.fill_(2.0), nbytes)):
+        fn()
+        best = 1e30
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            e1.synchronize()
+            best = min(best, e0.elapsed_time(e1))
+        out[name + "_GBs"] = round(moved / (best * 1e-3) / 1e9, 1)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -360,11 +400,16 @@ def main():
     ap.add_argument("--autotune", action="store_true", help="autotune FED kernels first")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 256 MiB device copy kernel first (byte-counter calibration for profiles)")
+    ap.add_argument("--hbm-probe", action="store_true",
+                    help="only time 1 GiB device copy / read / fill passes (the practical HBM rates) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
+    if args.hbm_probe:
+        print(json.dumps(hbm_probe()), flush=True)
+        return
 
     import akaze_amd as A
     if args.workload != "plain":
@@ -451,12 +496,9 @@ def main():
     achieved = (fed_bytes / fed_launches) / (avg_ms * 1e-3) / 1e9
     log("FED: %d launches, %.3f ms per call of %d images, avg launch %.2f us, %.1f GB/s algorithmic" %
         (fed_launches, fed_ms, B, avg_ms * 1e3, achieved))
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4),
-                # the counter runs were made on one 1920x1080 image per call
-                "traffic": fed_traffic() if (W, H, B) == (1920, 1080, 1) else None,
-                "kernel": "k_fed_multi (up to 8 FED steps per launch)", "avg_launch_us": round(avg_ms * 1e3, 3),
-                "alg_bytes_per_launch": round(fed_bytes / fed_launches)}
+    # the counter runs were made on one 1920x1080 image per call
+    roofline = fed_roofline(achieved, avg_ms, fed_bytes / fed_launches,
+                            fed_traffic() if (W, H, B) == (1920, 1080, 1) else None)
 
     batched = None
     if not args.no_batched and B == 1:

@@ -163,7 +163,7 @@ akz_status akz_read_batch_level(akz_ctx* ctx, int z, int lvl, int which, float* 
  * variant (LDS or register region shape, threads per block, 8 or 16 steps
  * per launch) over each level's whole FED run on the level's own buffers
  * with batches of nz images and keeps the fastest per level size.  Overwrites the level contents; captured graphs are rebuilt on
- * the next call.  Variants are readable and settable per level (2..15). */
+ * the next call.  Variants are readable and settable per level (2..16). */
 akz_status akz_autotune(akz_ctx* ctx, int nz, int reps);
 akz_status akz_get_fed_variants(const akz_ctx* ctx, int* variants);
 akz_status akz_set_fed_variants(akz_ctx* ctx, const int* variants);
