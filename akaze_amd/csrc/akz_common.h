@@ -52,14 +52,26 @@ struct akz_batch {
 };
 
 #ifdef __HIPCC__
+// The offset pointer goes through the global address space and back: the
+// integer round trip would otherwise hide where it points, and every access
+// through it would be a flat instruction (which also counts against the LDS
+// wait counter, so an LDS wait would wait for outstanding stores too).  From
+// the cast the compiler infers global accesses with a scalar base.
 template <class T>
 __device__ __forceinline__ T* akz_zoff(T* p, size_t zs, int z) {
-  return reinterpret_cast<T*>(reinterpret_cast<uintptr_t>(p) + (size_t)z * zs);
+  using G = __attribute__((address_space(1))) T*;
+  G g = (G)(reinterpret_cast<uintptr_t>(p) + (size_t)z * zs);
+  return (T*)g;
 }
 template <class T>
 __device__ __forceinline__ T* akz_zoff(T* p, size_t zs) {
   return akz_zoff(p, zs, (int)blockIdx.z);
 }
+
+// Index of the calling wavefront in its workgroup.  It is uniform across the
+// wavefront; read through readfirstlane the compiler knows that, and rows,
+// loop bounds and addresses derived from it become scalar work.
+__device__ __forceinline__ int akz_wave() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
 
 // Workgroups of a launch are dispatched round-robin over MI355X's 8 XCDs,
 // each with its own L2: workgroup i (x fastest, then y, then z) runs on XCD

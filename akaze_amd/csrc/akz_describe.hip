@@ -49,7 +49,7 @@ __global__ __launch_bounds__(64 * OR_WAVES) void k_orientation(akz_desc_params p
   __shared__ float s_dx[OR_WAVES][208], s_dy[OR_WAVES][208];
   __shared__ float s_rx[OR_WAVES][42], s_ry[OR_WAVES][42];
   __shared__ float s_ex[OR_WAVES][42], s_ey[OR_WAVES][42];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wv = akz_wave();
   kps = akz_zoff(kps, p.zs);
   const int n = *akz_zoff(nkps_p, p.zs);
   // the loop bound is uniform over the block so every barrier is reached by all waves

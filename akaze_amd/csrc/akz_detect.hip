@@ -46,7 +46,7 @@ __global__ __launch_bounds__(256) void k_deriv(akz_det_params p) {
   int x = (t % tiles_x) * 64 + (threadIdx.x & 63);
   int y0 = (t / tiles_x) * 16;
   if (x >= L.w) return;
-  for (int y = y0 + (threadIdx.x >> 6); y < y0 + 16 && y < L.h; y += 4) {
+  for (int y = y0 + akz_wave(); y < y0 + 16 && y < L.h; y += 4) {
     float dx, dy;
     deriv_xy(akz_zoff(p.Lsm[l], p.zs), L.pitch, L.w, L.h, x, y, L.deriv_step, p.fac1, p.fac2, &dx, &dy);
     akz_zoff(p.Lx[l], p.zs)[(size_t)y * L.pitch + x] = dx;
@@ -64,7 +64,7 @@ __global__ __launch_bounds__(256) void k_hess(akz_det_params p) {
   int x = (t % tiles_x) * 64 + (threadIdx.x & 63);
   int y0 = (t / tiles_x) * 16;
   if (x >= L.w) return;
-  for (int y = y0 + (threadIdx.x >> 6); y < y0 + 16 && y < L.h; y += 4) {
+  for (int y = y0 + akz_wave(); y < y0 + 16 && y < L.h; y += 4) {
     float lxx, lyx, lyy;
     deriv_xy(akz_zoff(p.Lx[l], p.zs), L.pitch, L.w, L.h, x, y, L.deriv_step, p.fac1, p.fac2, &lxx, &lyx);
     deriv_xy(akz_zoff(p.Ly[l], p.zs), L.pitch, L.w, L.h, x, y, L.deriv_step, p.fac1, p.fac2, nullptr, &lyy);
@@ -101,7 +101,7 @@ __global__ __launch_bounds__(256) void k_extrema(akz_det_params p, int32_t* __re
   int lane = threadIdx.x & 63;
   int x = tx * 64 + lane;
   int y0 = (t / L.nseg_x) * 16;
-  for (int y = y0 + (threadIdx.x >> 6); y < y0 + 16 && y < L.h; y += 4) {
+  for (int y = y0 + akz_wave(); y < y0 + 16 && y < L.h; y += 4) {
     bool pred = is_extremum(akz_zoff(p.det[l], p.zs), L.pitch, L.w, L.h, x, y, L.border, p.thr);
     uint64_t mask = __ballot(pred);
     int seg = L.seg_base + y * L.nseg_x + tx;
@@ -187,7 +187,7 @@ __global__ __launch_bounds__(512) void k_detect_fused(akz_det_params p, int32_t*
   const int w = L.w, h = L.h, pitch = L.pitch, s = L.deriv_step;
   const int H1 = 2 * s + 1, H2 = s + 1;
   const int W1 = 64 + 2 * H1, W2 = 64 + 2 * H2;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wv = akz_wave();
   constexpr int NW = 8;
   const float* lsm = akz_zoff(p.Lsm[l], p.zs);
   const bool interior = x0 >= H1 && x0 + 64 + H1 <= w && y0 >= H1 && y0 + DF_TH + H1 <= h;
@@ -308,11 +308,13 @@ __global__ __launch_bounds__(512) void k_detect_fused(akz_det_params p, int32_t*
       Ly[o] = s_y[i2];
       const float v = s_d[id];
       if (p.det_full) D[o] = v;
-      if (col_in && row_in) {
-        const int R = DF_RW;
-        pred = v > p.thr && v > s_d[id - R - 1] && v > s_d[id + R + 1] && v > s_d[id + R - 1] &&
-               v > s_d[id - R + 1] && v > s_d[id - 1] && v > s_d[id + 1] && v > s_d[id + R] && v > s_d[id - R];
-      }
+      // the eight neighbours are inside the tile's determinant rows for every
+      // lane: read them all, combine without branches
+      const int R = DF_RW;
+      const float n0 = s_d[id - R - 1], n1 = s_d[id + R + 1], n2 = s_d[id + R - 1], n3 = s_d[id - R + 1];
+      const float n4 = s_d[id - 1], n5 = s_d[id + 1], n6 = s_d[id + R], n7 = s_d[id - R];
+      pred = col_in & row_in & (v > p.thr) & (v > n0) & (v > n1) & (v > n2) & (v > n3) & (v > n4) & (v > n5) &
+             (v > n6) & (v > n7);
     }
     // without the full determinant image, the subpixel fit of k_scatter
     // still finds the 3x3 neighbourhood of every candidate (all inside the
@@ -349,7 +351,7 @@ __device__ __forceinline__ int4 load4(const int32_t* a, int i, int n) {
 
 // block-wide exclusive scan of one value per thread; returns the block total
 __device__ __forceinline__ int block_exclusive_scan(int v, int* excl, int32_t* s_w) {
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wv = akz_wave();
   int inc = v;
   for (int d = 1; d < 64; d <<= 1) {
     int o = __shfl_up(inc, d, 64);
@@ -546,7 +548,7 @@ constexpr int NMS_BITMAP_WORDS = 2048;  // bitmap ranking covers up to 65536 can
 
 // block-wide exclusive scan for the 1024-thread suppression kernel
 __device__ __forceinline__ int nms_block_scan(int v, int* excl, int32_t* s_w) {
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wv = akz_wave();
   int inc = v;
   for (int d = 1; d < 64; d <<= 1) {
     int o = __shfl_up(inc, d, 64);

@@ -115,7 +115,7 @@ __global__ __launch_bounds__(256) void k_tile_stats(akz_prop_src src, float* __r
   const int t = blockIdx.x - src.tile_start[l];
   const int ngx = (src.w[l] + 63) / 64;
   const int gx = t % ngx, ty = t / ngx;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wv = akz_wave();
   const int x = gx * 64 + lane;
   const int w = src.w[l], h = src.h[l], pitch = src.pitch[l];
   const float* D = akz_zoff(src.det[l], src.zs);
@@ -199,7 +199,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd(akz_prop_geom g, int total, co
                                                   const uint16_t* __restrict__ WT, const float* __restrict__ bias,
                                                   uint16_t* __restrict__ Y) {
   constexpr int KP = kpad32(9 * CIN), NT = COUT / 16;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wv = akz_wave();
   const int p0 = (blockIdx.x * 4 + wv) * 16;
   if (p0 >= total) return;
   const int i = lane & 15, grp = lane >> 4;
@@ -238,7 +238,7 @@ __global__ __launch_bounds__(256) void k_conv_bwd_data(akz_prop_geom g, int tota
                                                        const uint16_t* __restrict__ WB,
                                                        const uint16_t* __restrict__ Xin, uint16_t* __restrict__ Gin) {
   constexpr int COUT = 32, NT = CIN / 16;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wv = akz_wave();
   const int p0 = (blockIdx.x * 4 + wv) * 16;
   if (p0 >= total) return;
   const int i = lane & 15, grp = lane >> 4;
@@ -287,7 +287,7 @@ __global__ __launch_bounds__(256) void k_conv_bwd_w(akz_prop_geom g, int total, 
   constexpr int XS = KP + 8;          // LDS row stride (bf16), breaks bank alignment
   __shared__ uint16_t s_x[32 * XS];
   __shared__ uint16_t s_g[32 * COUT];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wv = akz_wave();
   const int i = lane & 15, grp = lane >> 4;
   f32x4 acc[TPW][MT];
 #pragma unroll
@@ -433,7 +433,7 @@ __global__ __launch_bounds__(HEAD_T) void k_head(int total, const uint16_t* __re
                           gq[v * 8 + 6] | ((uint32_t)gq[v * 8 + 7] << 16));
   }
   // block partials: dw4[c] = sum dz * h[c], db4 = sum dz, loss
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wv = akz_wave();
   for (int c = 0; c < C3 + 2; c++) {
     float v = c < C3 ? dz * h[c] : (c == C3 ? dz : loss);
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);

@@ -90,7 +90,7 @@ __global__ __launch_bounds__(256) void k_contrast_grad(const float* __restrict__
   blur = akz_zoff(blur, zs);
   grad = akz_zoff(grad, zs);
   blockmax = akz_zoff(blockmax, zs);
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wv = akz_wave();
   const int tiles_x = (w - 2 + 63) / 64, tiles_y = (h - 2 + 3) / 4;
   uint32_t bits = 0u;
   for (int t = blockIdx.x; t < tiles_x * tiles_y; t += gridDim.x) {
@@ -125,7 +125,7 @@ __global__ __launch_bounds__(1024) void k_contrast_max(const uint32_t* __restric
   __shared__ uint32_t s_w[16];
   blockmax = akz_zoff(blockmax, zs);
   maxbits = akz_zoff(maxbits, zs);
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wv = akz_wave();
   uint32_t m = 0u;
   for (int i = threadIdx.x; i < nb; i += 1024) m = m > blockmax[i] ? m : blockmax[i];
   for (int off = 32; off > 0; off >>= 1) {
@@ -154,7 +154,7 @@ __global__ __launch_bounds__(CT_HIST_THREADS) void k_contrast_hist(const float* 
   grad = akz_zoff(grad, zs);
   maxbits = akz_zoff(maxbits, zs);
   hist_part = akz_zoff(hist_part, zs);
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wv = akz_wave();
   constexpr int NW = CT_HIST_THREADS / 64;
   for (int i = threadIdx.x; i < 300 * 64; i += CT_HIST_THREADS) lh[i] = 0;
   for (int i = threadIdx.x; i < 212; i += CT_HIST_THREADS) tail[i] = 0;
@@ -214,7 +214,7 @@ __global__ __launch_bounds__(1024) void k_contrast_finish(const int32_t* __restr
   bin_tot = akz_zoff(bin_tot, zs);
   maxbits = akz_zoff(maxbits, zs);
   kc = akz_zoff(kc, zs);
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wv = akz_wave();
   for (int i = threadIdx.x; i < nbins; i += 1024) tot[i] = bin_tot[i];
   __syncthreads();
   // percentile: the loop of ContrastPercentile stops at k = 1 + the first bin
@@ -256,7 +256,7 @@ __global__ __launch_bounds__(256) void k_fed_step(const float* __restrict__ L, c
                                                   float* __restrict__ out, int pitch, int w, int h,
                                                   float stepsize, size_t zs) {
   int x = blockIdx.x * 64 + (threadIdx.x & 63);
-  int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  int y = blockIdx.y * 4 + akz_wave();
   if (x >= w || y >= h) return;
   L = akz_zoff(L, zs);
   g = akz_zoff(g, zs);
@@ -566,7 +566,7 @@ __global__ __launch_bounds__(256) void k_fed_tile(const float* __restrict__ L, c
   const int gx = x0 + c;
   if (gx >= w) return;
   const int de = (gx < w - 1) ? 1 : 0, dw = (gx > 0) ? 1 : 0;
-  for (int r = 1 + (tid >> 6); r <= TH; r += 4) {
+  for (int r = 1 + akz_wave(); r <= TH; r += 4) {
     const int gy = y0 + r;
     if (gy >= h) break;
     const int i = r * SW + c;
@@ -584,7 +584,7 @@ __global__ __launch_bounds__(256) void k_fed_tile(const float* __restrict__ L, c
 __global__ __launch_bounds__(256) void k_half2(const float* __restrict__ in, int ipitch, float* __restrict__ out,
                                                int ow, int oh, int opitch, size_t zs) {
   int x = blockIdx.x * 64 + (threadIdx.x & 63);
-  int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  int y = blockIdx.y * 4 + akz_wave();
   if (x >= ow || y >= oh) return;
   in = akz_zoff(in, zs);
   out = akz_zoff(out, zs);
@@ -606,7 +606,7 @@ __global__ __launch_bounds__(256) void k_half_general(const float* __restrict__ 
                                                       float* __restrict__ out, int ow, int oh, int opitch,
                                                       size_t zs) {
   int x = blockIdx.x * 64 + (threadIdx.x & 63);
-  int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  int y = blockIdx.y * 4 + akz_wave();
   if (x >= ow || y >= oh) return;
   in = akz_zoff(in, zs);
   out = akz_zoff(out, zs);
@@ -627,7 +627,7 @@ __global__ __launch_bounds__(256) void k_half_general(const float* __restrict__ 
 __global__ __launch_bounds__(256) void k_copy2d(const float* __restrict__ in, int ipitch, float* __restrict__ out,
                                                 int opitch, int w, int h, size_t zs) {
   int x = blockIdx.x * 64 + (threadIdx.x & 63);
-  int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  int y = blockIdx.y * 4 + akz_wave();
   if (x >= w || y >= h) return;
   in = akz_zoff(in, zs);
   out = akz_zoff(out, zs);

@@ -95,7 +95,7 @@ __global__ __launch_bounds__(256) void k_synth_noise(const uint64_t* __restrict_
     sa += __shfl_xor(sa, off, 64);
     sb += __shfl_xor(sb, off, 64);
   }
-  const int wv = threadIdx.x >> 6;
+  const int wv = akz_wave();
   if ((threadIdx.x & 63) == 0) { s_a[wv] = sa; s_b[wv] = sb; }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -130,7 +130,7 @@ __global__ __launch_bounds__(256) void k_synth_compose(const float* __restrict__
     s_scale = (float)(8.0 / sqrt((var > 0 ? var : 0.0) + 1e-30));
   }
   __syncthreads();
-  const int x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  const int x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + akz_wave();
   if (x >= w || y >= h) return;
   float v = bg[z];
   const float px = x + 0.5f, py = y + 0.5f;
@@ -162,7 +162,7 @@ struct Hinv { float m[9]; };
 
 __global__ __launch_bounds__(256) void k_warp(const float* __restrict__ src, int w, int h, int spitch, Hinv H,
                                               float noise_std, uint64_t seed, float* __restrict__ dst, int dpitch) {
-  const int x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  const int x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + akz_wave();
   if (x >= w || y >= h) return;
   const float u = x + 0.5f, v = y + 0.5f;
   const float zz = H.m[6] * u + H.m[7] * v + H.m[8];
