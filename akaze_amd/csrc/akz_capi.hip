@@ -426,7 +426,11 @@ void enqueue_scale_space(akz_ctx* c, const float* img, int in_pitch, size_t in_z
       base = P.Lt;  // Copy (AKAZE.cpp:258) is folded into the first step's read
       if (n == 0) akz_launch_copy2d(s, P.Lt, P.pitch, L.Lt, L.pitch, L.w, L.h, bt);
     }
-    akz_launch_smooth_flow(s, base, L.pitch, L.Lsm, c->g, L.w, L.h, c->opts.diffusivity, c->kc + i, c->taps1, bt);
+    // LDS variants build Lsmooth and g in the level's first FED launch; g
+    // goes through HBM only when further launches of the level read it
+    const bool fuse = bs > 1 && n > 0 && akz_fed_variant_fusable(c->fed_variant[i]);
+    if (!fuse)
+      akz_launch_smooth_flow(s, base, L.pitch, L.Lsm, c->g, L.w, L.h, c->opts.diffusivity, c->kc + i, c->taps1, bt);
     const float* cur = base;
     float* dst = first_dst;
     if (c->fed_profile) hipEventRecord(c->fed_ev[2 * i], s);
@@ -436,6 +440,9 @@ void enqueue_scale_space(akz_ctx* c, const float* img, int in_pitch, size_t in_z
       for (int q = 0; q < k; q++) ht[q] = 0.5 * L.tau[j + q];
       if (bs == 1)
         akz_launch_fed_step(s, cur, c->g, dst, L.pitch, L.w, L.h, ht[0], bt);
+      else if (fuse && j == 0)
+        akz_launch_fed_variant_sf(s, c->fed_variant[i], cur, L.Lsm, nl > 1 ? c->g : nullptr, dst, L.pitch, L.w, L.h,
+                                  c->opts.diffusivity, c->kc + i, c->taps1, ht, k, bt);
       else
         akz_launch_fed_variant(s, c->fed_variant[i], cur, c->g, dst, L.pitch, L.w, L.h, ht, k, bt);
       cur = dst;
@@ -953,8 +960,12 @@ akz_status akz_profile_fed_batch(akz_ctx* c, const float* d_imgs, size_t pitch_b
     float ms = 0.0f;
     AKZ_HIP_TRY(hipEventElapsedTime(&ms, c->fed_ev[2 * i], c->fed_ev[2 * i + 1]));
     total += ms;
-    // algorithmic traffic of one step: read L and g, write L (4 B each per pixel)
+    // algorithmic traffic of one step: read L and g, write L (4 B each per
+    // pixel); a level whose first launch also builds Lsmooth and g adds that
+    // pass's (read L, write Lsmooth and g)
     bytes += 12.0 * c->lev[i].w * c->lev[i].h * c->lev[i].nsteps * n;
+    if (level_block(c, i) > 1 && c->lev[i].nsteps > 0 && akz_fed_variant_fusable(c->fed_variant[i]))
+      bytes += 12.0 * c->lev[i].w * c->lev[i].h * n;
     launches += (c->lev[i].nsteps + level_block(c, i) - 1) / level_block(c, i);
   }
   if (fed_ms) *fed_ms = total;
@@ -1052,12 +1063,26 @@ akz_status akz_autotune(akz_ctx* c, int nz, int reps) {
       // the level's whole FED run with this variant (its steps per launch),
       // once to warm up, then `reps` timed times, ping-ponging Lt and the scratch
       const int bs = std::min(c->fed_block, akz_fed_variant_steps(v));
+      const int nl = (L.nsteps + bs - 1) / bs;
+      const bool fuse = bs > 1 && akz_fed_variant_fusable(v);
+      // as in the pipeline: smooth + flow, fused into the first launch for
+      // the LDS variants
       auto run_level = [&](int parity) {
         int launch = parity;
+        if (!fuse)
+          akz_launch_smooth_flow(c->stream, (parity & 1) ? c->fed_tmp : L.Lt, L.pitch, L.Lsm, c->g, L.w, L.h,
+                                 c->opts.diffusivity, c->kc + i, c->taps1, bt);
         for (int j = 0; j < L.nsteps; j += bs, launch++) {
           const bool even = (launch & 1) == 0;
-          akz_launch_fed_variant(c->stream, v, even ? L.Lt : c->fed_tmp, c->g, even ? c->fed_tmp : L.Lt, L.pitch,
-                                 L.w, L.h, ht + j, std::min(bs, L.nsteps - j), bt);
+          const float* src = even ? L.Lt : c->fed_tmp;
+          float* dst = even ? c->fed_tmp : L.Lt;
+          if (fuse && j == 0)
+            akz_launch_fed_variant_sf(c->stream, v, src, L.Lsm, nl > 1 ? c->g : nullptr, dst, L.pitch, L.w, L.h,
+                                      c->opts.diffusivity, c->kc + i, c->taps1, ht + j, std::min(bs, L.nsteps - j),
+                                      bt);
+          else
+            akz_launch_fed_variant(c->stream, v, src, c->g, dst, L.pitch, L.w, L.h, ht + j,
+                                   std::min(bs, L.nsteps - j), bt);
         }
       };
       run_level(0);

@@ -34,10 +34,17 @@ int akz_fed_variant_steps(int variant);
 void akz_launch_fed_multi(hipStream_t s, const float* Lin, const float* g, float* Lout, int pitch, int w, int h,
                           const float* half_tau, int n, akz_batch bt);
 // n FED steps with one kernel variant (0 single step, 1 single step LDS-tiled,
-// 2..8 blocked); variants 0 and 1 need Lin != Lout and n == 1 for a
-// ping-pong-free call, the blocked ones accept n <= akz_fed_block_steps()
+// 2..AKZ_FED_VARIANT_MAX blocked); variants 0 and 1 need Lin != Lout and
+// n == 1 for a ping-pong-free call, the blocked ones accept
+// n <= akz_fed_variant_steps(variant)
 void akz_launch_fed_variant(hipStream_t s, int variant, const float* Lin, const float* g, float* Lout, int pitch,
                             int w, int h, const float* half_tau, int n, akz_batch bt);
+// LDS variants can build the level's Lsmooth and conductance in their first
+// launch (k_fed_multi_sf): one launch instead of two
+bool akz_fed_variant_fusable(int variant);
+void akz_launch_fed_variant_sf(hipStream_t s, int variant, const float* in, float* lsm, float* gout, float* Lout,
+                               int pitch, int w, int h, int type, const float* kc, const akz_taps& taps,
+                               const float* half_tau, int n, akz_batch bt);
 void akz_launch_halfsample(hipStream_t s, const float* in, int iw, int ih, int ipitch, float* out, int ow, int oh,
                            int opitch, akz_batch bt);
 void akz_launch_copy2d(hipStream_t s, const float* in, int ipitch, float* out, int opitch, int w, int h,

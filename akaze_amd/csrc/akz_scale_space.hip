@@ -341,6 +341,132 @@ __global__ __launch_bounds__(NT) void k_fed_multi(const float* __restrict__ L, c
   }
 }
 
+// The first FED launch of a level fused with its Lsmooth and conductance
+// (the work of k_smooth_flow_reg: sigma=1 LowPass with replicate border,
+// unnormalised Scharr with reflect-101 neighbours, diffusivity TYPE; same
+// operands in the same order).  A block loads the level's input over its
+// RW x RH region plus 3 pixels (coordinates clamped), runs the row and column
+// Gaussian and the flow in LDS, writes Lsmooth (and g when later launches of
+// the level read it) for its tile, then runs the launch's FED steps as
+// k_fed_multi.  One launch instead of two, and g never goes through HBM when
+// the level takes one launch.  LDS: input window, row pass and Lsmooth with
+// its ring; the FED buffers reuse the first two once they are dead.
+template <int RW, int RH, int NT, int K = FB_K>
+__global__ __launch_bounds__(NT) void k_fed_multi_sf(const float* __restrict__ in, float* __restrict__ lsm,
+                                                     float* __restrict__ gout, float* __restrict__ out, int pitch,
+                                                     int w, int h, int type, const float* kcp, akz_taps taps,
+                                                     akz_fed_steps st, size_t zs) {
+  static_assert(RW > 2 * K && RH > 2 * K, "a tile remains inside the halo");
+  constexpr int AW = RW + 6, AH = RH + 6;  // input window
+  constexpr int BW = RW + 2;               // row pass (AH rows) and Lsmooth ring (RH + 2 rows)
+  constexpr int CH = RH + 2;
+  constexpr int LW = (RW == 128) ? 7 : (RW == 64) ? 6 : (RW == 32 ? 5 : 4);
+  static_assert((1 << LW) == RW, "power-of-two region width");
+  static_assert(RH * RW <= AH * AW && RH * RW <= AH * BW, "FED buffers fit the dead windows");
+  const akz_tile3 tl = akz_xcd_tile();
+  in = akz_zoff(in, zs, tl.z);
+  lsm = akz_zoff(lsm, zs, tl.z);
+  if (gout) gout = akz_zoff(gout, zs, tl.z);
+  out = akz_zoff(out, zs, tl.z);
+  kcp = akz_zoff(kcp, zs, tl.z);
+  __shared__ float sA[AH * AW];  // input window, then the second FED buffer
+  __shared__ float sB[AH * BW];  // row pass, then g
+  __shared__ float sC[CH * BW];  // Lsmooth with a one-pixel ring
+  __shared__ float sa[RH * RW];  // L at step 0
+  float* const sb = sA;
+  float* const sg = sB;
+  const int tid = threadIdx.x;
+  const int hn = st.n;
+  const int x0 = tl.x * (RW - 2 * hn) - hn, y0 = tl.y * (RH - 2 * hn) - hn;
+  load_window<AH, AW, NT>(sA, in, pitch, x0 - 3, y0 - 3, w, h);
+  __syncthreads();
+  // L of the region (the window without its 3-pixel ring), and the row pass
+  for (int idx = tid; idx < RH * RW; idx += NT) {
+    const int r = idx >> LW, c = idx & (RW - 1);
+    sa[idx] = sA[(r + 3) * AW + c + 3];
+  }
+  for (int idx = tid; idx < AH * BW; idx += NT) {
+    const int r = idx / BW, c = idx - r * BW;
+    float sum = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 5; i++) sum += sA[r * AW + c + i] * taps.k[i];
+    sB[idx] = sum;
+  }
+  __syncthreads();
+  // column pass: Lsmooth at image (y0 - 1 + r, x0 - 1 + c)
+  for (int idx = tid; idx < CH * BW; idx += NT) {
+    const int r = idx / BW, c = idx - r * BW;
+    float sum = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 5; i++) sum += sB[(r + i) * BW + c] * taps.k[i];
+    sC[idx] = sum;
+  }
+  __syncthreads();
+  // flow on the region's in-image pixels (the FED steps never read g
+  // outside the image); Lsmooth and g of the tile go out
+  const float kcontrast = *kcp;
+  const float invk = 1.0f / (kcontrast * kcontrast);
+  for (int idx = tid; idx < RH * RW; idx += NT) {
+    const int r = idx >> LW, c = idx & (RW - 1);
+    const int gx = x0 + c, gy = y0 + r;
+    if (gx < 0 || gy < 0 || gx >= w || gy >= h) continue;
+    const int cy = r + 1, cx = c + 1;
+    const int cym = akz_reflect101(gy - 1, h) - y0 + 1, cyp = akz_reflect101(gy + 1, h) - y0 + 1;
+    const int cxm = akz_reflect101(gx - 1, w) - x0 + 1, cxp = akz_reflect101(gx + 1, w) - x0 + 1;
+    const float ul = sC[cym * BW + cxm], ur = sC[cym * BW + cxp], ll = sC[cyp * BW + cxm], lr = sC[cyp * BW + cxp];
+    const float lx = 3.0f * (lr - ll + ur - ul) + 10.0f * (sC[cy * BW + cxp] - sC[cy * BW + cxm]);
+    const float ly = 3.0f * (lr + ll - ur - ul) + 10.0f * (sC[cyp * BW + cx] - sC[cym * BW + cx]);
+    const float dif2 = invk * (lx * lx + ly * ly);
+    float res;
+    if (type == 0)
+      res = expf(-dif2);
+    else if (type == 1)
+      res = 1.0f / (1.0f + dif2);
+    else if (type == 2)
+      res = 1.0f - (float)exp(-3.315 / (dif2 * dif2 * dif2 * dif2));
+    else
+      res = 1.0f / sqrtf(1.0f + dif2);
+    sg[idx] = res;
+    if (r >= hn && r < RH - hn && c >= hn && c < RW - hn) {
+      const size_t o = (size_t)gy * pitch + gx;
+      lsm[o] = sC[cy * BW + cx];
+      if (gout) gout[o] = res;
+    }
+  }
+  __syncthreads();
+  float* src = sa;
+  float* dst = sb;
+  for (int s = 1; s <= st.n; s++) {
+    const float stepsize = st.half_tau[s - 1];
+    const float* __restrict__ cs = src;
+    float* __restrict__ cd = dst;
+    for (int idx = s * RW + tid; idx < (RH - s) * RW; idx += NT) {
+      const int r = idx >> LW, c = idx & (RW - 1);
+      const int gx = x0 + c, gy = y0 + r;
+      if (c < s || c >= RW - s || gx < 0 || gy < 0 || gx >= w || gy >= h) continue;
+      const int e = (gx < w - 1) ? idx + 1 : idx, wi = (gx > 0) ? idx - 1 : idx;
+      const int so = (gy < h - 1) ? idx + RW : idx, no = (gy > 0) ? idx - RW : idx;
+      const float i0 = cs[idx], f0 = sg[idx];
+      const float le = cs[e], lw = cs[wi], ls = cs[so], ln = cs[no];
+      const float ge = sg[e], gw = sg[wi], gs = sg[so], gn = sg[no];
+      float xpos = (f0 + ge) * (le - i0);
+      float xneg = (f0 + gw) * (i0 - lw);
+      float ypos = (f0 + gs) * (ls - i0);
+      float yneg = (f0 + gn) * (i0 - ln);
+      cd[idx] = i0 + stepsize * (xpos - xneg + ypos - yneg);
+    }
+    __syncthreads();
+    float* t = src;
+    src = dst;
+    dst = t;
+  }
+  for (int idx = hn * RW + tid; idx < (RH - hn) * RW; idx += NT) {
+    const int r = idx >> LW, c = idx & (RW - 1);
+    const int gx = x0 + c, gy = y0 + r;
+    if (c >= hn && c < RW - hn && gx < w && gy < h) out[(size_t)gy * pitch + gx] = src[idx];
+  }
+}
+
 // FED steps with temporal blocking in registers: one wavefront per region of
 // 64 columns (one per lane) x R rows (R registers per lane), with a halo as
 // wide as the launch's step count.
@@ -783,6 +909,45 @@ void akz_launch_fed_variant(hipStream_t s, int variant, const float* Lin, const 
   } else {
     hipLaunchKernelGGL((k_fed_multi<64, 32, 256>), grid(64, 32), dim3(256), 0, s, Lin, g, Lout, pitch, w, h, st, zs);
   }
+}
+
+bool akz_fed_variant_fusable(int variant) {
+  return (variant >= 2 && variant <= 11) || variant == 14 || variant == 15;
+}
+
+// First launch of a level with an LDS variant, fused with the level's
+// smooth + flow (k_fed_multi_sf); g is written only when `gout` is given.
+void akz_launch_fed_variant_sf(hipStream_t s, int variant, const float* in, float* lsm, float* gout, float* Lout,
+                               int pitch, int w, int h, int type, const float* kc, const akz_taps& taps,
+                               const float* half_tau, int n, akz_batch bt) {
+  const int nz = bt.nz;
+  const size_t zs = bt.zs;
+  akz_fed_steps st;
+  if (n > akz_fed_variant_steps(variant)) n = akz_fed_variant_steps(variant);
+  st.n = n;
+  for (int i = 0; i < FB_KMAX; i++) st.half_tau[i] = i < n ? half_tau[i] : 0.0f;
+  auto grid = [&](int rw, int rh) {
+    const int tw = rw - 2 * n, th = rh - 2 * n;
+    return dim3((w + tw - 1) / tw, (h + th - 1) / th, nz);
+  };
+#define AKZ_SF(RW_, RH_, NT_, K_)                                                                               \
+  hipLaunchKernelGGL((k_fed_multi_sf<RW_, RH_, NT_, K_>), grid(RW_, RH_), dim3(NT_), 0, s, in, lsm, gout, Lout, \
+                     pitch, w, h, type, kc, taps, st, zs)
+  switch (variant) {
+    case 2: AKZ_SF(64, 64, 512, FB_K); break;
+    case 3: AKZ_SF(32, 32, 256, FB_K); break;
+    case 5: AKZ_SF(32, 32, 1024, FB_K); break;
+    case 6: AKZ_SF(64, 64, 1024, FB_K); break;
+    case 7: AKZ_SF(64, 48, 512, FB_K); break;
+    case 8: AKZ_SF(32, 48, 512, FB_K); break;
+    case 9: AKZ_SF(128, 64, 1024, FB_K); break;
+    case 10: AKZ_SF(64, 128, 1024, FB_K); break;
+    case 11: AKZ_SF(128, 32, 512, FB_K); break;
+    case 14: AKZ_SF(64, 64, 1024, FB_KMAX); break;
+    case 15: AKZ_SF(128, 64, 1024, FB_KMAX); break;
+    default: AKZ_SF(64, 32, 256, FB_K); break;
+  }
+#undef AKZ_SF
 }
 
 // Region shape by level size and images per launch, from the autotuner's

@@ -161,8 +161,10 @@ akz_status akz_batch_results_device(akz_ctx* ctx, int z, const int32_t** d_count
 akz_status akz_read_batch_level(akz_ctx* ctx, int z, int lvl, int which, float* dst);
 /* Autotuner (no reference counterpart): times every blocked FED kernel
  * variant (LDS or register region shape, threads per block, 8 or 16 steps
- * per launch) over each level's whole FED run on the level's own buffers
- * with batches of nz images and keeps the fastest per level size.  Overwrites the level contents; captured graphs are rebuilt on
+ * per launch) over each level's smooth + flow and whole FED run, as the
+ * pipeline runs them (the LDS variants build Lsmooth and the conductance in
+ * their first launch), on the level's own buffers with batches of nz images,
+ * and keeps the fastest per level size.  Overwrites the level contents; captured graphs are rebuilt on
  * the next call.  Variants are readable and settable per level (2..16). */
 akz_status akz_autotune(akz_ctx* ctx, int nz, int reps);
 akz_status akz_get_fed_variants(const akz_ctx* ctx, int* variants);
