@@ -238,7 +238,7 @@ akz_status allocate(akz_ctx* c) {
   size_t n0 = (size_t)L0.pitch * (L0.h + 1);
   size_t o_img = cv.take<float>(n0), o_g = cv.take<float>(n0), o_tmp = cv.take<float>(n0);
   size_t o_max = cv.take<uint32_t>(1), o_hist = cv.take<int32_t>(512 * 512 + 512);
-  size_t o_bmax = cv.take<uint32_t>(2048);
+  size_t o_bmax = cv.take<uint32_t>(akz_contrast_front_blocks(L0.w, L0.h));
   size_t o_kc = cv.take<float>(AKZ_MAX_LEVELS);
   // row segments of 64 pixels, level-major, raster order
   int seg_total = 0, tiles = 0;
@@ -399,13 +399,12 @@ void enqueue_scale_space(akz_ctx* c, const float* img, int in_pitch, size_t in_z
   const akz_batch bt{nz, c->zs};
   Level& L0 = c->lev[0];
   record(c, EV_START);
-  // contrast factor of the input (ContrastPercentile, cuda_akaze.cu:604-649);
-  // level-0 Lx / Ly serve as blur / gradient scratch until detection
-  akz_launch_gauss(s, img, in_pitch, in_zs, L0.Lx, L0.pitch, L0.w, L0.h, 2, c->taps_c, bt);
-  akz_launch_contrast(s, L0.Lx, L0.pitch, L0.Ly, L0.w, L0.h, std::min(c->opts.kcontrast_nbins, 512),
-                      c->opts.kcontrast_percentile, c->maxbits, c->blockmax, c->hist, c->kc, c->nlev, c->octs, bt);
-  // level 0: Gaussian of soffset; Lsmooth aliases Lt (AKAZE.cpp:234-236)
-  akz_launch_gauss(s, img, in_pitch, in_zs, L0.Lt, L0.pitch, L0.w, L0.h, c->R0, c->taps0, bt);
+  // level 0, the Gaussian of soffset (Lsmooth aliases Lt, AKAZE.cpp:234-236),
+  // and the contrast factor of the input (ContrastPercentile,
+  // cuda_akaze.cu:604-649); level-0 Ly holds the gradient until detection
+  akz_launch_contrast(s, img, in_pitch, in_zs, L0.Lt, L0.Ly, L0.pitch, L0.w, L0.h, c->R0, c->taps0, c->taps_c,
+                      std::min(c->opts.kcontrast_nbins, 512), c->opts.kcontrast_percentile, c->maxbits,
+                      c->blockmax, c->hist, c->kc, c->nlev, c->octs, bt);
   record(c, EV_CONTRAST);
   for (int i = 1; i < c->nlev; i++) {
     Level& L = c->lev[i];

@@ -7,15 +7,18 @@ struct akz_taps { float k[11]; };
 
 // ---- scale space (akz_scale_space.hip) ----
 // Every launcher takes the batch (akz_batch): pointers name image 0's arrays,
-// image z's sit z * bt.zs bytes further; the input of akz_launch_gauss has
-// its own image stride in_zs.
-void akz_launch_gauss(hipStream_t s, const float* in, int in_pitch, size_t in_zs, float* out, int out_pitch, int w,
-                      int h, int R, const akz_taps& taps, akz_batch bt);
-// Contrast factor of ContrastPercentile on the device: gradient of `blur`
-// (written to `grad`), its maximum, the histogram and the percentile, leaving
-// the per-level factor in kc[0..nlev).  blockmax: 2048 entries; hist_part:
-// 512 x 512 + 512 ints.
-void akz_launch_contrast(hipStream_t s, const float* blur, int pitch, float* grad, int w, int h, int nbins,
+// image z's sit z * bt.zs bytes further; the input image of
+// akz_launch_contrast has its own image stride in_zs.
+//
+// Level 0 and the contrast factor of ContrastPercentile on the device: the
+// soffset Gaussian of `img` into `lt` (radius R0, taps0) and the gradient of
+// its sigma=1 blur (taps_c) into `grad` in one launch, with the tile maxima
+// in `blockmax` (akz_contrast_front_blocks(w, h) entries); then the
+// histogram, the bin totals and the percentile, leaving the per-level factor
+// in kc[0..nlev).  hist_part: 512 x 512 + 512 ints; maxbits: one word.
+int akz_contrast_front_blocks(int w, int h);
+void akz_launch_contrast(hipStream_t s, const float* img, int in_pitch, size_t in_zs, float* lt, float* grad,
+                         int pitch, int w, int h, int R0, const akz_taps& taps0, const akz_taps& taps_c, int nbins,
                          float perc, uint32_t* maxbits, uint32_t* blockmax, int32_t* hist_part, float* kc, int nlev,
                          const int* octaves_host, akz_batch bt);
 // FED kernel variant chosen for a level size and the images per launch

@@ -32,78 +32,93 @@ __device__ __forceinline__ void load_window(float* __restrict__ dst, const float
   }
 }
 
-constexpr int GT_X = 64;   // Gaussian tile width (one wavefront per row)
-constexpr int GT_Y = 16;   // Gaussian tile height
-constexpr int GT_THREADS = 256;
+struct akz_octaves { int nlev; int oct[AKZ_MAX_LEVELS]; };
 
-// Separable Gaussian, row pass then column pass, replicate border
-// (ConvRowGPU/ConvColGPU, cuda_akaze.cu:56-128), fused through LDS: the tile
-// plus an R-pixel halo is loaded once, the row pass is kept in LDS and the
-// column pass reads it.  Sums run i = 0..2R in the reference's order.
-template <int R>
-__global__ __launch_bounds__(GT_THREADS) void k_gauss(const float* __restrict__ in, int in_pitch,
-                                                      float* __restrict__ out, int out_pitch, int w, int h,
-                                                      akz_taps taps, size_t in_zs, size_t out_zs) {
-  constexpr int AW = GT_X + 2 * R, AH = GT_Y + 2 * R;
+// Front end of the contrast factor and the level-0 image in one pass over the
+// input, per 64x16 tile with an H-pixel halo (H = max(R0, 3); coordinates
+// clamped: the replicate border of LowPass):
+// * Lt of level 0, the Gaussian of soffset with R0-pixel taps (ConvRowGPU /
+//   ConvColGPU, cuda_akaze.cu:56-128; AKAZE.cpp:234-236), row pass then
+//   column pass, sums in tap order from 0.0f;
+// * the sigma=1 blur of ContrastPercentile (:604-649), the same way, on the
+//   tile plus a one-pixel ring, kept in LDS only;
+// * the gradient magnitude of MaxContrast (:552-579): the unnormalised Scharr
+//   pair over the blur's 3x3 window with top-left (x, y), stored at
+//   (x+1, y+1), interior pixels only;
+// * the tile's largest gradient (float bits; all values are >= 0).
+constexpr int CF_TX = 64, CF_TY = 16, CF_NT = 256;
+
+template <int R0>
+__global__ __launch_bounds__(CF_NT) void k_contrast_front(const float* __restrict__ in, int in_pitch, size_t in_zs,
+                                                          float* __restrict__ lt, float* __restrict__ grad,
+                                                          int pitch, int w, int h, akz_taps taps_c, akz_taps taps0,
+                                                          uint32_t* __restrict__ blockmax, size_t zs) {
+  constexpr int H = R0 > 3 ? R0 : 3;
+  constexpr int AW = CF_TX + 2 * H, AH = CF_TY + 2 * H;
+  constexpr int BW = CF_TX + 2;  // blur columns x0-1 .. x0+64
+  constexpr int CH = CF_TY + 2;  // blur rows y0-1 .. y0+16
   in = akz_zoff(in, in_zs);
-  out = akz_zoff(out, out_zs);
-  __shared__ float A[AH][AW];
-  __shared__ float B[AH][GT_X];
-  const int x0 = blockIdx.x * GT_X, y0 = blockIdx.y * GT_Y;
-  const int tid = threadIdx.x;
-  load_window<AH, AW, GT_THREADS>(&A[0][0], in, in_pitch, x0 - R, y0 - R, w, h);
-  __syncthreads();
-  for (int idx = tid; idx < AH * GT_X; idx += GT_THREADS) {
-    int r = idx / GT_X, c = idx - r * GT_X;
-    float s = 0.0f;
-#pragma unroll
-    for (int i = 0; i <= 2 * R; i++) s += A[r][c + i] * taps.k[i];
-    B[r][c] = s;
-  }
-  __syncthreads();
-  const int c = tid & (GT_X - 1);
-  for (int r = tid / GT_X; r < GT_Y; r += GT_THREADS / GT_X) {
-    int gx = x0 + c, gy = y0 + r;
-    if (gx < w && gy < h) {
-      float s = 0.0f;
-#pragma unroll
-      for (int i = 0; i <= 2 * R; i++) s += B[r + i][c] * taps.k[i];
-      out[(size_t)gy * out_pitch + gx] = s;
-    }
-  }
-}
-
-constexpr int CT_GRAD_BLOCKS = 2048;  // grid-stride block counts of the contrast kernels
-constexpr int CT_HIST_BLOCKS = 512;
-constexpr int CT_HIST_THREADS = 512;
-
-// Gradient magnitude of the sigma=1 blur with the unnormalised Scharr pair of
-// MaxContrast (cuda_akaze.cu:552-579): the 3x3 window with top-left (x, y)
-// gives the value stored at (x+1, y+1); interior pixels only.  Each block
-// writes its maximum (float bits; all values are >= 0) to blockmax and the
-// single-block k_contrast_max reduces them.  No global atomics: device-scope
-// atomics on one address serialise across the whole chip.
-__global__ __launch_bounds__(256) void k_contrast_grad(const float* __restrict__ blur, int pitch,
-                                                       float* __restrict__ grad, int w, int h,
-                                                       uint32_t* __restrict__ blockmax, size_t zs) {
-  __shared__ uint32_t s_w[4];
-  blur = akz_zoff(blur, zs);
+  lt = akz_zoff(lt, zs);
   grad = akz_zoff(grad, zs);
   blockmax = akz_zoff(blockmax, zs);
-  const int lane = threadIdx.x & 63, wv = akz_wave();
-  const int tiles_x = (w - 2 + 63) / 64, tiles_y = (h - 2 + 3) / 4;
+  __shared__ float A[AH * AW];
+  __shared__ float B0[AH * CF_TX];
+  __shared__ float B1[AH * BW];
+  __shared__ float C[CH * BW];
+  __shared__ uint32_t s_w[CF_NT / 64];
+  const int tid = threadIdx.x;
+  const int x0 = blockIdx.x * CF_TX, y0 = blockIdx.y * CF_TY;
+  load_window<AH, AW, CF_NT>(A, in, in_pitch, x0 - H, y0 - H, w, h);
+  __syncthreads();
+  // row passes: soffset Gaussian over columns x0..x0+63, blur over x0-1..x0+64
+  for (int idx = tid; idx < AH * CF_TX; idx += CF_NT) {
+    const int r = idx / CF_TX, c = idx - r * CF_TX;
+    float sum = 0.0f;
+#pragma unroll
+    for (int i = 0; i <= 2 * R0; i++) sum += A[r * AW + c + (H - R0) + i] * taps0.k[i];
+    B0[idx] = sum;
+  }
+  for (int idx = tid; idx < AH * BW; idx += CF_NT) {
+    const int r = idx / BW, c = idx - r * BW;
+    float sum = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 5; i++) sum += A[r * AW + c + (H - 3) + i] * taps_c.k[i];
+    B1[idx] = sum;
+  }
+  __syncthreads();
+  // column passes: level-0 Lt of the tile, blur of the tile and its ring
+  for (int idx = tid; idx < CF_TY * CF_TX; idx += CF_NT) {
+    const int r = idx / CF_TX, c = idx - r * CF_TX;
+    const int gx = x0 + c, gy = y0 + r;
+    if (gx < w && gy < h) {
+      float sum = 0.0f;
+#pragma unroll
+      for (int i = 0; i <= 2 * R0; i++) sum += B0[(r + (H - R0) + i) * CF_TX + c] * taps0.k[i];
+      lt[(size_t)gy * pitch + gx] = sum;
+    }
+  }
+  for (int idx = tid; idx < CH * BW; idx += CF_NT) {
+    const int r = idx / BW, c = idx - r * BW;
+    float sum = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 5; i++) sum += B1[(r + (H - 3) + i) * BW + c] * taps_c.k[i];
+    C[idx] = sum;
+  }
+  __syncthreads();
+  // gradient at interior pixels (X, Y) of the tile from the blur window with
+  // top-left (X-1, Y-1), which is C[r][c] for the tile position (r, c)
   uint32_t bits = 0u;
-  for (int t = blockIdx.x; t < tiles_x * tiles_y; t += gridDim.x) {
-    int x = (t % tiles_x) * 64 + lane;
-    int y = (t / tiles_x) * 4 + wv;
-    if (x < w - 2 && y < h - 2) {
-      const float* b = blur + (size_t)y * pitch + x;
-      const int W = pitch;
+  for (int idx = tid; idx < CF_TY * CF_TX; idx += CF_NT) {
+    const int r = idx / CF_TX, c = idx - r * CF_TX;
+    const int X = x0 + c, Y = y0 + r;
+    if (X >= 1 && X <= w - 2 && Y >= 1 && Y <= h - 2) {
+      const float* b = C + r * BW + c;
+      const int W = BW;
       float dx = 3.0f * (b[0] - b[2] + b[2 * W] - b[2 * W + 2]) + 10.0f * (b[W] - b[W + 2]);
       float dy = 3.0f * (b[0] + b[2] - b[2 * W] - b[2 * W + 2]) + 10.0f * (b[1] - b[2 * W + 1]);
       float gv = sqrtf(dx * dx + dy * dy);
-      grad[(size_t)(y + 1) * pitch + (x + 1)] = gv;
-      uint32_t gb = __float_as_uint(gv);
+      grad[(size_t)Y * pitch + X] = gv;
+      const uint32_t gb = __float_as_uint(gv);
       bits = bits > gb ? bits : gb;
     }
   }
@@ -111,57 +126,52 @@ __global__ __launch_bounds__(256) void k_contrast_grad(const float* __restrict__
     uint32_t o = __shfl_xor(bits, off, 64);
     bits = bits > o ? bits : o;
   }
-  if (lane == 0) s_w[wv] = bits;
+  if ((tid & 63) == 0) s_w[akz_wave()] = bits;
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (tid == 0) {
     uint32_t m = s_w[0];
-    for (int i = 1; i < 4; i++) m = m > s_w[i] ? m : s_w[i];
-    blockmax[blockIdx.x] = m;
+    for (int i = 1; i < CF_NT / 64; i++) m = m > s_w[i] ? m : s_w[i];
+    blockmax[blockIdx.y * gridDim.x + blockIdx.x] = m;
   }
 }
 
-__global__ __launch_bounds__(1024) void k_contrast_max(const uint32_t* __restrict__ blockmax, int nb,
-                                                       uint32_t* maxbits, size_t zs) {
-  __shared__ uint32_t s_w[16];
-  blockmax = akz_zoff(blockmax, zs);
-  maxbits = akz_zoff(maxbits, zs);
-  const int lane = threadIdx.x & 63, wv = akz_wave();
-  uint32_t m = 0u;
-  for (int i = threadIdx.x; i < nb; i += 1024) m = m > blockmax[i] ? m : blockmax[i];
-  for (int off = 32; off > 0; off >>= 1) {
-    uint32_t o = __shfl_xor(m, off, 64);
-    m = m > o ? m : o;
-  }
-  if (lane == 0) s_w[wv] = m;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t r = s_w[0];
-    for (int i = 1; i < 16; i++) r = r > s_w[i] ? r : s_w[i];
-    *maxbits = r;
-  }
-}
+constexpr int CT_HIST_BLOCKS = 512;
+constexpr int CT_HIST_THREADS = 512;
 
-struct akz_octaves { int nlev; int oct[AKZ_MAX_LEVELS]; };
-
-// Histogram of grad/max over interior pixels (HistContrast, :581-602).  Bins
-// are kept per lane (LDS hist[bin][lane]) so the lanes of a wavefront never
-// collide on a bin; each block stores its bin totals in hist_part.
+// Histogram of grad/max over interior pixels (HistContrast, :581-602).  Every
+// block first reduces the tile maxima of k_contrast_front to the global
+// maximum (block 0 stores it for the percentile).  Bins are kept per lane
+// (LDS hist[bin][lane]) so the lanes of a wavefront never collide on a bin;
+// each wavefront walks whole rows; a block stores its bin totals in
+// hist_part.
 __global__ __launch_bounds__(CT_HIST_THREADS) void k_contrast_hist(const float* __restrict__ grad, int pitch,
-                                                                   int w, int h, const uint32_t* maxbits, int nbins,
-                                                                   int32_t* __restrict__ hist_part, size_t zs) {
+                                                                   int w, int h, const uint32_t* blockmax, int nbm,
+                                                                   int nbins, int32_t* __restrict__ hist_part,
+                                                                   uint32_t* maxbits, size_t zs) {
   __shared__ int32_t lh[300 * 64];  // bins >= 300 (nbins <= 512) go to `tail`
   __shared__ int32_t tail[212];
+  __shared__ uint32_t s_w[CT_HIST_THREADS / 64];
   grad = akz_zoff(grad, zs);
-  maxbits = akz_zoff(maxbits, zs);
+  blockmax = akz_zoff(blockmax, zs);
   hist_part = akz_zoff(hist_part, zs);
+  maxbits = akz_zoff(maxbits, zs);
   const int lane = threadIdx.x & 63, wv = akz_wave();
   constexpr int NW = CT_HIST_THREADS / 64;
   for (int i = threadIdx.x; i < 300 * 64; i += CT_HIST_THREADS) lh[i] = 0;
   for (int i = threadIdx.x; i < 212; i += CT_HIST_THREADS) tail[i] = 0;
+  uint32_t mb = 0u;
+  for (int i = threadIdx.x; i < nbm; i += CT_HIST_THREADS) mb = mb > blockmax[i] ? mb : blockmax[i];
+  for (int off = 32; off > 0; off >>= 1) {
+    uint32_t o = __shfl_xor(mb, off, 64);
+    mb = mb > o ? mb : o;
+  }
+  if (lane == 0) s_w[wv] = mb;
   __syncthreads();
-  const float maxv = __uint_as_float(*maxbits);
+  mb = s_w[0];
+  for (int i = 1; i < NW; i++) mb = mb > s_w[i] ? mb : s_w[i];
+  if (blockIdx.x == 0 && threadIdx.x == 0) *maxbits = mb;
+  const float maxv = __uint_as_float(mb);
   const float imaxval = 1.0f / maxv;
-  // each wavefront walks whole interior rows
   const int nwaves = gridDim.x * NW;
   for (int y = 1 + blockIdx.x * NW + wv; y < h - 1; y += nwaves) {
     const float* row = grad + (size_t)y * pitch;
@@ -764,19 +774,11 @@ inline dim3 grid64x4(int w, int h, int nz) { return dim3((w + 63) / 64, (h + 3) 
 
 }  // namespace
 
-void akz_launch_gauss(hipStream_t s, const float* in, int in_pitch, size_t in_zs, float* out, int out_pitch, int w,
-                      int h, int R, const akz_taps& taps, akz_batch bt) {
-  dim3 grid((w + GT_X - 1) / GT_X, (h + GT_Y - 1) / GT_Y, bt.nz);
-  const size_t zs = bt.zs;
-  switch (R) {
-    case 2: hipLaunchKernelGGL(k_gauss<2>, grid, dim3(GT_THREADS), 0, s, in, in_pitch, out, out_pitch, w, h, taps, in_zs, zs); break;
-    case 3: hipLaunchKernelGGL(k_gauss<3>, grid, dim3(GT_THREADS), 0, s, in, in_pitch, out, out_pitch, w, h, taps, in_zs, zs); break;
-    case 4: hipLaunchKernelGGL(k_gauss<4>, grid, dim3(GT_THREADS), 0, s, in, in_pitch, out, out_pitch, w, h, taps, in_zs, zs); break;
-    default: hipLaunchKernelGGL(k_gauss<5>, grid, dim3(GT_THREADS), 0, s, in, in_pitch, out, out_pitch, w, h, taps, in_zs, zs); break;
-  }
-}
 
-void akz_launch_contrast(hipStream_t s, const float* blur, int pitch, float* grad, int w, int h, int nbins,
+int akz_contrast_front_blocks(int w, int h) { return ((w + CF_TX - 1) / CF_TX) * ((h + CF_TY - 1) / CF_TY); }
+
+void akz_launch_contrast(hipStream_t s, const float* img, int in_pitch, size_t in_zs, float* lt, float* grad,
+                         int pitch, int w, int h, int R0, const akz_taps& taps0, const akz_taps& taps_c, int nbins,
                          float perc, uint32_t* maxbits, uint32_t* blockmax, int32_t* hist_part, float* kc, int nlev,
                          const int* octaves_host, akz_batch bt) {
   akz_octaves o;
@@ -784,17 +786,20 @@ void akz_launch_contrast(hipStream_t s, const float* blur, int pitch, float* gra
   for (int i = 0; i < nlev; i++) o.oct[i] = octaves_host[i];
   const int nz = bt.nz;
   const size_t zs = bt.zs;
-  // the histogram's blocks, which each clear and reduce 77 KB of LDS, are
-  // spread over the images of a batch; each wavefront gets a row at least
+  const dim3 fgrid((w + CF_TX - 1) / CF_TX, (h + CF_TY - 1) / CF_TY, nz);
+  switch (R0) {
+    case 2: hipLaunchKernelGGL(k_contrast_front<2>, fgrid, dim3(CF_NT), 0, s, img, in_pitch, in_zs, lt, grad, pitch, w, h, taps_c, taps0, blockmax, zs); break;
+    case 3: hipLaunchKernelGGL(k_contrast_front<3>, fgrid, dim3(CF_NT), 0, s, img, in_pitch, in_zs, lt, grad, pitch, w, h, taps_c, taps0, blockmax, zs); break;
+    case 4: hipLaunchKernelGGL(k_contrast_front<4>, fgrid, dim3(CF_NT), 0, s, img, in_pitch, in_zs, lt, grad, pitch, w, h, taps_c, taps0, blockmax, zs); break;
+    default: hipLaunchKernelGGL(k_contrast_front<5>, fgrid, dim3(CF_NT), 0, s, img, in_pitch, in_zs, lt, grad, pitch, w, h, taps_c, taps0, blockmax, zs); break;
+  }
+  // histogram blocks, which each clear and reduce 77 KB of LDS, are spread
+  // over the images of a batch; each wavefront gets a row at least
   const int per = nz > 0 ? nz : 1;
-  int gtiles = ((w - 2 + 63) / 64) * ((h - 2 + 3) / 4);
-  int gb = gtiles < CT_GRAD_BLOCKS ? gtiles : CT_GRAD_BLOCKS;
-  hipLaunchKernelGGL(k_contrast_grad, dim3(gb, 1, nz), dim3(256), 0, s, blur, pitch, grad, w, h, blockmax, zs);
-  hipLaunchKernelGGL(k_contrast_max, dim3(1, 1, nz), dim3(1024), 0, s, blockmax, gb, maxbits, zs);
-  int hb = akz_imin((h - 2 + CT_HIST_THREADS / 64 - 1) / (CT_HIST_THREADS / 64),
-                    akz_imax(CT_HIST_BLOCKS / per, 32));
-  hipLaunchKernelGGL(k_contrast_hist, dim3(hb, 1, nz), dim3(CT_HIST_THREADS), 0, s, grad, pitch, w, h, maxbits,
-                     nbins, hist_part, zs);
+  const int hb = akz_imin((h - 2 + CT_HIST_THREADS / 64 - 1) / (CT_HIST_THREADS / 64),
+                          akz_imax(CT_HIST_BLOCKS / per, 32));
+  hipLaunchKernelGGL(k_contrast_hist, dim3(hb, 1, nz), dim3(CT_HIST_THREADS), 0, s, grad, pitch, w, h, blockmax,
+                     (int)(fgrid.x * fgrid.y), nbins, hist_part, maxbits, zs);
   int32_t* bin_tot = hist_part + (size_t)512 * CT_HIST_BLOCKS;
   hipLaunchKernelGGL(k_contrast_bins, dim3(nbins, 1, nz), dim3(64), 0, s, hist_part, hb, bin_tot, zs);
   hipLaunchKernelGGL(k_contrast_finish, dim3(1, 1, nz), dim3(1024), 0, s, bin_tot, nbins, maxbits, w, h, perc, kc, o,
