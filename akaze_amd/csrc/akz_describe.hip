@@ -136,14 +136,38 @@ __global__ __launch_bounds__(64 * OR_WAVES) void k_orientation(akz_desc_params p
 
 // MLDB descriptor (ExtractDescriptors + BuildDescriptor, cuda_akaze.cu:
 // 1420-1533, 1786-1807).  Lane l accumulates the samples i = l, l+64, ... of
-// the 21x21 rotated grid into its own 87 cell sums (LDS, value-major so the
-// lanes hit distinct banks); the 64 partials of each value are then combined
-// as the reference does: p[t] + p[t+32], then a pairwise tree over 32.
+// the rotated sampling grid into its own partial sums of the 87 values (29
+// cells x 3 channels, LDS, value-major); the 64 partials of each value are
+// then combined as the reference does: p[t] + p[t+32], then a pairwise tree
+// over 32.  A value's 64 partials sit at a stride of 65 floats: the lanes
+// writing them hit consecutive words, the lanes reducing them (one value
+// each) hit different banks.
 constexpr int MLDB_WAVES = 1;
+constexpr int MLDB_VS = 65;
+
+// Cells of sample i in the 2x2, 3x3 and 4x4 grids (-1: outside that grid).
+struct mldb_cells { int c2, c3, c4; };
+__device__ inline mldb_cells mldb_cell(int i, int size2, int size3, int size4, int winsize) {
+  const int y = i / winsize, x = i - winsize * y;
+  const int mm = x > y ? x : y;
+  mldb_cells c{-1, -1, -1};
+  if (mm < 2 * size2) c.c2 = (y < size2 ? 0 : 1) * 2 + (x < size2 ? 0 : 1);
+  if (mm < 3 * size3) {
+    int x3 = (x < size3 ? 0 : (x < 2 * size3 ? 1 : 2));
+    int y3 = (y < size3 ? 0 : (y < 2 * size3 ? 1 : 2));
+    c.c3 = 4 + y3 * 3 + x3;
+  }
+  if (mm < 4 * size4) {
+    int x4 = (x < 2 * size4 ? (x < size4 ? 0 : 1) : (x < 3 * size4 ? 2 : 3));
+    int y4 = (y < 2 * size4 ? (y < size4 ? 0 : 1) : (y < 3 * size4 ? 2 : 3));
+    c.c4 = 13 + y4 * 4 + x4;
+  }
+  return c;
+}
 
 __global__ __launch_bounds__(64 * MLDB_WAVES) void k_mldb(akz_desc_params p, const akz_kp* __restrict__ kps,
                                                           const int32_t* nkps_p, uint8_t* __restrict__ desc) {
-  __shared__ float acc[87 * 64];
+  __shared__ float acc[87 * MLDB_VS];
   __shared__ float vals[88];
   const int lane = threadIdx.x & 63;
   kps = akz_zoff(kps, p.zs);
@@ -153,6 +177,28 @@ __global__ __launch_bounds__(64 * MLDB_WAVES) void k_mldb(akz_desc_params p, con
   const int size3 = (int)ceilf(2.0f * p.patsize / 3.0f);
   const int size4 = (int)ceilf(0.5f * p.patsize);
   const int winsize = akz_imax(3 * size3, 4 * size4);
+  constexpr int MS = 7;  // samples per lane (m < 7 covers grids up to 21x21)
+  // comparison pairs of the lane's output byte (byte 60 has 6 bits; its
+  // table entries 486 and 487 are (0, 0), which compare false): loaded once,
+  // the same for every keypoint
+  int ia[8], ib[8];
+#pragma unroll
+  for (int b = 0; b < 8; b++) {
+    const int c = lane < 61 ? lane * 8 + b : 486;
+    ia[b] = d_cmp.a[c];
+    ib[b] = d_cmp.b[c];
+  }
+  // cells of the lane's samples: the same for every keypoint
+  int cg[3][MS];
+#pragma unroll
+  for (int m = 0; m < MS; m++) {
+    const int i = lane + 64 * m;
+    mldb_cells cc{-1, -1, -1};
+    if (i < winsize * winsize) cc = mldb_cell(i, size2, size3, size4, winsize);
+    cg[0][m] = cc.c2;
+    cg[1][m] = cc.c3;
+    cg[2][m] = cc.c4;
+  }
   for (int kp = blockIdx.x; kp < n; kp += gridDim.x) {
     const akz_kp k = kps[kp];
     const int lev = k.class_id;
@@ -166,15 +212,13 @@ __global__ __launch_bounds__(64 * MLDB_WAVES) void k_mldb(akz_desc_params p, con
     const float yf = k.y * iratio;
     float co, si;
     akz_sincosf(k.angle, &si, &co);
-    for (int v = 0; v < 87; v++) acc[v * 64 + lane] = 0.0f;
-    // the lane's samples i = lane + 64m (m < 7 covers the 21x21 grid): all
-    // gathers are issued first, then accumulated in increasing i
-    constexpr int MS = 7;
-    float imv[MS], dxv[MS], dyv[MS];
+    for (int v = 0; v < 87; v++) acc[v * MLDB_VS + lane] = 0.0f;
+    // all gathers are issued first, then accumulated in increasing i
+    float ch[3][MS];
 #pragma unroll
     for (int m = 0; m < MS; m++) {
       const int i = lane + 64 * m;
-      imv[m] = dxv[m] = dyv[m] = 0.0f;
+      ch[0][m] = ch[1][m] = ch[2][m] = 0.0f;
       if (i < winsize * winsize) {
         const int y = i / winsize, x = i - winsize * y;
         const int l = x - size2, kk = y - size2;
@@ -183,50 +227,35 @@ __global__ __launch_bounds__(64 * MLDB_WAVES) void k_mldb(akz_desc_params p, con
         xp = akz_iclamp(xp, 0, w - 1);
         yp = akz_iclamp(yp, 0, h - 1);
         const size_t pos = (size_t)yp * pitch + xp;
-        imv[m] = imd[pos];
-        dxv[m] = dxd[pos];
-        dyv[m] = dyd[pos];
+        ch[0][m] = imd[pos];
+        ch[1][m] = dxd[pos];
+        ch[2][m] = dyd[pos];
       }
     }
+    // channels: L, rx = -dx sin + dy cos, ry = dx cos + dy sin
 #pragma unroll
     for (int m = 0; m < MS; m++) {
-      const int i = lane + 64 * m;
-      if (i >= winsize * winsize) break;
-      const int y = i / winsize;
-      const int x = i - winsize * y;
-      const int mm = akz_imax(x, y);
-      const float im = imv[m], dx = dxv[m], dy = dyv[m];
-      const float rx = -dx * si + dy * co;
-      const float ry = dx * co + dy * si;
-      if (mm < 2 * size2) {
-        int x2 = (x < size2 ? 0 : 1), y2 = (y < size2 ? 0 : 1);
-        int c = 3 * (y2 * 2 + x2);
-        acc[(c + 0) * 64 + lane] += im;
-        acc[(c + 1) * 64 + lane] += rx;
-        acc[(c + 2) * 64 + lane] += ry;
-      }
-      if (mm < 3 * size3) {
-        int x3 = (x < size3 ? 0 : (x < 2 * size3 ? 1 : 2));
-        int y3 = (y < size3 ? 0 : (y < 2 * size3 ? 1 : 2));
-        int c = 3 * (4 + y3 * 3 + x3);
-        acc[(c + 0) * 64 + lane] += im;
-        acc[(c + 1) * 64 + lane] += rx;
-        acc[(c + 2) * 64 + lane] += ry;
-      }
-      if (mm < 4 * size4) {
-        int x4 = (x < 2 * size4 ? (x < size4 ? 0 : 1) : (x < 3 * size4 ? 2 : 3));
-        int y4 = (y < 2 * size4 ? (y < size4 ? 0 : 1) : (y < 3 * size4 ? 2 : 3));
-        int c = 3 * (13 + y4 * 4 + x4);
-        acc[(c + 0) * 64 + lane] += im;
-        acc[(c + 1) * 64 + lane] += rx;
-        acc[(c + 2) * 64 + lane] += ry;
+      const float dx = ch[1][m], dy = ch[2][m];
+      ch[1][m] = -dx * si + dy * co;
+      ch[2][m] = dx * co + dy * si;
+    }
+    // samples in increasing i; within a sample the 2x2, 3x3, 4x4 cells
+#pragma unroll
+    for (int m = 0; m < MS; m++) {
+#pragma unroll
+      for (int g = 0; g < 3; g++) {
+        const int cm = cg[g][m];
+        if (cm < 0) continue;
+        acc[(3 * cm + 0) * MLDB_VS + lane] += ch[0][m];
+        acc[(3 * cm + 1) * MLDB_VS + lane] += ch[1][m];
+        acc[(3 * cm + 2) * MLDB_VS + lane] += ch[2][m];
       }
     }
     __syncthreads();
     for (int v = lane; v < 87; v += 64) {
       float a[32];
 #pragma unroll
-      for (int t = 0; t < 32; t++) a[t] = acc[v * 64 + t] + acc[v * 64 + t + 32];
+      for (int t = 0; t < 32; t++) a[t] = acc[v * MLDB_VS + t] + acc[v * MLDB_VS + t + 32];
 #pragma unroll
       for (int s = 1; s < 32; s <<= 1)
 #pragma unroll
@@ -234,15 +263,11 @@ __global__ __launch_bounds__(64 * MLDB_WAVES) void k_mldb(akz_desc_params p, con
       vals[v] = a[0];
     }
     __syncthreads();
-    uint8_t byte = 0;
-    if (lane < 61) {
-      const int nb = (lane == 60 ? 6 : 8);
-      for (int b = 0; b < nb; b++) {
-        int c = lane * 8 + b;
-        byte |= (uint8_t)((vals[d_cmp.a[c]] > vals[d_cmp.b[c]] ? 1 : 0) << b);
-      }
-    }
-    desc[(size_t)kp * AKZ_DESC_STRIDE + lane] = byte;
+    uint32_t byte = 0;
+#pragma unroll
+    for (int b = 0; b < 8; b++) byte |= (vals[ia[b]] > vals[ib[b]] ? 1u : 0u) << b;
+    if (lane >= 61) byte = 0;
+    desc[(size_t)kp * AKZ_DESC_STRIDE + lane] = (uint8_t)byte;
     __syncthreads();
   }
 }

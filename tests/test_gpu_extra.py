@@ -143,3 +143,22 @@ def test_proposal_in_graph_batches_equals_eager(gpu_lib):
         for z in range(3):
             kps, desc, n = bk.read_results(z, int(c[z]))
             assert kps.tobytes() == ref[z][0].tobytes() and np.array_equal(desc, ref[z][1]), (rep, z)
+
+
+@pytest.mark.parametrize("patsize", [8, 9])
+def test_mldb_pattern_sizes(gpu_lib, patsize):
+    """Other descriptor pattern sizes (smaller grids and cells than the
+    default 10, and for 8 lanes whose samples leave a cell and come back to
+    it) give the oracle's descriptor bits."""
+    A = gpu_lib
+    W, H = 640, 480
+    img = A.to_float(A.synth_image(23, W, H))
+    ak = A.AKAZE(A.AKAZEOptions(img_width=W, img_height=H, descriptor_pattern_size=patsize))
+    ak.Create_Nonlinear_Scale_Space(img)
+    ak.Feature_Detection()
+    kps, desc = ak.Compute_Descriptors()
+    orc = O.Pipeline(W, H, descriptor_pattern_size=patsize)
+    okp, odesc = orc.run(img)
+    assert len(kps) == len(okp) > 100
+    assert np.array_equal(kps["angle"], okp["angle"])
+    assert np.array_equal(desc, odesc), int(np.sum(desc != odesc))
