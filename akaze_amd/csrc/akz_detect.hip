@@ -332,9 +332,9 @@ __global__ __launch_bounds__(512) void k_detect_fused(akz_det_params p, int32_t*
   }
 }
 
-// Exclusive scans over per-image count arrays: block sums (k_xscan_reduce),
-// a scan of the block sums per image (k_xscan_top), then each block scans its
-// 4096 counts with its carry (k_xscan_down).  Counts at indices >= nvalid
+// Exclusive scans over per-image count arrays in two launches: block totals
+// (k_xscan_reduce), then each block adds the totals of the blocks before it
+// and scans its 4096 counts (k_xscan_down).  Counts at indices >= nvalid
 // count as zero, where nvalid is *n_ptr when given and `len` otherwise;
 // offsets are written for indices 0..nvalid, so off[nvalid] is the total.
 // Each thread owns four consecutive counts, read as one int4.
@@ -384,21 +384,6 @@ __global__ __launch_bounds__(SCAN_T) void k_xscan_reduce(const int32_t* __restri
   if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 
-__global__ __launch_bounds__(SCAN_T) void k_xscan_top(int32_t* bsum, int nblk, int32_t* total_out, size_t zs) {
-  __shared__ int32_t s_w[17];
-  bsum = akz_zoff(bsum, zs);
-  int carry = 0;
-  for (int base = 0; base < nblk; base += SCAN_T) {
-    int i = base + threadIdx.x;
-    int v = i < nblk ? bsum[i] : 0;
-    int ex;
-    int tot = block_exclusive_scan(v, &ex, s_w);
-    if (i < nblk) bsum[i] = carry + ex;
-    carry += tot;
-  }
-  if (threadIdx.x == 0) *akz_zoff(total_out, zs) = carry;
-}
-
 __global__ __launch_bounds__(SCAN_T) void k_xscan_down(const int32_t* __restrict__ cnt, int len, const int32_t* n_ptr,
                                                        const int32_t* __restrict__ bsum, int32_t* __restrict__ off,
                                                        size_t zs) {
@@ -409,34 +394,39 @@ __global__ __launch_bounds__(SCAN_T) void k_xscan_down(const int32_t* __restrict
   const int nvalid = n_ptr ? *akz_zoff(n_ptr, zs) : len;
   const int i = blockIdx.x * SCAN_BLK + threadIdx.x * SCAN_PER;
   int4 v = load4(cnt, i, nvalid);
+  // the block's carry: the totals of the blocks before it (a few hundred at
+  // most), summed here instead of in a separate launch
+  int part = 0;
+  for (int j = threadIdx.x; j < (int)blockIdx.x; j += SCAN_T) part += bsum[j];
   int ex;
+  const int carry = block_exclusive_scan(part, &ex, s_w);
   block_exclusive_scan(v.x + v.y + v.z + v.w, &ex, s_w);
-  const int o0 = bsum[blockIdx.x] + ex;
+  const int o0 = carry + ex;
   const int o[4] = {o0, o0 + v.x, o0 + v.x + v.y, o0 + v.x + v.y + v.z};
 #pragma unroll
   for (int k = 0; k < 4; k++)
     if (i + k <= nvalid) off[i + k] = o[k];
 }
 
-// After the segment scan: per-level start offsets, the candidate count (cut
-// at the capacity) and the cleared overflow flags of this detection run.
-__global__ void k_seg_finish(akz_det_params p, akz_det_bufs b) {
-  const int32_t* off = akz_zoff(b.seg_off, b.zs);
-  int32_t* ls = akz_zoff(b.level_start, b.zs);
-  int32_t* st = akz_zoff(b.status, b.zs);
-  const int t = threadIdx.x;
-  if (t < p.nlev) ls[t] = off[p.lev[t].seg_base];
-  if (t == 0) {
-    const int total = off[p.seg_total];
-    ls[p.nlev] = total;
-    st[AKZ_ST_TOTAL] = total;
-    st[AKZ_ST_FLAGS] = 0;
-    *akz_zoff(b.ncand, b.zs) = akz_imin(total, b.cap);
-  }
-}
-
 // Materialise candidates in keypoint layout (FindExtrema, cuda_akaze.cu:744-772).
+// Block 0 also writes what the later stages read of the segment scan: the
+// per-level start offsets, the candidate count (cut at the capacity) and the
+// cleared overflow flags of this detection run.
 __global__ __launch_bounds__(256) void k_scatter(akz_det_params p, akz_det_bufs b) {
+  if (blockIdx.x == 0) {
+    const int32_t* off = akz_zoff(b.seg_off, b.zs);
+    int32_t* ls = akz_zoff(b.level_start, b.zs);
+    int32_t* st = akz_zoff(b.status, b.zs);
+    const int t = threadIdx.x;
+    if (t < p.nlev) ls[t] = off[p.lev[t].seg_base];
+    if (t == 0) {
+      const int total = off[p.seg_total];
+      ls[p.nlev] = total;
+      st[AKZ_ST_TOTAL] = total;
+      st[AKZ_ST_FLAGS] = 0;
+      *akz_zoff(b.ncand, b.zs) = akz_imin(total, b.cap);
+    }
+  }
   const int seg = blockIdx.x * 256 + threadIdx.x;
   if (seg >= p.seg_total) return;
   const int32_t* seg_count = akz_zoff(b.seg_count, b.zs);
@@ -862,12 +852,11 @@ inline int grid_stride_blocks(int cap) {
   return b < 2048 ? (b > 0 ? b : 1) : 2048;
 }
 
-void xscan(hipStream_t s, const int32_t* cnt, int len, const int32_t* n_ptr, int32_t* bsum, int32_t* off,
-           int32_t* total, int nz, size_t zs) {
+void xscan(hipStream_t s, const int32_t* cnt, int len, const int32_t* n_ptr, int32_t* bsum, int32_t* off, int nz,
+           size_t zs) {
   // grid covers indices 0..len (len + 1 offsets)
   const int nblk = (len + 1 + SCAN_BLK - 1) / SCAN_BLK;
   hipLaunchKernelGGL(k_xscan_reduce, dim3(nblk, 1, nz), dim3(SCAN_T), 0, s, cnt, len, n_ptr, bsum, zs);
-  hipLaunchKernelGGL(k_xscan_top, dim3(1, 1, nz), dim3(SCAN_T), 0, s, bsum, nblk, total, zs);
   hipLaunchKernelGGL(k_xscan_down, dim3(nblk, 1, nz), dim3(SCAN_T), 0, s, cnt, len, n_ptr, bsum, off, zs);
 }
 
@@ -895,15 +884,14 @@ void akz_launch_extrema(hipStream_t s, const akz_det_params& p, const akz_det_bu
 }
 
 void akz_launch_candidates(hipStream_t s, const akz_det_params& p, const akz_det_bufs& b, int nz) {
-  xscan(s, b.seg_count, p.seg_total, nullptr, b.scan_bsum, b.seg_off, b.scan_total, nz, b.zs);
-  hipLaunchKernelGGL(k_seg_finish, dim3(1, 1, nz), dim3(64), 0, s, p, b);
+  xscan(s, b.seg_count, p.seg_total, nullptr, b.scan_bsum, b.seg_off, nz, b.zs);
   hipLaunchKernelGGL(k_scatter, dim3((p.seg_total + 255) / 256, 1, nz), dim3(256), 0, s, p, b);
 }
 
 void akz_launch_suppression(hipStream_t s, const akz_det_bufs& b, int nz) {
   const int g = grid_stride_blocks(b.cap);
   hipLaunchKernelGGL(k_neighbors<false>, dim3(g, 1, nz), dim3(256), 0, s, b);
-  xscan(s, b.nb_count, b.cap, b.ncand, b.nb_bsum, b.nb_off, b.nb_total, nz, b.zs);
+  xscan(s, b.nb_count, b.cap, b.ncand, b.nb_bsum, b.nb_off, nz, b.zs);
   hipLaunchKernelGGL(k_neighbors<true>, dim3(g, 1, nz), dim3(256), 0, s, b);
   hipLaunchKernelGGL(k_nms, dim3(1, 1, nz), dim3(NMS_THREADS), 0, s, b);
 }

@@ -14,8 +14,8 @@ struct akz_taps { float k[11]; };
 // soffset Gaussian of `img` into `lt` (radius R0, taps0) and the gradient of
 // its sigma=1 blur (taps_c) into `grad` in one launch, with the tile maxima
 // in `blockmax` (akz_contrast_front_blocks(w, h) entries); then the
-// histogram, the bin totals and the percentile, leaving the per-level factor
-// in kc[0..nlev).  hist_part: 512 x 512 + 512 ints; maxbits: one word.
+// histogram and the percentile (bin totals included), leaving the per-level factor
+// in kc[0..nlev).  hist_part: 512 x 512 ints; maxbits: one word.
 int akz_contrast_front_blocks(int w, int h);
 void akz_launch_contrast(hipStream_t s, const float* img, int in_pitch, size_t in_zs, float* lt, float* grad,
                          int pitch, int w, int h, int R0, const akz_taps& taps0, const akz_taps& taps_c, int nbins,

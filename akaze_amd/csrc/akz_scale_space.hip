@@ -198,34 +198,23 @@ __global__ __launch_bounds__(CT_HIST_THREADS) void k_contrast_hist(const float* 
 // Percentile search of ContrastPercentile (:637-642) over the summed block
 // histograms, on the device so no value round-trips through the host; the
 // per-level factor decays by 0.75 at each new octave (AKAZE.cpp:254-256).
-// Counts are integers: the result does not depend on the summation order.
-// Bin totals over the block histograms: one wavefront per bin, lanes read
-// consecutive block partials, a single round of loads per lane.
-__global__ __launch_bounds__(64) void k_contrast_bins(const int32_t* __restrict__ hist_part, int nb,
-                                                      int32_t* __restrict__ tot, size_t zs) {
-  const int lane = threadIdx.x, i = blockIdx.x;
-  hist_part = akz_zoff(hist_part, zs);
-  tot = akz_zoff(tot, zs);
-  const int32_t* row = hist_part + (size_t)i * CT_HIST_BLOCKS;
-  int sum = 0;
-#pragma unroll
-  for (int k = 0; k < CT_HIST_BLOCKS / 64; k++) {
-    const int b = lane + 64 * k;
-    if (b < nb) sum += row[b];
-  }
-  for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
-  if (lane == 0) tot[i] = sum;
-}
-
-__global__ __launch_bounds__(1024) void k_contrast_finish(const int32_t* __restrict__ bin_tot, int nbins,
+// Thread i adds bin i over the nb block histograms (counts are integers: the
+// result does not depend on the order of the additions).
+__global__ __launch_bounds__(1024) void k_contrast_finish(const int32_t* __restrict__ hist_part, int nb, int nbins,
                                                           const uint32_t* maxbits, int w, int h, float perc, float* kc,
                                                           akz_octaves octs, size_t zs) {
   __shared__ int32_t tot[512];
-  bin_tot = akz_zoff(bin_tot, zs);
+  hist_part = akz_zoff(hist_part, zs);
   maxbits = akz_zoff(maxbits, zs);
   kc = akz_zoff(kc, zs);
   const int lane = threadIdx.x & 63, wv = akz_wave();
-  for (int i = threadIdx.x; i < nbins; i += 1024) tot[i] = bin_tot[i];
+  if ((int)threadIdx.x < nbins) {
+    const int32_t* row = hist_part + (size_t)threadIdx.x * CT_HIST_BLOCKS;
+    int sum = 0;
+#pragma unroll 8
+    for (int b = 0; b < nb; b++) sum += row[b];
+    tot[threadIdx.x] = sum;
+  }
   __syncthreads();
   // percentile: the loop of ContrastPercentile stops at k = 1 + the first bin
   // whose cumulative count reaches nthreshold (k = 0 when nthreshold <= 0);
@@ -800,10 +789,8 @@ void akz_launch_contrast(hipStream_t s, const float* img, int in_pitch, size_t i
                           akz_imax(CT_HIST_BLOCKS / per, 32));
   hipLaunchKernelGGL(k_contrast_hist, dim3(hb, 1, nz), dim3(CT_HIST_THREADS), 0, s, grad, pitch, w, h, blockmax,
                      (int)(fgrid.x * fgrid.y), nbins, hist_part, maxbits, zs);
-  int32_t* bin_tot = hist_part + (size_t)512 * CT_HIST_BLOCKS;
-  hipLaunchKernelGGL(k_contrast_bins, dim3(nbins, 1, nz), dim3(64), 0, s, hist_part, hb, bin_tot, zs);
-  hipLaunchKernelGGL(k_contrast_finish, dim3(1, 1, nz), dim3(1024), 0, s, bin_tot, nbins, maxbits, w, h, perc, kc, o,
-                     zs);
+  hipLaunchKernelGGL(k_contrast_finish, dim3(1, 1, nz), dim3(1024), 0, s, hist_part, hb, nbins, maxbits, w, h, perc,
+                     kc, o, zs);
 }
 
 template <int TH>
