@@ -151,7 +151,11 @@ __device__ __forceinline__ void detect_interior(const float* s_l, float* s_x, fl
     s_x[r * R + c] = fac1 * (ur + lr - ul - ll) + fac2 * (cr - cl);
     s_y[r * R + c] = fac1 * (lr + ll - ur - ul) + fac2 * (lc - uc);
   };
-  for (int r = wv; r < R2n; r += NW) deriv(r, lane);
+  // rows wv, wv + NW, ...: the first R2n / NW rows of every wavefront
+  // exist, unrolled without tests; then at most one more
+#pragma unroll
+  for (int k = 0; k < R2n / NW; k++) deriv(wv + k * NW, lane);
+  if (R2n % NW && wv + (R2n / NW) * NW < R2n) deriv(wv + (R2n / NW) * NW, lane);
   for (int k = threadIdx.x; k < R2n * E2; k += 64 * NW) deriv(k % R2n, 64 + k / R2n);
   __syncthreads();
   auto det = [&](int r, int c) {
@@ -167,7 +171,9 @@ __device__ __forceinline__ void detect_interior(const float* s_l, float* s_x, fl
     const float lyy = fac1 * (lr + ll - ur - ul) + fac2 * (lc - uc);
     s_d[r * R + c] = lxx * lyy - lyx * lyx;
   };
-  for (int r = wv; r < DF_RD; r += NW) det(r, lane);
+#pragma unroll
+  for (int k = 0; k < DF_RD / NW; k++) det(wv + k * NW, lane);
+  if (DF_RD % NW && wv + (DF_RD / NW) * NW < DF_RD) det(wv + (DF_RD / NW) * NW, lane);
   for (int k = threadIdx.x; k < DF_RD * 2; k += 64 * NW) det(k % DF_RD, 64 + k / DF_RD);
 }
 
@@ -578,15 +584,16 @@ __device__ __forceinline__ akz_kp emit(const akz_kp& ok) {
 }
 
 // Output of sortFiltered_kernel (cuda_akaze.cu:1227-1316): per 2048-slot
-// chunk, the kept member values whose point is strong, ascending.  The
+// chunk, the kept member values whose point is strong (strong_at: size >= 0),
+// ascending.  The
 // member values of a chunk are distinct candidate indices, so a bitmap over
 // [0, n) and a prefix popcount of its words give every value its rank;
 // candidate counts beyond the bitmap use a bitonic sort.  Returns the number
 // of keypoints (thread 0's value).
-template <class MemAt>
-__device__ int nms_output(const akz_kp* __restrict__ cand, int n, MemAt mem_at, akz_kp* __restrict__ out,
-                          int32_t* __restrict__ kidx, int cap, int32_t* s_sort, uint32_t* s_bits, int32_t* s_pre,
-                          int32_t* s_w, int32_t* s_run) {
+template <class MemAt, class StrongAt>
+__device__ int nms_output(const akz_kp* __restrict__ cand, int n, MemAt mem_at, StrongAt strong_at,
+                          akz_kp* __restrict__ out, int32_t* __restrict__ kidx, int cap, int32_t* s_sort,
+                          uint32_t* s_bits, int32_t* s_pre, int32_t* s_w, int32_t* s_run) {
   const int t = threadIdx.x;
   if (t == 0) *s_run = 0;
   __syncthreads();
@@ -598,7 +605,7 @@ __device__ int nms_output(const akz_kp* __restrict__ cand, int n, MemAt mem_at, 
       for (int e = t; e < 2048; e += NMS_THREADS) {
         const int i = base + e;
         const int v = (i < n) ? mem_at(i) : -1;
-        if (v >= 0 && !(cand[v].size < 0)) atomicOr(&s_bits[v >> 5], 1u << (v & 31));
+        if (v >= 0 && strong_at(v)) atomicOr(&s_bits[v >> 5], 1u << (v & 31));
       }
       __syncthreads();
       // exclusive prefix of word popcounts, two words per thread
@@ -613,7 +620,7 @@ __device__ int nms_output(const akz_kp* __restrict__ cand, int n, MemAt mem_at, 
       for (int e = t; e < 2048; e += NMS_THREADS) {
         const int i = base + e;
         const int v = (i < n) ? mem_at(i) : -1;
-        if (v >= 0 && !(cand[v].size < 0)) {
+        if (v >= 0 && strong_at(v)) {
           const int rank = s_pre[v >> 5] + __popc(s_bits[v >> 5] & ((1u << (v & 31)) - 1u));
           if (run + rank < cap) {
             out[run + rank] = emit(cand[v]);
@@ -628,7 +635,7 @@ __device__ int nms_output(const akz_kp* __restrict__ cand, int n, MemAt mem_at, 
       for (int e = t; e < 2048; e += NMS_THREADS) {
         const int i = base + e;
         const int v = (i < n) ? mem_at(i) : -1;
-        const bool ok = v >= 0 && !(cand[v].size < 0);
+        const bool ok = v >= 0 && strong_at(v);
         s_sort[e] = ok ? v : INT32_MAX;
       }
       __syncthreads();
@@ -677,10 +684,10 @@ __device__ int nms_output(const akz_kp* __restrict__ cand, int n, MemAt mem_at, 
 // of the unprocessed candidates.
 __global__ __launch_bounds__(NMS_THREADS) void k_nms(akz_det_bufs b) {
   __shared__ int16_t s_mem[NMS_LDS_N];
-  __shared__ int16_t s_code[NMS_LDS_N];
   __shared__ int32_t s_minnb[NMS_LDS_N];
   __shared__ uint16_t s_off[NMS_LDS_N + 2];
   __shared__ float s_resp[NMS_LDS_N];
+  __shared__ uint8_t s_strong[NMS_LDS_N];  // 1: size >= 0 (emitted when kept)
   __shared__ __align__(16) int16_t s_nb[NMS_LDS_NB];  // the output phase reuses it (24 KB)
   __shared__ int32_t s_w[17];
   __shared__ int32_t s_nact[2];
@@ -702,69 +709,158 @@ __global__ __launch_bounds__(NMS_THREADS) void k_nms(akz_det_bufs b) {
   int wave = 0;
   int kept;
   if (lds) {
-    for (int i = t; i < n; i += NMS_THREADS) {
-      s_mem[i] = -1;
-      s_code[i] = CODE_WAIT;
-      s_minnb[i] = INT32_MAX;
-      s_resp[i] = cand[i].response;
+    // thread t owns candidates t + j * NMS_THREADS; their pending bits, the
+    // readiness-scan resume points, list ends and codes stay in registers
+    constexpr int PER = NMS_LDS_N / NMS_THREADS;
+    int rs[PER], e1[PER], cd[PER];
+    uint32_t pend = 0;
+    // every global load of a thread is issued before its LDS stores
+    {
+      float rv[PER], sz[PER];
+      int ov[PER + 1];
+#pragma unroll
+      for (int j = 0; j < PER; j++) {
+        const int i = t + j * NMS_THREADS;
+        if (i < n) { rv[j] = cand[i].response; sz[j] = cand[i].size; }
+      }
+#pragma unroll
+      for (int j = 0; j <= PER; j++) {
+        const int i = t + j * NMS_THREADS;
+        if (i <= n) ov[j] = nb_off[i];
+      }
+      constexpr int PER4 = NMS_LDS_NB / 4 / NMS_THREADS;
+      int4 lv[PER4];
+      const int n4 = (ntot + 3) >> 2;
+#pragma unroll
+      for (int j = 0; j < PER4; j++) {
+        const int k4 = t + j * NMS_THREADS;
+        if (4 * k4 + 3 < ntot) {
+          lv[j] = reinterpret_cast<const int4*>(nb_list)[k4];
+        } else if (k4 < n4) {
+          lv[j].x = nb_list[4 * k4];
+          lv[j].y = 4 * k4 + 1 < ntot ? nb_list[4 * k4 + 1] : 0;
+          lv[j].z = 4 * k4 + 2 < ntot ? nb_list[4 * k4 + 2] : 0;
+          lv[j].w = 0;
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < PER; j++) {
+        const int i = t + j * NMS_THREADS;
+        if (i < n) {
+          s_mem[i] = -1;
+          s_minnb[i] = INT32_MAX;
+          s_resp[i] = rv[j];
+          s_strong[i] = !(sz[j] < 0);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j <= PER; j++) {
+        const int i = t + j * NMS_THREADS;
+        if (i <= n) s_off[i] = (uint16_t)akz_imin(ov[j], b.nb_cap);
+        if (j < PER) {
+          if (i < n) {
+            rs[j] = akz_imin(ov[j], b.nb_cap);
+            pend |= 1u << j;
+          } else {
+            rs[j] = 0;
+          }
+          cd[j] = CODE_WAIT;
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < PER4; j++) {
+        const int k4 = t + j * NMS_THREADS;
+        if (k4 < n4) {
+          // 4 * k4 + 3 < 4 * n4 <= NMS_LDS_NB: inside s_nb
+          s_nb[4 * k4] = (int16_t)lv[j].x;
+          s_nb[4 * k4 + 1] = (int16_t)lv[j].y;
+          s_nb[4 * k4 + 2] = (int16_t)lv[j].z;
+          s_nb[4 * k4 + 3] = (int16_t)lv[j].w;
+        }
+      }
     }
-    for (int i = t; i <= n; i += NMS_THREADS) s_off[i] = (uint16_t)akz_imin(nb_off[i], b.nb_cap);
-    for (int k = t; k < ntot; k += NMS_THREADS) s_nb[k] = (int16_t)nb_list[k];
     __syncthreads();
+    // list ends: the next candidate's start
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+      const int i = t + j * NMS_THREADS;
+      e1[j] = i < n ? (int)s_off[i + 1] : 0;
+    }
     bool any = n > 0;
     while (wave < b.max_waves && any) {
+      // Arbitration key of this wave: every neighbour of a candidate has a
+      // lower index, so the lowest pending candidate is always ready and
+      // wins; a wave with work left therefore has wave < n <= NMS_LDS_N and
+      // keys of later waves are smaller than every key left from earlier
+      // ones, which makes a reset of the slots unnecessary.
+      const int tag = (NMS_LDS_N - 1 - wave) << 13;
       // phase 1: classify ready candidates and arbitrate contention
-      for (int c = t; c < n; c += NMS_THREADS) {
-        if (s_mem[c] != -1) continue;
-        const int e0 = s_off[c], e1 = s_off[c + 1];
-        bool ready = true, anykept = false;
-        int best = -1, bestv = INT32_MAX;
-        for (int k = e0; k < e1; k++) {
-          const int q = s_nb[k];
-          const int m = s_mem[q];
-          if (m == -1) { ready = false; break; }
-          if (m >= 0) {
-            anykept = true;
-            if (m < bestv) { bestv = m; best = q; }
-          }
+#pragma unroll
+      for (int j = 0; j < PER; j++) {
+        if (!(pend & (1u << j))) continue;
+        const int c = t + j * NMS_THREADS;
+        // a processed neighbour never becomes pending again: resume the
+        // readiness scan at the first neighbour that was pending
+        // (two neighbours per step: both loads in flight together)
+        int k = rs[j];
+        const int e = e1[j];
+        for (; k + 1 < e; k += 2) {
+          const int ma = s_mem[s_nb[k]], mb = s_mem[s_nb[k + 1]];
+          if (ma == -1) break;
+          if (mb == -1) { k++; break; }
         }
+        if (k == e - 1 && s_mem[s_nb[k]] != -1) k = e;
+        rs[j] = k;
         int code = CODE_WAIT;
-        if (ready) {
-          if (!anykept) code = CODE_ADD;
-          else { code = best; atomicMin(&s_minnb[best], c); }
+        if (k == e) {
+          // the kept neighbour with the smallest member value, first in
+          // list order on ties; four neighbours per step
+          int best = -1, bestv = INT32_MAX;
+          int q0 = s_off[c];
+          for (; q0 + 3 < e; q0 += 4) {
+            const int qa = s_nb[q0], qb = s_nb[q0 + 1], qc = s_nb[q0 + 2], qd = s_nb[q0 + 3];
+            const int ma = s_mem[qa], mb = s_mem[qb], mc = s_mem[qc], md = s_mem[qd];
+            if (ma >= 0 && ma < bestv) { bestv = ma; best = qa; }
+            if (mb >= 0 && mb < bestv) { bestv = mb; best = qb; }
+            if (mc >= 0 && mc < bestv) { bestv = mc; best = qc; }
+            if (md >= 0 && md < bestv) { bestv = md; best = qd; }
+          }
+          for (; q0 < e; q0++) {
+            const int q = s_nb[q0];
+            const int m = s_mem[q];
+            if (m >= 0 && m < bestv) { bestv = m; best = q; }
+          }
+          if (best < 0) code = CODE_ADD;
+          else { code = best; atomicMin(&s_minnb[best], tag | c); }
         }
-        s_code[c] = (int16_t)code;
+        cd[j] = code;
       }
       __syncthreads();
-      // phase 2: additions and replacements (targets are kept, never active)
-      for (int c = t; c < n; c += NMS_THREADS) {
-        if (s_mem[c] != -1) continue;
-        const int code = s_code[c];
+      // phase 2: additions and replacements (targets are kept, never pending)
+#pragma unroll
+      for (int j = 0; j < PER; j++) {
+        if (!(pend & (1u << j))) continue;
+        const int c = t + j * NMS_THREADS;
+        const int code = cd[j];
         if (code == CODE_ADD) {
           s_mem[c] = (int16_t)c;
-        } else if (code >= 0 && s_minnb[code] == c) {
+          pend &= ~(1u << j);
+        } else if (code >= 0 && s_minnb[code] == (tag | c)) {
           if (s_resp[c] > s_resp[code]) {
             s_mem[c] = (int16_t)code;
             s_mem[code] = -2;
           } else {
             s_mem[c] = -2;
           }
+          pend &= ~(1u << j);
         }
       }
-      __syncthreads();
-      // phase 3: reset the arbitration slots and codes, look for work left
-      bool left = false;
-      for (int c = t; c < n; c += NMS_THREADS) {
-        const int code = s_code[c];
-        if (code >= 0) s_minnb[code] = INT32_MAX;
-        if (code != CODE_WAIT) s_code[c] = CODE_WAIT;
-        left |= s_mem[c] == -1;
-      }
       wave++;
-      any = __syncthreads_or(left);
+      any = __syncthreads_or(pend != 0);
     }
-    kept = nms_output(cand, n, [&](int i) { return (int)s_mem[i]; }, out, kidx, cap, s_sort, s_bits, s_pre, s_w,
-                      &s_run);
+    kept = nms_output(
+        cand, n, [&](int i) { return (int)s_mem[i]; }, [&](int v) { return s_strong[v] != 0; }, out, kidx, cap,
+        s_sort, s_bits, s_pre, s_w, &s_run);
   } else {
     int32_t* mem = akz_zoff(b.mem, b.zs);
     int32_t* minnb = akz_zoff(b.minnb, b.zs);
@@ -835,7 +931,9 @@ __global__ __launch_bounds__(NMS_THREADS) void k_nms(akz_det_bufs b) {
       wave++;
       __syncthreads();
     }
-    kept = nms_output(cand, n, [&](int i) { return mem[i]; }, out, kidx, cap, s_sort, s_bits, s_pre, s_w, &s_run);
+    kept = nms_output(
+        cand, n, [&](int i) { return mem[i]; }, [&](int v) { return !(cand[v].size < 0); }, out, kidx, cap, s_sort,
+        s_bits, s_pre, s_w, &s_run);
   }
   if (t == 0) {
     st[AKZ_ST_WAVES] = wave;
