@@ -425,9 +425,10 @@ void enqueue_scale_space(akz_ctx* c, const float* img, int in_pitch, size_t in_z
       base = P.Lt;  // Copy (AKAZE.cpp:258) is folded into the first step's read
       if (n == 0) akz_launch_copy2d(s, P.Lt, P.pitch, L.Lt, L.pitch, L.w, L.h, bt);
     }
-    // LDS variants build Lsmooth and g in the level's first FED launch; g
-    // goes through HBM only when further launches of the level read it
-    const bool fuse = bs > 1 && n > 0 && akz_fed_variant_fusable(c->fed_variant[i]);
+    // the level's first FED launch builds Lsmooth and g where that pays
+    // (akz_fed_variant_fusable); g goes through HBM only when further
+    // launches of the level read it
+    const bool fuse = bs > 1 && n > 0 && akz_fed_variant_fusable(c->fed_variant[i], std::min(bs, n));
     if (!fuse)
       akz_launch_smooth_flow(s, base, L.pitch, L.Lsm, c->g, L.w, L.h, c->opts.diffusivity, c->kc + i, c->taps1, bt);
     const float* cur = base;
@@ -963,7 +964,8 @@ akz_status akz_profile_fed_batch(akz_ctx* c, const float* d_imgs, size_t pitch_b
     // pixel); a level whose first launch also builds Lsmooth and g adds that
     // pass's (read L, write Lsmooth and g)
     bytes += 12.0 * c->lev[i].w * c->lev[i].h * c->lev[i].nsteps * n;
-    if (level_block(c, i) > 1 && c->lev[i].nsteps > 0 && akz_fed_variant_fusable(c->fed_variant[i]))
+    if (level_block(c, i) > 1 && c->lev[i].nsteps > 0 &&
+        akz_fed_variant_fusable(c->fed_variant[i], std::min(level_block(c, i), c->lev[i].nsteps)))
       bytes += 12.0 * c->lev[i].w * c->lev[i].h * n;
     launches += (c->lev[i].nsteps + level_block(c, i) - 1) / level_block(c, i);
   }
@@ -1063,9 +1065,9 @@ akz_status akz_autotune(akz_ctx* c, int nz, int reps) {
       // once to warm up, then `reps` timed times, ping-ponging Lt and the scratch
       const int bs = std::min(c->fed_block, akz_fed_variant_steps(v));
       const int nl = (L.nsteps + bs - 1) / bs;
-      const bool fuse = bs > 1 && akz_fed_variant_fusable(v);
-      // as in the pipeline: smooth + flow, fused into the first launch for
-      // the LDS variants
+      const bool fuse = bs > 1 && akz_fed_variant_fusable(v, std::min(bs, L.nsteps));
+      // as in the pipeline: smooth + flow, fused into the first launch of
+      // the blocked variants
       auto run_level = [&](int parity) {
         int launch = parity;
         if (!fuse)

@@ -42,9 +42,11 @@ void akz_launch_fed_multi(hipStream_t s, const float* Lin, const float* g, float
 // n <= akz_fed_variant_steps(variant)
 void akz_launch_fed_variant(hipStream_t s, int variant, const float* Lin, const float* g, float* Lout, int pitch,
                             int w, int h, const float* half_tau, int n, akz_batch bt);
-// LDS variants can build the level's Lsmooth and conductance in their first
-// launch (k_fed_multi_sf): one launch instead of two
-bool akz_fed_variant_fusable(int variant);
+// whether a level's first launch of n steps with this variant also builds
+// the level's Lsmooth and conductance (k_fed_multi_sf, k_fed_reg_sf): one
+// launch instead of two; every LDS variant does, the register variants when
+// the region's halo is small
+bool akz_fed_variant_fusable(int variant, int n);
 void akz_launch_fed_variant_sf(hipStream_t s, int variant, const float* in, float* lsm, float* gout, float* Lout,
                                int pitch, int w, int h, int type, const float* kc, const akz_taps& taps,
                                const float* half_tau, int n, akz_batch bt);

@@ -565,6 +565,16 @@ void k_fed_reg(const float* __restrict__ L, const float* __restrict__ g,
   else fed_reg_body<R, true>(L, g, out, pitch, w, h, st, x0, y0);
 }
 
+// Diffusivity of Flow (cuda_akaze.cu:240-287) from dif2 = (Lx^2 + Ly^2) / k^2:
+// 0 Perona-Malik g1, 1 Perona-Malik g2, 2 Weickert, 3 Charbonnier.
+template <int TYPE>
+__device__ __forceinline__ float diffusivity(float dif2) {
+  if (TYPE == 0) return expf(-dif2);
+  if (TYPE == 1) return 1.0f / (1.0f + dif2);
+  if (TYPE == 2) return 1.0f - (float)exp(-3.315 / (dif2 * dif2 * dif2 * dif2));
+  return 1.0f / sqrtf(1.0f + dif2);
+}
+
 // Lsmooth and conductance of one level in one pass (AKAZE.cpp:260-261): the
 // sigma=1 Gaussian of LowPass (5 taps, replicate border, row pass then
 // column pass, sums in tap order from 0.0f) fused with Flow (unnormalised
@@ -633,16 +643,7 @@ __device__ __forceinline__ void smooth_flow_reg_body(const float* __restrict__ i
     const float ul = CL[im], ur = CR[im], ll = CL[ip], lr = CR[ip];
     const float lx = 3.0f * (lr - ll + ur - ul) + 10.0f * (CR[r + 1] - CL[r + 1]);
     const float ly = 3.0f * (lr + ll - ur - ul) + 10.0f * (C[ip] - C[im]);
-    const float dif2 = invk * (lx * lx + ly * ly);
-    float res;
-    if (TYPE == 0)
-      res = expf(-dif2);
-    else if (TYPE == 1)
-      res = 1.0f / (1.0f + dif2);
-    else if (TYPE == 2)
-      res = 1.0f - (float)exp(-3.315 / (dif2 * dif2 * dif2 * dif2));
-    else
-      res = 1.0f / sqrtf(1.0f + dif2);
+    const float res = diffusivity<TYPE>(invk * (lx * lx + ly * ly));
     if (out_col) {
       const size_t o = (size_t)y * pitch + gx;
       lsm[o] = C[r + 1];
@@ -670,6 +671,149 @@ __global__ __launch_bounds__(64) void k_smooth_flow_reg(const float* __restrict_
     smooth_flow_reg_body<TH, TYPE, false>(in, pitch, lsm, g, w, h, invk, taps, x0, y0);
   else
     smooth_flow_reg_body<TH, TYPE, true>(in, pitch, lsm, g, w, h, invk, taps, x0, y0);
+}
+
+// The first FED launch of a level with the register variants, fused with
+// the level's smooth + flow: the work of k_smooth_flow_reg and k_fed_reg in
+// one pass, with the same arithmetic in the same order.  One wavefront per
+// region of 64 columns x R rows: the R + 6 input rows (3 above, 3 below)
+// are loaded once and become the FED rows; the row pass, column pass and
+// Scharr/diffusivity stream down the region, leaving the conductance of
+// every region row in registers, and Lsmooth (and g, when later launches of
+// the level read it) is written for the tile.  g is right on lanes 3..60,
+// so after n steps lanes 3+n..60-n and rows n..R-1-n are the tile.  Outside
+// the image the conductance is computed from clamped input: finite, and
+// only ever multiplied by a zero flux difference at the border.
+template <int R, int TYPE, bool BORDER>
+__device__ __forceinline__ void fed_reg_sf_body(const float* __restrict__ in, float* __restrict__ lsm,
+                                                float* __restrict__ gout, float* __restrict__ out, int pitch, int w,
+                                                int h, float invk, const akz_taps& taps, const akz_fed_steps& st,
+                                                int x0, int y0) {
+  constexpr int NA = R + 6;
+  const int lane = threadIdx.x;
+  const int gx = x0 + lane;
+  const int cx = BORDER ? akz_iclamp(gx, 0, w - 1) : gx;
+  const int hn = st.n;
+  // A[i]: input row y0 - 3 + i; rows 3..R+2 are the FED rows
+  float A[NA];
+#pragma unroll
+  for (int i = 0; i < NA; i++) {
+    const int gy = BORDER ? akz_iclamp(y0 - 3 + i, 0, h - 1) : y0 - 3 + i;
+    A[i] = in[(size_t)gy * pitch + cx];
+  }
+  const float k0 = taps.k[0], k1 = taps.k[1], k2 = taps.k[2], k3 = taps.k[3], k4 = taps.k[4];
+  const bool first_col = BORDER && gx == 0, last_col = BORDER && gx == w - 1;
+  const bool out_col = lane >= 3 + hn && lane < 61 - hn && gx < w;
+  float Wv[R], Wh[R];
+  // rings: B rows i-4..i, C / CL / CR rows j-2..j (C[j]: row y0 - 1 + j)
+  float B[5], C[3], CL[3], CR[3];
+#pragma unroll
+  for (int i = 0; i < NA; i++) {
+    {
+      const float m1 = from_left(A[i]), p1 = from_right(A[i]);
+      const float m2 = from_left(m1), p2 = from_right(p1);
+      float sum = 0.0f;
+      sum += m2 * k0;
+      sum += m1 * k1;
+      sum += A[i] * k2;
+      sum += p1 * k3;
+      sum += p2 * k4;
+      B[i % 5] = sum;
+    }
+    const int j = i - 4;
+    if (j < 0) continue;
+    {
+      float sum = 0.0f;
+      sum += B[(j + 0) % 5] * k0;
+      sum += B[(j + 1) % 5] * k1;
+      sum += B[(j + 2) % 5] * k2;
+      sum += B[(j + 3) % 5] * k3;
+      sum += B[(j + 4) % 5] * k4;
+      C[j % 3] = sum;
+      const float fl = from_left(sum), fr = from_right(sum);
+      CL[j % 3] = first_col ? fr : fl;
+      CR[j % 3] = last_col ? fl : fr;
+    }
+    const int r = j - 2;  // region row y0 + r: C rows r, r+1, r+2 exist
+    if (r < 0) continue;
+    // opaque per row, so the row tests stay scalar compares here instead
+    // of R hoisted masks (which spill)
+    int ytop = y0, hk = hn;
+    asm volatile("" : "+s"(ytop), "+s"(hk));
+    const int y = ytop + r;
+    const bool top = BORDER && y == 0, bot = BORDER && y == h - 1;
+    const int im = (top ? r + 2 : r) % 3, ip = (bot ? r : r + 2) % 3, ic = (r + 1) % 3;
+    const float ul = CL[im], ur = CR[im], ll = CL[ip], lr = CR[ip];
+    const float lx = 3.0f * (lr - ll + ur - ul) + 10.0f * (CR[ic] - CL[ic]);
+    const float ly = 3.0f * (lr + ll - ur - ul) + 10.0f * (C[ip] - C[im]);
+    const float res = diffusivity<TYPE>(invk * (lx * lx + ly * ly));
+    Wv[r] = res;  // g for now
+    if (out_col && r >= hk && r < R - hk && (!BORDER || y < h)) {
+      const size_t o = (size_t)y * pitch + gx;
+      lsm[o] = C[ic];
+      if (gout) gout[o] = res;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < R; r++) Wh[r] = Wv[r] + from_right(Wv[r]);
+#pragma unroll
+  for (int r = 0; r < R - 1; r++) Wv[r] = Wv[r] + Wv[r + 1];
+  const bool east_self = BORDER && gx >= w - 1, west_self = BORDER && gx <= 0;
+  for (int s = 0; s < st.n; s++) {
+    const float stepsize = st.half_tau[s];
+    int ytop = y0;
+    if (BORDER) asm volatile("" : "+s"(ytop));
+    float up = A[3];
+#pragma unroll
+    for (int r = 1; r < R - 1; r++) {
+      const float i0 = A[r + 3];
+      float le = from_right(i0), lw = from_left(i0);
+      float ln = up, ls = A[r + 4];
+      if (BORDER) {
+        if (east_self) le = i0;
+        if (west_self) lw = i0;
+        if (ytop + r <= 0) ln = i0;
+        if (ytop + r >= h - 1) ls = i0;
+      }
+      const float ww = from_left(Wh[r]);
+      const float xpos = Wh[r] * (le - i0);
+      const float xneg = ww * (i0 - lw);
+      const float ypos = Wv[r] * (ls - i0);
+      const float yneg = Wv[r - 1] * (i0 - ln);
+      up = i0;
+      A[r + 3] = i0 + stepsize * (xpos - xneg + ypos - yneg);
+    }
+  }
+  if (!out_col) return;
+#pragma unroll
+  for (int r = 1; r < R - 1; r++) {
+    const int gy = y0 + r;
+    if (r >= hn && r < R - hn && gy < h) out[(size_t)gy * pitch + gx] = A[r + 3];
+  }
+}
+
+template <int R, int TYPE>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(R == 48 ? 3 : 1)))
+void k_fed_reg_sf(const float* __restrict__ in, float* __restrict__ lsm, float* __restrict__ gout,
+                  float* __restrict__ out, int pitch, int w, int h, const float* kcp, akz_taps taps,
+                  akz_fed_steps st, size_t zs) {
+  const akz_tile3 tl = akz_xcd_tile();
+  in = akz_zoff(in, zs, tl.z);
+  lsm = akz_zoff(lsm, zs, tl.z);
+  if (gout) gout = akz_zoff(gout, zs, tl.z);
+  out = akz_zoff(out, zs, tl.z);
+  kcp = akz_zoff(kcp, zs, tl.z);
+  const float kcontrast = *kcp;
+  const float invk = 1.0f / (kcontrast * kcontrast);
+  const int hn = st.n;
+  const int x0 = tl.x * (58 - 2 * hn) - 3 - hn, y0 = tl.y * (R - 2 * hn) - hn;
+  // every input pixel inside the image, no Scharr reflection and no region
+  // pixel on or beyond an image border
+  const bool interior = x0 >= 1 && x0 + 65 <= w && y0 >= 3 && y0 + R + 3 <= h;
+  if (interior)
+    fed_reg_sf_body<R, TYPE, false>(in, lsm, gout, out, pitch, w, h, invk, taps, st, x0, y0);
+  else
+    fed_reg_sf_body<R, TYPE, true>(in, lsm, gout, out, pitch, w, h, invk, taps, st, x0, y0);
 }
 
 // One FED step, LDS-tiled: a 64x16 output tile plus a one-pixel halo of L and
@@ -903,12 +1047,34 @@ void akz_launch_fed_variant(hipStream_t s, int variant, const float* Lin, const 
   }
 }
 
-bool akz_fed_variant_fusable(int variant) {
-  return (variant >= 2 && variant <= 11) || variant == 14 || variant == 15;
+bool akz_fed_variant_fusable(int variant, int n) {
+  if (variant < 2 || variant > AKZ_FED_VARIANT_MAX || n < 1) return false;
+  if (variant != 12 && variant != 13 && variant != 16) return true;
+  // a register region computes smooth + flow on all of its 64 x R pixels
+  // for a tile of (58 - 2n) x (R - 2n); fused while that is at most 1.56
+  // times the tile (measured: 1920x1080 with 3-4 steps gains, 960x540 with
+  // 4-7 steps on 32 rows loses)
+  const int R = variant == 12 ? 64 : variant == 13 ? 48 : 32;
+  return 64 * R * 100 <= 156 * (58 - 2 * n) * (R - 2 * n);
 }
 
-// First launch of a level with an LDS variant, fused with the level's
-// smooth + flow (k_fed_multi_sf); g is written only when `gout` is given.
+template <int R>
+static void launch_fed_reg_sf(hipStream_t s, dim3 grid, const float* in, float* lsm, float* gout, float* Lout,
+                              int pitch, int w, int h, int type, const float* kc, const akz_taps& taps,
+                              const akz_fed_steps& st, size_t zs) {
+  if (type == 0)
+    hipLaunchKernelGGL((k_fed_reg_sf<R, 0>), grid, dim3(64), 0, s, in, lsm, gout, Lout, pitch, w, h, kc, taps, st, zs);
+  else if (type == 1)
+    hipLaunchKernelGGL((k_fed_reg_sf<R, 1>), grid, dim3(64), 0, s, in, lsm, gout, Lout, pitch, w, h, kc, taps, st, zs);
+  else if (type == 2)
+    hipLaunchKernelGGL((k_fed_reg_sf<R, 2>), grid, dim3(64), 0, s, in, lsm, gout, Lout, pitch, w, h, kc, taps, st, zs);
+  else
+    hipLaunchKernelGGL((k_fed_reg_sf<R, 3>), grid, dim3(64), 0, s, in, lsm, gout, Lout, pitch, w, h, kc, taps, st, zs);
+}
+
+// First launch of a level, fused with the level's smooth + flow
+// (k_fed_multi_sf for the LDS variants, k_fed_reg_sf for the register
+// ones); g is written only when `gout` is given.
 void akz_launch_fed_variant_sf(hipStream_t s, int variant, const float* in, float* lsm, float* gout, float* Lout,
                                int pitch, int w, int h, int type, const float* kc, const akz_taps& taps,
                                const float* half_tau, int n, akz_batch bt) {
@@ -937,6 +1103,17 @@ void akz_launch_fed_variant_sf(hipStream_t s, int variant, const float* in, floa
     case 11: AKZ_SF(128, 32, 512, FB_K); break;
     case 14: AKZ_SF(64, 64, 1024, FB_KMAX); break;
     case 15: AKZ_SF(128, 64, 1024, FB_KMAX); break;
+    case 12:
+    case 13:
+    case 16: {
+      // 58 columns carry g; the tile is those minus the halo of n steps
+      const int R = variant == 12 ? 64 : variant == 13 ? 48 : 32;
+      const dim3 g((w + 57 - 2 * n) / (58 - 2 * n), (h + R - 1 - 2 * n) / (R - 2 * n), nz);
+      if (R == 64) launch_fed_reg_sf<64>(s, g, in, lsm, gout, Lout, pitch, w, h, type, kc, taps, st, zs);
+      else if (R == 48) launch_fed_reg_sf<48>(s, g, in, lsm, gout, Lout, pitch, w, h, type, kc, taps, st, zs);
+      else launch_fed_reg_sf<32>(s, g, in, lsm, gout, Lout, pitch, w, h, type, kc, taps, st, zs);
+      break;
+    }
     default: AKZ_SF(64, 32, 256, FB_K); break;
   }
 #undef AKZ_SF
