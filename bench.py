@@ -42,13 +42,14 @@ def log(*a):
 def fed_roofline(achieved, avg_ms, alg_bytes, traffic):
     """Roofline object of the FED launches.  `achieved` counts the algorithmic
     bytes (12 per pixel and step: read L and g, write L, as if every step were
-    its own pass) over the measured average launch; temporal blocking moves
-    fewer, so `frac` can pass 1.  `traffic` is the HBM bytes per launch from
+    its own pass, plus 12 per pixel for a launch that also builds its level's
+    Lsmooth and conductance) over the measured average launch; temporal
+    blocking moves fewer, so `frac` can pass 1.  `traffic` is the HBM bytes per launch from
     the committed counter runs, and `traffic_GBs` the rate they imply at the
     live launch time."""
     r = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
          "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-         "kernel": "FED launches: k_fed_reg / k_fed_multi, up to 8 steps each, halo = steps",
+         "kernel": "FED launches: k_fed_reg / k_fed_multi (first launch of a level: k_fed_reg_sf / k_fed_multi_sf, with its smooth + flow), up to 8 steps each, halo = steps",
          "avg_launch_us": round(avg_ms * 1e3, 3), "alg_bytes_per_launch": round(alg_bytes)}
     if traffic:
         rate = traffic / (avg_ms * 1e-3) / 1e9
