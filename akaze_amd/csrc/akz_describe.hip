@@ -167,7 +167,9 @@ __device__ inline mldb_cells mldb_cell(int i, int size2, int size3, int size4, i
 
 __global__ __launch_bounds__(64 * MLDB_WAVES) void k_mldb(akz_desc_params p, const akz_kp* __restrict__ kps,
                                                           const int32_t* nkps_p, uint8_t* __restrict__ desc) {
-  __shared__ float acc[87 * MLDB_VS];
+  // 29 cells x 3 channels, plus cell 29: where a sample outside a grid's
+  // cells adds (never read)
+  __shared__ float acc[90 * MLDB_VS];
   __shared__ float vals[88];
   const int lane = threadIdx.x & 63;
   kps = akz_zoff(kps, p.zs);
@@ -188,16 +190,17 @@ __global__ __launch_bounds__(64 * MLDB_WAVES) void k_mldb(akz_desc_params p, con
     ia[b] = d_cmp.a[c];
     ib[b] = d_cmp.b[c];
   }
-  // cells of the lane's samples: the same for every keypoint
-  int cg[3][MS];
+  // first accumulator row (3 x cell) of the lane's samples in each grid: the
+  // same for every keypoint
+  int rb[3][MS];
 #pragma unroll
   for (int m = 0; m < MS; m++) {
     const int i = lane + 64 * m;
     mldb_cells cc{-1, -1, -1};
     if (i < winsize * winsize) cc = mldb_cell(i, size2, size3, size4, winsize);
-    cg[0][m] = cc.c2;
-    cg[1][m] = cc.c3;
-    cg[2][m] = cc.c4;
+    rb[0][m] = 3 * (cc.c2 < 0 ? 29 : cc.c2);
+    rb[1][m] = 3 * (cc.c3 < 0 ? 29 : cc.c3);
+    rb[2][m] = 3 * (cc.c4 < 0 ? 29 : cc.c4);
   }
   for (int kp = blockIdx.x; kp < n; kp += gridDim.x) {
     const akz_kp k = kps[kp];
@@ -239,17 +242,21 @@ __global__ __launch_bounds__(64 * MLDB_WAVES) void k_mldb(akz_desc_params p, con
       ch[1][m] = -dx * si + dy * co;
       ch[2][m] = dx * co + dy * si;
     }
-    // samples in increasing i; within a sample the 2x2, 3x3, 4x4 cells
+    // samples in increasing i.  The 2x2, 3x3 and 4x4 cells of one sample
+    // are different accumulators (disjoint cell ranges), so the nine of a
+    // sample are read together, then written: one LDS round trip per sample
+    // instead of three
 #pragma unroll
     for (int m = 0; m < MS; m++) {
+      float r[3][3];
 #pragma unroll
-      for (int g = 0; g < 3; g++) {
-        const int cm = cg[g][m];
-        if (cm < 0) continue;
-        acc[(3 * cm + 0) * MLDB_VS + lane] += ch[0][m];
-        acc[(3 * cm + 1) * MLDB_VS + lane] += ch[1][m];
-        acc[(3 * cm + 2) * MLDB_VS + lane] += ch[2][m];
-      }
+      for (int g = 0; g < 3; g++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) r[g][c] = acc[(rb[g][m] + c) * MLDB_VS + lane];
+#pragma unroll
+      for (int g = 0; g < 3; g++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) acc[(rb[g][m] + c) * MLDB_VS + lane] = r[g][c] + ch[c][m];
     }
     __syncthreads();
     for (int v = lane; v < 87; v += 64) {
