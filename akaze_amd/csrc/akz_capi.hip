@@ -456,15 +456,31 @@ void enqueue_scale_space(akz_ctx* c, const float* img, int in_pitch, size_t in_z
 // Feature_Detection (AKAZE.cpp:316-378) + FilterExtrema (cuda_akaze.cu:1318-1402).
 // full_det: write every determinant pixel (parity readback, tile features);
 // otherwise only the candidates' 3x3 neighbourhoods the subpixel fit reads.
+// With a learned proposal the fused detector keeps the tile maxima of the
+// determinant itself; the unfused kernels write whole determinant images,
+// which the tile statistics then read.
 void enqueue_detect(akz_ctx* c, int nz, bool full_det) {
   hipStream_t s = c->stream;
   c->status_synced = 0;
-  full_det = full_det || c->prop_net != nullptr;
-  c->det32.det_full = full_det ? 1 : 0;
   int smax = 0;
   for (int i = 0; i < c->nlev; i++) smax = std::max(smax, c->lev[i].deriv_step);
-  c->det_full_last = full_det || smax > akz_detect_max_step();  // the unfused kernels write all of it
-  if (smax <= akz_detect_max_step()) {
+  const bool fused = smax <= akz_detect_max_step();
+  const bool prop_keys = c->prop_net != nullptr && fused;
+  full_det = full_det || (c->prop_net != nullptr && !fused);
+  c->det32.det_full = full_det ? 1 : 0;
+  c->det32.tile_key = prop_keys ? c->pact.tkey : nullptr;
+  if (prop_keys) {
+    const akz_prop_geom& g = c->psrc.g;
+    for (int i = 0; i < c->nlev; i++) {
+      c->det32.tile_base[i] = g.base[i];
+      c->det32.tile_w[i] = g.tw[i];
+      c->det32.tile_h[i] = g.th[i];
+    }
+    c->det32.tiles_per_image = g.P;
+    akz_prop_launch_clear_keys(s, g, c->pact, nz);
+  }
+  c->det_full_last = full_det || !fused;  // the unfused kernels write all of it
+  if (fused) {
     akz_launch_detect_fused(s, c->det32, c->db, nz);
     record(c, EV_DERIV);
   } else {
@@ -473,7 +489,7 @@ void enqueue_detect(akz_ctx* c, int nz, bool full_det) {
     akz_launch_extrema(s, c->det, c->db, nz);
   }
   if (c->prop_net) {
-    akz_prop_launch_features(s, c->psrc, c->pact, nz);
+    akz_prop_launch_features(s, c->psrc, c->pact, nz, prop_keys);
     akz_prop_launch_infer(s, akz_prop_net_weights(c->prop_net), c->psrc.g, c->pact, nz, c->prop_thr);
     akz_prop_launch_mask_segments(s, c->det, c->db, c->psrc.g, c->pact.mask, nz);
   }
@@ -524,7 +540,7 @@ akz_status ensure_prop_act(akz_ctx* c) {
   Carver cv;
   size_t o_x0 = cv.take<uint16_t>(P * 4), o_h1 = cv.take<uint16_t>(P * 16), o_h2 = cv.take<uint16_t>(P * 32);
   size_t o_h3 = cv.take<uint16_t>(P * 32), o_tm = cv.take<float>(P), o_ts = cv.take<float>(P);
-  size_t o_pr = cv.take<float>(P), o_mk = cv.take<uint8_t>(P);
+  size_t o_pr = cv.take<float>(P), o_mk = cv.take<uint8_t>(P), o_tk = cv.take<uint32_t>(P);
   AKZ_HIP_TRY(hipMalloc(&c->prop_mem, cv.off));
   char* b = (char*)c->prop_mem;
   c->pact.x0 = (uint16_t*)(b + o_x0);
@@ -535,6 +551,7 @@ akz_status ensure_prop_act(akz_ctx* c) {
   c->pact.tmean = (float*)(b + o_ts);
   c->pact.prob = (float*)(b + o_pr);
   c->pact.mask = (uint8_t*)(b + o_mk);
+  c->pact.tkey = (uint32_t*)(b + o_tk);
   return AKZ_OK;
 }
 

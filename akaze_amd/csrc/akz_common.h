@@ -68,6 +68,16 @@ __device__ __forceinline__ T* akz_zoff(T* p, size_t zs) {
   return akz_zoff(p, zs, (int)blockIdx.z);
 }
 
+// Order-preserving key of a float (a < b for finite a, b gives key(a) <
+// key(b)), so that an unsigned atomicMax keeps a float maximum; and back.
+__device__ __forceinline__ uint32_t akz_det_key(float v) {
+  const uint32_t u = __float_as_uint(v);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float akz_det_unkey(uint32_t k) {
+  return __uint_as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k);
+}
+
 // Index of the calling wavefront in its workgroup.  It is uniform across the
 // wavefront; read through readfirstlane the compiler knows that, and rows,
 // loop bounds and addresses derived from it become scalar work.

@@ -114,23 +114,26 @@ __global__ __launch_bounds__(256) void k_extrema(akz_det_params p, int32_t* __re
 }
 
 // Derivate + HessianDeterminant + FindExtrema (cuda_akaze.cu:651-774) fused
-// per 64x32 tile of every level in one launch.  Lsmooth is loaded with a
+// per 64x30 tile of every level in one launch.  Lsmooth is loaded with a
 // (2s+1)-pixel halo, Lx and Ly are computed on an (s+1)-pixel halo and the
-// determinant on a one-pixel ring, all in LDS (rows padded to 97 floats: a
-// wavefront's lanes walk one row, the few columns past 64 are walked down).
-// The determinant reuses the Lsmooth buffer, dead once the first derivatives
-// exist, which keeps a block at 52 KB of LDS: three blocks per CU.  Tiles
-// whose halo stays inside the image load without clamps or bounds tests.  Every stencil
-// addresses its reflect-101 neighbours through image coordinates, which stay
-// inside the loaded regions.  Lx, Ly and det are written for the tile's own
-// pixels and each 64-pixel row is one wavefront whose ballot gives the
-// candidate count and ranks, as in k_extrema.
-constexpr int DF_SMAX = 4, DF_TH = 32;
+// determinant on a one-pixel ring, all in LDS (a wavefront's lanes walk one
+// row, the few columns past 64 are walked down).  The determinant reuses the
+// Lsmooth buffer, dead once the first derivatives exist; with 30 rows and row
+// strides of 83 (Lsmooth, determinant) and 75 floats (Lx, Ly) a block holds
+// 40320 bytes of LDS (with the proposal's tile maxima), so four blocks share a
+// CU (32 rows and one 97-float stride took 52 KB: three blocks, 6% slower).  Tiles whose halo stays inside
+// the image load without clamps or bounds tests.  Every stencil addresses its
+// reflect-101 neighbours through image coordinates, which stay inside the
+// loaded regions.  Lx, Ly and det are written for the tile's own pixels and
+// each 64-pixel row is one wavefront whose ballot gives the candidate count
+// and ranks, as in k_extrema.
+constexpr int DF_SMAX = 4, DF_TH = 30;
 constexpr int DF_H1 = 2 * DF_SMAX + 1, DF_H2 = DF_SMAX + 1;
-// LDS row stride, >= 64 + 2*DF_H1 and odd: the halo columns beyond the
-// first 64 are walked down their columns, 32 consecutive rows of one column
-// then fall into 32 different banks
-constexpr int DF_RW = 97;
+// LDS row strides, at least the loaded width and odd: the halo columns
+// beyond the first 64 are walked down their columns, and consecutive rows of
+// one column then fall into different banks
+constexpr int DF_RL = 83, DF_RX = 75;
+static_assert(DF_RL >= 64 + 2 * DF_H1 && DF_RX >= 64 + 2 * DF_H2 && (DF_RL & 1) && (DF_RX & 1), "odd row strides");
 constexpr int DF_R1 = DF_TH + 2 * DF_H1, DF_R2 = DF_TH + 2 * DF_H2, DF_RD = DF_TH + 2;
 static_assert(DF_RD <= DF_R1, "the determinant fits the Lsmooth buffer");
 
@@ -142,14 +145,14 @@ static_assert(DF_RD <= DF_R1, "the determinant fits the Lsmooth buffer");
 template <int S>
 __device__ __forceinline__ void detect_interior(const float* s_l, float* s_x, float* s_y, float* s_d, float fac1,
                                                 float fac2, int wv, int lane) {
-  constexpr int NW = 8, H2 = S + 1, R2n = DF_TH + 2 * H2, E2 = 2 * H2, R = DF_RW;
+  constexpr int NW = 8, H2 = S + 1, R2n = DF_TH + 2 * H2, E2 = 2 * H2, R = DF_RL, X = DF_RX;
   auto deriv = [&](int r, int c) {
     const float* b = s_l + r * R + c;
     const float ul = b[0], ur = b[2 * S], ll = b[2 * S * R], lr = b[2 * S * R + 2 * S];
     const float cl = b[S * R], cr = b[S * R + 2 * S];
     const float uc = b[S], lc = b[2 * S * R + S];
-    s_x[r * R + c] = fac1 * (ur + lr - ul - ll) + fac2 * (cr - cl);
-    s_y[r * R + c] = fac1 * (lr + ll - ur - ul) + fac2 * (lc - uc);
+    s_x[r * X + c] = fac1 * (ur + lr - ul - ll) + fac2 * (cr - cl);
+    s_y[r * X + c] = fac1 * (lr + ll - ur - ul) + fac2 * (lc - uc);
   };
   // rows wv, wv + NW, ...: the first R2n / NW rows of every wavefront
   // exist, unrolled without tests; then at most one more
@@ -159,15 +162,15 @@ __device__ __forceinline__ void detect_interior(const float* s_l, float* s_x, fl
   for (int k = threadIdx.x; k < R2n * E2; k += 64 * NW) deriv(k % R2n, 64 + k / R2n);
   __syncthreads();
   auto det = [&](int r, int c) {
-    const float* bx = s_x + r * R + c;
-    const float* by = s_y + r * R + c;
-    float ul = bx[0], ur = bx[2 * S], ll = bx[2 * S * R], lr = bx[2 * S * R + 2 * S];
-    const float cl = bx[S * R], cr = bx[S * R + 2 * S];
+    const float* bx = s_x + r * X + c;
+    const float* by = s_y + r * X + c;
+    float ul = bx[0], ur = bx[2 * S], ll = bx[2 * S * X], lr = bx[2 * S * X + 2 * S];
+    const float cl = bx[S * X], cr = bx[S * X + 2 * S];
     const float lxx = fac1 * (ur + lr - ul - ll) + fac2 * (cr - cl);
-    float uc = bx[S], lc = bx[2 * S * R + S];
+    float uc = bx[S], lc = bx[2 * S * X + S];
     const float lyx = fac1 * (lr + ll - ur - ul) + fac2 * (lc - uc);
-    ul = by[0]; ur = by[2 * S]; ll = by[2 * S * R]; lr = by[2 * S * R + 2 * S];
-    uc = by[S]; lc = by[2 * S * R + S];
+    ul = by[0]; ur = by[2 * S]; ll = by[2 * S * X]; lr = by[2 * S * X + 2 * S];
+    uc = by[S]; lc = by[2 * S * X + S];
     const float lyy = fac1 * (lr + ll - ur - ul) + fac2 * (lc - uc);
     s_d[r * R + c] = lxx * lyy - lyx * lyx;
   };
@@ -181,9 +184,9 @@ __global__ __launch_bounds__(512) void k_detect_fused(akz_det_params p, int32_t*
                                                       uint8_t* __restrict__ seg_slots) {
   seg_count = akz_zoff(seg_count, p.zs);
   seg_slots = akz_zoff(seg_slots, p.zs);
-  __shared__ float s_l[DF_R1 * DF_RW];
-  __shared__ float s_x[DF_R2 * DF_RW];
-  __shared__ float s_y[DF_R2 * DF_RW];
+  __shared__ float s_l[DF_R1 * DF_RL];
+  __shared__ float s_x[DF_R2 * DF_RX];
+  __shared__ float s_y[DF_R2 * DF_RX];
   float* const s_d = s_l;  // DF_RD <= DF_R1 rows
   const int l = find_level(p, blockIdx.x);
   const akz_level_desc& L = p.lev[l];
@@ -216,8 +219,8 @@ __global__ __launch_bounds__(512) void k_detect_fused(akz_det_params p, int32_t*
     for (int k = 0; k < ROWS; k++) {
       const int r = wv + k * NW;
       if (r < R1n) {
-        s_l[r * DF_RW + lane] = v[k][0];
-        if (lane < W1 - 64) s_l[r * DF_RW + 64 + lane] = v[k][1];
+        s_l[r * DF_RL + lane] = v[k][0];
+        if (lane < W1 - 64) s_l[r * DF_RL + 64 + lane] = v[k][1];
       }
     }
   } else {
@@ -239,7 +242,7 @@ __global__ __launch_bounds__(512) void k_detect_fused(akz_det_params p, int32_t*
 #pragma unroll
       for (int q = 0; q < COLS; q++) {
         const int c = lane + q * 64, gx = x0 - H1 + c;
-        if (r < DF_TH + 2 * H1 && c < W1 && gy >= 0 && gy < h && gx >= 0 && gx < w) s_l[r * DF_RW + c] = v[k][q];
+        if (r < DF_TH + 2 * H1 && c < W1 && gy >= 0 && gy < h && gx >= 0 && gx < w) s_l[r * DF_RL + c] = v[k][q];
       }
     }
   }
@@ -255,17 +258,17 @@ __global__ __launch_bounds__(512) void k_detect_fused(akz_det_params p, int32_t*
     auto deriv_at = [&](int r, int c) {
       const int y = y0 - H2 + r, x = x0 - H2 + c;
       if (x < 0 || x >= w || y < 0 || y >= h) return;
-      const int yl = ((y < s ? s - y : y - s) - (y0 - H1)) * DF_RW;
-      const int yh = ((y >= h - s ? 2 * h - y - s - 2 : y + s) - (y0 - H1)) * DF_RW;
-      const int yc = (y - (y0 - H1)) * DF_RW;
+      const int yl = ((y < s ? s - y : y - s) - (y0 - H1)) * DF_RL;
+      const int yh = ((y >= h - s ? 2 * h - y - s - 2 : y + s) - (y0 - H1)) * DF_RL;
+      const int yc = (y - (y0 - H1)) * DF_RL;
       const int xl = (x < s ? s - x : x - s) - (x0 - H1);
       const int xh = (x >= w - s ? 2 * w - x - s - 2 : x + s) - (x0 - H1);
       const int xc = x - (x0 - H1);
       float ul = s_l[yl + xl], ur = s_l[yl + xh], ll = s_l[yh + xl], lr = s_l[yh + xh];
       float cl = s_l[yc + xl], cr = s_l[yc + xh];
       float uc = s_l[yl + xc], lc = s_l[yh + xc];
-      s_x[r * DF_RW + c] = fac1 * (ur + lr - ul - ll) + fac2 * (cr - cl);
-      s_y[r * DF_RW + c] = fac1 * (lr + ll - ur - ul) + fac2 * (lc - uc);
+      s_x[r * DF_RX + c] = fac1 * (ur + lr - ul - ll) + fac2 * (cr - cl);
+      s_y[r * DF_RX + c] = fac1 * (lr + ll - ur - ul) + fac2 * (lc - uc);
     };
     const int R2n = DF_TH + 2 * H2, E2 = 2 * H2;
     for (int r = wv; r < R2n; r += NW) deriv_at(r, lane);
@@ -275,9 +278,9 @@ __global__ __launch_bounds__(512) void k_detect_fused(akz_det_params p, int32_t*
     auto det_at = [&](int r, int c) {
       const int y = y0 - 1 + r, x = x0 - 1 + c;
       if (x < 0 || x >= w || y < 0 || y >= h) return;
-      const int yl = ((y < s ? s - y : y - s) - (y0 - H2)) * DF_RW;
-      const int yh = ((y >= h - s ? 2 * h - y - s - 2 : y + s) - (y0 - H2)) * DF_RW;
-      const int yc = (y - (y0 - H2)) * DF_RW;
+      const int yl = ((y < s ? s - y : y - s) - (y0 - H2)) * DF_RX;
+      const int yh = ((y >= h - s ? 2 * h - y - s - 2 : y + s) - (y0 - H2)) * DF_RX;
+      const int yc = (y - (y0 - H2)) * DF_RX;
       const int xl = (x < s ? s - x : x - s) - (x0 - H2);
       const int xh = (x >= w - s ? 2 * w - x - s - 2 : x + s) - (x0 - H2);
       const int xc = x - (x0 - H2);
@@ -289,7 +292,7 @@ __global__ __launch_bounds__(512) void k_detect_fused(akz_det_params p, int32_t*
       ul = s_y[yl + xl]; ur = s_y[yl + xh]; ll = s_y[yh + xl]; lr = s_y[yh + xh];
       uc = s_y[yl + xc]; lc = s_y[yh + xc];
       float lyy = fac1 * (lr + ll - ur - ul) + fac2 * (lc - uc);
-      s_d[r * DF_RW + c] = lxx * lyy - lyx * lyx;
+      s_d[r * DF_RL + c] = lxx * lyy - lyx * lyx;
     };
     for (int r = wv; r < DF_RD; r += NW) det_at(r, lane);
     for (int k = threadIdx.x; k < DF_RD * 2; k += 64 * NW) det_at(k % DF_RD, 64 + k / DF_RD);
@@ -305,18 +308,18 @@ __global__ __launch_bounds__(512) void k_detect_fused(akz_det_params p, int32_t*
     const int y = y0 + r;
     if (y >= h) break;
     bool pred = false;
-    const int id = (r + 1) * DF_RW + lane + 1;
+    const int id = (r + 1) * DF_RL + lane + 1;
     const size_t o = (size_t)y * pitch + x;
     const bool row_in = (int)(y - L.border + 0.5f) - 1 >= 0 && (int)(y + L.border + 0.5f) + 1 < h;
     if (x < w) {
-      const int i2 = (r + H2) * DF_RW + lane + H2;
+      const int i2 = (r + H2) * DF_RX + lane + H2;
       Lx[o] = s_x[i2];
       Ly[o] = s_y[i2];
       const float v = s_d[id];
       if (p.det_full) D[o] = v;
       // the eight neighbours are inside the tile's determinant rows for every
       // lane: read them all, combine without branches
-      const int R = DF_RW;
+      const int R = DF_RL;
       const float n0 = s_d[id - R - 1], n1 = s_d[id + R + 1], n2 = s_d[id + R - 1], n3 = s_d[id - R + 1];
       const float n4 = s_d[id - 1], n5 = s_d[id + 1], n6 = s_d[id + R], n7 = s_d[id - R];
       pred = col_in & row_in & (v > p.thr) & (v > n0) & (v > n1) & (v > n2) & (v > n3) & (v > n4) & (v > n5) &
@@ -329,12 +332,55 @@ __global__ __launch_bounds__(512) void k_detect_fused(akz_det_params p, int32_t*
 #pragma unroll
       for (int dy = -1; dy <= 1; dy++)
 #pragma unroll
-        for (int dx = -1; dx <= 1; dx++) D[o + (ptrdiff_t)dy * pitch + dx] = s_d[id + dy * DF_RW + dx];
+        for (int dx = -1; dx <= 1; dx++) D[o + (ptrdiff_t)dy * pitch + dx] = s_d[id + dy * DF_RL + dx];
     }
     uint64_t mask = __ballot(pred);
     int seg = L.seg_base + y * L.nseg_x + tx;
     if (lane == 0) seg_count[seg] = __popcll(mask);
     if (pred) seg_slots[(size_t)seg * 32 + __popcll(mask & ((1ull << lane) - 1ull))] = (uint8_t)lane;
+  }
+  if (p.tile_key) {
+    // Maxima of the determinant over the learned proposal's 16x16 tiles (the
+    // statistic k_tile_stats takes from a full determinant image): the tile's
+    // 64 columns are four proposal tiles, its rows touch at most three
+    // proposal tile rows; per wavefront, per lane group of 16, then over the
+    // wavefronts, and into the tile's key with atomicMax (a proposal tile
+    // row can span two detector tiles).
+    static_assert(DF_TH <= 2 * AKZ_PROP_TILE + 1, "a tile's rows touch three proposal tile rows at most");
+    __shared__ float s_tm[NW][3][4];
+    const int pr0 = y0 / AKZ_PROP_TILE;
+    float m0 = -INFINITY, m1 = -INFINITY, m2 = -INFINITY;
+    for (int r = wv; r < DF_TH; r += NW) {
+      const int y = y0 + r;
+      if (y >= h) break;
+      if (x < w) {
+        const float v = s_d[(r + 1) * DF_RL + lane + 1];
+        const int k = y / AKZ_PROP_TILE - pr0;
+        if (k == 0) m0 = fmaxf(m0, v);
+        else if (k == 1) m1 = fmaxf(m1, v);
+        else m2 = fmaxf(m2, v);
+      }
+    }
+    for (int off = 1; off < 16; off <<= 1) {
+      m0 = fmaxf(m0, __shfl_xor(m0, off, 64));
+      m1 = fmaxf(m1, __shfl_xor(m1, off, 64));
+      m2 = fmaxf(m2, __shfl_xor(m2, off, 64));
+    }
+    if ((lane & 15) == 0) {
+      s_tm[wv][0][lane >> 4] = m0;
+      s_tm[wv][1][lane >> 4] = m1;
+      s_tm[wv][2][lane >> 4] = m2;
+    }
+    __syncthreads();
+    if (threadIdx.x < 12) {
+      const int k = threadIdx.x >> 2, c = threadIdx.x & 3;
+      float mm = -INFINITY;
+      for (int q = 0; q < NW; q++) mm = fmaxf(mm, s_tm[q][k][c]);
+      const int pty = pr0 + k, ptx = x0 / AKZ_PROP_TILE + c;
+      if (mm > -INFINITY && pty < p.tile_h[l] && ptx < p.tile_w[l])
+        atomicMax(p.tile_key + (size_t)blockIdx.z * p.tiles_per_image + p.tile_base[l] + pty * p.tile_w[l] + ptx,
+                  akz_det_key(mm));
+    }
   }
 }
 

@@ -69,6 +69,12 @@ struct akz_det_params {
   float fac1, fac2;
   int det_full;                        // 1: write every determinant; 0: only candidates' 3x3
   size_t zs;                           // image stride of the batch in bytes
+  // maxima of the determinant over the 16x16 tiles of the learned proposal,
+  // kept by the fused detector when tile_key is set: order-preserving keys
+  // (akz_det_key) raised with atomicMax, image z's tiles at z * tiles_per_image
+  uint32_t* tile_key;
+  int tile_base[AKZ_MAX_LEVELS], tile_w[AKZ_MAX_LEVELS], tile_h[AKZ_MAX_LEVELS];
+  int tiles_per_image;
 };
 // Detector work arrays of image 0 (image z's sit z * zs bytes further).
 struct akz_det_bufs {
@@ -133,12 +139,18 @@ akz_prop_geom akz_prop_make_geom(int nlev, const int* w, const int* h, const int
 struct akz_prop_act {
   uint16_t *x0, *h1, *h2, *h3;  // bf16 activations, P x {4, 16, 32, 32} per image
   float *tmax, *tmean;          // tile statistics, P per image
+  uint32_t* tkey;               // P per image: determinant maxima from the fused detector (keys)
   float* prob;                  // P per image
   uint8_t* mask;                // P per image
 };
 struct akz_prop_weights;        // device weights of a network (opaque here)
-// features of images 0..nz-1 into act.x0 (tile statistics in act.tmax / tmean)
-void akz_prop_launch_features(hipStream_t s, const akz_prop_src& src, const akz_prop_act& act, int nz);
+// features of images 0..nz-1 into act.x0 (tile statistics in act.tmax / tmean);
+// with det_keys the determinant maxima come from act.tkey (filled by the
+// fused detector) instead of the full determinant images
+void akz_prop_launch_features(hipStream_t s, const akz_prop_src& src, const akz_prop_act& act, int nz,
+                              bool det_keys = false);
+// act.tkey of nz images to the key of -infinity, before the detector raises them
+void akz_prop_launch_clear_keys(hipStream_t s, const akz_prop_geom& g, const akz_prop_act& act, int nz);
 // inference on nz images' features: probabilities and the tile mask (prob >= thr)
 void akz_prop_launch_infer(hipStream_t s, const akz_prop_weights* w, const akz_prop_geom& g, const akz_prop_act& act,
                            int nz, float thr);

@@ -105,48 +105,62 @@ __device__ __forceinline__ uint4 im2col8(const uint16_t* __restrict__ X, const P
 
 // ---- features -------------------------------------------------------------
 // Tile statistics per level: maximum of the determinant and mean of Lt over
-// each 16x16 tile (pixels inside the level).  A block covers four tiles of a
-// tile row: 64 columns x 16 rows, each thread four rows of one column.
-__global__ __launch_bounds__(256) void k_tile_stats(akz_prop_src src, float* __restrict__ tmax,
-                                                    float* __restrict__ tmean) {
-  __shared__ float s_m[4][64], s_s[4][64];
+// each 16x16 tile (pixels inside the level).  One wavefront covers four tiles
+// of a tile row (64 columns x 16 rows); each lane loads the 16 rows of its
+// column at once.  The sum of a column is ((S0 + S1) + (S2 + S3)) with Sw
+// the rows w, w+4, w+8, w+12 summed in order, then a butterfly over the 16
+// columns of a tile.  With `keys` the maxima were kept by the fused detector
+// (order-preserving keys) and the determinant image is not read.
+constexpr int TS_WAVES = 4;
+__global__ __launch_bounds__(64 * TS_WAVES) void k_tile_stats(akz_prop_src src, float* __restrict__ tmax,
+                                                              float* __restrict__ tmean,
+                                                              const uint32_t* __restrict__ keys) {
+  const int grp = blockIdx.x * TS_WAVES + akz_wave();
+  if (grp >= src.tile_start[src.g.nlev]) return;
   int l = 0;
-  while (l + 1 < src.g.nlev && (int)blockIdx.x >= src.tile_start[l + 1]) l++;
-  const int t = blockIdx.x - src.tile_start[l];
+  while (l + 1 < src.g.nlev && grp >= src.tile_start[l + 1]) l++;
+  const int t = grp - src.tile_start[l];
   const int ngx = (src.w[l] + 63) / 64;
   const int gx = t % ngx, ty = t / ngx;
-  const int lane = threadIdx.x & 63, wv = akz_wave();
+  const int lane = threadIdx.x & 63;
   const int x = gx * 64 + lane;
   const int w = src.w[l], h = src.h[l], pitch = src.pitch[l];
   const float* D = akz_zoff(src.det[l], src.zs);
   const float* Lt = akz_zoff(src.Lt[l], src.zs);
   float m = -INFINITY, sum = 0.0f;
   if (x < w) {
-    for (int r = wv; r < AKZ_PROP_TILE; r += 4) {
-      const int y = ty * AKZ_PROP_TILE + r;
-      if (y >= h) break;
-      m = fmaxf(m, D[(size_t)y * pitch + x]);
-      sum += Lt[(size_t)y * pitch + x];
+    const int nr = akz_imin(AKZ_PROP_TILE, h - ty * AKZ_PROP_TILE);
+    const size_t o0 = (size_t)(ty * AKZ_PROP_TILE) * pitch + x;
+    float v[AKZ_PROP_TILE];
+#pragma unroll
+    for (int r = 0; r < AKZ_PROP_TILE; r++) v[r] = r < nr ? Lt[o0 + (size_t)r * pitch] : 0.0f;
+    float sw[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      float a = 0.0f;
+#pragma unroll
+      for (int r = q; r < AKZ_PROP_TILE; r += 4)
+        if (r < nr) a += v[r];
+      sw[q] = a;
+    }
+    sum = (sw[0] + sw[1]) + (sw[2] + sw[3]);
+    if (!keys) {
+#pragma unroll
+      for (int r = 0; r < AKZ_PROP_TILE; r++)
+        if (r < nr) m = fmaxf(m, D[o0 + (size_t)r * pitch]);
     }
   }
-  s_m[wv][lane] = m;
-  s_s[wv][lane] = sum;
-  __syncthreads();
-  if (wv == 0) {
-    m = fmaxf(fmaxf(s_m[0][lane], s_m[1][lane]), fmaxf(s_m[2][lane], s_m[3][lane]));
-    sum = (s_s[0][lane] + s_s[1][lane]) + (s_s[2][lane] + s_s[3][lane]);
-    for (int off = 1; off < 16; off <<= 1) {
-      m = fmaxf(m, __shfl_xor(m, off, 64));
-      sum += __shfl_xor(sum, off, 64);
-    }
-    const int tx = gx * 4 + (lane >> 4);
-    if ((lane & 15) == 0 && tx < src.g.tw[l]) {
-      const int cw = akz_imin(AKZ_PROP_TILE, w - tx * AKZ_PROP_TILE);
-      const int ch = akz_imin(AKZ_PROP_TILE, h - ty * AKZ_PROP_TILE);
-      const size_t o = (size_t)blockIdx.z * src.g.P + src.g.base[l] + (size_t)ty * src.g.tw[l] + tx;
-      tmax[o] = m;
-      tmean[o] = sum / (float)(cw * ch);
-    }
+  for (int off = 1; off < 16; off <<= 1) {
+    m = fmaxf(m, __shfl_xor(m, off, 64));
+    sum += __shfl_xor(sum, off, 64);
+  }
+  const int tx = gx * 4 + (lane >> 4);
+  if ((lane & 15) == 0 && tx < src.g.tw[l]) {
+    const int cw = akz_imin(AKZ_PROP_TILE, w - tx * AKZ_PROP_TILE);
+    const int ch = akz_imin(AKZ_PROP_TILE, h - ty * AKZ_PROP_TILE);
+    const size_t o = (size_t)blockIdx.z * src.g.P + src.g.base[l] + (size_t)ty * src.g.tw[l] + tx;
+    tmax[o] = keys ? akz_det_unkey(keys[o]) : m;
+    tmean[o] = sum / (float)(cw * ch);
   }
 }
 
@@ -496,6 +510,13 @@ __global__ __launch_bounds__(256) void k_pack(const float* __restrict__ params, 
 }
 
 // ---- pipeline hooks ---------------------------------------------------------
+// detector tile-maximum keys to the key of -infinity (a kernel, not a memset:
+// the pipeline is captured in graphs)
+__global__ __launch_bounds__(256) void k_fill_keys(uint32_t* __restrict__ key, int n) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) key[i] = akz_det_key(-INFINITY);
+}
+
 __global__ __launch_bounds__(256) void k_threshold(int total, const float* __restrict__ prob, float thr,
                                                    uint8_t* __restrict__ mask) {
   const int i = blockIdx.x * 256 + threadIdx.x;
@@ -564,9 +585,11 @@ akz_prop_geom akz_prop_make_geom(int nlev, const int* w, const int* h, const int
   return g;
 }
 
-void akz_prop_launch_features(hipStream_t s, const akz_prop_src& src, const akz_prop_act& act, int nz) {
-  hipLaunchKernelGGL(k_tile_stats, dim3(src.tile_start[src.g.nlev], 1, nz), dim3(256), 0, s, src, act.tmax,
-                     act.tmean);
+void akz_prop_launch_features(hipStream_t s, const akz_prop_src& src, const akz_prop_act& act, int nz,
+                              bool det_keys) {
+  hipLaunchKernelGGL(k_tile_stats, dim3(blocks_for(src.tile_start[src.g.nlev], TS_WAVES), 1, nz),
+                     dim3(64 * TS_WAVES), 0, s, src, act.tmax, act.tmean,
+                     det_keys ? (const uint32_t*)act.tkey : (const uint32_t*)nullptr);
   const int total = src.g.P * nz;
   hipLaunchKernelGGL(k_features, dim3(blocks_for(total, 256)), dim3(256), 0, s, src.g, total, src.thr,
                      (const float*)act.tmax, (const float*)act.tmean, act.x0);
@@ -589,6 +612,11 @@ void akz_prop_launch_infer(hipStream_t s, const akz_prop_weights* w, const akz_p
                      (uint16_t*)nullptr, (float*)nullptr);
   hipLaunchKernelGGL(k_threshold, dim3(blocks_for(total, 256)), dim3(256), 0, s, total, (const float*)act.prob, thr,
                      act.mask);
+}
+
+void akz_prop_launch_clear_keys(hipStream_t s, const akz_prop_geom& g, const akz_prop_act& act, int nz) {
+  const int total = g.P * nz;
+  hipLaunchKernelGGL(k_fill_keys, dim3(blocks_for(total, 256)), dim3(256), 0, s, act.tkey, total);
 }
 
 void akz_prop_launch_mask_segments(hipStream_t s, const akz_det_params& p, const akz_det_bufs& b,

@@ -163,6 +163,39 @@ def test_masked_detection_equals_oracle(gpu_lib):
     assert kps.tobytes() == okp.tobytes()
 
 
+def test_detector_tile_maxima_match_full_determinant_features(gpu_lib):
+    """With a proposal the fused detector keeps the determinant maxima of the
+    16x16 tiles itself (order-preserving keys, atomicMax across its 30-row
+    tiles) and no whole determinant image is written; the probabilities are
+    those of a forward pass over features taken from whole determinant
+    images.  Two image sets in turn: keys left from the first set would raise
+    the second set's maxima."""
+    A = gpu_lib
+    from akaze_amd.proposal import ProposalNet
+    W, H, B = 640, 480, 2
+    sets = [[A.to_float(A.synth_image(41 + 2 * k + i, W, H)) for i in range(B)] for k in range(2)]
+    net = ProposalNet(seed=7)
+    net.set_params(random_params(13))
+    bk = A.BatchAKAZE(A.AKAZEOptions(img_width=W, img_height=H), B)
+    P = bk.proposal_positions()
+    feat = A.DeviceBuffer(B * P * 8)
+    refs = []
+    bk.set_keep_determinant(True)
+    for imgs in sets:
+        bk.upload(imgs)
+        bk.process_async(B)
+        bk.extract(B, feat.ptr)
+        refs.append(net.forward(bk, feat.ptr, B))
+    bk.set_keep_determinant(False)
+    bk.set_proposal(net, 0.5)
+    for k, imgs in enumerate(sets):
+        bk.upload(imgs)
+        bk.process_async(B)
+        for z in range(B):
+            assert np.array_equal(bk.read_probs(z), refs[k][z * P:(z + 1) * P]), (k, z)
+    bk.set_proposal(None)
+
+
 def test_short_training_learns_the_labels(gpu_lib):
     """64 training and 32 held-out 640x480 device-rendered images: the loss
     falls and the held-out tile recall at the threshold that keeps 99% of the
