@@ -79,7 +79,8 @@ struct Level {
   float tau[AKZ_MAX_TAU];
   int deriv_step;
   float border, det_size;
-  float *Lt, *Lsm, *Lx, *Ly, *det;
+  float *Lt, *Lsm, *det;
+  float2* Lxy;  // Lx and Ly interleaved, element (y, x) at y * pitch + x
 };
 
 enum { EV_START, EV_CONTRAST, EV_SCALE, EV_DERIV, EV_EXTREMA, EV_NMS, EV_DESC, EV_COUNT };
@@ -229,10 +230,11 @@ struct Carver {
 
 akz_status allocate(akz_ctx* c) {
   Carver cv;
-  size_t o_lev[AKZ_MAX_LEVELS][5];
+  // per level: Lt, Lsmooth, the interleaved first derivatives, det
+  size_t o_lev[AKZ_MAX_LEVELS][4];
   for (int i = 0; i < c->nlev; i++) {
     size_t n = (size_t)c->lev[i].pitch * c->lev[i].h;
-    for (int k = 0; k < 5; k++) o_lev[i][k] = cv.take<float>(n);
+    for (int k = 0; k < 4; k++) o_lev[i][k] = cv.take<float>(k == 2 ? 2 * n : n);
   }
   const Level& L0 = c->lev[0];
   size_t n0 = (size_t)L0.pitch * (L0.h + 1);
@@ -291,9 +293,8 @@ akz_status allocate(akz_ctx* c) {
     Level& L = c->lev[i];
     L.Lt = (float*)(b + o_lev[i][0]);
     L.Lsm = (float*)(b + o_lev[i][1]);
-    L.Lx = (float*)(b + o_lev[i][2]);
-    L.Ly = (float*)(b + o_lev[i][3]);
-    L.det = (float*)(b + o_lev[i][4]);
+    L.Lxy = (float2*)(b + o_lev[i][2]);
+    L.det = (float*)(b + o_lev[i][3]);
   }
   c->img_in = (float*)(b + o_img);
   c->g = (float*)(b + o_g);
@@ -340,12 +341,10 @@ akz_status allocate(akz_ctx* c) {
   for (int i = 0; i < c->nlev; i++) {
     Level& L = c->lev[i];
     c->det.Lsm[i] = (i == 0) ? L.Lt : L.Lsm;  // level 0: Lsmooth is Lt (AKAZE.cpp:236)
-    c->det.Lx[i] = L.Lx;
-    c->det.Ly[i] = L.Ly;
+    c->det.Lxy[i] = L.Lxy;
     c->det.det[i] = L.det;
     c->dp.Lt[i] = L.Lt;
-    c->dp.Lx[i] = L.Lx;
-    c->dp.Ly[i] = L.Ly;
+    c->dp.Lxy[i] = L.Lxy;
     c->dp.w[i] = L.w;
     c->dp.h[i] = L.h;
     c->dp.pitch[i] = L.pitch;
@@ -402,7 +401,7 @@ void enqueue_scale_space(akz_ctx* c, const float* img, int in_pitch, size_t in_z
   // level 0, the Gaussian of soffset (Lsmooth aliases Lt, AKAZE.cpp:234-236),
   // and the contrast factor of the input (ContrastPercentile,
   // cuda_akaze.cu:604-649); level-0 Ly holds the gradient until detection
-  akz_launch_contrast(s, img, in_pitch, in_zs, L0.Lt, L0.Ly, L0.pitch, L0.w, L0.h, c->R0, c->taps0, c->taps_c,
+  akz_launch_contrast(s, img, in_pitch, in_zs, L0.Lt, (float*)L0.Lxy, L0.pitch, L0.w, L0.h, c->R0, c->taps0, c->taps_c,
                       std::min(c->opts.kcontrast_nbins, 512), c->opts.kcontrast_percentile, c->maxbits,
                       c->blockmax, c->hist, c->kc, c->nlev, c->octs, bt);
   record(c, EV_CONTRAST);
@@ -903,8 +902,14 @@ akz_status akz_read_batch_level(akz_ctx* c, int z, int lvl, int which, float* ds
   if (which == 4 && !c->det_full_last) return AKZ_ERR_STATE;  // see akz_set_keep_determinant
   AKZ_HIP_TRY(hipStreamSynchronize(c->stream));
   const Level& L = c->lev[lvl];
-  const float* src = which == 0 ? L.Lt : which == 1 ? (lvl == 0 ? L.Lt : L.Lsm) : which == 2 ? L.Lx
-                     : which == 3 ? L.Ly : L.det;
+  if (which == 2 || which == 3) {  // one component of the interleaved derivatives
+    std::vector<float> tmp((size_t)2 * L.w * L.h);
+    AKZ_HIP_TRY(hipMemcpy2D(tmp.data(), sizeof(float2) * L.w, zptr(c, L.Lxy, z), sizeof(float2) * L.pitch,
+                            sizeof(float2) * L.w, L.h, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < (size_t)L.w * L.h; i++) dst[i] = tmp[2 * i + (which - 2)];
+    return AKZ_OK;
+  }
+  const float* src = which == 0 ? L.Lt : which == 1 ? (lvl == 0 ? L.Lt : L.Lsm) : L.det;
   AKZ_HIP_TRY(hipMemcpy2D(dst, sizeof(float) * L.w, zptr(c, src, z), sizeof(float) * L.pitch, sizeof(float) * L.w,
                           L.h, hipMemcpyDeviceToHost));
   return AKZ_OK;

@@ -59,8 +59,7 @@ __global__ __launch_bounds__(64 * OR_WAVES) void k_orientation(akz_desc_params p
     if (active) {
       akz_kp k = kps[kp];
       const int lev = k.class_id;
-      const float* dxd = akz_zoff(p.Lx[lev], p.zs);
-      const float* dyd = akz_zoff(p.Ly[lev], p.zs);
+      const float2* lxy = akz_zoff(p.Lxy[lev], p.zs);
       const int pitch = p.pitch[lev], w = p.w[lev], h = p.h[lev];
       const int octave = k.octave;
       const int step = (int)(0.5f * k.size + 0.5f) >> octave;
@@ -78,9 +77,9 @@ __global__ __launch_bounds__(64 * OR_WAVES) void k_orientation(akz_desc_params p
         if (tx < 208 && r2 < 36) {
           gwv[k] = wt.w[r2];
           const int xs = akz_iclamp(x + step * i, 0, w - 1), ys = akz_iclamp(y + step * j, 0, h - 1);
-          const size_t pos = (size_t)ys * pitch + xs;
-          gxv[k] = dxd[pos];
-          gyv[k] = dyd[pos];
+          const float2 d = lxy[(size_t)ys * pitch + xs];
+          gxv[k] = d.x;
+          gyv[k] = d.y;
         }
       }
 #pragma unroll
@@ -206,8 +205,7 @@ __global__ __launch_bounds__(64 * MLDB_WAVES) void k_mldb(akz_desc_params p, con
     const akz_kp k = kps[kp];
     const int lev = k.class_id;
     const float* imd = akz_zoff(p.Lt[lev], p.zs);
-    const float* dxd = akz_zoff(p.Lx[lev], p.zs);
-    const float* dyd = akz_zoff(p.Ly[lev], p.zs);
+    const float2* lxy = akz_zoff(p.Lxy[lev], p.zs);
     const int pitch = p.pitch[lev], w = p.w[lev], h = p.h[lev];
     const float iratio = 1.0f / (1 << k.octave);
     const int scale = (int)(0.5f * k.size * iratio + 0.5f);
@@ -230,9 +228,10 @@ __global__ __launch_bounds__(64 * MLDB_WAVES) void k_mldb(akz_desc_params p, con
         xp = akz_iclamp(xp, 0, w - 1);
         yp = akz_iclamp(yp, 0, h - 1);
         const size_t pos = (size_t)yp * pitch + xp;
+        const float2 d = lxy[pos];
         ch[0][m] = imd[pos];
-        ch[1][m] = dxd[pos];
-        ch[2][m] = dyd[pos];
+        ch[1][m] = d.x;
+        ch[2][m] = d.y;
       }
     }
     // channels: L, rx = -dx sin + dy cos, ry = dx cos + dy sin

@@ -19,21 +19,24 @@ __device__ __forceinline__ int find_level(const akz_det_params& p, int b) {
 }
 
 // Derivate (cuda_akaze.cu:651-671): first derivatives at integer step s,
-// reflect-101 at offset s.
+// reflect-101 at offset s.  Element (y, x) of the source is f[ST * (y * pitch
+// + x)]: ST = 2 reads one component of the interleaved Lx / Ly.
+template <int ST>
 __device__ __forceinline__ void deriv_xy(const float* f, int pitch, int w, int h, int x, int y, int s, float fac1,
                                          float fac2, float* dx, float* dy) {
   int xl = (x < s ? s - x : x - s);
   int xh = (x >= w - s ? 2 * w - x - s - 2 : x + s);
   int yl = (y < s ? s - y : y - s);
   int yh = (y >= h - s ? 2 * h - y - s - 2 : y + s);
-  float ul = f[(size_t)yl * pitch + xl], ur = f[(size_t)yl * pitch + xh];
-  float ll = f[(size_t)yh * pitch + xl], lr = f[(size_t)yh * pitch + xh];
+  auto at = [&](int yy, int xx) { return f[ST * ((size_t)yy * pitch + xx)]; };
+  float ul = at(yl, xl), ur = at(yl, xh);
+  float ll = at(yh, xl), lr = at(yh, xh);
   if (dx) {
-    float cl = f[(size_t)y * pitch + xl], cr = f[(size_t)y * pitch + xh];
+    float cl = at(y, xl), cr = at(y, xh);
     *dx = fac1 * (ur + lr - ul - ll) + fac2 * (cr - cl);
   }
   if (dy) {
-    float uc = f[(size_t)yl * pitch + x], lc = f[(size_t)yh * pitch + x];
+    float uc = at(yl, x), lc = at(yh, x);
     *dy = fac1 * (lr + ll - ur - ul) + fac2 * (lc - uc);
   }
 }
@@ -48,9 +51,8 @@ __global__ __launch_bounds__(256) void k_deriv(akz_det_params p) {
   if (x >= L.w) return;
   for (int y = y0 + akz_wave(); y < y0 + 16 && y < L.h; y += 4) {
     float dx, dy;
-    deriv_xy(akz_zoff(p.Lsm[l], p.zs), L.pitch, L.w, L.h, x, y, L.deriv_step, p.fac1, p.fac2, &dx, &dy);
-    akz_zoff(p.Lx[l], p.zs)[(size_t)y * L.pitch + x] = dx;
-    akz_zoff(p.Ly[l], p.zs)[(size_t)y * L.pitch + x] = dy;
+    deriv_xy<1>(akz_zoff(p.Lsm[l], p.zs), L.pitch, L.w, L.h, x, y, L.deriv_step, p.fac1, p.fac2, &dx, &dy);
+    akz_zoff(p.Lxy[l], p.zs)[(size_t)y * L.pitch + x] = make_float2(dx, dy);
   }
 }
 
@@ -66,8 +68,9 @@ __global__ __launch_bounds__(256) void k_hess(akz_det_params p) {
   if (x >= L.w) return;
   for (int y = y0 + akz_wave(); y < y0 + 16 && y < L.h; y += 4) {
     float lxx, lyx, lyy;
-    deriv_xy(akz_zoff(p.Lx[l], p.zs), L.pitch, L.w, L.h, x, y, L.deriv_step, p.fac1, p.fac2, &lxx, &lyx);
-    deriv_xy(akz_zoff(p.Ly[l], p.zs), L.pitch, L.w, L.h, x, y, L.deriv_step, p.fac1, p.fac2, nullptr, &lyy);
+    const float* lxy = reinterpret_cast<const float*>(akz_zoff(p.Lxy[l], p.zs));
+    deriv_xy<2>(lxy, L.pitch, L.w, L.h, x, y, L.deriv_step, p.fac1, p.fac2, &lxx, &lyx);
+    deriv_xy<2>(lxy + 1, L.pitch, L.w, L.h, x, y, L.deriv_step, p.fac1, p.fac2, nullptr, &lyy);
     akz_zoff(p.det[l], p.zs)[(size_t)y * L.pitch + x] = lxx * lyy - lyx * lyx;
   }
 }
@@ -299,8 +302,7 @@ __global__ __launch_bounds__(512) void k_detect_fused(akz_det_params p, int32_t*
   }
   __syncthreads();
   const int x = x0 + lane;
-  float* Lx = akz_zoff(p.Lx[l], p.zs);
-  float* Ly = akz_zoff(p.Ly[l], p.zs);
+  float2* Lxy = akz_zoff(p.Lxy[l], p.zs);
   float* D = akz_zoff(p.det[l], p.zs);
   // the detector border test splits into a column part and a row part
   const bool col_in = (int)(x - L.border + 0.5f) - 1 >= 0 && (int)(x + L.border + 0.5f) + 1 < w;
@@ -313,8 +315,7 @@ __global__ __launch_bounds__(512) void k_detect_fused(akz_det_params p, int32_t*
     const bool row_in = (int)(y - L.border + 0.5f) - 1 >= 0 && (int)(y + L.border + 0.5f) + 1 < h;
     if (x < w) {
       const int i2 = (r + H2) * DF_RX + lane + H2;
-      Lx[o] = s_x[i2];
-      Ly[o] = s_y[i2];
+      Lxy[o] = make_float2(s_x[i2], s_y[i2]);
       const float v = s_d[id];
       if (p.det_full) D[o] = v;
       // the eight neighbours are inside the tile's determinant rows for every

@@ -60,8 +60,7 @@ struct akz_det_params {
   int nlev;
   akz_level_desc lev[AKZ_MAX_LEVELS];
   const float* Lsm[AKZ_MAX_LEVELS];
-  float* Lx[AKZ_MAX_LEVELS];
-  float* Ly[AKZ_MAX_LEVELS];
+  float2* Lxy[AKZ_MAX_LEVELS];  // Lx and Ly interleaved: element (y, x) at y * pitch + x
   float* det[AKZ_MAX_LEVELS];
   int tile_start[AKZ_MAX_LEVELS + 1];  // prefix of 64x16 tiles per level
   int seg_total;                       // row segments over all levels
@@ -173,8 +172,7 @@ akz_status akz_prop_grad_geom(akz_prop_net* n, const akz_prop_geom& g, const uin
 // ---- description (akz_describe.hip) ----
 struct akz_desc_params {
   const float* Lt[AKZ_MAX_LEVELS];
-  const float* Lx[AKZ_MAX_LEVELS];
-  const float* Ly[AKZ_MAX_LEVELS];
+  const float2* Lxy[AKZ_MAX_LEVELS];  // Lx and Ly interleaved (one gather reads both)
   int w[AKZ_MAX_LEVELS], h[AKZ_MAX_LEVELS], pitch[AKZ_MAX_LEVELS];
   int patsize;
   size_t zs;
