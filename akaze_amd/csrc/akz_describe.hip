@@ -34,19 +34,41 @@ constexpr CmpTables make_cmp_tables() {
 
 __constant__ CmpTables d_cmp = make_cmp_tables();
 
+// The samples of FindOrientation's radius-6 disc (cuda_akaze.cu:2114-2167),
+// offsets (i, j) with i^2 + j^2 < 36, in the order of their slot tx = 16 (j +
+// 6) + (i + 6) in the reference's 16-wide sample grid: packed as (i + 6) |
+// (j + 6) << 4 | (i^2 + j^2) << 8.
+constexpr int OR_NS = 109;
+struct OrientPts {
+  int16_t v[OR_NS];
+};
+constexpr OrientPts make_orient_pts() {
+  OrientPts t{};
+  int c = 0;
+  for (int tx = 0; tx < 208; tx++) {
+    const int i = (tx & 15) - 6, j = (tx >> 4) - 6, r2 = i * i + j * j;
+    if (r2 < 36) t.v[c++] = (int16_t)((i + 6) | ((j + 6) << 4) | (r2 << 8));
+  }
+  return t;
+}
+static_assert(make_orient_pts().v[OR_NS - 1] != 0, "109 disc samples");
+__constant__ OrientPts d_orpts = make_orient_pts();
+
 // exp(-r2 / 12.5) for r2 = 0..35 (cuda_akaze.cu:2133), rounded once on the host
 struct OrientWeights { float w[36]; };
 
 constexpr int OR_WAVES = 4;
 
 // Dominant orientation (FindOrientation, cuda_akaze.cu:2114-2167).  The 109
-// samples of the radius-6 disc are distributed over the lanes; their bin and
-// weighted derivatives go to LDS, then lane b (b < 42) sums bin b in sample
-// order, so the float sums do not depend on scheduling.
+// samples of the radius-6 disc go to the lanes in the reference's sample
+// order (two per lane); their bin and weighted derivatives go to LDS, then
+// lane b (b < 42) sums bin b in sample order, so the float sums do not depend
+// on scheduling.  The strongest window is found by a wavefront maximum: the
+// first window reaching it, as the reference's sequential scan keeps.
 __global__ __launch_bounds__(64 * OR_WAVES) void k_orientation(akz_desc_params p, akz_kp* __restrict__ kps,
                                                                const int32_t* nkps_p, OrientWeights wt) {
-  __shared__ int s_bin[OR_WAVES][208];
-  __shared__ float s_dx[OR_WAVES][208], s_dy[OR_WAVES][208];
+  __shared__ int8_t s_bin[OR_WAVES][OR_NS + 3];
+  __shared__ float s_dx[OR_WAVES][OR_NS + 3], s_dy[OR_WAVES][OR_NS + 3];
   __shared__ float s_rx[OR_WAVES][42], s_ry[OR_WAVES][42];
   __shared__ float s_ex[OR_WAVES][42], s_ey[OR_WAVES][42];
   const int lane = threadIdx.x & 63, wv = akz_wave();
@@ -65,50 +87,44 @@ __global__ __launch_bounds__(64 * OR_WAVES) void k_orientation(akz_desc_params p
       const int step = (int)(0.5f * k.size + 0.5f) >> octave;
       const int x = (int)(k.x + 0.5f) >> octave;
       const int y = (int)(k.y + 0.5f) >> octave;
-      // the lane's 4 sample slots: all gathers are issued before any is used
-      float gxv[4], gyv[4], gwv[4];
+      // the lane's 2 samples: both gathers are issued before either is used
+      float gxv[2], gyv[2], gwv[2];
 #pragma unroll
-      for (int k = 0; k < 4; k++) {
-        const int tx = lane + 64 * k;
-        const int i = (tx & 15) - 6, j = (tx / 16) - 6;
-        const int r2 = i * i + j * j;
-        gwv[k] = 0.0f;
-        gxv[k] = gyv[k] = 0.0f;
-        if (tx < 208 && r2 < 36) {
-          gwv[k] = wt.w[r2];
+      for (int q = 0; q < 2; q++) {
+        const int sn = lane + 64 * q;
+        gwv[q] = gxv[q] = gyv[q] = 0.0f;
+        if (sn < OR_NS) {
+          const int pt = d_orpts.v[sn];
+          const int i = (pt & 15) - 6, j = ((pt >> 4) & 15) - 6;
+          gwv[q] = wt.w[pt >> 8];
           const int xs = akz_iclamp(x + step * i, 0, w - 1), ys = akz_iclamp(y + step * j, 0, h - 1);
           const float2 d = lxy[(size_t)ys * pitch + xs];
-          gxv[k] = d.x;
-          gyv[k] = d.y;
+          gxv[q] = d.x;
+          gyv[q] = d.y;
         }
       }
 #pragma unroll
-      for (int k = 0; k < 4; k++) {
-        const int tx = lane + 64 * k;
-        if (tx >= 208) break;
-        const int i = (tx & 15) - 6, j = (tx / 16) - 6;
-        int bin = -1;
-        float dx = 0.0f, dy = 0.0f;
-        if (i * i + j * j < 36) {
-          dx = gwv[k] * gxv[k];
-          dy = gwv[k] * gyv[k];
-          float angle = akz_atan2f(dy, dx);
-          bin = akz_imax(akz_imin((int)(angle * (21 / 3.14159265358979323846)) + 21, 41), 0);
-        }
-        s_bin[wv][tx] = bin;
-        s_dx[wv][tx] = dx;
-        s_dy[wv][tx] = dy;
+      for (int q = 0; q < 2; q++) {
+        const int sn = lane + 64 * q;
+        if (sn >= OR_NS) break;
+        const float dx = gwv[q] * gxv[q];
+        const float dy = gwv[q] * gyv[q];
+        const float angle = akz_atan2f(dy, dx);
+        s_bin[wv][sn] = (int8_t)akz_imax(akz_imin((int)(angle * (21 / 3.14159265358979323846)) + 21, 41), 0);
+        s_dx[wv][sn] = dx;
+        s_dy[wv][sn] = dy;
       }
     }
     __syncthreads();
     if (active && lane < 42) {
       float rx = 0.0f, ry = 0.0f;
-      for (int tx = 0; tx < 208; tx++)
-        if (s_bin[wv][tx] == lane) { rx += s_dx[wv][tx]; ry += s_dy[wv][tx]; }
+      for (int sn = 0; sn < OR_NS; sn++)
+        if (s_bin[wv][sn] == lane) { rx += s_dx[wv][sn]; ry += s_dy[wv][sn]; }
       s_rx[wv][lane] = rx;
       s_ry[wv][lane] = ry;
     }
     __syncthreads();
+    float r = -1.0f;
     if (active && lane < 42) {
       float ex = s_rx[wv][lane], ey = s_ry[wv][lane];
       for (int q = lane + 1; q < lane + 7; q++) {
@@ -117,15 +133,17 @@ __global__ __launch_bounds__(64 * OR_WAVES) void k_orientation(akz_desc_params p
       }
       s_ex[wv][lane] = ex;
       s_ey[wv][lane] = ey;
+      r = ex * ex + ey * ey;
     }
+    // the reference keeps the first window whose r exceeds the running
+    // maximum (from 0): the lowest window with the largest r, or window 0
+    // when no r is positive
+    float m = r;
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+    const uint64_t at = __ballot(r == m && m > 0.0f);
+    const int maxk = at ? __ffsll((unsigned long long)at) - 1 : 0;
     __syncthreads();
     if (active && lane == 0) {
-      float maxr = 0.0f;
-      int maxk = 0;
-      for (int q = 0; q < 42; q++) {
-        float r = s_ex[wv][q] * s_ex[wv][q] + s_ey[wv][q] * s_ey[wv][q];
-        if (r > maxr) { maxr = r; maxk = q; }
-      }
       float angle = akz_atan2f(s_ey[wv][maxk], s_ex[wv][maxk]);
       kps[kp].angle = (angle < 0.0f ? angle + 2.0f * 3.14159265358979323846 : angle);
     }
