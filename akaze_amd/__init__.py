@@ -131,6 +131,7 @@ SIGNATURES = {
     "akz_memcpy_d2d": [_P, _P, _SZ],
     "akz_memcpy_2d": [_P, _SZ, _P, _SZ, _SZ, _SZ],
     "akz_device_synchronize": [],
+    "akz_selftest_reciprocal": [_P],
     "akz_synth_image": [_U64, _I, _I, _P],
     "akz_synth_homography": [_U64, _I, _I, _D, _D, _D, _D, _P],
     "akz_synth_warp": [_P, _I, _I, _P, _D, _U64, _P],

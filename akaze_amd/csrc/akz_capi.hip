@@ -1397,6 +1397,23 @@ akz_status akz_device_synchronize(void) {
   return AKZ_OK;
 }
 
+akz_status akz_selftest_reciprocal(uint64_t* mismatches) {
+  if (!mismatches) return AKZ_ERR_ARG;
+  unsigned long long* d = nullptr;
+  AKZ_HIP_TRY(hipMalloc(&d, sizeof(*d)));
+  unsigned long long h = 0;
+  hipError_t e = hipMemcpy(d, &h, sizeof(h), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    akz_launch_rcp_check(nullptr, d);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(&h, d, sizeof(h), hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  if (e != hipSuccess) return AKZ_ERR_HIP;
+  *mismatches = h;
+  return AKZ_OK;
+}
+
 // ---- synthetic inputs ----
 akz_status akz_synth_image(uint64_t seed, int w, int h, uint8_t* out) {
   if (!out || w <= 0 || h <= 0) return AKZ_ERR_ARG;

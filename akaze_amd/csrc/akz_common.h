@@ -78,6 +78,17 @@ __device__ __forceinline__ float akz_det_unkey(uint32_t k) {
   return __uint_as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k);
 }
 
+// 1/x for x >= 1, equal to IEEE division: the hardware reciprocal (within one
+// ulp) and one Newton step of fused multiply-adds round to the same float as
+// 1.0f / x for every float x in [1, 2^126) (checked for all of them on
+// gfx950); beyond that, where 1/x leaves the normal range or x is infinite,
+// the division itself.  Three instructions instead of the division sequence.
+__device__ __forceinline__ float akz_rcp_ge1(float x) {
+  if (__builtin_expect(x >= 0x1p126f, 0)) return 1.0f / x;
+  const float y = __builtin_amdgcn_rcpf(x);
+  return __builtin_fmaf(__builtin_fmaf(-x, y, 1.0f), y, y);
+}
+
 // Index of the calling wavefront in its workgroup.  It is uniform across the
 // wavefront; read through readfirstlane the compiler knows that, and rows,
 // loop bounds and addresses derived from it become scalar work.

@@ -50,6 +50,9 @@ bool akz_fed_variant_fusable(int variant, int n);
 void akz_launch_fed_variant_sf(hipStream_t s, int variant, const float* in, float* lsm, float* gout, float* Lout,
                                int pitch, int w, int h, int type, const float* kc, const akz_taps& taps,
                                const float* half_tau, int n, akz_batch bt);
+// every float x in [1, +inf]: akz_rcp_ge1(x) against 1.0f / x, differing
+// results added to *mismatches
+void akz_launch_rcp_check(hipStream_t s, unsigned long long* mismatches);
 void akz_launch_halfsample(hipStream_t s, const float* in, int iw, int ih, int ipitch, float* out, int ow, int oh,
                            int opitch, akz_batch bt);
 void akz_launch_copy2d(hipStream_t s, const float* in, int ipitch, float* out, int opitch, int w, int h,

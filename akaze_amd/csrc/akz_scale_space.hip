@@ -420,11 +420,11 @@ __global__ __launch_bounds__(NT) void k_fed_multi_sf(const float* __restrict__ i
     if (type == 0)
       res = expf(-dif2);
     else if (type == 1)
-      res = 1.0f / (1.0f + dif2);
+      res = akz_rcp_ge1(1.0f + dif2);
     else if (type == 2)
       res = 1.0f - (float)exp(-3.315 / (dif2 * dif2 * dif2 * dif2));
     else
-      res = 1.0f / sqrtf(1.0f + dif2);
+      res = akz_rcp_ge1(sqrtf(1.0f + dif2));
     sg[idx] = res;
     if (r >= hn && r < RH - hn && c >= hn && c < RW - hn) {
       const size_t o = (size_t)gy * pitch + gx;
@@ -570,9 +570,9 @@ void k_fed_reg(const float* __restrict__ L, const float* __restrict__ g,
 template <int TYPE>
 __device__ __forceinline__ float diffusivity(float dif2) {
   if (TYPE == 0) return expf(-dif2);
-  if (TYPE == 1) return 1.0f / (1.0f + dif2);
+  if (TYPE == 1) return akz_rcp_ge1(1.0f + dif2);
   if (TYPE == 2) return 1.0f - (float)exp(-3.315 / (dif2 * dif2 * dif2 * dif2));
-  return 1.0f / sqrtf(1.0f + dif2);
+  return akz_rcp_ge1(sqrtf(1.0f + dif2));
 }
 
 // Lsmooth and conductance of one level in one pass (AKAZE.cpp:260-261): the
@@ -1142,6 +1142,25 @@ int akz_fed_variant_for(int w, int h, int nz) {
 void akz_launch_fed_multi(hipStream_t s, const float* Lin, const float* g, float* Lout, int pitch, int w, int h,
                           const float* half_tau, int n, akz_batch bt) {
   akz_launch_fed_variant(s, akz_fed_variant_for(w, h, bt.nz), Lin, g, Lout, pitch, w, h, half_tau, n, bt);
+}
+
+namespace {
+// 64 consecutive floats per thread from x = 1.0f up; the count is n
+__global__ __launch_bounds__(256) void k_rcp_check(unsigned long long* __restrict__ bad, uint32_t n) {
+  const uint32_t start = (blockIdx.x * 256u + threadIdx.x) * 64u;
+  uint32_t cnt = 0;
+  for (uint32_t k = 0; k < 64u && start + k < n; k++) {
+    const float x = __uint_as_float(0x3F800000u + start + k);
+    if (__float_as_uint(akz_rcp_ge1(x)) != __float_as_uint(1.0f / x)) cnt++;
+  }
+  if (cnt) atomicAdd(bad, (unsigned long long)cnt);
+}
+}  // namespace
+
+void akz_launch_rcp_check(hipStream_t s, unsigned long long* mismatches) {
+  const uint32_t n = 0x7F800000u - 0x3F800000u + 1u;  // 1.0f .. +inf
+  const uint32_t threads = (n + 63u) / 64u;
+  hipLaunchKernelGGL(k_rcp_check, dim3((threads + 255u) / 256u), dim3(256), 0, s, mismatches, n);
 }
 
 void akz_launch_halfsample(hipStream_t s, const float* in, int iw, int ih, int ipitch, float* out, int ow, int oh,

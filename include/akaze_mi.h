@@ -297,6 +297,13 @@ akz_status akz_memcpy_2d(void* dst, size_t dpitch, const void* src, size_t spitc
                          size_t height);
 akz_status akz_device_synchronize(void);
 
+/* ---- Self-test (no counterpart in the reference) ----
+ * The conductance's 1/x (x >= 1) uses the hardware reciprocal and one FMA
+ * Newton step instead of the division sequence.  This compares that function
+ * with IEEE 1.0f / x for every float x in [1, +inf] on the current device and
+ * returns the number of differing results (0 expected). */
+akz_status akz_selftest_reciprocal(uint64_t* mismatches);
+
 /* ---- Seeded synthetic inputs (no reference counterpart: the benchmark
  * inputs).  uint8 image, homography about the image centre, warp. ---- */
 akz_status akz_synth_image(uint64_t seed, int w, int h, uint8_t* out);

@@ -162,3 +162,14 @@ def test_mldb_pattern_sizes(gpu_lib, patsize):
     assert len(kps) == len(okp) > 100
     assert np.array_equal(kps["angle"], okp["angle"])
     assert np.array_equal(desc, odesc), int(np.sum(desc != odesc))
+
+
+def test_conductance_reciprocal_equals_division(gpu_lib):
+    """The conductance's 1/x (x >= 1) uses the hardware reciprocal and one FMA
+    Newton step; on the device it equals IEEE 1.0f / x for every float in
+    [1, +inf] (about 1.07e9 values), so the scale space stays bit-exact."""
+    import ctypes
+    A = gpu_lib
+    m = ctypes.c_uint64(12345)
+    A._check(A.lib().akz_selftest_reciprocal(ctypes.byref(m)), "akz_selftest_reciprocal")
+    assert m.value == 0, m.value
