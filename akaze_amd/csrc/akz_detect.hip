@@ -714,6 +714,44 @@ __device__ int nms_output(const akz_kp* __restrict__ cand, int n, MemAt mem_at, 
   return *s_run;
 }
 
+// The same output in one pass when every chunk's bitmap fits side by side
+// (chunks x words <= NMS_BITMAP_WORDS): chunk c's bitmap follows chunk c-1's,
+// so one prefix scan over all of them ranks every value at its position
+// behind the earlier chunks' keypoints.  Returns the number of keypoints.
+template <class MemAt, class StrongAt>
+__device__ int nms_output_flat(const akz_kp* __restrict__ cand, int n, MemAt mem_at, StrongAt strong_at,
+                               akz_kp* __restrict__ out, int32_t* __restrict__ kidx, int cap, uint32_t* s_bits,
+                               int32_t* s_pre, int32_t* s_w) {
+  const int t = threadIdx.x;
+  const int nwords = (n + 31) >> 5, total = ((n + 2047) >> 11) * nwords;
+  for (int i = t; i < total; i += NMS_THREADS) s_bits[i] = 0u;
+  __syncthreads();
+  for (int i = t; i < n; i += NMS_THREADS) {
+    const int v = mem_at(i);
+    if (v >= 0 && strong_at(v)) atomicOr(&s_bits[(i >> 11) * nwords + (v >> 5)], 1u << (v & 31));
+  }
+  __syncthreads();
+  const int c0 = (2 * t < total) ? __popc(s_bits[2 * t]) : 0;
+  const int c1 = (2 * t + 1 < total) ? __popc(s_bits[2 * t + 1]) : 0;
+  int ex;
+  const int kept = nms_block_scan(c0 + c1, &ex, s_w);
+  if (2 * t < total) s_pre[2 * t] = ex;
+  if (2 * t + 1 < total) s_pre[2 * t + 1] = ex + c0;
+  __syncthreads();
+  for (int i = t; i < n; i += NMS_THREADS) {
+    const int v = mem_at(i);
+    if (v >= 0 && strong_at(v)) {
+      const int wi = (i >> 11) * nwords + (v >> 5);
+      const int rank = s_pre[wi] + __popc(s_bits[wi] & ((1u << (v & 31)) - 1u));
+      if (rank < cap) {
+        out[rank] = emit(cand[v]);
+        kidx[rank] = v;
+      }
+    }
+  }
+  return kept;
+}
+
 // Wave-synchronous greedy suppression of FilterExtrema_kernel
 // (cuda_akaze.cu:1072-1224), one workgroup per image.  Per wave: a candidate
 // whose neighbours are all processed is "ready"; a ready candidate with no
@@ -738,6 +776,7 @@ __global__ __launch_bounds__(NMS_THREADS) void k_nms(akz_det_bufs b) {
   __shared__ __align__(16) int16_t s_nb[NMS_LDS_NB];  // the output phase reuses it (24 KB)
   __shared__ int32_t s_w[17];
   __shared__ int32_t s_nact[2];
+  __shared__ int32_t s_left[2];  // work left after a wave, by the parity of the wave
   __shared__ int32_t s_run;
   int32_t* s_sort = reinterpret_cast<int32_t*>(s_nb);
   uint32_t* s_bits = reinterpret_cast<uint32_t*>(s_nb) + 2048;
@@ -835,6 +874,10 @@ __global__ __launch_bounds__(NMS_THREADS) void k_nms(akz_det_bufs b) {
     }
     bool any = n > 0;
     while (wave < b.max_waves && any) {
+      // the flag of this wave's parity was last read before the previous
+      // wave's first barrier; thread 0 clears it, wavefronts with work left
+      // set it after this wave's second phase
+      if (t == 0) s_left[wave & 1] = 0;
       // Arbitration key of this wave: every neighbour of a candidate has a
       // lower index, so the lowest pending candidate is always ready and
       // wins; a wave with work left therefore has wave < n <= NMS_LDS_N and
@@ -902,12 +945,17 @@ __global__ __launch_bounds__(NMS_THREADS) void k_nms(akz_det_bufs b) {
           pend &= ~(1u << j);
         }
       }
+      const bool wave_left = __ballot(pend != 0) != 0;
+      if (wave_left && (t & 63) == 0) s_left[wave & 1] = 1;
+      __syncthreads();
+      any = s_left[wave & 1] != 0;
       wave++;
-      any = __syncthreads_or(pend != 0);
     }
-    kept = nms_output(
+    // n <= NMS_LDS_N: at most 4 chunks of 256 words, side by side in s_bits
+    static_assert(((NMS_LDS_N + 2047) / 2048) * (NMS_LDS_N / 32) <= NMS_BITMAP_WORDS, "chunk bitmaps fit");
+    kept = nms_output_flat(
         cand, n, [&](int i) { return (int)s_mem[i]; }, [&](int v) { return s_strong[v] != 0; }, out, kidx, cap,
-        s_sort, s_bits, s_pre, s_w, &s_run);
+        s_bits, s_pre, s_w);
   } else {
     int32_t* mem = akz_zoff(b.mem, b.zs);
     int32_t* minnb = akz_zoff(b.minnb, b.zs);
