@@ -430,6 +430,11 @@ __global__ __launch_bounds__(SCAN_T) void k_xscan_reduce(const int32_t* __restri
   cnt = akz_zoff(cnt, zs);
   bsum = akz_zoff(bsum, zs);
   const int nvalid = n_ptr ? *akz_zoff(n_ptr, zs) : len;
+  // the grid covers the capacity; blocks past the device-side count sum to 0
+  if ((int)blockIdx.x * SCAN_BLK > nvalid) {
+    if (threadIdx.x == 0) bsum[blockIdx.x] = 0;
+    return;
+  }
   const int i = blockIdx.x * SCAN_BLK + threadIdx.x * SCAN_PER;
   int4 v = load4(cnt, i, nvalid);
   int ex;
@@ -445,6 +450,7 @@ __global__ __launch_bounds__(SCAN_T) void k_xscan_down(const int32_t* __restrict
   bsum = akz_zoff(bsum, zs);
   off = akz_zoff(off, zs);
   const int nvalid = n_ptr ? *akz_zoff(n_ptr, zs) : len;
+  if ((int)blockIdx.x * SCAN_BLK > nvalid) return;  // no offset to write
   const int i = blockIdx.x * SCAN_BLK + threadIdx.x * SCAN_PER;
   int4 v = load4(cnt, i, nvalid);
   // the block's carry: the totals of the blocks before it (a few hundred at
