@@ -34,112 +34,132 @@ __device__ __forceinline__ void load_window(float* __restrict__ dst, const float
 
 struct akz_octaves { int nlev; int oct[AKZ_MAX_LEVELS]; };
 
+// Wavefront lane shifts by DPP with zero fill (one instruction, no copy of an
+// old value): the lane with no source receives 0.
+__device__ __forceinline__ float from_right(float v) {  // lane i receives lane i+1 (lane 63 receives 0)
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xf, 0xf, true));
+}
+__device__ __forceinline__ float from_left(float v) {  // lane i receives lane i-1 (lane 0 receives 0)
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xf, 0xf, true));
+}
+
+
 // Front end of the contrast factor and the level-0 image in one pass over the
-// input, per 64x16 tile with an H-pixel halo (H = max(R0, 3); coordinates
-// clamped: the replicate border of LowPass):
+// input (coordinates clamped: the replicate border of LowPass):
 // * Lt of level 0, the Gaussian of soffset with R0-pixel taps (ConvRowGPU /
 //   ConvColGPU, cuda_akaze.cu:56-128; AKAZE.cpp:234-236), row pass then
 //   column pass, sums in tap order from 0.0f;
-// * the sigma=1 blur of ContrastPercentile (:604-649), the same way, on the
-//   tile plus a one-pixel ring, kept in LDS only;
+// * the sigma=1 blur of ContrastPercentile (:604-649), the same way, in
+//   registers only;
 // * the gradient magnitude of MaxContrast (:552-579): the unnormalised Scharr
-//   pair over the blur's 3x3 window with top-left (x, y), stored at
-//   (x+1, y+1), interior pixels only;
-// * the tile's largest gradient (float bits; all values are >= 0).
-constexpr int CF_TX = 64, CF_TY = 16, CF_NT = 256;
-
-template <int R0>
-__global__ __launch_bounds__(CF_NT) void k_contrast_front(const float* __restrict__ in, int in_pitch, size_t in_zs,
-                                                          float* __restrict__ lt, float* __restrict__ grad,
-                                                          int pitch, int w, int h, akz_taps taps_c, akz_taps taps0,
-                                                          uint32_t* __restrict__ blockmax, size_t zs) {
-  constexpr int H = R0 > 3 ? R0 : 3;
-  constexpr int AW = CF_TX + 2 * H, AH = CF_TY + 2 * H;
-  constexpr int BW = CF_TX + 2;  // blur columns x0-1 .. x0+64
-  constexpr int CH = CF_TY + 2;  // blur rows y0-1 .. y0+16
-  in = akz_zoff(in, in_zs);
-  lt = akz_zoff(lt, zs);
-  grad = akz_zoff(grad, zs);
-  blockmax = akz_zoff(blockmax, zs);
-  __shared__ float A[AH * AW];
-  __shared__ float B0[AH * CF_TX];
-  __shared__ float B1[AH * BW];
-  __shared__ float C[CH * BW];
-  __shared__ uint32_t s_w[CF_NT / 64];
-  const int tid = threadIdx.x;
-  const int x0 = blockIdx.x * CF_TX, y0 = blockIdx.y * CF_TY;
-  load_window<AH, AW, CF_NT>(A, in, in_pitch, x0 - H, y0 - H, w, h);
-  __syncthreads();
-  // row passes: soffset Gaussian over columns x0..x0+63, blur over x0-1..x0+64
-  for (int idx = tid; idx < AH * CF_TX; idx += CF_NT) {
-    const int r = idx / CF_TX, c = idx - r * CF_TX;
-    float sum = 0.0f;
+//   pair over the blur's 3x3 window centred on the pixel, interior pixels only;
+// * the strip's largest gradient (float bits; all values are >= 0).
+// One wavefront per 64-column strip whose lanes M..63-M (M = max(R0, 3)) are
+// output columns, TH output rows.  The TH + 2M input rows stream through the
+// wavefront once; each row's horizontal taps come from DPP lane shifts,
+// shared by the soffset Gaussian and the sigma=1 blur; the column passes and
+// the Scharr rows are register rings.  No LDS and no barriers (an LDS-tiled
+// form with 64x16 tiles took 8% longer on 16-image batches).
+template <int R0, int TH>
+__global__ __launch_bounds__(64) void k_contrast_front(const float* __restrict__ in, int in_pitch,
+                                                           size_t in_zs, float* __restrict__ lt,
+                                                           float* __restrict__ grad, int pitch, int w, int h,
+                                                           akz_taps taps_c, akz_taps taps0,
+                                                           uint32_t* __restrict__ blockmax, size_t zs) {
+  constexpr int M = R0 > 3 ? R0 : 3, TW = 64 - 2 * M, NA = TH + 2 * M;
+  constexpr int K0 = 2 * R0 + 1;
+  const akz_tile3 tl = akz_xcd_tile();
+  in = akz_zoff(in, in_zs, tl.z);
+  lt = akz_zoff(lt, zs, tl.z);
+  grad = akz_zoff(grad, zs, tl.z);
+  blockmax = akz_zoff(blockmax, zs, tl.z);
+  const int lane = threadIdx.x;
+  const int x0 = tl.x * TW - M, y0 = tl.y * TH;
+  const int gx = x0 + lane;
+  const int cx = akz_iclamp(gx, 0, w - 1);
+  const bool out_col = lane >= M && lane < 64 - M && gx < w;
+  const bool grad_col = out_col && gx >= 1 && gx <= w - 2;
+  float k0[K0], kc[5];
 #pragma unroll
-    for (int i = 0; i <= 2 * R0; i++) sum += A[r * AW + c + (H - R0) + i] * taps0.k[i];
-    B0[idx] = sum;
-  }
-  for (int idx = tid; idx < AH * BW; idx += CF_NT) {
-    const int r = idx / BW, c = idx - r * BW;
-    float sum = 0.0f;
+  for (int i = 0; i < K0; i++) k0[i] = taps0.k[i];
 #pragma unroll
-    for (int i = 0; i < 5; i++) sum += A[r * AW + c + (H - 3) + i] * taps_c.k[i];
-    B1[idx] = sum;
-  }
-  __syncthreads();
-  // column passes: level-0 Lt of the tile, blur of the tile and its ring
-  for (int idx = tid; idx < CF_TY * CF_TX; idx += CF_NT) {
-    const int r = idx / CF_TX, c = idx - r * CF_TX;
-    const int gx = x0 + c, gy = y0 + r;
-    if (gx < w && gy < h) {
+  for (int i = 0; i < 5; i++) kc[i] = taps_c.k[i];
+  // rings: B0 (Gaussian row pass) rows i-2R0..i, B1 (blur row pass) rows
+  // i-4..i, C (blur) rows with their x-1 / x+1 neighbours
+  float B0[K0], B1[5], C[3], CL[3], CR[3];
+  uint32_t bits = 0u;
+#pragma unroll
+  for (int i = 0; i < NA; i++) {
+    const int yi = y0 - M + i;  // input row of this step
+    const float a = in[(size_t)akz_iclamp(yi, 0, h - 1) * in_pitch + cx];
+    // v[d + R0]: the row at column x + d, d = -R0..R0
+    float v[K0];
+    v[R0] = a;
+#pragma unroll
+    for (int d = 1; d <= R0; d++) {
+      v[R0 - d] = from_left(v[R0 - d + 1]);
+      v[R0 + d] = from_right(v[R0 + d - 1]);
+    }
+    {
       float sum = 0.0f;
 #pragma unroll
-      for (int i = 0; i <= 2 * R0; i++) sum += B0[(r + (H - R0) + i) * CF_TX + c] * taps0.k[i];
-      lt[(size_t)gy * pitch + gx] = sum;
+      for (int t = 0; t < K0; t++) sum += v[t] * k0[t];
+      B0[i % K0] = sum;
     }
-  }
-  for (int idx = tid; idx < CH * BW; idx += CF_NT) {
-    const int r = idx / BW, c = idx - r * BW;
-    float sum = 0.0f;
+    {
+      float sum = 0.0f;
 #pragma unroll
-    for (int i = 0; i < 5; i++) sum += B1[(r + (H - 3) + i) * BW + c] * taps_c.k[i];
-    C[idx] = sum;
-  }
-  __syncthreads();
-  // gradient at interior pixels (X, Y) of the tile from the blur window with
-  // top-left (X-1, Y-1), which is C[r][c] for the tile position (r, c)
-  uint32_t bits = 0u;
-  for (int idx = tid; idx < CF_TY * CF_TX; idx += CF_NT) {
-    const int r = idx / CF_TX, c = idx - r * CF_TX;
-    const int X = x0 + c, Y = y0 + r;
-    if (X >= 1 && X <= w - 2 && Y >= 1 && Y <= h - 2) {
-      const float* b = C + r * BW + c;
-      const int W = BW;
-      float dx = 3.0f * (b[0] - b[2] + b[2 * W] - b[2 * W + 2]) + 10.0f * (b[W] - b[W + 2]);
-      float dy = 3.0f * (b[0] + b[2] - b[2 * W] - b[2 * W + 2]) + 10.0f * (b[1] - b[2 * W + 1]);
-      float gv = sqrtf(dx * dx + dy * dy);
-      grad[(size_t)Y * pitch + X] = gv;
-      const uint32_t gb = __float_as_uint(gv);
-      bits = bits > gb ? bits : gb;
+      for (int t = 0; t < 5; t++) sum += v[R0 - 2 + t] * kc[t];
+      B1[i % 5] = sum;
+    }
+    // Lt row y0 + i - R0 - M: B0 rows of that row -R0 .. +R0
+    {
+      const int r = i - R0 - M;
+      if (r >= 0 && r < TH) {
+        float sum = 0.0f;
+#pragma unroll
+        for (int t = 0; t < K0; t++) sum += B0[(i - 2 * R0 + t) % K0] * k0[t];
+        const int y = y0 + r;
+        if (out_col && y < h) lt[(size_t)y * pitch + gx] = sum;
+      }
+    }
+    // blur row yc = y0 - M + i - 2 (B1 rows yc-2 .. yc+2), kept for rows
+    // y0-1 .. y0+TH; the gradient of row yc - 1 once row yc exists
+    const int jc = i - M - 2 + 1;  // blur ring index: yc = y0 - 1 + jc
+    if (jc >= 0 && jc <= TH + 1) {
+      float sum = 0.0f;
+#pragma unroll
+      for (int t = 0; t < 5; t++) sum += B1[(i - 4 + t) % 5] * kc[t];
+      C[jc % 3] = sum;
+      CL[jc % 3] = from_left(sum);
+      CR[jc % 3] = from_right(sum);
+      const int r = jc - 2;  // gradient row y0 + r: blur rows r-1, r, r+1 are jc-2, jc-1, jc
+      if (r >= 0) {
+        const int Y = y0 + r;
+        const int u = (jc - 2) % 3, m = (jc - 1) % 3, d = jc % 3;
+        if (grad_col && Y >= 1 && Y <= h - 2) {
+          const float dx = 3.0f * (CL[u] - CR[u] + CL[d] - CR[d]) + 10.0f * (CL[m] - CR[m]);
+          const float dy = 3.0f * (CL[u] + CR[u] - CL[d] - CR[d]) + 10.0f * (C[u] - C[d]);
+          const float gv = sqrtf(dx * dx + dy * dy);
+          grad[(size_t)Y * pitch + gx] = gv;
+          const uint32_t gb = __float_as_uint(gv);
+          bits = bits > gb ? bits : gb;
+        }
+      }
     }
   }
   for (int off = 32; off > 0; off >>= 1) {
-    uint32_t o = __shfl_xor(bits, off, 64);
+    const uint32_t o = __shfl_xor(bits, off, 64);
     bits = bits > o ? bits : o;
   }
-  if ((tid & 63) == 0) s_w[akz_wave()] = bits;
-  __syncthreads();
-  if (tid == 0) {
-    uint32_t m = s_w[0];
-    for (int i = 1; i < CF_NT / 64; i++) m = m > s_w[i] ? m : s_w[i];
-    blockmax[blockIdx.y * gridDim.x + blockIdx.x] = m;
-  }
+  if (lane == 0) blockmax[tl.y * (int)gridDim.x + tl.x] = bits;
 }
 
 constexpr int CT_HIST_BLOCKS = 512;
 constexpr int CT_HIST_THREADS = 512;
 
 // Histogram of grad/max over interior pixels (HistContrast, :581-602).  Every
-// block first reduces the tile maxima of k_contrast_front to the global
+// block first reduces the strip maxima of k_contrast_front to the global
 // maximum (block 0 stores it for the percentile).  Bins are kept in 16
 // columns (LDS hist[bin][lane % 16]): at most four lanes of a wavefront add
 // to one counter, and the 19 KB table clears and sums four times faster
@@ -485,12 +505,6 @@ __global__ __launch_bounds__(NT) void k_fed_multi_sf(const float* __restrict__ i
 // The shifts fill the lane with no source with zero (one instruction, no
 // copy of an old value); both kernels that use them read that lane's result
 // only in their halo, which never reaches an output.
-__device__ __forceinline__ float from_right(float v) {  // lane i receives lane i+1 (lane 63 receives 0)
-  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xf, 0xf, true));
-}
-__device__ __forceinline__ float from_left(float v) {  // lane i receives lane i-1 (lane 0 receives 0)
-  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xf, 0xf, true));
-}
 
 template <int R, bool BORDER>
 __device__ __forceinline__ void fed_reg_body(const float* __restrict__ L, const float* __restrict__ g,
@@ -911,7 +925,16 @@ inline dim3 grid64x4(int w, int h, int nz) { return dim3((w + 63) / 64, (h + 3) 
 }  // namespace
 
 
-int akz_contrast_front_blocks(int w, int h) { return ((w + CF_TX - 1) / CF_TX) * ((h + CF_TY - 1) / CF_TY); }
+// k_contrast_front: 64 - 2 max(R0, 3) columns per wavefront (54 at least:
+// R0 <= 5) and TH rows: 32 for batches, fewer for one image, whose strips
+// would otherwise be too few to fill the GPU
+constexpr int CFR_TH = 32, CFR_TH1 = 8;
+template <int R0, int TH>
+static dim3 cfr_grid(int w, int h, int nz) {
+  constexpr int TW = 64 - 2 * (R0 > 3 ? R0 : 3);
+  return dim3((w + TW - 1) / TW, (h + TH - 1) / TH, nz);
+}
+int akz_contrast_front_blocks(int w, int h) { return ((w + 53) / 54) * ((h + CFR_TH1 - 1) / CFR_TH1); }
 
 void akz_launch_contrast(hipStream_t s, const float* img, int in_pitch, size_t in_zs, float* lt, float* grad,
                          int pitch, int w, int h, int R0, const akz_taps& taps0, const akz_taps& taps_c, int nbins,
@@ -922,14 +945,26 @@ void akz_launch_contrast(hipStream_t s, const float* img, int in_pitch, size_t i
   for (int i = 0; i < nlev; i++) o.oct[i] = octaves_host[i];
   const int nz = bt.nz;
   const size_t zs = bt.zs;
-  const dim3 fgrid((w + CF_TX - 1) / CF_TX, (h + CF_TY - 1) / CF_TY, nz);
-  switch (R0) {
-    case 2: hipLaunchKernelGGL(k_contrast_front<2>, fgrid, dim3(CF_NT), 0, s, img, in_pitch, in_zs, lt, grad, pitch, w, h, taps_c, taps0, blockmax, zs); break;
-    case 3: hipLaunchKernelGGL(k_contrast_front<3>, fgrid, dim3(CF_NT), 0, s, img, in_pitch, in_zs, lt, grad, pitch, w, h, taps_c, taps0, blockmax, zs); break;
-    case 4: hipLaunchKernelGGL(k_contrast_front<4>, fgrid, dim3(CF_NT), 0, s, img, in_pitch, in_zs, lt, grad, pitch, w, h, taps_c, taps0, blockmax, zs); break;
-    default: hipLaunchKernelGGL(k_contrast_front<5>, fgrid, dim3(CF_NT), 0, s, img, in_pitch, in_zs, lt, grad, pitch, w, h, taps_c, taps0, blockmax, zs); break;
+  dim3 fgrid;
+#define AKZ_CF(R, TH)                                                                                    \
+  fgrid = cfr_grid<R, TH>(w, h, nz);                                                                     \
+  hipLaunchKernelGGL((k_contrast_front<R, TH>), fgrid, dim3(64), 0, s, img, in_pitch, in_zs, lt, grad, \
+                     pitch, w, h, taps_c, taps0, blockmax, zs)
+#define AKZ_CF2(R)    \
+  if (nz > 1) {       \
+    AKZ_CF(R, CFR_TH);  \
+  } else {            \
+    AKZ_CF(R, CFR_TH1); \
   }
-  // histogram blocks, which each clear and reduce 77 KB of LDS, are spread
+  switch (R0) {
+    case 2: AKZ_CF2(2); break;
+    case 3: AKZ_CF2(3); break;
+    case 4: AKZ_CF2(4); break;
+    default: AKZ_CF2(5); break;
+  }
+#undef AKZ_CF2
+#undef AKZ_CF
+  // histogram blocks, which each clear and reduce 19 KB of LDS, are spread
   // over the images of a batch; each wavefront gets a row at least
   const int per = nz > 0 ? nz : 1;
   const int hb = akz_imin((h - 2 + CT_HIST_THREADS / 64 - 1) / (CT_HIST_THREADS / 64),
