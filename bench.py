@@ -210,7 +210,7 @@ def streams_workload(A, args):
     a seeded base image under drifting homographies plus noise of std 2/255.
     Ragged batches of 16 frames (4 per stream) run with the learned tile
     proposal; every 8th frame of a stream is audited by the full scan, whose
-    keypoints judge the proposal's recall; every frame is matched 2-NN to its
+    keypoints measure the proposal's recall; every frame is matched 2-NN to its
     predecessor in the stream."""
     from akaze_amd.proposal import ProposalNet, keypoint_recall
     here = os.path.join(REPO, "akaze_amd")

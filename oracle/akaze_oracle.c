@@ -2,7 +2,8 @@
  * akaze_oracle.c -- CPU restatement of the A-KAZE hot path (TEST INFRASTRUCTURE).
  *
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load this
- * library, and only as the checker.  The product (akaze_amd/) never links it.
+ * library, and only as the reference the results are compared with.  The
+ * product (akaze_amd/) never links it.
  *
  * Each function restates the semantics of the reference CUDA fork of libAKAZE;
  * citations are file:line in that repository.  Operand order inside every

@@ -12,8 +12,9 @@
  * keeps the reference's operand order, so results are reproducible bit-for-bit
  * by any IEEE-754 implementation that evaluates the same expressions.
  *
- * Parity status of each function (how it is pinned) is listed in DESIGN.md; the
- * functions with no external pin are marked "parity unpinned" below.
+ * How each function is pinned is in tests/test_oracle_stages.py and
+ * tests/test_proposal_oracle.py; the functions with no external pin are marked
+ * "parity unpinned" below.
  */
 #ifndef AKAZE_ORACLE_H
 #define AKAZE_ORACLE_H
@@ -87,6 +88,10 @@ int orc_ksize_radius(int ksize);
 void orc_gauss_taps(double var, int R, float* taps);
 void orc_lowpass(const float* in, float* out, int w, int h, double var, int ksize);
 float orc_contrast(const float* img, int w, int h, float perc, int nbins, int32_t* hist_out, float* max_out);
+/* Conductance of Lsm (cuda_akaze.cu:240-287).  type 1 (PM g2) and 3
+ * (Charbonnier) are pinned (closed form of a ramp; bit-exact against the
+ * device).  type 0 (PM g1) and 2 (Weickert): parity unpinned -- they call
+ * expf / exp, whose last bit differs between glibc and the device library. */
 void orc_flow(const float* Lsm, float* g, int w, int h, int type, float kcontrast);
 void orc_fed_step(const float* L, const float* g, float* out, int w, int h, float tau);
 void orc_halfsample(const float* in, int iw, int ih, float* out, int ow, int oh);

@@ -17,5 +17,5 @@ def gpu_lib():
     """The HIP library with a visible device; fails (does not skip) otherwise."""
     import akaze_amd
     n = akaze_amd.device_count()
-    assert n >= 1, "no HIP device visible: GPU tests must run on the GPU box"
+    assert n >= 1, "no HIP device visible: GPU tests need an MI355X"
     return akaze_amd
