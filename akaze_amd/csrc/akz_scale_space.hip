@@ -182,8 +182,21 @@ __global__ __launch_bounds__(CT_HIST_THREADS) void k_contrast_hist(const float* 
   constexpr int NW = CT_HIST_THREADS / 64;
   for (int i = threadIdx.x; i < 300 * 16; i += CT_HIST_THREADS) lh[i] = 0;
   for (int i = threadIdx.x; i < 212; i += CT_HIST_THREADS) tail[i] = 0;
+  // strip maxima and gradient rows: eight independent loads in flight per
+  // lane before any is used (a loop that waits on each load leaves the
+  // kernel bound by memory latency at one wavefront per SIMD)
+  constexpr int U = 8;
   uint32_t mb = 0u;
-  for (int i = threadIdx.x; i < nbm; i += CT_HIST_THREADS) mb = mb > blockmax[i] ? mb : blockmax[i];
+  for (int i0 = threadIdx.x; i0 < nbm; i0 += U * CT_HIST_THREADS) {
+    uint32_t b[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int i = i0 + u * CT_HIST_THREADS;
+      b[u] = i < nbm ? blockmax[i] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) mb = mb > b[u] ? mb : b[u];
+  }
   for (int off = 32; off > 0; off >>= 1) {
     uint32_t o = __shfl_xor(mb, off, 64);
     mb = mb > o ? mb : o;
@@ -198,12 +211,22 @@ __global__ __launch_bounds__(CT_HIST_THREADS) void k_contrast_hist(const float* 
   const int nwaves = gridDim.x * NW;
   for (int y = 1 + blockIdx.x * NW + wv; y < h - 1; y += nwaves) {
     const float* row = grad + (size_t)y * pitch;
-    for (int x = 1 + lane; x < w - 1; x += 64) {
-      float v = nbins * row[x] * imaxval;
-      int idx = (v != v) ? 0 : (int)v;
-      idx = akz_imin(idx, nbins - 1);
-      if (idx < 300) atomicAdd(&lh[idx * 16 + (lane & 15)], 1);
-      else atomicAdd(&tail[idx - 300], 1);
+    for (int x0 = 1 + lane; x0 < w - 1; x0 += U * 64) {
+      float g[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int x = x0 + u * 64;
+        g[u] = x < w - 1 ? row[x] : 0.0f;
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        if (x0 + u * 64 >= w - 1) break;
+        float v = nbins * g[u] * imaxval;
+        int idx = (v != v) ? 0 : (int)v;
+        idx = akz_imin(idx, nbins - 1);
+        if (idx < 300) atomicAdd(&lh[idx * 16 + (lane & 15)], 1);
+        else atomicAdd(&tail[idx - 300], 1);
+      }
     }
   }
   __syncthreads();
