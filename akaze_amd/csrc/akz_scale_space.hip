@@ -88,10 +88,15 @@ __global__ __launch_bounds__(64) void k_contrast_front(const float* __restrict__
   // i-4..i, C (blur) rows with their x-1 / x+1 neighbours
   float B0[K0], B1[5], C[3], CL[3], CR[3];
   uint32_t bits = 0u;
+  // every input row of the strip is loaded before the first is used: loaded
+  // row by row, the loads wait one after another and the kernel is bound by
+  // memory latency, not by its arithmetic
+  float rin[NA];
+#pragma unroll
+  for (int i = 0; i < NA; i++) rin[i] = in[(size_t)akz_iclamp(y0 - M + i, 0, h - 1) * in_pitch + cx];
 #pragma unroll
   for (int i = 0; i < NA; i++) {
-    const int yi = y0 - M + i;  // input row of this step
-    const float a = in[(size_t)akz_iclamp(yi, 0, h - 1) * in_pitch + cx];
+    const float a = rin[i];  // input row y0 - M + i
     // v[d + R0]: the row at column x + d, d = -R0..R0
     float v[K0];
     v[R0] = a;
