@@ -127,8 +127,9 @@ struct akz_ctx {
   akz_taps taps0{}, taps1{}, taps_c{};
   int R0 = 4;
   hipEvent_t ev[EV_COUNT] = {};
-  bool timing_valid = false;
-  akz_timing timing{};
+  // events of the last eager run (bit e: ev[e] recorded); graph captures and
+  // replays record none, so their stage times are not reported as current
+  unsigned ev_mask = 0;
   int state = 0;  // 0 created, 1 scale space, 2 detected, 3 described
   bool graph_mode = false;
   bool capturing = false;
@@ -176,6 +177,17 @@ akz_status compute_geometry(akz_ctx* c) {
   if (o.descriptor_size != 0 || o.descriptor_channels != 3) return AKZ_ERR_UNSUPPORTED;
   // the MLDB grid (max(3*ceil(2p/3), 4*ceil(p/2)))^2 must fit 7 x 64 samples per wavefront
   if (o.descriptor_pattern_size < 1 || o.descriptor_pattern_size > 10) return AKZ_ERR_UNSUPPORTED;
+  // scales and thresholds: finite and positive (the level sigmas, FED times and
+  // derivative steps are built from them)
+  if (!(std::isfinite(o.soffset) && o.soffset > 0.0f)) return AKZ_ERR_ARG;
+  if (!(std::isfinite(o.derivative_factor) && o.derivative_factor > 0.0f)) return AKZ_ERR_ARG;
+  if (!std::isfinite(o.dthreshold) || !std::isfinite(o.min_dthreshold)) return AKZ_ERR_ARG;
+  // ContrastPercentile (cuda_akaze.cu:619-641) caps the bins at 512 and takes a
+  // fraction of the interior pixels; fewer than one bin or a fraction outside
+  // (0, 1] has no percentile
+  if (o.kcontrast_nbins < 1 || o.kcontrast_nbins > 512) return AKZ_ERR_ARG;
+  if (!(std::isfinite(o.kcontrast_percentile) && o.kcontrast_percentile > 0.0f && o.kcontrast_percentile <= 1.0f))
+    return AKZ_ERR_ARG;
   int nlev = 0;
   // level geometry and scales, AKAZE.cpp:151-184
   for (int i = 0; i <= o.omax - 1; i++) {
@@ -214,6 +226,9 @@ akz_status compute_geometry(akz_ctx* c) {
     L.det_size = L.esigma * o.derivative_factor;
     L.border = smax * fround(L.det_size / ratio);
     c->octs[i] = L.octave;
+    // the reflect-101 indices of Derivate (cuda_akaze.cu:651-671) stay inside a
+    // level only for steps below its width and height
+    if (L.deriv_step < 0 || L.deriv_step >= std::min(L.w, L.h)) return AKZ_ERR_ARG;
   }
   return AKZ_OK;
 }
@@ -389,7 +404,10 @@ int level_block(const akz_ctx* c, int i) {
 
 // Create_Nonlinear_Scale_Space (AKAZE.cpp:211-276) on the context stream.
 inline void record(akz_ctx* c, int e) {
-  if (!c->capturing) hipEventRecord(c->ev[e], c->stream);
+  if (e == EV_START) c->ev_mask = 0;
+  if (c->capturing) return;
+  hipEventRecord(c->ev[e], c->stream);
+  c->ev_mask |= 1u << e;
 }
 
 // Images 0..nz-1 of the batch: input z at img + z * in_zs bytes.
@@ -709,7 +727,6 @@ akz_status akz_build_scale_space(akz_ctx* c, const float* img, size_t pitch_byte
   AKZ_HIP_TRY(hipGetLastError());
   c->state = 1;
   c->last_nz = 1;
-  c->timing_valid = false;
   return AKZ_OK;
 }
 
@@ -722,7 +739,6 @@ akz_status akz_build_scale_space_device(akz_ctx* c, const float* d_img, size_t p
   AKZ_HIP_TRY(hipGetLastError());
   c->state = 1;
   c->last_nz = 1;
-  c->timing_valid = false;
   return AKZ_OK;
 }
 
@@ -838,7 +854,6 @@ akz_status akz_process_batch_device_async(akz_ctx* c, const float* d_imgs, size_
   }
   c->state = 3;
   c->last_nz = n;
-  c->timing_valid = false;
   return AKZ_OK;
 }
 
@@ -941,18 +956,19 @@ akz_status akz_read_candidates(akz_ctx* c, akz_keypoint* cand, int cap, int* n_o
 akz_status akz_get_timing(akz_ctx* c, akz_timing* t) {
   if (!c || !t) return AKZ_ERR_ARG;
   AKZ_HIP_TRY(hipStreamSynchronize(c->stream));
+  auto have = [&](int a, int b) { return (c->ev_mask >> a & 1u) && (c->ev_mask >> b & 1u); };
+  // no stage timed by the last run (a graph replay records no events)
+  if (!have(EV_START, EV_SCALE)) return AKZ_ERR_STATE;
   akz_timing r{};
-  if (c->state >= 1) {
-    r.kcontrast = ev_ms(c, EV_START, EV_CONTRAST);
-    r.scale = ev_ms(c, EV_START, EV_SCALE);
-  }
-  if (c->state >= 2) {
-    r.derivatives = ev_ms(c, EV_SCALE, EV_DERIV);
-    r.extrema = ev_ms(c, EV_DERIV, EV_NMS);
+  r.kcontrast = have(EV_START, EV_CONTRAST) ? ev_ms(c, EV_START, EV_CONTRAST) : 0.0;
+  r.scale = ev_ms(c, EV_START, EV_SCALE);
+  if (c->state >= 2 && have(EV_SCALE, EV_NMS)) {
+    r.derivatives = have(EV_SCALE, EV_DERIV) ? ev_ms(c, EV_SCALE, EV_DERIV) : 0.0;
+    r.extrema = have(EV_DERIV, EV_NMS) ? ev_ms(c, EV_DERIV, EV_NMS) : 0.0;
     r.detector = ev_ms(c, EV_SCALE, EV_NMS);
     r.subpixel = 0.0;  // folded into the extrema kernels, as in the reference
   }
-  if (c->state >= 3) r.descriptor = ev_ms(c, EV_NMS, EV_DESC);
+  if (c->state >= 3 && have(EV_NMS, EV_DESC)) r.descriptor = ev_ms(c, EV_NMS, EV_DESC);
   *t = r;
   return AKZ_OK;
 }

@@ -39,7 +39,9 @@ typedef struct {
   int nsublevels;               /* sublevels per octave, default 4 */
   int img_width, img_height;    /* image size fixed at creation (AKAZEConfig.h:113-114) */
   float soffset;                /* 1.6 */
-  float derivative_factor;      /* 1.5 */
+  float derivative_factor;      /* 1.5; with soffset finite and > 0, and every
+                                   level's derivative step below its width and
+                                   height, else AKZ_ERR_ARG */
   float sderivatives;           /* 1.0 (unused by the path, kept for layout) */
   int diffusivity;              /* 0 PM_G1, 1 PM_G2 (default), 2 WEICKERT, 3 CHARBONNIER */
   float dthreshold;             /* 0.001 */
@@ -49,8 +51,8 @@ typedef struct {
   int descriptor_channels;      /* 3 */
   int descriptor_pattern_size;  /* 10 */
   float kcontrast;              /* output of the contrast stage */
-  float kcontrast_percentile;   /* 0.7 */
-  int kcontrast_nbins;          /* 300, at most 512 */
+  float kcontrast_percentile;   /* 0.7; in (0, 1], else AKZ_ERR_ARG */
+  int kcontrast_nbins;          /* 300; 1..512, else AKZ_ERR_ARG */
   int maxkeypoints;             /* 0 = capacity for every possible candidate */
   int nms_max_waves;            /* 10000, cuda_akaze.cu:1100 */
   int verbosity;
@@ -217,7 +219,9 @@ akz_status akz_profile_fed(akz_ctx* ctx, const float* d_img, size_t pitch_bytes,
  * `iters` times on the null stream; average ms per launch.  Used to calibrate
  * hardware byte counters against a kernel of known traffic. */
 akz_status akz_copy_kernel_run(const float* d_src, float* d_dst, size_t nfloats, int iters, float* avg_ms);
-/* AKAZE::Get_Computation_Times (AKAZE.h:139-141). */
+/* AKAZE::Get_Computation_Times (AKAZE.h:139-141): stage times of the last
+ * eager run.  AKZ_ERR_STATE when that run recorded none (a captured graph
+ * replays without events). */
 akz_status akz_get_timing(akz_ctx* ctx, akz_timing* t);
 
 /* ---- Learned tile proposal.  No counterpart in the reference.  A small

@@ -52,6 +52,49 @@ def test_create_validates_arguments_before_touching_the_device():
     assert L.akz_create(None, ctypes.byref(p)) == 1
 
 
+def test_create_rejects_options_that_would_index_out_of_bounds():
+    """Options whose kernels would read or write outside their buffers are
+    refused with AKZ_ERR_ARG (1) before any device call: a derivative step at
+    or beyond a level's width or height (reflect-101 indices of Derivate,
+    cuda_akaze.cu:651-671), histogram bins outside 1..512 (ContrastPercentile
+    caps them at 512, cuda_akaze.cu:619-624), a percentile outside (0, 1], and
+    non-finite or non-positive scales."""
+    L = A.lib()
+    p = ctypes.c_void_p()
+    nan, inf = float("nan"), float("inf")
+    bad = [
+        # 16x16: level 0's step fRound(1.6 * 6 * 2^(3/4)) = 16 reaches the width
+        dict(img_width=16, img_height=16, omax=1, derivative_factor=6.0),
+        dict(img_width=640, img_height=480, derivative_factor=0.0),
+        dict(img_width=640, img_height=480, derivative_factor=nan),
+        dict(img_width=640, img_height=480, soffset=-1.6),
+        dict(img_width=640, img_height=480, soffset=inf),
+        dict(img_width=640, img_height=480, kcontrast_nbins=0),
+        dict(img_width=640, img_height=480, kcontrast_nbins=-3),
+        dict(img_width=640, img_height=480, kcontrast_nbins=513),
+        dict(img_width=640, img_height=480, kcontrast_percentile=0.0),
+        dict(img_width=640, img_height=480, kcontrast_percentile=1.5),
+        dict(img_width=640, img_height=480, kcontrast_percentile=nan),
+        dict(img_width=640, img_height=480, dthreshold=nan),
+    ]
+    for kw in bad:
+        o = A.AKAZEOptions(**kw)
+        st = L.akz_create(ctypes.byref(o._c), ctypes.byref(p))
+        assert st == 1, (kw, st)
+    # the limits themselves are accepted (the call then fails only for want
+    # of a device here, or succeeds on a GPU)
+    ok = [dict(img_width=640, img_height=480, kcontrast_nbins=1),
+          dict(img_width=640, img_height=480, kcontrast_nbins=512),
+          dict(img_width=640, img_height=480, kcontrast_percentile=1.0),
+          dict(img_width=16, img_height=16, omax=1, derivative_factor=5.0)]
+    for kw in ok:
+        o = A.AKAZEOptions(**kw)
+        st = L.akz_create(ctypes.byref(o._c), ctypes.byref(p))
+        assert st in (0, 7), (kw, st)
+        if st == 0:
+            assert L.akz_destroy(p) == 0
+
+
 @pytest.mark.skipif(A.device_count() > 0, reason="checks the no-device path")
 def test_no_device_is_an_error_not_a_fallback():
     o = A.AKAZEOptions(img_width=640, img_height=480)
