@@ -156,16 +156,34 @@ def test_contrast_ramp_closed_form():
 
 
 # ---------------------------------------------------------------- conductance
-def test_flow_pm_g2_on_ramp():
-    a, b, kc = 0.003, -0.002, 0.05
+FLOW_FORMULAS = {
+    # Flow, cuda_akaze.cu:261-268: PM_G1, PM_G2, WEICKERT, CHARBONNIER of
+    # d = |grad|^2 / k^2
+    0: lambda d: math.exp(-d),
+    1: lambda d: 1.0 / (1.0 + d),
+    2: lambda d: 1.0 - math.exp(-3.315 / d ** 4),
+    3: lambda d: 1.0 / math.sqrt(1.0 + d),
+}
+
+
+@pytest.mark.parametrize("kc", [0.05, 0.1])
+@pytest.mark.parametrize("dtype", [0, 1, 2, 3])
+def test_flow_on_ramp(dtype, kc):
+    """On L = a x + b y the unnormalised Scharr gives (32 a, 32 b) inside and
+    (0, 32 b) on the reflect-101 edge columns, so g is the closed form of the
+    diffusivity at d = |grad|^2 / k^2."""
+    a, b = 0.003, -0.002
     y, x = np.mgrid[0:30, 0:40].astype(np.float32)
     L = (a * x + b * y).astype(np.float32)
-    g = O.flow(L, kc, 1)
-    grad2 = (32 * a) ** 2 + (32 * b) ** 2
-    expect = 1.0 / (1.0 + grad2 / kc ** 2)
-    assert np.allclose(g[1:-1, 1:-1], expect, rtol=1e-5)
-    # reflect-101 border: no gradient across the edge column / row
-    assert np.allclose(g[5, 0], 1.0 / (1.0 + (32 * b) ** 2 / kc ** 2), rtol=1e-5)
+    g = O.flow(L, kc, dtype)
+    f = FLOW_FORMULAS[dtype]
+    d_in = ((32 * a) ** 2 + (32 * b) ** 2) / kc ** 2
+    d_edge = (32 * b) ** 2 / kc ** 2
+    assert np.allclose(g[1:-1, 1:-1], f(d_in), rtol=2e-5, atol=1e-7)
+    assert np.allclose(g[1:-1, 0], f(d_edge), rtol=2e-5, atol=1e-7)
+    # the four formulas differ from each other far beyond the tolerance here
+    others = [FLOW_FORMULAS[t](d_in) for t in FLOW_FORMULAS if t != dtype]
+    assert all(abs(v - f(d_in)) > 1e-3 * abs(f(d_in)) for v in others)
 
 
 # ---------------------------------------------------------------- FED step
