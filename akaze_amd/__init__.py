@@ -122,6 +122,7 @@ SIGNATURES = {
     "akz_matcher_destroy": [_P],
     "akz_match2nn": [_P, _P, _I, _P, _I, _SZ, _P],
     "akz_match2nn_device": [_P, _P, _I, _P, _I, _P],
+    "akz_match2nn_device_noalloc": [_P, _I, _P, _I, _P, _P],
     "akz_matcher_wait": [_P],
     "akz_matcher_last_ms": [_P, _P],
     "akz_dev_alloc": [_SZ, _P],

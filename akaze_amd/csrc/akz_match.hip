@@ -97,3 +97,66 @@ void akz_launch_match(hipStream_t s, const uint8_t* q64, int nq, const uint8_t* 
 void akz_launch_match_merge(hipStream_t s, const uint32_t* partial, int nq, int nsplit, akz_dmatch* out) {
   hipLaunchKernelGGL(k_match_merge, dim3((nq + 255) / 256), dim3(256), 0, s, partial, nq, nsplit, out);
 }
+
+// ---- Allocation-free 2-NN: the per-slice top-2 partials of k_match live in a
+// file-scope device array instead of a matcher's grow-only buffer.  Each GPU
+// holds its own copy of the array; its address on a device is taken once, with
+// that device current, by hipGetSymbolAddress, and only that address is handed
+// to kernels (the array's name is a host-side stand-in in host code).  Every
+// (slice, query) pair the merge reads is written by k_match of the same call,
+// so nothing needs clearing; calls that overlap on one GPU share the array and
+// race.
+#include <mutex>
+
+namespace {
+
+constexpr size_t NOALLOC_WORDS = (size_t)2 * AKZ_NOALLOC_MAX_QUERIES;  // 8 MiB
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static __device__ __align__(256) uint32_t scratch[NOALLOC_WORDS];
+
+std::mutex g_scratch_mu;
+uint32_t* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+akz_status scratch_on_current_device(uint32_t** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    (void)hipGetLastError();
+    return AKZ_ERR_NO_DEVICE;
+  }
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return AKZ_ERR_UNSUPPORTED;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return AKZ_ERR_HIP;
+    }
+    g_scratch_ptr[dev] = static_cast<uint32_t*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return AKZ_OK;
+}
+
+}  // namespace
+
+extern "C" akz_status akz_match2nn_device_noalloc(const uint8_t* d_q64, int nq, const uint8_t* d_t64, int nt,
+                                                  akz_dmatch* d_out, void* stream) {
+  if (nq < 0 || nt < 0 || (nq > 0 && (!d_q64 || !d_out)) || (nt > 0 && !d_t64)) return AKZ_ERR_ARG;
+  if (nq > AKZ_NOALLOC_MAX_QUERIES) return AKZ_ERR_ARG;  // the partials would not fit the scratch array
+  if (nt >= (1 << IDX_BITS)) return AKZ_ERR_UNSUPPORTED;
+  if (nq == 0) return AKZ_OK;
+  uint32_t* partial = nullptr;
+  akz_status st = scratch_on_current_device(&partial);
+  if (st != AKZ_OK) return st;
+  // as many train slices as the scratch holds, at most the allocating path's
+  // choice; the merged pairs do not depend on the split
+  int ns = akz_match_splits(nq, nt);
+  int fit = (int)(NOALLOC_WORDS / (2 * (size_t)nq));
+  if (ns > fit) ns = fit;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  akz_launch_match(s, d_q64, nq, d_t64, nt, partial, ns);
+  akz_launch_match_merge(s, partial, nq, ns, d_out);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? AKZ_OK : AKZ_ERR_HIP;
+}

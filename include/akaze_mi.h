@@ -286,6 +286,17 @@ akz_status akz_match2nn(akz_matcher* m, const uint8_t* q, int nq, const uint8_t*
  * memory; asynchronous on the matcher's stream. */
 akz_status akz_match2nn_device(akz_matcher* m, const uint8_t* d_q64, int nq, const uint8_t* d_t64, int nt,
                                akz_dmatch* d_out);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops): the same 2-NN as
+ * akz_match2nn_device on device-resident 64-byte rows, enqueued on `stream`
+ * (a hipStream_t; null = the null stream), with no matcher object and no
+ * allocation of any kind.  Its partial results live in a fixed 8 MiB array
+ * per GPU, held until the process ends, so nq is at most
+ * AKZ_NOALLOC_MAX_QUERIES (else AKZ_ERR_ARG); calls that overlap on one GPU
+ * share that array and race.  Output equals akz_match2nn_device's. */
+#define AKZ_NOALLOC_MAX_QUERIES 1048576
+akz_status akz_match2nn_device_noalloc(const uint8_t* d_q64, int nq, const uint8_t* d_t64, int nt,
+                                       akz_dmatch* d_out, void* stream);
 akz_status akz_matcher_wait(akz_matcher* m);
 /* Average duration in ms of the matcher kernels of the last call (HIP events). */
 akz_status akz_matcher_last_ms(akz_matcher* m, float* ms);
