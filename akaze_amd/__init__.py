@@ -1,10 +1,15 @@
 """MI355X A-KAZE: Python bindings of libakaze_amd.so (ctypes over the C-ABI).
 
-Mirrors the reference's Python module (python/python_bindings.cpp:27-40):
+Carries the reference's Python module (python/python_bindings.cpp:27-40):
 AKAZEOptions with setWidth/setHeight, AKAZE with Create_Nonlinear_Scale_Space,
-Feature_Detection and Compute_Descriptors (returning (desc n x 61 uint8,
-kpts n x 7 float32 in [x, y, size, angle, response, octave, class_id] order,
-AKAZE.cpp:290-298), and Matcher.bfmatch_ returning n x 8 float32.
+Feature_Detection (n x 7 float32 keypoints) and Compute_Descriptors (desc
+n x 61 uint8, kpts n x 7 float32 in [x, y, size, angle, response, octave,
+class_id] order, AKAZE.cpp:290-298), and Matcher.BFMatch returning n x 8
+float32; akaze_amd.libakaze_pybindings exposes exactly those names.  The
+structured-array forms are AKAZE.detect / describe / run and
+Matcher.match2nn.  The evaluation helpers of src/lib/utils.cpp (PGM input,
+save_keypoints, read_homography, NNDR, homography and RANSAC inliers) are
+module functions.
 
 The HIP library must be built (python -m akaze_amd.build); there is no CPU
 fallback: every call runs the HIP kernels or raises.
@@ -136,6 +141,12 @@ SIGNATURES = {
     "akz_synth_image": [_U64, _I, _I, _P],
     "akz_synth_homography": [_U64, _I, _I, _D, _D, _D, _D, _P],
     "akz_synth_warp": [_P, _I, _I, _P, _D, _U64, _P],
+    "akz_read_pgm": [ctypes.c_char_p, _P, _SZ, _P, _P],
+    "akz_save_keypoints": [ctypes.c_char_p, _P, _I, _P, _I, _I],
+    "akz_read_homography": [ctypes.c_char_p, _P],
+    "akz_matches2points_nndr": [_P, _I, _P, _I, _P, _I, ctypes.c_float, _P, _P],
+    "akz_inliers_homography": [_P, _I, _P, ctypes.c_float, _P, _P],
+    "akz_inliers_ransac": [_P, _I, ctypes.c_float, _U64, _I, _D, _P, _P, _P],
 }
 
 _lib = None
@@ -195,6 +206,99 @@ def synth_warp(img, H, noise_std=0.0, noise_seed=0):
 def to_float(img_u8):
     """uint8 -> float32 in [0, 1] (the reference's input normalisation)."""
     return img_u8.astype(np.float32) / np.float32(255.0)
+
+
+# ---------------------------------------------------------------- I/O and evaluation (src/lib/utils.cpp)
+def _path(p):
+    return os.fsencode(os.fspath(p))
+
+
+def read_pgm(path):
+    """cv::imread(path, 0) for PGM (P5/P2) files: uint8 H x W."""
+    w, h = ctypes.c_int(), ctypes.c_int()
+    _check(lib().akz_read_pgm(_path(path), None, 0, ctypes.byref(w), ctypes.byref(h)), "akz_read_pgm")
+    out = np.empty((h.value, w.value), np.uint8)
+    _check(lib().akz_read_pgm(_path(path), out.ctypes.data, out.nbytes, ctypes.byref(w), ctypes.byref(h)),
+           "akz_read_pgm")
+    return out
+
+
+def save_keypoints(path, kpts, desc=None, save_desc=True):
+    """save_keypoints (utils.cpp:97-151).  kpts: KP_DTYPE array or n x 7
+    float32 [x, y, size, angle, response, octave, class_id]."""
+    kps = _as_keypoints(kpts)
+    d = np.zeros((len(kps), 0), np.uint8) if desc is None else np.ascontiguousarray(desc, np.uint8)
+    _check(lib().akz_save_keypoints(_path(path), kps.ctypes.data, len(kps), d.ctypes.data, d.shape[1] if d.ndim == 2
+                                    else 0, int(bool(save_desc))), "akz_save_keypoints")
+
+
+def read_homography(path):
+    """read_homography (utils.cpp:340-380): 3 x 3 float64 (float precision), or None."""
+    H = np.zeros(9, np.float64)
+    st = lib().akz_read_homography(_path(path), H.ctypes.data)
+    return None if st != 0 else H.reshape(3, 3)
+
+
+def matches2points_nndr(train, query, matches, nndr=0.8):
+    """matches2points_nndr (utils.cpp:154-170): k x 4 float32 rows
+    (train[queryIdx].pt, query[trainIdx].pt) of the rows with best < nndr * second."""
+    tr, qu = _as_keypoints(train), _as_keypoints(query)
+    m = np.ascontiguousarray(matches, DM_DTYPE).reshape(-1, 2)
+    out = np.zeros((max(len(m), 1), 4), np.float32)
+    k = ctypes.c_int()
+    _check(lib().akz_matches2points_nndr(tr.ctypes.data, len(tr), qu.ctypes.data, len(qu), m.ctypes.data, len(m),
+                                         nndr, out.ctypes.data, ctypes.byref(k)), "akz_matches2points_nndr")
+    return out[:k.value].copy()
+
+
+def inliers_homography(pairs, H, min_error=2.5):
+    """compute_inliers_homography (utils.cpp:203-243): boolean mask."""
+    p = np.ascontiguousarray(pairs, np.float32).reshape(-1, 4)
+    Hd = np.ascontiguousarray(H, np.float64).reshape(9)
+    mask = np.zeros(max(len(p), 1), np.uint8)
+    k = ctypes.c_int()
+    _check(lib().akz_inliers_homography(p.ctypes.data, len(p), Hd.ctypes.data, min_error, mask.ctypes.data,
+                                        ctypes.byref(k)), "akz_inliers_homography")
+    return mask[:len(p)].astype(bool)
+
+
+def inliers_ransac(pairs, error=2.5, seed=0x5EED, max_iters=2000, confidence=0.995):
+    """compute_inliers_ransac (utils.cpp:173-200) with the library's own
+    RANSAC homography: (H 3 x 3, boolean mask)."""
+    p = np.ascontiguousarray(pairs, np.float32).reshape(-1, 4)
+    H = np.zeros(9, np.float64)
+    mask = np.zeros(max(len(p), 1), np.uint8)
+    k = ctypes.c_int()
+    _check(lib().akz_inliers_ransac(p.ctypes.data, len(p), error, seed, max_iters, confidence, H.ctypes.data,
+                                    mask.ctypes.data, ctypes.byref(k)), "akz_inliers_ransac")
+    return H.reshape(3, 3), mask[:len(p)].astype(bool)
+
+
+def kpts_to_mat(kps):
+    """KeyPoint array -> n x 7 float32 [x, y, size, angle, response, octave,
+    class_id], the order of mat2kpvec (AKAZE.cpp:290-298)."""
+    kps = np.asarray(kps, KP_DTYPE)
+    return np.stack([kps["x"], kps["y"], kps["size"], kps["angle"], kps["response"],
+                     kps["octave"].astype(np.float32), kps["class_id"].astype(np.float32)],
+                    axis=1).astype(np.float32).reshape(-1, 7)
+
+
+def mat_to_kpts(mat):
+    """n x 7 float32 -> KeyPoint array (mat2kpvec, AKAZE.cpp:290-298)."""
+    mat = np.asarray(mat, np.float32).reshape(-1, 7)
+    kps = np.zeros(len(mat), KP_DTYPE)
+    for i, n in enumerate(("x", "y", "size", "angle", "response")):
+        kps[n] = mat[:, i]
+    kps["octave"] = mat[:, 5].astype(np.int32)
+    kps["class_id"] = mat[:, 6].astype(np.int32)
+    return kps
+
+
+def _as_keypoints(k):
+    a = np.asarray(k)
+    if a.dtype == KP_DTYPE:
+        return np.ascontiguousarray(a)
+    return mat_to_kpts(a)
 
 
 # ---------------------------------------------------------------- device memory
@@ -290,18 +394,21 @@ class AKAZE:
         if img.shape != (self.H, self.W):
             return -1
         _check(lib().akz_build_scale_space(self.ctx, img.ctypes.data, img.strides[0]), "akz_build_scale_space")
+        self._detected = False
         return 0
 
-    def Feature_Detection(self):
+    def detect(self):
+        """AKAZE::Feature_Detection on the device; returns the kept count."""
         n = ctypes.c_int()
         st = lib().akz_detect(self.ctx, ctypes.byref(n))
         _check(st, "akz_detect", allow=(AKZ_ERR_OVERFLOW,))
         self.nump = n.value
         self.overflow = st == AKZ_ERR_OVERFLOW
+        self._detected = True
         return self.nump
 
-    def Compute_Descriptors(self):
-        """Returns (kpts structured array, desc n x 61 uint8)."""
+    def describe(self):
+        """AKAZE::Compute_Descriptors; returns (kpts KP_DTYPE array, desc n x 61 uint8)."""
         cap = max(self.nump, 1)
         kps = np.zeros(cap, KP_DTYPE)
         desc = np.zeros((cap, 61), np.uint8)
@@ -311,13 +418,35 @@ class AKAZE:
         m = min(n.value, cap)
         return kps[:m].copy(), desc[:m].copy()
 
-    def Compute_Descriptors_(self):
-        """Python-binding form (AKAZE.cpp:381-395): (desc, kpts n x 7 float32)."""
-        self.Feature_Detection()
-        kps, desc = self.Compute_Descriptors()
-        mat = np.stack([kps["x"], kps["y"], kps["size"], kps["angle"], kps["response"],
-                        kps["octave"].astype(np.float32), kps["class_id"].astype(np.float32)], axis=1)
-        return desc, mat.astype(np.float32)
+    def detected_keypoints(self):
+        """The detector's kept keypoints (angle 0 until described)."""
+        cap = max(self.nump, 1)
+        kps = np.zeros(cap, KP_DTYPE)
+        n = ctypes.c_int()
+        st = lib().akz_read_results(self.ctx, kps.ctypes.data, None, cap, ctypes.byref(n))
+        _check(st, "akz_read_results", allow=(AKZ_ERR_OVERFLOW,))
+        return kps[:min(n.value, cap)].copy()
+
+    def Feature_Detection(self):
+        """Python-binding form (python_bindings.cpp:34, AKAZE.cpp:302-310):
+        n x 7 float32 keypoints [x, y, size, angle, response, octave,
+        class_id].  The reference returns an unfilled matrix (its kpvec2mat
+        is empty, AKAZE.cpp:279-287); this one holds the kept keypoints."""
+        self.detect()
+        return kpts_to_mat(self.detected_keypoints())
+
+    def Compute_Descriptors(self):
+        """Python-binding form (python_bindings.cpp:35, AKAZE.cpp:381-395):
+        (desc n x 61 uint8, kpts n x 7 float32).  Detection runs first unless
+        it already ran on the current scale space (the reference runs it
+        again)."""
+        if not getattr(self, "_detected", False):
+            self.detect()
+        kps, desc = self.describe()
+        return desc, kpts_to_mat(kps)
+
+    Feature_Detection_ = Feature_Detection
+    Compute_Descriptors_ = Compute_Descriptors
 
     def Get_Computation_Times(self):
         t = CTiming()
@@ -383,8 +512,8 @@ class AKAZE:
         """Scale space + detection + description of one host image."""
         if self.Create_Nonlinear_Scale_Space(img) != 0:
             raise ValueError("image shape does not match the options")
-        self.Feature_Detection()
-        return self.Compute_Descriptors()
+        self.detect()
+        return self.describe()
 
 
 class BatchAKAZE:
@@ -636,6 +765,10 @@ class Matcher:
                                   out.ctypes.data), "akz_match2nn")
         return out[:2 * len(q)].reshape(-1, 2)
 
+    def bfmatch(self, q, t):
+        """Matcher::bfmatch (AKAZE.cpp:36-79): n_query x 2 DMatch rows."""
+        return self.match2nn(q, t)
+
     def bfmatch_(self, q, t):
         """n x 8 float32 [q, t1, 0, d1, q, t2, 0, d2] (AKAZE.cpp:82-105)."""
         m = self.match2nn(q, t)
@@ -645,6 +778,8 @@ class Matcher:
             out[:, 4 * k + 1] = m[:, k]["trainIdx"]
             out[:, 4 * k + 3] = m[:, k]["distance"]
         return out
+
+    BFMatch = bfmatch_  # python_bindings.cpp:38
 
     def match2nn_device(self, d_q64, nq, d_t64, nt, d_out):
         _check(lib().akz_match2nn_device(self.m, d_q64, nq, d_t64, nt, d_out), "akz_match2nn_device")

@@ -31,6 +31,7 @@ SOURCES = [
     "akz_synth_dev.hip",
     "akz_capi.hip",
     "akz_synth.cpp",
+    "akz_utils.cpp",
     "AKAZE.cpp",
 ]
 
@@ -54,6 +55,28 @@ def _compile(src):
     return out
 
 
+APPS = ["akaze_features", "akaze_match"]
+BIN = os.path.join(HERE, "bin")
+
+
+def build_apps(verbose=True):
+    """The GUI-free command-line programs, linked against the library in-tree
+    (rpath $ORIGIN/..)."""
+    os.makedirs(BIN, exist_ok=True)
+    cxx = os.environ.get("CXX", "g++")
+    for app in APPS:
+        out = os.path.join(BIN, app)
+        cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-I", os.path.join(REPO, "include"),
+               os.path.join(HERE, "apps", app + ".cpp"), "-o", out + ".tmp", "-L", HERE, "-l:libakaze_amd.so",
+               "-Wl,-rpath,$ORIGIN/.."]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("app build failed: %s\n%s" % (" ".join(cmd), r.stderr))
+        os.replace(out + ".tmp", out)
+        if verbose:
+            print("built", out)
+
+
 def build_oracle(verbose=True):
     r = subprocess.run(["make", "-C", os.path.join(REPO, "oracle")], capture_output=True, text=True)
     if verbose:
@@ -75,6 +98,7 @@ def build(verbose=True):
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_apps(verbose)
     return LIB
 
 

@@ -301,6 +301,38 @@ akz_status akz_matcher_wait(akz_matcher* m);
 /* Average duration in ms of the matcher kernels of the last call (HIP events). */
 akz_status akz_matcher_last_ms(akz_matcher* m, float* ms);
 
+/* ---- Evaluation and I/O utilities of the reference programs
+ * (src/lib/utils.cpp), host only, no OpenCV.  C++ forms with the
+ * reference's names are in include/akaze/utils.h. ---- */
+/* cv::imread(path, 0) for netpbm greyscale files (P5 binary or P2 text;
+ * 16-bit samples keep their high byte).  dst null: only the size. */
+akz_status akz_read_pgm(const char* path, uint8_t* dst, size_t cap, int* width, int* height);
+/* save_keypoints (utils.cpp:97-151): Mikolajczyk's text format, "dsize\nN\n"
+ * then "x y 1/r^2 0 1/r^2 [desc bytes]" per keypoint, r = size; dsize is
+ * desc_cols with save_desc, else 1.  desc rows are desc_cols bytes. */
+akz_status akz_save_keypoints(const char* path, const akz_keypoint* kps, int n, const uint8_t* desc, int desc_cols,
+                              int save_desc);
+/* read_homography (utils.cpp:340-380): 3x3 floats divided by h33, row-major. */
+akz_status akz_read_homography(const char* path, double* H9);
+/* matches2points_nndr (utils.cpp:154-170): for each of the nrows 2-NN rows
+ * (2 DMatches each) with best < nndr * second, the pair
+ * (train[queryIdx].pt, query[trainIdx].pt) as 4 floats; pairs holds up to
+ * 4 * nrows floats. */
+akz_status akz_matches2points_nndr(const akz_keypoint* train, int ntrain, const akz_keypoint* query, int nquery,
+                                   const akz_dmatch* matches, int nrows, float nndr, float* pairs, int* npairs);
+/* compute_inliers_homography (utils.cpp:203-243): mask[i] = 1 where H maps
+ * pair i's first point within min_error px of its second (float arithmetic
+ * as the reference). */
+akz_status akz_inliers_homography(const float* pairs, int npairs, const double* H9, float min_error, uint8_t* mask,
+                                  int* ninliers);
+/* compute_inliers_ransac (utils.cpp:173-200) with this library's own RANSAC
+ * homography in place of cv::findHomography: seeded 4-point samples,
+ * normalised DLT, reprojection distance <= error, adaptive sample count for
+ * `confidence` (at most max_iters), a final DLT refit on the consensus set.
+ * Writes H (optional), the inlier mask and count; npairs <= 8 gives none. */
+akz_status akz_inliers_ransac(const float* pairs, int npairs, float error, uint64_t seed, int max_iters,
+                              double confidence, double* H9, uint8_t* mask, int* ninliers);
+
 /* ---- Device memory helpers (callers without their own HIP runtime) ---- */
 akz_status akz_dev_alloc(size_t bytes, void** ptr);
 akz_status akz_dev_free(void* ptr);

@@ -56,7 +56,7 @@ def test_keypoint_cap_reports_overflow(gpu_lib):
     img = A.to_float(A.synth_image(9, W, H))
     ak = A.AKAZE(A.AKAZEOptions(img_width=W, img_height=H, maxkeypoints=cap))
     ak.Create_Nonlinear_Scale_Space(img)
-    n = ak.Feature_Detection()
+    n = ak.detect()
     assert ak.overflow
     orc = O.Pipeline(W, H)
     orc.scale_space(img)
@@ -66,7 +66,7 @@ def test_keypoint_cap_reports_overflow(gpu_lib):
     ls = np.minimum(orc.level_start(), cap).astype(np.int32)
     okp, _ = O.nms(cand[:cap], ls)
     okp, odesc = O.orient_describe(orc, okp)
-    kps, desc = ak.Compute_Descriptors()
+    kps, desc = ak.describe()
     assert n == len(okp) == len(kps)
     assert kps.tobytes() == okp.tobytes() and np.array_equal(desc, odesc)
 
@@ -100,7 +100,7 @@ def test_large_derivative_steps_use_the_unfused_detector(gpu_lib, factor):
     img = A.to_float(A.synth_image(12, W, H))
     ak = A.AKAZE(A.AKAZEOptions(img_width=W, img_height=H, derivative_factor=factor))
     ak.Create_Nonlinear_Scale_Space(img)
-    n = ak.Feature_Detection()
+    n = ak.detect()
     orc = O.Pipeline(W, H, derivative_factor=factor)
     orc.scale_space(img)
     on = orc.detect()
@@ -109,7 +109,7 @@ def test_large_derivative_steps_use_the_unfused_detector(gpu_lib, factor):
             assert np.array_equal(ak.read_level(lvl, which), orc.level(lvl, which)), (lvl, which)
     cand, ls, waves = ak.read_candidates()
     assert cand.tobytes() == orc.candidates().tobytes() and waves == orc.waves()
-    kps, desc = ak.Compute_Descriptors()
+    kps, desc = ak.describe()
     okp = orc.keypoints()
     odesc = orc.describe()
     assert n == on and kps.tobytes() == orc.keypoints().tobytes() and np.array_equal(desc, odesc)
@@ -155,8 +155,8 @@ def test_mldb_pattern_sizes(gpu_lib, patsize):
     img = A.to_float(A.synth_image(23, W, H))
     ak = A.AKAZE(A.AKAZEOptions(img_width=W, img_height=H, descriptor_pattern_size=patsize))
     ak.Create_Nonlinear_Scale_Space(img)
-    ak.Feature_Detection()
-    kps, desc = ak.Compute_Descriptors()
+    ak.detect()
+    kps, desc = ak.describe()
     orc = O.Pipeline(W, H, descriptor_pattern_size=patsize)
     okp, odesc = orc.run(img)
     assert len(kps) == len(okp) > 100

@@ -53,7 +53,7 @@ def test_scale_space_levels(gpu_lib, W, H, seed, fed_steps):
 def test_detection_and_description(gpu_lib, W, H, seed):
     A = gpu_lib
     ak, orc, _ = run_both(A, W, H, seed)
-    n = ak.Feature_Detection()
+    n = ak.detect()
     orc.detect()
     for lvl in range(ak.nlev):
         for which in (2, 3, 4):
@@ -66,7 +66,7 @@ def test_detection_and_description(gpu_lib, W, H, seed):
     assert len(cand) == len(ocand)
     assert cand.tobytes() == ocand.tobytes()
     assert waves == orc.waves()
-    kps, desc = ak.Compute_Descriptors()
+    kps, desc = ak.describe()
     okp = orc.keypoints()
     odesc = orc.describe()
     okp = orc.keypoints()
@@ -126,7 +126,7 @@ def test_many_candidates(gpu_lib, W, H, thr):
     img = rng.random((H, W)).astype(np.float32)
     ak = A.AKAZE(A.AKAZEOptions(img_width=W, img_height=H, dthreshold=thr, min_dthreshold=thr * 0.1))
     ak.Create_Nonlinear_Scale_Space(img)
-    n = ak.Feature_Detection()
+    n = ak.detect()
     orc = O.Pipeline(W, H, dthreshold=thr, min_dthreshold=thr * 0.1)
     orc.scale_space(img)
     on = orc.detect()
@@ -134,7 +134,7 @@ def test_many_candidates(gpu_lib, W, H, thr):
     assert len(cand) == orc.candidates().shape[0] and len(cand) > 8192
     assert cand.tobytes() == orc.candidates().tobytes()
     assert waves == orc.waves()
-    kps, desc = ak.Compute_Descriptors()
+    kps, desc = ak.describe()
     odesc = orc.describe()
     assert n == on and kps.tobytes() == orc.keypoints().tobytes()
     assert np.array_equal(desc, odesc)
