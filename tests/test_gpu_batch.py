@@ -108,6 +108,28 @@ def test_ragged_batches_match_oracle(gpu_lib):
                 assert np.array_equal(desc, ref[i][1]), (graph, rep, i)
 
 
+def test_ragged_five_size_classes_match_oracle(gpu_lib):
+    """The five size classes of the ragged configuration (320x240 ...
+    2560x1440) in one ragged call, eager and then replayed from graphs: each
+    image's keypoints and descriptors are the oracle's."""
+    A = gpu_lib
+    sizes = [(320, 240), (640, 480), (1280, 720), (1920, 1080), (2560, 1440)]
+    order = [4, 2, 3, 0, 1, 3, 2, 4, 1]
+    imgs = [A.to_float(A.synth_image(120 + i, *sizes[k])) for i, k in enumerate(order)]
+    ref = [oracle_run(im.shape[1], im.shape[0], im) for im in imgs]
+    caps = [order.count(k) for k in range(len(sizes))]
+    rg = A.Ragged(A.AKAZEOptions(), sizes, caps)
+    for graph in (False, True):
+        rg.set_graph_mode(graph)
+        for rep in range(2 if graph else 1):
+            print("ragged 5 classes: graph %s rep %d" % (graph, rep), flush=True)
+            res = rg.process(imgs)
+            for i, (kps, desc) in enumerate(res):
+                assert len(kps) > 0
+                assert kps.tobytes() == ref[i][0].tobytes(), (graph, rep, i, sizes[order[i]])
+                assert np.array_equal(desc, ref[i][1]), (graph, rep, i, sizes[order[i]])
+
+
 def test_every_fed_variant_and_the_autotuner(gpu_lib):
     """Each blocked FED kernel variant (region shapes 2..16) on every level,
     and the autotuner's choice, give the oracle's levels bit for bit."""
