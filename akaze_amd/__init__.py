@@ -88,7 +88,7 @@ SIGNATURES = {
     "akz_read_kcontrast": [_P, _I, _P],
     "akz_read_candidates": [_P, _P, _I, _P, _P, _P],
     "akz_get_timing": [_P, _P],
-    "akz_profile_fed": [_P, _P, _SZ, _P, _P, _P],
+    "akz_profile_fed": [_P, _P, _SZ, _P, _P, _P, _P],
     "akz_copy_kernel_run": [_P, _P, _SZ, _I, _P],
     "akz_create_batch": [_P, _I, _P],
     "akz_batch_capacity": [_P, _P],
@@ -98,7 +98,7 @@ SIGNATURES = {
     "akz_read_batch_results": [_P, _I, _P, _P, _I, _P],
     "akz_batch_results_device": [_P, _I, _P, _P, _P],
     "akz_read_batch_level": [_P, _I, _I, _I, _P],
-    "akz_profile_fed_batch": [_P, _P, _SZ, _SZ, _I, _P, _P, _P],
+    "akz_profile_fed_batch": [_P, _P, _SZ, _SZ, _I, _P, _P, _P, _P],
     "akz_prop_num_params": [_P],
     "akz_prop_create": [_U64, _P],
     "akz_prop_destroy": [_P],
@@ -493,11 +493,12 @@ class AKAZE:
         _check(lib().akz_wait(self.ctx), "akz_wait")
 
     def profile_fed(self, d_img, pitch_bytes):
-        """(FED ms per image, algorithmic bytes, launches) from HIP events on the context stream."""
-        ms = ctypes.c_float(); by = ctypes.c_double(); n = ctypes.c_int()
+        """(FED ms per image, algorithmic bytes, launches, must-move bytes)
+        from HIP events on the context stream."""
+        ms = ctypes.c_float(); by = ctypes.c_double(); n = ctypes.c_int(); mb = ctypes.c_double()
         _check(lib().akz_profile_fed(self.ctx, d_img, pitch_bytes, ctypes.byref(ms), ctypes.byref(by),
-                                     ctypes.byref(n)), "akz_profile_fed")
-        return ms.value, by.value, n.value
+                                     ctypes.byref(n), ctypes.byref(mb)), "akz_profile_fed")
+        return ms.value, by.value, n.value, mb.value
 
     def read_results(self, cap):
         kps = np.zeros(max(cap, 1), KP_DTYPE)
@@ -597,11 +598,13 @@ class BatchAKAZE:
     level_size = AKAZE.level_size
 
     def profile_fed(self, n):
+        """(FED ms per call of n images, algorithmic bytes (12 B per pixel and
+        step), launches, bytes the launches must move)."""
         d, pitch, zs = self.input_buffer()
-        ms = ctypes.c_float(); by = ctypes.c_double(); k = ctypes.c_int()
+        ms = ctypes.c_float(); by = ctypes.c_double(); k = ctypes.c_int(); mb = ctypes.c_double()
         _check(lib().akz_profile_fed_batch(self.ctx, d, pitch, zs, int(n), ctypes.byref(ms), ctypes.byref(by),
-                                           ctypes.byref(k)), "akz_profile_fed_batch")
-        return ms.value, by.value, k.value
+                                           ctypes.byref(k), ctypes.byref(mb)), "akz_profile_fed_batch")
+        return ms.value, by.value, k.value, mb.value
 
     def wait(self):
         _check(lib().akz_wait(self.ctx), "akz_wait")

@@ -974,7 +974,7 @@ akz_status akz_get_timing(akz_ctx* c, akz_timing* t) {
 }
 
 akz_status akz_profile_fed_batch(akz_ctx* c, const float* d_imgs, size_t pitch_bytes, size_t image_stride_bytes,
-                                 int n, float* fed_ms, double* fed_bytes, int* fed_launches) {
+                                 int n, float* fed_ms, double* fed_bytes, int* fed_launches, double* min_bytes) {
   if (!c || !d_imgs || n < 1 || n > c->batch) return AKZ_ERR_ARG;
   const Level& L0 = c->lev[0];
   size_t sp = pitch_bytes ? pitch_bytes : sizeof(float) * L0.w;
@@ -992,30 +992,37 @@ akz_status akz_profile_fed_batch(akz_ctx* c, const float* d_imgs, size_t pitch_b
   c->state = 3;
   c->last_nz = n;
   float total = 0.0f;
-  double bytes = 0.0;
+  double bytes = 0.0, least = 0.0;
   int launches = 0;
   for (int i = 1; i < c->nlev; i++) {
     float ms = 0.0f;
     AKZ_HIP_TRY(hipEventElapsedTime(&ms, c->fed_ev[2 * i], c->fed_ev[2 * i + 1]));
     total += ms;
+    const double px = (double)c->lev[i].w * c->lev[i].h * n;
+    const int bs = level_block(c, i), ns = c->lev[i].nsteps, nl = (ns + bs - 1) / bs;
+    const bool fused = bs > 1 && ns > 0 && akz_fed_variant_fusable(c->fed_variant[i], std::min(bs, ns));
     // algorithmic traffic of one step: read L and g, write L (4 B each per
     // pixel); a level whose first launch also builds Lsmooth and g adds that
     // pass's (read L, write Lsmooth and g)
-    bytes += 12.0 * c->lev[i].w * c->lev[i].h * c->lev[i].nsteps * n;
-    if (level_block(c, i) > 1 && c->lev[i].nsteps > 0 &&
-        akz_fed_variant_fusable(c->fed_variant[i], std::min(level_block(c, i), c->lev[i].nsteps)))
-      bytes += 12.0 * c->lev[i].w * c->lev[i].h * n;
-    launches += (c->lev[i].nsteps + level_block(c, i) - 1) / level_block(c, i);
+    bytes += 12.0 * px * ns;
+    if (fused) bytes += 12.0 * px;
+    // what the launches must move however many steps each runs: every launch
+    // reads L and g and writes L once; a fused first launch reads its input
+    // and writes Lsmooth, L and (when later launches read it) g
+    least += 12.0 * px * nl;
+    if (fused && nl > 1) least += 4.0 * px;
+    launches += nl;
   }
   if (fed_ms) *fed_ms = total;
   if (fed_bytes) *fed_bytes = bytes;
   if (fed_launches) *fed_launches = launches;
+  if (min_bytes) *min_bytes = least;
   return AKZ_OK;
 }
 
 akz_status akz_profile_fed(akz_ctx* c, const float* d_img, size_t pitch_bytes, float* fed_ms, double* fed_bytes,
-                           int* fed_launches) {
-  return akz_profile_fed_batch(c, d_img, pitch_bytes, 0, 1, fed_ms, fed_bytes, fed_launches);
+                           int* fed_launches, double* min_bytes) {
+  return akz_profile_fed_batch(c, d_img, pitch_bytes, 0, 1, fed_ms, fed_bytes, fed_launches, min_bytes);
 }
 
 akz_status akz_prop_positions(const akz_ctx* c, int* per_image) {

@@ -11,10 +11,11 @@ Prints one JSON line (the last line on stdout).  Extra objects:
   roofline      FED kernel (the dominant kernel, up to 8 steps per launch):
                 algorithmic bytes (12 B per pixel and step) over its duration,
                 measured with HIP events around the FED launches on the
-                context stream;
+                context stream, beside the bytes the launches must move and
+                the counter traffic (see fed_roofline);
   cpu_baseline  the CPU oracle (the C restatement of the reference
-                semantics, all host threads) on a bounded sample of the same
-                workload;
+                semantics) on bounded samples of the same workload, with all
+                host threads and with one thread;
   batched       16-image batches (one launch per stage, HIP graph) with the
                 full scan and with the committed learned tile proposal, and
                 the proposal's keypoint recall against the full scan.
@@ -39,18 +40,33 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def fed_roofline(achieved, avg_ms, alg_bytes, traffic):
-    """Roofline object of the FED launches.  `achieved` counts the algorithmic
-    bytes (12 per pixel and step: read L and g, write L, as if every step were
-    its own pass, plus 12 per pixel for a launch that also builds its level's
-    Lsmooth and conductance) over the measured average launch; temporal
-    blocking moves fewer, so `frac` can pass 1.  `traffic` is the HBM bytes per launch from
-    the committed counter runs, and `traffic_GBs` the rate they imply at the
-    live launch time."""
+def fed_roofline(avg_ms, alg_bytes, least_bytes, traffic, basis):
+    """Roofline object of the FED launches at the measured average launch.
+
+    Two byte counts per launch:
+      alg_bytes    12 B per pixel and step (read L and g, write L, as if every
+                   step were a pass of its own; the per-step unit of the
+                   FED roofline), plus 12 B per pixel for a launch that also
+                   builds its level's Lsmooth and conductance;
+      least_bytes  what the launches must move however many steps each runs
+                   (L and g read once, L written once per launch).
+    basis "per_step": `achieved` counts alg_bytes (one image per call: the
+    launches are short and latency-bound, far from either bound);
+    basis "per_launch": `achieved` counts least_bytes, because temporal
+    blocking (several steps per launch) lets a launch do more steps than its
+    HBM traffic pays for, and per-step bytes would then exceed the HBM peak.
+    `traffic` is the HBM bytes per launch of the committed counter runs."""
+    alg_rate = alg_bytes / (avg_ms * 1e-3) / 1e9
+    least_rate = least_bytes / (avg_ms * 1e-3) / 1e9
+    achieved = alg_rate if basis == "per_step" else least_rate
     r = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-         "kernel": "FED launches: k_fed_reg / k_fed_multi (first launch of a level: k_fed_reg_sf / k_fed_multi_sf, with its smooth + flow), up to 8 steps each, halo = steps",
-         "avg_launch_us": round(avg_ms * 1e3, 3), "alg_bytes_per_launch": round(alg_bytes)}
+         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "basis": basis,
+         "kernel": "FED launches: k_fed_reg / k_fed_multi (first launch of a level: k_fed_reg_sf / k_fed_multi_sf, "
+                   "with its smooth + flow), up to 8 steps each, halo = steps",
+         "avg_launch_us": round(avg_ms * 1e3, 3), "alg_bytes_per_launch": round(alg_bytes),
+         "least_bytes_per_launch": round(least_bytes),
+         "alg_GBs": round(alg_rate, 1), "least_GBs": round(least_rate, 1),
+         "least_frac": round(least_rate / HBM_PEAK_GBS, 4)}
     if traffic:
         rate = traffic / (avg_ms * 1e-3) / 1e9
         r["traffic_GBs"] = round(rate, 1)
@@ -71,11 +87,18 @@ def fed_traffic(profile="fed_plain_1080p_pmc.md", key="traffic_bytes_per_launch"
     return None
 
 
-def cpu_baseline(img, W, H, budget_s):
-    """CPU oracle images/s on a bounded sample (at most `budget_s` of work)."""
-    sys.path.insert(0, os.path.join(REPO, "tests"))
-    import oracle_lib as O
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
+def host_threads():
+    """Threads this process may run on (the box's CPU share), capped by
+    OMP_NUM_THREADS when that is set."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    env = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return min(n, env) if env > 0 else n
+
+
+def cpu_rate(O, img, W, H, threads, budget_s):
     P = O.Pipeline(W, H, threads=threads)
     P.run(img)  # warm-up (page faults, thread pool)
     n, t0 = 0, time.perf_counter()
@@ -84,9 +107,23 @@ def cpu_baseline(img, W, H, budget_s):
         n += 1
         el = time.perf_counter() - t0
         if el >= budget_s:
-            break
+            return n, el
+
+
+def cpu_baseline(img, W, H, budget_s):
+    """CPU oracle images/s on bounded samples (about `budget_s` of work each)
+    with every host thread available and with one thread."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import oracle_lib as O
+    threads = host_threads()
+    n, el = cpu_rate(O, img, W, H, threads, budget_s)
+    n1, el1 = cpu_rate(O, img, W, H, 1, budget_s)
+    O.lib().orc_set_threads(threads)
+    sample = "%dx%d synthetic image, full detect+MLDB on the CPU oracle (-O3 -march=x86-64-v3, OpenMP)" % (W, H)
     return {"value": n / el, "unit": "images/s", "cores": threads, "kind": "port",
-            "sample": "%d x %dx%d synthetic image, full detect+MLDB on the CPU oracle (%.1f s)" % (n, W, H, el)}
+            "sample": "%d images in %.1f s on %d threads; %s" % (n, el, threads, sample),
+            "one_thread": {"value": n1 / el1, "unit": "images/s", "cores": 1,
+                           "sample": "%d images in %.1f s on 1 thread" % (n1, el1)}}
 
 
 def batched_comparison(A, W, H, B=16, reps=20):
@@ -120,12 +157,11 @@ def batched_comparison(A, W, H, B=16, reps=20):
         bk.set_graph_mode(False)
     # FED roofline of the batched launches (live HIP events, as for the headline)
     bk.set_proposal(None)
-    fms, fby, fl = min(bk.profile_fed(B), bk.profile_fed(B))
-    ach = (fby / fl) / (fms / fl * 1e-3) / 1e9
-    out["fed_roofline"] = fed_roofline(ach, fms / fl, fby / fl,
+    fms, fby, fl, fmin = min(bk.profile_fed(B), bk.profile_fed(B))
+    out["fed_roofline"] = fed_roofline(fms / fl, fby / fl, fmin / fl,
                                        fed_traffic("akaze_batched16_1080p_kernel_stats.md",
                                                    "traffic_bytes_per_launch_batched16")
-                                       if (W, H, B) == (1920, 1080, 16) else None)
+                                       if (W, H, B) == (1920, 1080, 16) else None, "per_launch")
     full, prop = kps["full_scan"], kps["learned_proposal"]
     total = sum(len(f) for f in full)
     out["proposal_kp_recall"] = round(sum(len(f) * keypoint_recall(f, p) for f, p in zip(full, prop)) / total, 4)
@@ -490,16 +526,14 @@ def main():
 
     # roofline of the dominant kernel (FED step), live HIP events on the context stream
     bk = ctxs[0]
-    fed_ms, fed_bytes, fed_launches = bk.profile_fed(B)
-    fed_ms2, _, _ = bk.profile_fed(B)
-    fed_ms = min(fed_ms, fed_ms2)
+    fed_ms, fed_bytes, fed_launches, fed_min = min(bk.profile_fed(B), bk.profile_fed(B))
     avg_ms = fed_ms / fed_launches
-    achieved = (fed_bytes / fed_launches) / (avg_ms * 1e-3) / 1e9
-    log("FED: %d launches, %.3f ms per call of %d images, avg launch %.2f us, %.1f GB/s algorithmic" %
-        (fed_launches, fed_ms, B, avg_ms * 1e3, achieved))
     # the counter runs were made on one 1920x1080 image per call
-    roofline = fed_roofline(achieved, avg_ms, fed_bytes / fed_launches,
-                            fed_traffic() if (W, H, B) == (1920, 1080, 1) else None)
+    roofline = fed_roofline(avg_ms, fed_bytes / fed_launches, fed_min / fed_launches,
+                            fed_traffic() if (W, H, B) == (1920, 1080, 1) else None,
+                            "per_step" if B == 1 else "per_launch")
+    log("FED: %d launches, %.3f ms per call of %d images, avg launch %.2f us, %.1f GB/s per-step algorithmic, "
+        "%.1f GB/s must-move" % (fed_launches, fed_ms, B, avg_ms * 1e3, roofline["alg_GBs"], roofline["least_GBs"]))
 
     batched = None
     if not args.no_batched and B == 1:

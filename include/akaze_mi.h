@@ -196,7 +196,7 @@ akz_status akz_ragged_results_device(akz_ragged* r, int i, const int32_t** d_cou
 
 /* akz_profile_fed over a batch of n images (bytes count all n images). */
 akz_status akz_profile_fed_batch(akz_ctx* ctx, const float* d_imgs, size_t pitch_bytes, size_t image_stride_bytes,
-                                 int n, float* fed_ms, double* fed_bytes, int* fed_launches);
+                                 int n, float* fed_ms, double* fed_bytes, int* fed_launches, double* min_bytes);
 
 /* Debug readback of a level image (AKAZE::Save_Scale_Space reads Lt back,
  * AKAZE.cpp:453-463).  which: 0 Lt, 1 Lsmooth, 2 Lx, 3 Ly, 4 Ldet.
@@ -212,9 +212,12 @@ akz_status akz_read_candidates(akz_ctx* ctx, akz_keypoint* cand, int cap, int* n
                                int* waves);
 /* Measurement: runs the whole pipeline on `d_img` with HIP events around each
  * level's FED launches and returns their summed time, the algorithmic bytes of
- * those launches (12 B per pixel and step) and the launch count.  Waits. */
+ * those launches (12 B per pixel and step, as if every step were a pass of
+ * its own), the launch count, and the bytes the launches must move however
+ * many steps each runs (L and g read once and L written once per launch;
+ * min_bytes may be null).  Waits. */
 akz_status akz_profile_fed(akz_ctx* ctx, const float* d_img, size_t pitch_bytes, float* fed_ms, double* fed_bytes,
-                           int* fed_launches);
+                           int* fed_launches, double* min_bytes);
 /* Measurement: a plain device copy of `nfloats` floats (a multiple of 4096),
  * `iters` times on the null stream; average ms per launch.  Used to calibrate
  * hardware byte counters against a kernel of known traffic. */
