@@ -72,6 +72,7 @@ SIGNATURES = {
     "akz_create": [_P, _P],
     "akz_destroy": [_P],
     "akz_num_levels": [_P, _P],
+    "akz_descriptor_bytes": [_P, _P, _P],
     "akz_level_size": [_P, _I, _P, _P, _P, _P, _P],
     "akz_build_scale_space": [_P, _P, _SZ],
     "akz_build_scale_space_device": [_P, _P, _SZ],
@@ -372,6 +373,9 @@ class AKAZE:
         self.nlev = n.value
         self.W, self.H = options.img_width, options.img_height
         self.nump = 0
+        nb = ctypes.c_int()
+        _check(lib().akz_descriptor_bytes(self.ctx, ctypes.byref(nb), None), "akz_descriptor_bytes")
+        self.desc_bytes = nb.value
 
     def close(self):
         if getattr(self, "ctx", None) and _lib is not None:
@@ -408,10 +412,11 @@ class AKAZE:
         return self.nump
 
     def describe(self):
-        """AKAZE::Compute_Descriptors; returns (kpts KP_DTYPE array, desc n x 61 uint8)."""
+        """AKAZE::Compute_Descriptors; returns (kpts KP_DTYPE array, desc
+        n x desc_bytes uint8: 61 for the default 486-bit M-LDB)."""
         cap = max(self.nump, 1)
         kps = np.zeros(cap, KP_DTYPE)
-        desc = np.zeros((cap, 61), np.uint8)
+        desc = np.zeros((cap, self.desc_bytes), np.uint8)
         n = ctypes.c_int()
         st = lib().akz_describe(self.ctx, kps.ctypes.data, desc.ctypes.data, cap, ctypes.byref(n))
         _check(st, "akz_describe", allow=(AKZ_ERR_OVERFLOW,))
@@ -502,7 +507,7 @@ class AKAZE:
 
     def read_results(self, cap):
         kps = np.zeros(max(cap, 1), KP_DTYPE)
-        desc = np.zeros((max(cap, 1), 61), np.uint8)
+        desc = np.zeros((max(cap, 1), self.desc_bytes), np.uint8)
         n = ctypes.c_int()
         st = lib().akz_read_results(self.ctx, kps.ctypes.data, desc.ctypes.data, cap, ctypes.byref(n))
         _check(st, "akz_read_results", allow=(AKZ_ERR_OVERFLOW,))
@@ -532,6 +537,8 @@ class BatchAKAZE:
         n = ctypes.c_int()
         lib().akz_num_levels(self.ctx, ctypes.byref(n))
         self.nlev = n.value
+        _check(lib().akz_descriptor_bytes(self.ctx, ctypes.byref(n), None), "akz_descriptor_bytes")
+        self.desc_bytes = n.value
 
     def close(self):
         if getattr(self, "ctx", None) and _lib is not None:
@@ -582,7 +589,7 @@ class BatchAKAZE:
 
     def read_results(self, z, cap):
         kps = np.zeros(max(cap, 1), KP_DTYPE)
-        desc = np.zeros((max(cap, 1), 61), np.uint8)
+        desc = np.zeros((max(cap, 1), self.desc_bytes), np.uint8)
         n = ctypes.c_int()
         st = lib().akz_read_batch_results(self.ctx, int(z), kps.ctypes.data, desc.ctypes.data, cap, ctypes.byref(n))
         _check(st, "akz_read_batch_results", allow=(AKZ_ERR_OVERFLOW,))
@@ -683,6 +690,10 @@ class Ragged:
         self.r = p.value
         self.sizes = [tuple(s) for s in sizes]
         self._bufs = []
+        c, nb = ctypes.c_void_p(), ctypes.c_int()
+        _check(lib().akz_ragged_context(self.r, 0, ctypes.byref(c)), "akz_ragged_context")
+        _check(lib().akz_descriptor_bytes(c, ctypes.byref(nb), None), "akz_descriptor_bytes")
+        self.desc_bytes = nb.value
 
     def close(self):
         if getattr(self, "r", None) and _lib is not None:
@@ -727,7 +738,7 @@ class Ragged:
 
     def read(self, i, cap):
         kps = np.zeros(max(cap, 1), KP_DTYPE)
-        desc = np.zeros((max(cap, 1), 61), np.uint8)
+        desc = np.zeros((max(cap, 1), self.desc_bytes), np.uint8)
         n = ctypes.c_int()
         st = lib().akz_ragged_read(self.r, int(i), kps.ctypes.data, desc.ctypes.data, cap, ctypes.byref(n))
         _check(st, "akz_ragged_read", allow=(AKZ_ERR_OVERFLOW,))
@@ -759,9 +770,9 @@ class Matcher:
         """2-NN structured array (n_query x 2) of DMatch."""
         q = np.ascontiguousarray(q, np.uint8)
         t = np.ascontiguousarray(t, np.uint8)
-        if q.ndim != 2 or t.ndim != 2 or q.shape[1] < 61 or t.shape[1] < 61:
-            raise ValueError("descriptors must be n x 61 uint8")
-        if q.shape[1] != t.shape[1]:
+        if q.ndim != 2 or t.ndim != 2 or q.shape[1] < 1 or q.shape[1] != t.shape[1]:
+            raise ValueError("descriptors must be n x b uint8 with the same b (61 for 486-bit M-LDB)")
+        if q.shape[1] > 61:
             t = np.ascontiguousarray(t[:, :61]); q = np.ascontiguousarray(q[:, :61])
         out = np.zeros(max(2 * len(q), 1), DM_DTYPE)
         _check(lib().akz_match2nn(self.m, q.ctypes.data, len(q), t.ctypes.data, len(t), q.shape[1],

@@ -32,6 +32,7 @@ SOURCES = [
     "akz_capi.hip",
     "akz_synth.cpp",
     "akz_utils.cpp",
+    "akz_mldb_table.cpp",
     "AKAZE.cpp",
 ]
 

@@ -2,6 +2,7 @@
 // Follows the call structure of the reference's AKAZE.cpp:36-495.
 #include "../../include/akaze/AKAZE.h"
 
+#include <algorithm>
 #include <iomanip>
 #include <iostream>
 
@@ -60,18 +61,20 @@ Matcher::~Matcher() {
 }
 
 void Matcher::bfmatch(Mat& q, Mat& t, std::vector<std::vector<DMatch>>& dmatches) {
-  if (q.type != CV_8U || t.type != CV_8U || (q.rows && q.cols < 61) || (t.rows && t.cols < 61))
+  if (q.type != CV_8U || t.type != CV_8U || (q.rows && t.rows && q.cols != t.cols) || (q.rows && q.cols < 1))
     throw Error(AKZ_ERR_ARG, "Matcher::bfmatch");
+  const int cols = q.rows ? std::min(q.cols, 61) : 61;
   std::vector<akz_dmatch> out(2 * (size_t)q.rows);
   // rows of a Mat are `step` bytes apart; both sets must share one stride for the C call
   Mat tq = q, tt = t;
   if (q.step != t.step) {
-    tq = Mat(q.rows, 61, CV_8U);
-    for (int r = 0; r < q.rows; r++) std::memcpy(tq.ptr<uint8_t>(r), q.ptr<uint8_t>(r), 61);
-    tt = Mat(t.rows, 61, CV_8U);
-    for (int r = 0; r < t.rows; r++) std::memcpy(tt.ptr<uint8_t>(r), t.ptr<uint8_t>(r), 61);
+    tq = Mat(q.rows, cols, CV_8U);
+    for (int r = 0; r < q.rows; r++) std::memcpy(tq.ptr<uint8_t>(r), q.ptr<uint8_t>(r), cols);
+    tt = Mat(t.rows, cols, CV_8U);
+    for (int r = 0; r < t.rows; r++) std::memcpy(tt.ptr<uint8_t>(r), t.ptr<uint8_t>(r), cols);
   }
-  check(akz_match2nn(m_, tq.data, tq.rows, tt.data, tt.rows, tq.rows ? tq.step : 61, out.data()), "Matcher::bfmatch");
+  check(akz_match2nn(m_, tq.data, tq.rows, tt.data, tt.rows, tq.rows ? tq.step : (size_t)cols, out.data()),
+        "Matcher::bfmatch");
   dmatches.clear();
   dmatches.reserve(q.rows);
   for (int i = 0; i < q.rows; ++i) {
@@ -157,7 +160,9 @@ Mat AKAZE::Feature_Detection_() {
 void AKAZE::Compute_Descriptors(std::vector<KeyPoint>& kpts, Mat& desc) {
   int cap = nump_;
   kpts.resize(cap);
-  desc = Mat(cap, 61, CV_8U);
+  int nbytes = 61;
+  check(akz_descriptor_bytes(ctx_, &nbytes, nullptr), "AKAZE::Compute_Descriptors");
+  desc = Mat(cap, nbytes, CV_8U);  // ceil(bits / 8) columns, AKAZE.cpp:412-423
   int n = 0;
   check(akz_describe(ctx_, reinterpret_cast<akz_keypoint*>(kpts.data()), desc.data, cap, &n),
         "AKAZE::Compute_Descriptors");

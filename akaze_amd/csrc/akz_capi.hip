@@ -130,6 +130,9 @@ struct akz_ctx {
   // events of the last eager run (bit e: ev[e] recorded); graph captures and
   // replays record none, so their stage times are not reported as current
   unsigned ev_mask = 0;
+  // M-LDB variant: comparison list on the device, bits and bytes per descriptor
+  int16_t* d_cmp = nullptr;
+  int desc_bits = 486, desc_bytes = 61;
   int state = 0;  // 0 created, 1 scale space, 2 detected, 3 described
   bool graph_mode = false;
   bool capturing = false;
@@ -173,8 +176,12 @@ akz_status compute_geometry(akz_ctx* c) {
   if (o.img_width < 16 || o.img_height < 16 || o.omax < 1 || o.nsublevels < 1) return AKZ_ERR_ARG;
   if (o.img_width >= 32768 || o.img_height >= 32768) return AKZ_ERR_ARG;
   if (o.diffusivity < 0 || o.diffusivity > 3 || o.fed_steps_per_launch < 0) return AKZ_ERR_ARG;
-  if (o.descriptor != 5) return AKZ_ERR_UNSUPPORTED;  // AKAZE.cpp:428-443: only MLDB exists on the GPU
-  if (o.descriptor_size != 0 || o.descriptor_channels != 3) return AKZ_ERR_UNSUPPORTED;
+  // M-LDB (5) and M-LDB upright (4); the reference's GPU path has only the
+  // former (AKAZE.cpp:428-443).  SURF and M-SURF (0-3) are not provided.
+  if (o.descriptor != 5 && o.descriptor != 4) return AKZ_ERR_UNSUPPORTED;
+  if (o.descriptor_channels < 1 || o.descriptor_channels > 3) return AKZ_ERR_ARG;
+  // the subsample picks from the 162 comparisons per channel (AKAZE.cpp:506-515)
+  if (o.descriptor_size < 0 || o.descriptor_size > 162 * o.descriptor_channels) return AKZ_ERR_ARG;
   // the MLDB grid (max(3*ceil(2p/3), 4*ceil(p/2)))^2 must fit 7 x 64 samples per wavefront
   if (o.descriptor_pattern_size < 1 || o.descriptor_pattern_size > 10) return AKZ_ERR_UNSUPPORTED;
   // scales and thresholds: finite and positive (the level sigmas, FED times and
@@ -519,7 +526,8 @@ void enqueue_detect(akz_ctx* c, int nz, bool full_det) {
 // Compute_Descriptors (AKAZE.cpp:405-449): orientation, then MLDB.
 void enqueue_describe(akz_ctx* c, int nz) {
   hipStream_t s = c->stream;
-  akz_launch_orientation(s, c->dp, c->kps, c->status + AKZ_ST_KEPT, c->cap, nz);
+  // M-LDB upright keeps the detector's angle 0 (no orientation)
+  if (c->opts.descriptor == 5) akz_launch_orientation(s, c->dp, c->kps, c->status + AKZ_ST_KEPT, c->cap, nz);
   akz_launch_mldb(s, c->dp, c->kps, c->status + AKZ_ST_KEPT, c->desc, c->cap, nz);
   record(c, EV_DESC);
 }
@@ -666,6 +674,18 @@ akz_status akz_create_batch(const akz_options* opts, int batch, akz_ctx** out) {
   c->graph.assign(batch, nullptr);
   st = allocate(c);
   if (st != AKZ_OK) { akz_destroy(c); return st; }
+  {
+    int16_t tab[2 * AKZ_MLDB_TABLE];
+    int nb = akz_mldb_table(c->opts.descriptor_size, c->opts.descriptor_pattern_size, c->opts.descriptor_channels,
+                            tab, tab + AKZ_MLDB_TABLE);
+    if (nb < 1) { akz_destroy(c); return AKZ_ERR_ARG; }
+    c->desc_bits = nb;
+    c->desc_bytes = (nb + 7) / 8;
+    if (hipMalloc((void**)&c->d_cmp, sizeof tab) != hipSuccess) { (void)hipGetLastError(); akz_destroy(c); return AKZ_ERR_OOM; }
+    if (hipMemcpy(c->d_cmp, tab, sizeof tab, hipMemcpyHostToDevice) != hipSuccess) { akz_destroy(c); return AKZ_ERR_HIP; }
+    c->dp.cmp = c->d_cmp;
+    c->dp.nch = c->opts.descriptor_channels;
+  }
   if (hipStreamSynchronize(c->stream) != hipSuccess) { akz_destroy(c); return AKZ_ERR_HIP; }
   *out = c;
   return AKZ_OK;
@@ -686,6 +706,7 @@ akz_status akz_destroy(akz_ctx* c) {
     if (c->fed_ev[i]) hipEventDestroy(c->fed_ev[i]);
   if (c->slab) hipFree(c->slab);
   if (c->prop_mem) hipFree(c->prop_mem);
+  if (c->d_cmp) hipFree(c->d_cmp);
   if (c->h_small) hipHostFree(c->h_small);
   if (c->h_stage) hipHostFree(c->h_stage);
   if (c->stream) hipStreamDestroy(c->stream);
@@ -696,6 +717,13 @@ akz_status akz_destroy(akz_ctx* c) {
 akz_status akz_batch_capacity(const akz_ctx* c, int* batch) {
   if (!c || !batch) return AKZ_ERR_ARG;
   *batch = c->batch;
+  return AKZ_OK;
+}
+
+akz_status akz_descriptor_bytes(const akz_ctx* c, int* nbytes, int* nbits) {
+  if (!c || !nbytes) return AKZ_ERR_ARG;
+  *nbytes = c->desc_bytes;
+  if (nbits) *nbits = c->desc_bits;
   return AKZ_OK;
 }
 
@@ -776,7 +804,8 @@ akz_status akz_read_batch_results(akz_ctx* c, int z, akz_keypoint* kps, uint8_t*
   st = image_status(c, z, &nk);
   int m = std::min(nk, cap);
   if (m > 0 && (kps || desc)) {
-    // one contiguous copy each into pinned staging, then repack 64-byte rows to 61
+    // one contiguous copy each into pinned staging, then repack the 64-byte
+    // rows to the descriptor's bytes
     size_t kb = sizeof(akz_kp) * (size_t)m, db = (size_t)AKZ_DESC_STRIDE * m;
     if (c->h_stage_bytes < kb + db) {
       if (c->h_stage) hipHostFree(c->h_stage);
@@ -792,7 +821,8 @@ akz_status akz_read_batch_results(akz_ctx* c, int z, akz_keypoint* kps, uint8_t*
     AKZ_HIP_TRY(hipStreamSynchronize(c->stream));
     if (kps) memcpy(kps, c->h_stage, kb);
     if (desc)
-      for (int i = 0; i < m; i++) memcpy(desc + (size_t)i * 61, c->h_stage + kb + (size_t)i * AKZ_DESC_STRIDE, 61);
+      for (int i = 0; i < m; i++)
+        memcpy(desc + (size_t)i * c->desc_bytes, c->h_stage + kb + (size_t)i * AKZ_DESC_STRIDE, c->desc_bytes);
   }
   if (n_out) *n_out = nk;
   return (nk > cap) ? AKZ_ERR_OVERFLOW : st;
@@ -1351,18 +1381,19 @@ akz_status akz_match2nn_device(akz_matcher* m, const uint8_t* d_q64, int nq, con
 
 akz_status akz_match2nn(akz_matcher* m, const uint8_t* q, int nq, const uint8_t* t, int nt, size_t stride,
                         akz_dmatch* out) {
-  if (!m || nq < 0 || nt < 0 || stride < 61 || (nq > 0 && (!q || !out)) || (nt > 0 && !t)) return AKZ_ERR_ARG;
+  if (!m || nq < 0 || nt < 0 || stride < 1 || (nq > 0 && (!q || !out)) || (nt > 0 && !t)) return AKZ_ERR_ARG;
+  const size_t row = stride < 61 ? stride : 61;  // descriptor bytes used per row
   if (nq == 0) return AKZ_OK;
   akz_status st;
   if ((st = grow((void**)&m->q, &m->qcap, (size_t)nq * 64)) != AKZ_OK) return st;
   if ((st = grow((void**)&m->t, &m->tcap, (size_t)std::max(nt, 1) * 64)) != AKZ_OK) return st;
   if ((st = grow((void**)&m->out, &m->ocap, sizeof(akz_dmatch) * 2 * (size_t)nq)) != AKZ_OK) return st;
-  // 61-byte rows into zero-padded 64-byte rows (AKAZE.cpp:39-60)
+  // rows into zero-padded 64-byte rows (AKAZE.cpp:39-60)
   AKZ_HIP_TRY(hipMemsetAsync(m->q, 0, (size_t)nq * 64, m->stream));
-  AKZ_HIP_TRY(hipMemcpy2DAsync(m->q, 64, q, stride, 61, nq, hipMemcpyHostToDevice, m->stream));
+  AKZ_HIP_TRY(hipMemcpy2DAsync(m->q, 64, q, stride, row, nq, hipMemcpyHostToDevice, m->stream));
   if (nt > 0) {
     AKZ_HIP_TRY(hipMemsetAsync(m->t, 0, (size_t)nt * 64, m->stream));
-    AKZ_HIP_TRY(hipMemcpy2DAsync(m->t, 64, t, stride, 61, nt, hipMemcpyHostToDevice, m->stream));
+    AKZ_HIP_TRY(hipMemcpy2DAsync(m->t, 64, t, stride, row, nt, hipMemcpyHostToDevice, m->stream));
   }
   st = akz_match2nn_device(m, m->q, nq, m->t, nt, m->out);
   if (st != AKZ_OK) return st;

@@ -9,31 +9,6 @@
 
 namespace {
 
-// MLDB comparison pairs of InitCompareIndices (cuda_akaze.cu:2034-2111): for
-// the 2x2, 3x3 and 4x4 grids in turn, for channel 0, 1, 2, every cell pair
-// j < i compares vals[3j+ch] > vals[3i+ch]; 6+36+120 pairs x 3 = 486 bits.
-struct CmpTables {
-  int16_t a[488], b[488];
-};
-
-constexpr CmpTables make_cmp_tables() {
-  CmpTables t{};
-  int c = 0;
-  const int lo[3] = {0, 4, 13}, hi[3] = {4, 13, 29};
-  for (int g = 0; g < 3; g++)
-    for (int ch = 0; ch < 3; ch++)
-      for (int j = lo[g]; j < hi[g] - 1; j++)
-        for (int i = j + 1; i < hi[g]; i++) {
-          t.a[c] = (int16_t)(3 * j + ch);
-          t.b[c] = (int16_t)(3 * i + ch);
-          c++;
-        }
-  for (; c < 488; c++) t.a[c] = t.b[c] = 0;
-  return t;
-}
-
-__constant__ CmpTables d_cmp = make_cmp_tables();
-
 // The samples of FindOrientation's radius-6 disc (cuda_akaze.cu:2114-2167),
 // offsets (i, j) with i^2 + j^2 < 36, in the order of their slot tx = 16 (j +
 // 6) + (i + 6) in the reference's 16-wide sample grid: packed as (i + 6) |
@@ -197,15 +172,15 @@ __global__ __launch_bounds__(64 * MLDB_WAVES) void k_mldb(akz_desc_params p, con
   const int size4 = (int)ceilf(0.5f * p.patsize);
   const int winsize = akz_imax(3 * size3, 4 * size4);
   constexpr int MS = 7;  // samples per lane (m < 7 covers grids up to 21x21)
-  // comparison pairs of the lane's output byte (byte 60 has 6 bits; its
-  // table entries 486 and 487 are (0, 0), which compare false): loaded once,
-  // the same for every keypoint
+  // comparison pairs of the lane's output byte (the variant's list, entries
+  // past its last bit are (0, 0) and compare false; lanes past byte 60 read
+  // the (0, 0) entry 487): loaded once, the same for every keypoint
   int ia[8], ib[8];
 #pragma unroll
   for (int b = 0; b < 8; b++) {
-    const int c = lane < 61 ? lane * 8 + b : 486;
-    ia[b] = d_cmp.a[c];
-    ib[b] = d_cmp.b[c];
+    const int c = lane < 61 ? lane * 8 + b : AKZ_MLDB_TABLE - 1;
+    ia[b] = p.cmp[c];
+    ib[b] = p.cmp[AKZ_MLDB_TABLE + c];
   }
   // first accumulator row (3 x cell) of the lane's samples in each grid: the
   // same for every keypoint
@@ -252,12 +227,22 @@ __global__ __launch_bounds__(64 * MLDB_WAVES) void k_mldb(akz_desc_params p, con
         ch[2][m] = d.y;
       }
     }
-    // channels: L, rx = -dx sin + dy cos, ry = dx cos + dy sin
+    // channels: L, rx = -dx sin + dy cos, ry = dx cos + dy sin; with two
+    // channels L and the gradient magnitude sqrt(dx^2 + dy^2)
+    if (p.nch == 2) {
 #pragma unroll
-    for (int m = 0; m < MS; m++) {
-      const float dx = ch[1][m], dy = ch[2][m];
-      ch[1][m] = -dx * si + dy * co;
-      ch[2][m] = dx * co + dy * si;
+      for (int m = 0; m < MS; m++) {
+        const float dx = ch[1][m], dy = ch[2][m];
+        ch[1][m] = sqrtf(dx * dx + dy * dy);
+        ch[2][m] = 0.0f;
+      }
+    } else {
+#pragma unroll
+      for (int m = 0; m < MS; m++) {
+        const float dx = ch[1][m], dy = ch[2][m];
+        ch[1][m] = -dx * si + dy * co;
+        ch[2][m] = dx * co + dy * si;
+      }
     }
     // samples in increasing i.  The 2x2, 3x3 and 4x4 cells of one sample
     // are different accumulators (disjoint cell ranges), so the nine of a

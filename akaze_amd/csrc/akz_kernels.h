@@ -178,8 +178,15 @@ struct akz_desc_params {
   const float2* Lxy[AKZ_MAX_LEVELS];  // Lx and Ly interleaved (one gather reads both)
   int w[AKZ_MAX_LEVELS], h[AKZ_MAX_LEVELS], pitch[AKZ_MAX_LEVELS];
   int patsize;
+  int nch;                 // M-LDB channels: 1 intensity, 2 + gradient magnitude, 3 + rotated derivatives
+  const int16_t* cmp;      // AKZ_MLDB_TABLE first then AKZ_MLDB_TABLE second value slots of each bit
   size_t zs;
 };
+// M-LDB comparison list of a descriptor variant (akz_mldb_table.cpp): value
+// slots (3 * cell + channel) of each bit's two cell sums, (0, 0) past the last
+// bit.  Returns the bit count, -1 for an invalid variant.
+constexpr int AKZ_MLDB_TABLE = 488;
+int akz_mldb_table(int descriptor_size, int patsize, int nch, int16_t* a, int16_t* b);
 void akz_describe_init_tables();
 void akz_launch_orientation(hipStream_t s, const akz_desc_params& p, akz_kp* kps, const int32_t* nkps, int cap,
                             int nz);

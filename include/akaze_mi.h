@@ -46,9 +46,12 @@ typedef struct {
   int diffusivity;              /* 0 PM_G1, 1 PM_G2 (default), 2 WEICKERT, 3 CHARBONNIER */
   float dthreshold;             /* 0.001 */
   float min_dthreshold;         /* 0.00001 */
-  int descriptor;               /* 5 = MLDB (the only descriptor the reference implements) */
-  int descriptor_size;          /* 0 = full 486 bits */
-  int descriptor_channels;      /* 3 */
+  int descriptor;               /* 5 MLDB (the reference's GPU descriptor), 4 MLDB_UPRIGHT (angle 0, no
+                                   orientation); 0-3 (SURF, M-SURF) give AKZ_ERR_UNSUPPORTED */
+  int descriptor_size;          /* 0 = full length (162 bits per channel); 1..162*channels: the bits
+                                   generateDescriptorSubsample picks (AKAZE.cpp:498-603) */
+  int descriptor_channels;      /* 3 intensity + rotated derivatives; 2 intensity + gradient magnitude;
+                                   1 intensity */
   int descriptor_pattern_size;  /* 10 */
   float kcontrast;              /* output of the contrast stage */
   float kcontrast_percentile;   /* 0.7; in (0, 1], else AKZ_ERR_ARG */
@@ -107,10 +110,16 @@ akz_status akz_build_scale_space_device(akz_ctx* ctx, const float* d_img, size_t
  * (cuda_akaze.cu:1318-1402).  Writes the number of kept keypoints. */
 akz_status akz_detect(akz_ctx* ctx, int* n_out);
 /* AKAZE::Compute_Descriptors (AKAZE.cpp:405-449): orientation
- * (FindOrientation, cuda_akaze.cu:2114-2188) and 486-bit MLDB descriptors
+ * (FindOrientation, cuda_akaze.cu:2114-2188) and MLDB descriptors
  * (ExtractDescriptors + BuildDescriptor, :1420-1533, :1786-1837).  Copies at
- * most `cap` keypoints (28 B each) and descriptors (61 B each) to the host. */
+ * most `cap` keypoints (28 B each) and descriptors (akz_descriptor_bytes
+ * each: 61 for the default 486 bits) to the host. */
 akz_status akz_describe(akz_ctx* ctx, akz_keypoint* kps, uint8_t* desc, int cap, int* n_out);
+/* Bytes (and bits) of one descriptor row of the context's M-LDB variant:
+ * ceil(bits / 8), bits = descriptor_size or 162 * descriptor_channels
+ * (the reference's desc.cols, AKAZE.cpp:412-423).  Host readbacks use rows of
+ * this many bytes; device rows stay 64 bytes, zero past the last bit. */
+akz_status akz_descriptor_bytes(const akz_ctx* ctx, int* nbytes, int* nbits);
 
 /* Asynchronous whole-image pipeline for throughput: scale space + detection +
  * description of a device-resident image, results left on the device.  No
@@ -280,9 +289,10 @@ akz_status akz_synth_device_warp(const float* d_src, int w, int h, size_t src_pi
  * 36-121) and MatchDescriptors (cuda_akaze.cu:1842-2031). ---- */
 akz_status akz_matcher_create(akz_matcher** out);
 akz_status akz_matcher_destroy(akz_matcher* m);
-/* Matcher::bfmatch: q is nq x 61 bytes, t is nt x 61 bytes (row stride
- * `stride` bytes, >= 61).  Writes 2*nq DMatches: best then second best by
- * (distance, trainIdx). */
+/* Matcher::bfmatch: q is nq rows, t is nt rows, `stride` bytes apart, of
+ * which the first min(stride, 61) are the descriptor (61 for 486 bits; fewer
+ * for the shorter M-LDB variants).  Writes 2*nq DMatches: best then second
+ * best by (distance, trainIdx). */
 akz_status akz_match2nn(akz_matcher* m, const uint8_t* q, int nq, const uint8_t* t, int nt, size_t stride,
                         akz_dmatch* out);
 /* Device-resident variant: 64-byte rows (61 used, 3 zero), output in device

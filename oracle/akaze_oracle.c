@@ -27,6 +27,7 @@
  *    keypoints inside the detector border (AKAZE.cpp:353-363) the clamp never
  *    changes a coordinate.
  */
+#define _DEFAULT_SOURCE /* random_r / initstate_r: glibc's rand() with private state */
 #include "akaze_oracle.h"
 
 #include <math.h>
@@ -69,6 +70,9 @@ void orc_options_default(orc_options* o) {
   o->kcontrast_percentile = 0.7f;
   o->kcontrast_nbins = 300;
   o->nms_max_waves = 10000;
+  o->descriptor = 5;
+  o->descriptor_size = 0;
+  o->descriptor_channels = 3;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -721,8 +725,8 @@ void orc_compare_tables(int32_t* idx1, int32_t* idx2) {
   for (; c < 488; c++) idx1[c] = idx2[c] = 0;
 }
 
-void orc_mldb(const orc_kp* kp, const float* Lt, const float* Lx, const float* Ly, int w, int h, int patsize,
-              uint8_t* desc, float* vals_out) {
+void orc_mldb_ex(const orc_kp* kp, const float* Lt, const float* Lx, const float* Ly, int w, int h, int patsize,
+                 int nch, const int32_t* idx1, const int32_t* idx2, int nbits, uint8_t* desc, float* vals_out) {
   int size2 = patsize;
   int size3 = (int)ceilf(2.0f * patsize / 3.0f);
   int size4 = (int)ceilf(0.5f * patsize);
@@ -749,8 +753,16 @@ void orc_mldb(const orc_kp* kp, const float* Lt, const float* Lx, const float* L
     yp = iclamp(yp, 0, h - 1);
     size_t pos = (size_t)yp * w + xp;
     float im = Lt[pos], dx = Lx[pos], dy = Ly[pos];
-    float rx = -dx * si + dy * co;
-    float ry = dx * co + dy * si;
+    float rx, ry;
+    if (nch == 2) {
+      /* intensity + gradient magnitude (the two-channel M-LDB of
+       * AKAZEOptions::descriptor_channels, AKAZEConfig.h:131) */
+      rx = sqrtf(dx * dx + dy * dy);
+      ry = 0.0f;
+    } else {
+      rx = -dx * si + dy * co;
+      ry = dx * co + dy * si;
+    }
     if (m < 2 * size2) {
       int x2 = (x < size2 ? 0 : 1), y2 = (y < size2 ? 0 : 1);
       int cell = y2 * 2 + x2;
@@ -783,14 +795,126 @@ void orc_mldb(const orc_kp* kp, const float* Lt, const float* Lx, const float* L
       for (int t = 0; t + s < 32; t += 2 * s) a[t] = a[t] + a[t + s];
     vals[v] = a[0];
   }
-  int32_t i1[488], i2[488];
-  orc_compare_tables(i1, i2);
-  for (int b = 0; b < 61; b++) {
+  int nbytes = (nbits + 7) / 8;
+  for (int b = 0; b < nbytes; b++) {
     uint8_t r = 0;
-    for (int i = 0; i < (b == 60 ? 6 : 8); i++) r |= (uint8_t)((vals[i1[b * 8 + i]] > vals[i2[b * 8 + i]] ? 1 : 0) << i);
+    for (int i = 0; i < 8 && b * 8 + i < nbits; i++)
+      r |= (uint8_t)((vals[idx1[b * 8 + i]] > vals[idx2[b * 8 + i]] ? 1 : 0) << i);
     desc[b] = r;
   }
   if (vals_out) memcpy(vals_out, vals, sizeof(vals));
+}
+
+void orc_mldb(const orc_kp* kp, const float* Lt, const float* Lx, const float* Ly, int w, int h, int patsize,
+              uint8_t* desc, float* vals_out) {
+  int32_t i1[488], i2[488];
+  orc_compare_tables(i1, i2);
+  orc_mldb_ex(kp, Lt, Lx, Ly, w, h, patsize, 3, i1, i2, 486, desc, vals_out);
+}
+
+/* generateDescriptorSubsample, AKAZE.cpp:498-603: the quasi-random bit
+ * selection of an nbits-long M-LDB descriptor.  The full comparison list
+ * (every cell pair j < k of the 2x2, 3x3 and 4x4 grids, cells at pattern
+ * offsets psz * (j % g) - P, psz * (j / g) - P, psz = ceil(2P / g)) is drawn
+ * from with glibc's rand() after srand(1024) (here random_r on a private
+ * TYPE_3 state, the generator rand() uses), the first 6 picks forced to the
+ * 2x2 grid; a pick adds its two cells to the sample list unless present and
+ * yields one comparison per channel.  The reference writes three comparison
+ * rows per pick whatever the channel count (AKAZE.cpp:556-590), which for
+ * fewer than three channels overwrites rows the next pick rewrites and runs
+ * two rows past the matrix on the last pick; the rows it keeps are the
+ * per-channel rows written here.  Returns the sample count, or -1 when nbits
+ * exceeds the full descriptor. */
+int orc_descriptor_subsample(int nbits, int patsize, int nch, int32_t* samples, int32_t* comps) {
+  int ssz = 0;
+  for (int i = 0; i < 3; i++) {
+    int gz = (i + 2) * (i + 2);
+    ssz += gz * (gz - 1) / 2;
+  }
+  if (nbits < 1 || nbits > ssz * nch) return -1;
+  enum { FULL = 162 };
+  int fullM[FULL][5];
+  for (int i = 0, c = 0; i < 3; i++) {
+    int gdiv = i + 2, gsz = gdiv * gdiv;
+    int psz = (int)ceil(2. * patsize / (float)gdiv);
+    for (int j = 0; j < gsz; j++)
+      for (int k = j + 1; k < gsz; k++, c++) {
+        fullM[c][0] = i;
+        fullM[c][1] = psz * (j % gdiv) - patsize;
+        fullM[c][2] = psz * (j / gdiv) - patsize;
+        fullM[c][3] = psz * (k % gdiv) - patsize;
+        fullM[c][4] = psz * (k / gdiv) - patsize;
+      }
+  }
+  struct random_data rd;
+  char state[128];
+  memset(&rd, 0, sizeof rd);
+  initstate_r(1024, state, sizeof state, &rd);
+  int npicks = (int)ceil(nbits / (float)nch);
+  int count = 0;
+  for (int i = 0; i < npicks; i++) {
+    int32_t r = 0;
+    random_r(&rd, &r);
+    size_t k = (size_t)r % (size_t)(FULL - i);
+    if (i < 6) k = (size_t)i;
+    for (int end = 0; end < 2; end++) {
+      int sx = fullM[k][1 + 2 * end], sy = fullM[k][2 + 2 * end], found = -1;
+      for (int j = 0; j < count; j++)
+        if (samples[3 * j] == fullM[k][0] && samples[3 * j + 1] == sx && samples[3 * j + 2] == sy) {
+          found = j;
+          break;
+        }
+      if (found < 0) {
+        samples[3 * count] = fullM[k][0];
+        samples[3 * count + 1] = sx;
+        samples[3 * count + 2] = sy;
+        found = count++;
+      }
+      for (int ch = 0; ch < nch; ch++)
+        if (i * nch + ch < nbits) comps[2 * (i * nch + ch) + end] = nch * found + ch;
+    }
+    memcpy(fullM[k], fullM[FULL - i - 1], sizeof fullM[k]);
+  }
+  return count;
+}
+
+/* The comparison list of the M-LDB variant, as indices into the 87 cell
+ * values (3 * cell + channel, cells 0-3 of the 2x2 grid, 4-12 of the 3x3,
+ * 13-28 of the 4x4): with descriptor_size 0, every cell pair j < i of each
+ * grid for channels 0 .. nch-1 in turn (InitCompareIndices,
+ * cuda_akaze.cu:2034-2111, for three channels); otherwise the comparisons of
+ * orc_descriptor_subsample mapped from sample slots to cells.  Returns the bit
+ * count, or -1. */
+int orc_mldb_table(int descriptor_size, int patsize, int nch, int32_t* idx1, int32_t* idx2) {
+  static const int lo[3] = {0, 4, 13}, hi[3] = {4, 13, 29};
+  if (nch < 1 || nch > 3) return -1;
+  int c = 0;
+  if (descriptor_size == 0) {
+    for (int g = 0; g < 3; g++)
+      for (int ch = 0; ch < nch; ch++)
+        for (int j = lo[g]; j < hi[g] - 1; j++)
+          for (int i = j + 1; i < hi[g]; i++) {
+            idx1[c] = 3 * j + ch;
+            idx2[c] = 3 * i + ch;
+            c++;
+          }
+    return c;
+  }
+  int32_t samples[3 * 29], comps[2 * 486];
+  int ns = orc_descriptor_subsample(descriptor_size, patsize, nch, samples, comps);
+  if (ns < 0) return -1;
+  int cell_of[29];
+  for (int s = 0; s < ns; s++) {
+    int g = samples[3 * s], gdiv = g + 2;
+    int psz = (int)ceil(2. * patsize / (float)gdiv);
+    int cx = (samples[3 * s + 1] + patsize) / psz, cy = (samples[3 * s + 2] + patsize) / psz;
+    cell_of[s] = lo[g] + cy * gdiv + cx;
+  }
+  for (int b = 0; b < descriptor_size; b++) {
+    idx1[b] = 3 * cell_of[comps[2 * b] / nch] + comps[2 * b] % nch;
+    idx2[b] = 3 * cell_of[comps[2 * b + 1] / nch] + comps[2 * b + 1] % nch;
+  }
+  return descriptor_size;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -801,11 +925,12 @@ void orc_match2nn(const uint8_t* q, int nq, const uint8_t* t, int nt, int stride
 #pragma omp parallel for schedule(dynamic, 64)
   for (int a = 0; a < nq; a++) {
     const uint8_t* qa = q + (size_t)a * stride;
+    const int nb = stride < 61 ? stride : 61; /* shorter rows: the M-LDB subsets and channel variants */
     int b1 = 0, d1 = 512, b2 = 0, d2 = 512;
     for (int b = 0; b < nt; b++) {
       const uint8_t* tb = t + (size_t)b * stride;
       int d = 0;
-      for (int k = 0; k < 61; k++) d += __builtin_popcount((unsigned)(qa[k] ^ tb[k]));
+      for (int k = 0; k < nb; k++) d += __builtin_popcount((unsigned)(qa[k] ^ tb[k]));
       if (d < d1) {
         d2 = d1; b2 = b1;
         d1 = d; b1 = b;
@@ -836,11 +961,19 @@ struct orc_ctx {
   int nkps;
   int waves;
   uint8_t* desc;
+  int32_t cmp1[488], cmp2[488];
+  int nbits;
 };
 
 orc_ctx* orc_create(const orc_options* o, int W, int H) {
+  if (o->descriptor != 4 && o->descriptor != 5) return NULL;
+  int nbits_probe[2][488];
+  if (orc_mldb_table(o->descriptor_size, o->descriptor_pattern_size, o->descriptor_channels, nbits_probe[0],
+                     nbits_probe[1]) < 0)
+    return NULL;
   orc_ctx* c = (orc_ctx*)calloc(1, sizeof(orc_ctx));
   c->o = *o;
+  c->nbits = orc_mldb_table(o->descriptor_size, o->descriptor_pattern_size, o->descriptor_channels, c->cmp1, c->cmp2);
   c->W = W;
   c->H = H;
   if (orc_geometry(o, W, H, &c->g) <= 0) { free(c); return NULL; }
@@ -938,18 +1071,27 @@ const int* orc_kept_candidates(const orc_ctx* c) { return c->kidx; }
 const orc_kp* orc_keypoints(const orc_ctx* c) { return c->kps; }
 const uint8_t* orc_descriptors(const orc_ctx* c) { return c->desc; }
 
+int orc_descriptor_bytes(const orc_ctx* c) { return (c->nbits + 7) / 8; }
+
+/* Compute_Descriptors (AKAZE.cpp:405-449).  MLDB_UPRIGHT (descriptor 4) skips
+ * the orientation: angle 0, the unrotated pattern.  Rows are
+ * orc_descriptor_bytes() bytes. */
 int orc_describe(orc_ctx* c) {
+  int nb = orc_descriptor_bytes(c);
   free(c->desc);
-  c->desc = (uint8_t*)calloc((size_t)c->nkps * 61 + 1, 1);
+  c->desc = (uint8_t*)calloc((size_t)c->nkps * nb + 1, 1);
   int n = c->nkps;
 #pragma omp parallel for schedule(dynamic, 16)
   for (int p = 0; p < n; p++) {
     orc_kp* k = &c->kps[p];
     int l = k->class_id;
     int w = c->g.w[l], h = c->g.h[l];
-    orc_orientation(k, c->lev[l][2], c->lev[l][3], w, h);
-    orc_mldb(k, c->lev[l][0], c->lev[l][2], c->lev[l][3], w, h, c->o.descriptor_pattern_size,
-             c->desc + (size_t)p * 61, NULL);
+    if (c->o.descriptor == 5)
+      orc_orientation(k, c->lev[l][2], c->lev[l][3], w, h);
+    else
+      k->angle = 0.0f;
+    orc_mldb_ex(k, c->lev[l][0], c->lev[l][2], c->lev[l][3], w, h, c->o.descriptor_pattern_size,
+                c->o.descriptor_channels, c->cmp1, c->cmp2, c->nbits, c->desc + (size_t)p * nb, NULL);
   }
   return n;
 }

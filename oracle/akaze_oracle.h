@@ -42,6 +42,9 @@ typedef struct {
   float kcontrast_percentile;
   int kcontrast_nbins;
   int nms_max_waves; /* cuda_akaze.cu:1100 uses 10000 */
+  int descriptor;          /* 5 MLDB, 4 MLDB_UPRIGHT (AKAZEConfig.h:41-57) */
+  int descriptor_size;     /* 0 = full length, else the bits of the subsample */
+  int descriptor_channels; /* 1 intensity, 2 + gradient magnitude, 3 + rotated derivatives */
 } orc_options;
 
 /* cv::KeyPoint wire layout (28 bytes).  For detector candidates `octave` holds
@@ -106,6 +109,12 @@ int orc_nms_ex(const orc_kp* cand, int n, const int* level_start, int nlev, int 
 void orc_orientation(orc_kp* kp, const float* Lx, const float* Ly, int w, int h);
 void orc_mldb(const orc_kp* kp, const float* Lt, const float* Lx, const float* Ly, int w, int h,
               int patsize, uint8_t* desc61, float* vals87);
+/* M-LDB with nch channels and a comparison list (orc_mldb_table) of nbits
+ * entries into the 87 cell values; desc receives ceil(nbits / 8) bytes. */
+void orc_mldb_ex(const orc_kp* kp, const float* Lt, const float* Lx, const float* Ly, int w, int h, int patsize,
+                 int nch, const int32_t* idx1, const int32_t* idx2, int nbits, uint8_t* desc, float* vals87);
+int orc_descriptor_subsample(int nbits, int patsize, int nch, int32_t* samples3, int32_t* comps2);
+int orc_mldb_table(int descriptor_size, int patsize, int nch, int32_t* idx1, int32_t* idx2);
 void orc_match2nn(const uint8_t* q, int nq, const uint8_t* t, int nt, int stride, orc_dmatch* out);
 void orc_compare_tables(int32_t* idx1, int32_t* idx2); /* 488 entries each, cuda_akaze.cu:2034-2111 */
 
@@ -159,6 +168,7 @@ double orc_prop_loss_grad(const orc_prop_geom* g, int nimg, const float* params,
  * counts from 1. */
 void orc_prop_adam(float* p, float* m, float* v, const double* g, int n, float lr, int step);
 int orc_describe(orc_ctx* c);
+int orc_descriptor_bytes(const orc_ctx* c);
 const uint8_t* orc_descriptors(const orc_ctx* c);
 int orc_set_threads(int n);
 

@@ -26,7 +26,9 @@ class Options(ctypes.Structure):
                 ("derivative_factor", ctypes.c_float), ("diffusivity", ctypes.c_int),
                 ("dthreshold", ctypes.c_float), ("min_dthreshold", ctypes.c_float),
                 ("descriptor_pattern_size", ctypes.c_int), ("kcontrast_percentile", ctypes.c_float),
-                ("kcontrast_nbins", ctypes.c_int), ("nms_max_waves", ctypes.c_int)]
+                ("kcontrast_nbins", ctypes.c_int), ("nms_max_waves", ctypes.c_int),
+                ("descriptor", ctypes.c_int), ("descriptor_size", ctypes.c_int),
+                ("descriptor_channels", ctypes.c_int)]
 
 
 MAXL = 32
@@ -79,6 +81,14 @@ def lib():
                                ctypes.c_int, c_u8_p, c_float_p]
         L.orc_match2nn.argtypes = [c_u8_p, ctypes.c_int, c_u8_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
         L.orc_compare_tables.argtypes = [c_int_p, c_int_p]
+        L.orc_mldb_ex.argtypes = [ctypes.c_void_p, c_float_p, c_float_p, c_float_p, ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_int, ctypes.c_int, c_int_p, c_int_p, ctypes.c_int, c_u8_p, c_float_p]
+        L.orc_descriptor_subsample.restype = ctypes.c_int
+        L.orc_descriptor_subsample.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, c_int_p, c_int_p]
+        L.orc_mldb_table.restype = ctypes.c_int
+        L.orc_mldb_table.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, c_int_p, c_int_p]
+        L.orc_descriptor_bytes.restype = ctypes.c_int
+        L.orc_descriptor_bytes.argtypes = [ctypes.c_void_p]
         L.orc_atan2f.restype = ctypes.c_float
         L.orc_atan2f.argtypes = [ctypes.c_float, ctypes.c_float]
         L.orc_sincosf.argtypes = [ctypes.c_float, c_float_p, c_float_p]
@@ -243,6 +253,38 @@ def compare_tables():
     return a, b
 
 
+def descriptor_subsample(nbits, patsize=10, nch=3):
+    """orc_descriptor_subsample: (samples n x 3 [grid, x, y], comparisons nbits x 2)."""
+    smp = np.zeros(3 * 29, np.int32)
+    cmp = np.zeros(2 * 486, np.int32)
+    n = lib().orc_descriptor_subsample(nbits, patsize, nch, smp.ctypes.data_as(c_int_p), cmp.ctypes.data_as(c_int_p))
+    if n < 0:
+        return None
+    return smp[:3 * n].reshape(n, 3), cmp[:2 * nbits].reshape(nbits, 2)
+
+
+def mldb_table(descriptor_size=0, patsize=10, nch=3):
+    """orc_mldb_table: (idx1, idx2) into the 87 cell values, one pair per bit."""
+    a = np.zeros(488, np.int32)
+    b = np.zeros(488, np.int32)
+    n = lib().orc_mldb_table(descriptor_size, patsize, nch, a.ctypes.data_as(c_int_p), b.ctypes.data_as(c_int_p))
+    if n < 0:
+        return None
+    return a[:n], b[:n]
+
+
+def mldb_ex(kp, Lt, Lx, Ly, patsize, nch, table):
+    """orc_mldb_ex on one keypoint: (descriptor bytes, 87 values)."""
+    a, b = (np.ascontiguousarray(t, np.int32) for t in table)
+    h, w = Lt.shape
+    desc = np.zeros((len(a) + 7) // 8, np.uint8)
+    vals = np.zeros(87, np.float32)
+    lib().orc_mldb_ex(np.ascontiguousarray(kp, KP_DTYPE).ctypes.data, fptr(Lt), fptr(Lx), fptr(Ly), w, h, patsize,
+                      nch, a.ctypes.data_as(c_int_p), b.ctypes.data_as(c_int_p), len(a),
+                      desc.ctypes.data_as(c_u8_p), fptr(vals))
+    return desc, vals
+
+
 def atan2f(y, x):
     return np.float32(lib().orc_atan2f(y, x))
 
@@ -318,10 +360,11 @@ class Pipeline:
     def describe(self):
         L = lib()
         n = L.orc_describe(self.ctx)
+        nb = L.orc_descriptor_bytes(self.ctx)
         if n == 0:
-            return np.zeros((0, 61), np.uint8)
-        buf = (ctypes.c_char * (n * 61)).from_address(L.orc_descriptors(self.ctx))
-        return np.frombuffer(bytes(buf), np.uint8).reshape(n, 61).copy()
+            return np.zeros((0, nb), np.uint8)
+        buf = (ctypes.c_char * (n * nb)).from_address(L.orc_descriptors(self.ctx))
+        return np.frombuffer(bytes(buf), np.uint8).reshape(n, nb).copy()
 
     def prop_geom(self):
         g = PropGeom()
