@@ -122,6 +122,7 @@ SIGNATURES = {
     "akz_ragged_set_graph_mode": [_P, _I],
     "akz_ragged_autotune": [_P, _I],
     "akz_ragged_process": [_P, _I, _P, _P, _P, _P, _P],
+    "akz_ragged_set_work_lists": [_P, _I],
     "akz_ragged_read": [_P, _I, _P, _P, _I, _P],
     "akz_ragged_results_device": [_P, _I, _P, _P, _P],
     "akz_matcher_create": [_P],
@@ -705,6 +706,11 @@ class Ragged:
 
     def set_graph_mode(self, on):
         _check(lib().akz_ragged_set_graph_mode(self.r, int(bool(on))), "akz_ragged_set_graph_mode")
+
+    def set_work_lists(self, on):
+        """Persistent detector / orientation / MLDB launches over the whole
+        ragged call (default) or one batch call per size class."""
+        _check(lib().akz_ragged_set_work_lists(self.r, int(bool(on))), "akz_ragged_set_work_lists")
 
     def autotune(self, reps=3):
         _check(lib().akz_ragged_autotune(self.r, int(reps)), "akz_ragged_autotune")

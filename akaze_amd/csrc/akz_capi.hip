@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <new>
 #include <vector>
 
@@ -133,6 +134,10 @@ struct akz_ctx {
   // M-LDB variant: comparison list on the device, bits and bytes per descriptor
   int16_t* d_cmp = nullptr;
   int desc_bits = 486, desc_bytes = 61;
+  // bumped whenever captured graphs of this context go stale (proposal,
+  // FED variants, determinant setting): graphs built over several contexts
+  // compare it
+  unsigned config_gen = 0;
   int state = 0;  // 0 created, 1 scale space, 2 detected, 3 described
   bool graph_mode = false;
   bool capturing = false;
@@ -156,6 +161,20 @@ struct akz_ragged {
   int ncls = 0;
   akz_ctx* cls[AKZ_MAX_CLASSES] = {};
   std::vector<int> img_cls, img_slot;  // class and batch slot of each image of the last call
+  // work-list mode: the detector, orientation and MLDB run as one persistent
+  // launch each over every image of every class (ragged_enqueue_lists)
+  bool lists = false;
+  bool graph = false;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_mid = nullptr;
+  hipEvent_t ev_join[AKZ_MAX_CLASSES] = {}, ev_join2[AKZ_MAX_CLASSES] = {};
+  akz_det_params* d_detp = nullptr;    // the classes' fused-detector parameters
+  akz_desc_params* d_descp = nullptr;  // and descriptor parameters, in device memory
+  std::vector<akz_det_params> detp_host;
+  bool descp_ready = false;
+  unsigned gen = ~0u;  // sum of the class contexts' config_gen the graphs were built with
+  // one captured graph per class composition (images per class)
+  std::map<std::vector<int>, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
 };
 
 struct akz_matcher {
@@ -477,15 +496,11 @@ void enqueue_scale_space(akz_ctx* c, const float* img, int in_pitch, size_t in_z
   record(c, EV_SCALE);
 }
 
-// Feature_Detection (AKAZE.cpp:316-378) + FilterExtrema (cuda_akaze.cu:1318-1402).
-// full_det: write every determinant pixel (parity readback, tile features);
-// otherwise only the candidates' 3x3 neighbourhoods the subpixel fit reads.
-// With a learned proposal the fused detector keeps the tile maxima of the
-// determinant itself; the unfused kernels write whole determinant images,
-// which the tile statistics then read.
-void enqueue_detect(akz_ctx* c, int nz, bool full_det) {
-  hipStream_t s = c->stream;
-  c->status_synced = 0;
+// Detection in three parts, so that a ragged batch can run the middle one as
+// a single persistent launch over all its size classes: the parameters and
+// the proposal keys (returns whether the fused detector applies), the
+// detector kernels, and everything after them.
+bool detect_params(akz_ctx* c, bool full_det) {
   int smax = 0;
   for (int i = 0; i < c->nlev; i++) smax = std::max(smax, c->lev[i].deriv_step);
   const bool fused = smax <= akz_detect_max_step();
@@ -501,9 +516,20 @@ void enqueue_detect(akz_ctx* c, int nz, bool full_det) {
       c->det32.tile_h[i] = g.th[i];
     }
     c->det32.tiles_per_image = g.P;
-    akz_prop_launch_clear_keys(s, g, c->pact, nz);
   }
   c->det_full_last = full_det || !fused;  // the unfused kernels write all of it
+  return fused;
+}
+
+bool detect_prepare(akz_ctx* c, int nz, bool full_det) {
+  c->status_synced = 0;
+  const bool fused = detect_params(c, full_det);
+  if (c->det32.tile_key) akz_prop_launch_clear_keys(c->stream, c->psrc.g, c->pact, nz);
+  return fused;
+}
+
+void detect_kernels(akz_ctx* c, int nz, bool fused) {
+  hipStream_t s = c->stream;
   if (fused) {
     akz_launch_detect_fused(s, c->det32, c->db, nz);
     record(c, EV_DERIV);
@@ -512,8 +538,12 @@ void enqueue_detect(akz_ctx* c, int nz, bool full_det) {
     record(c, EV_DERIV);
     akz_launch_extrema(s, c->det, c->db, nz);
   }
+}
+
+void detect_finish(akz_ctx* c, int nz, bool fused) {
+  hipStream_t s = c->stream;
   if (c->prop_net) {
-    akz_prop_launch_features(s, c->psrc, c->pact, nz, prop_keys);
+    akz_prop_launch_features(s, c->psrc, c->pact, nz, c->prop_net != nullptr && fused);
     akz_prop_launch_infer(s, akz_prop_net_weights(c->prop_net), c->psrc.g, c->pact, nz, c->prop_thr);
     akz_prop_launch_mask_segments(s, c->det, c->db, c->psrc.g, c->pact.mask, nz);
   }
@@ -521,6 +551,18 @@ void enqueue_detect(akz_ctx* c, int nz, bool full_det) {
   record(c, EV_EXTREMA);
   akz_launch_suppression(s, c->db, nz);
   record(c, EV_NMS);
+}
+
+// Feature_Detection (AKAZE.cpp:316-378) + FilterExtrema (cuda_akaze.cu:1318-1402).
+// full_det: write every determinant pixel (parity readback, tile features);
+// otherwise only the candidates' 3x3 neighbourhoods the subpixel fit reads.
+// With a learned proposal the fused detector keeps the tile maxima of the
+// determinant itself; the unfused kernels write whole determinant images,
+// which the tile statistics then read.
+void enqueue_detect(akz_ctx* c, int nz, bool full_det) {
+  const bool fused = detect_prepare(c, nz, full_det);
+  detect_kernels(c, nz, fused);
+  detect_finish(c, nz, fused);
 }
 
 // Compute_Descriptors (AKAZE.cpp:405-449): orientation, then MLDB.
@@ -581,6 +623,7 @@ akz_status ensure_prop_act(akz_ctx* c) {
 }
 
 void drop_graphs(akz_ctx* c) {
+  c->config_gen++;  // graphs that include this context's kernels are stale
   for (size_t i = 0; i < c->graph_exec.size(); i++) {
     if (c->graph_exec[i]) (void)hipGraphExecDestroy(c->graph_exec[i]);
     if (c->graph[i]) (void)hipGraphDestroy(c->graph[i]);
@@ -1218,15 +1261,126 @@ akz_status akz_ragged_create(const akz_options* base, int nclass, const int* wid
     akz_status st = akz_create_batch(&o, capacity[k], &r->cls[k]);
     if (st != AKZ_OK) { akz_ragged_destroy(r); return st; }
   }
+  bool ok = hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking) == hipSuccess &&
+            hipEventCreateWithFlags(&r->ev_fork, hipEventDisableTiming) == hipSuccess &&
+            hipEventCreateWithFlags(&r->ev_mid, hipEventDisableTiming) == hipSuccess &&
+            hipMalloc((void**)&r->d_detp, sizeof(akz_det_params) * nclass) == hipSuccess &&
+            hipMalloc((void**)&r->d_descp, sizeof(akz_desc_params) * nclass) == hipSuccess;
+  for (int k = 0; k < nclass && ok; k++)
+    ok = hipEventCreateWithFlags(&r->ev_join[k], hipEventDisableTiming) == hipSuccess &&
+         hipEventCreateWithFlags(&r->ev_join2[k], hipEventDisableTiming) == hipSuccess;
+  if (!ok) {
+    (void)hipGetLastError();
+    akz_ragged_destroy(r);
+    return AKZ_ERR_HIP;
+  }
+  r->detp_host.resize(nclass);
+  std::memset(r->detp_host.data(), 0xff, sizeof(akz_det_params) * nclass);  // uploaded on first use
   *out = r;
   return AKZ_OK;
 }
 
+namespace {
+
+void ragged_drop_graphs(akz_ragged* r) {
+  for (auto& g : r->graphs) {
+    if (g.second.second) hipGraphExecDestroy(g.second.second);
+    if (g.second.first) hipGraphDestroy(g.second.first);
+  }
+  r->graphs.clear();
+}
+
+// Work-list execution of a ragged call whose images are staged in slots
+// 0..cnt[k]-1 of the class contexts: per class on its own stream, forked
+// from r->stream, the contrast factor and scale space; then on r->stream one
+// persistent detector launch walking the (image, level, tile) work list of
+// all classes; per class again the candidates and the suppression; then one
+// persistent orientation and one MLDB launch walking the (image, keypoint)
+// lists.  Everything joins back into r->stream.
+akz_status ragged_enqueue_lists(akz_ragged* r, const int* cnt) {
+  hipStream_t s = r->stream;
+  AKZ_HIP_TRY(hipEventRecord(r->ev_fork, s));
+  akz_det_worklist dw{};
+  akz_desc_worklist ew{};
+  int tiles = 0;
+  for (int k = 0; k < r->ncls; k++) {
+    if (cnt[k] == 0) continue;
+    akz_ctx* c = r->cls[k];
+    AKZ_HIP_TRY(hipStreamWaitEvent(c->stream, r->ev_fork, 0));
+    enqueue_scale_space(c, c->img_in, c->lev[0].pitch, c->zs, cnt[k]);
+    detect_prepare(c, cnt[k], c->keep_det);
+    AKZ_HIP_TRY(hipEventRecord(r->ev_join[k], c->stream));
+    AKZ_HIP_TRY(hipStreamWaitEvent(s, r->ev_join[k], 0));
+    dw.seg_count[k] = c->db.seg_count;
+    dw.seg_slots[k] = c->db.seg_slots;
+    ew.kps[k] = c->kps;
+    ew.nkps[k] = c->status + AKZ_ST_KEPT;
+    ew.desc[k] = c->desc;
+    for (int z = 0; z < cnt[k]; z++) {
+      dw.cls[dw.nimg] = k;
+      dw.z[dw.nimg] = z;
+      dw.start[dw.nimg] = tiles;
+      tiles += c->det32.tile_start[c->nlev];
+      dw.nimg++;
+      ew.cls[ew.nimg] = k;
+      ew.z[ew.nimg] = z;
+      ew.nimg++;
+    }
+  }
+  dw.start[dw.nimg] = tiles;
+  dw.P = r->d_detp;
+  ew.P = r->d_descp;
+  if (dw.nimg > 0) akz_launch_detect_fused_list(s, dw, std::min(tiles, 1024));
+  AKZ_HIP_TRY(hipEventRecord(r->ev_mid, s));
+  bool orient = false;
+  for (int k = 0; k < r->ncls; k++) {
+    if (cnt[k] == 0) continue;
+    akz_ctx* c = r->cls[k];
+    AKZ_HIP_TRY(hipStreamWaitEvent(c->stream, r->ev_mid, 0));
+    detect_finish(c, cnt[k], true);
+    AKZ_HIP_TRY(hipEventRecord(r->ev_join2[k], c->stream));
+    AKZ_HIP_TRY(hipStreamWaitEvent(s, r->ev_join2[k], 0));
+    orient |= c->opts.descriptor == 5;
+  }
+  if (ew.nimg > 0) akz_launch_describe_list(s, ew, orient, 2048, 4096);
+  return AKZ_OK;
+}
+
+// Whether every class can use the persistent detector (derivative steps
+// within the fused kernel's limit) and the shared descriptor settings.
+bool ragged_lists_apply(const akz_ragged* r) {
+  for (int k = 0; k < r->ncls; k++) {
+    const akz_ctx* c = r->cls[k];
+    for (int i = 0; i < c->nlev; i++)
+      if (c->lev[i].deriv_step > akz_detect_max_step()) return false;
+  }
+  return true;
+}
+
+}  // namespace
+
 akz_status akz_ragged_destroy(akz_ragged* r) {
   if (!r) return AKZ_ERR_ARG;
+  if (r->stream) hipStreamSynchronize(r->stream);
+  ragged_drop_graphs(r);
   for (int k = 0; k < r->ncls; k++)
     if (r->cls[k]) akz_destroy(r->cls[k]);
+  if (r->d_detp) hipFree(r->d_detp);
+  if (r->d_descp) hipFree(r->d_descp);
+  if (r->ev_fork) hipEventDestroy(r->ev_fork);
+  if (r->ev_mid) hipEventDestroy(r->ev_mid);
+  for (int k = 0; k < AKZ_MAX_CLASSES; k++) {
+    if (r->ev_join[k]) hipEventDestroy(r->ev_join[k]);
+    if (r->ev_join2[k]) hipEventDestroy(r->ev_join2[k]);
+  }
+  if (r->stream) hipStreamDestroy(r->stream);
   delete r;
+  return AKZ_OK;
+}
+
+akz_status akz_ragged_set_work_lists(akz_ragged* r, int enabled) {
+  if (!r) return AKZ_ERR_ARG;
+  r->lists = enabled != 0;
   return AKZ_OK;
 }
 
@@ -1251,20 +1405,97 @@ akz_status akz_ragged_process(akz_ragged* r, int n, const float* const* d_imgs, 
     slot[i] = cnt[k]++;
     if (cnt[k] > r->cls[k]->batch) return AKZ_ERR_ARG;
   }
-  // stage every image into its class context's input slot, then one batch
-  // call per class; the classes run concurrently on their own streams
-  for (int i = 0; i < n; i++) {
-    akz_ctx* c = r->cls[cls[i]];
-    const Level& L0 = c->lev[0];
-    const size_t sp = pitch_bytes && pitch_bytes[i] ? pitch_bytes[i] : sizeof(float) * widths[i];
-    AKZ_HIP_TRY(hipMemcpy2DAsync(zptr(c, c->img_in, slot[i]), sizeof(float) * L0.pitch, d_imgs[i], sp,
-                                 sizeof(float) * L0.w, L0.h, hipMemcpyDeviceToDevice, c->stream));
-  }
-  for (int k = 0; k < r->ncls; k++) {
-    if (cnt[k] == 0) continue;
-    akz_ctx* c = r->cls[k];
-    akz_status st = akz_process_batch_device_async(c, c->img_in, sizeof(float) * c->lev[0].pitch, c->zs, cnt[k]);
-    if (st != AKZ_OK) return st;
+  if (r->lists && ragged_lists_apply(r)) {
+    // stage every image into its class context's input slot on r->stream,
+    // then the work-list pipeline (captured once per class composition in
+    // graph mode)
+    for (int i = 0; i < n; i++) {
+      akz_ctx* c = r->cls[cls[i]];
+      const Level& L0 = c->lev[0];
+      const size_t sp = pitch_bytes && pitch_bytes[i] ? pitch_bytes[i] : sizeof(float) * widths[i];
+      AKZ_HIP_TRY(hipMemcpy2DAsync(zptr(c, c->img_in, slot[i]), sizeof(float) * L0.pitch, d_imgs[i], sp,
+                                   sizeof(float) * L0.w, L0.h, hipMemcpyDeviceToDevice, r->stream));
+    }
+    if (!r->descp_ready) {
+      for (int k = 0; k < r->ncls; k++)
+        AKZ_HIP_TRY(hipMemcpy(r->d_descp + k, &r->cls[k]->dp, sizeof(akz_desc_params), hipMemcpyHostToDevice));
+      r->descp_ready = true;
+    }
+    // the detector parameters as the class contexts hold them now (flags and
+    // proposal keys change with their settings), in device memory before
+    // anything reads them; graphs built with other settings are dropped
+    bool changed = false;
+    unsigned gen = 0;
+    for (int k = 0; k < r->ncls; k++) {
+      akz_ctx* c = r->cls[k];
+      detect_params(c, c->keep_det);
+      gen += c->config_gen;
+      if (std::memcmp(&r->detp_host[k], &c->det32, sizeof(akz_det_params)) != 0) {
+        r->detp_host[k] = c->det32;
+        changed = true;
+      }
+    }
+    if (changed) {
+      AKZ_HIP_TRY(hipStreamSynchronize(r->stream));
+      AKZ_HIP_TRY(hipMemcpy(r->d_detp, r->detp_host.data(), sizeof(akz_det_params) * r->ncls,
+                            hipMemcpyHostToDevice));
+    }
+    if (changed || gen != r->gen) {
+      ragged_drop_graphs(r);
+      r->gen = gen;
+    }
+    std::vector<int> key(cnt, cnt + r->ncls);
+    if (!r->graph) {
+      akz_status st = ragged_enqueue_lists(r, cnt);
+      if (st != AKZ_OK) return st;
+    } else {
+      auto it = r->graphs.find(key);
+      if (it == r->graphs.end()) {
+        AKZ_HIP_TRY(hipStreamBeginCapture(r->stream, hipStreamCaptureModeThreadLocal));
+        for (int k = 0; k < r->ncls; k++) r->cls[k]->capturing = true;
+        akz_status st = ragged_enqueue_lists(r, cnt);
+        for (int k = 0; k < r->ncls; k++) r->cls[k]->capturing = false;
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(r->stream, &g);
+        if (st != AKZ_OK) {
+          if (g) hipGraphDestroy(g);
+          return st;
+        }
+        if (e != hipSuccess) return akz_hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
+        hipGraphExec_t ge = nullptr;
+        e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+          hipGraphDestroy(g);
+          return akz_hip_fail(e, "hipGraphInstantiate", __FILE__, __LINE__);
+        }
+        it = r->graphs.emplace(key, std::make_pair(g, ge)).first;
+      }
+      AKZ_HIP_TRY(hipGraphLaunch(it->second.second, r->stream));
+    }
+    AKZ_HIP_TRY(hipGetLastError());
+    AKZ_HIP_TRY(hipStreamSynchronize(r->stream));
+    for (int k = 0; k < r->ncls; k++) {
+      if (cnt[k] == 0) continue;
+      r->cls[k]->state = 3;
+      r->cls[k]->last_nz = cnt[k];
+      r->cls[k]->status_synced = 0;
+    }
+  } else {
+    // stage every image into its class context's input slot, then one batch
+    // call per class; the classes run concurrently on their own streams
+    for (int i = 0; i < n; i++) {
+      akz_ctx* c = r->cls[cls[i]];
+      const Level& L0 = c->lev[0];
+      const size_t sp = pitch_bytes && pitch_bytes[i] ? pitch_bytes[i] : sizeof(float) * widths[i];
+      AKZ_HIP_TRY(hipMemcpy2DAsync(zptr(c, c->img_in, slot[i]), sizeof(float) * L0.pitch, d_imgs[i], sp,
+                                   sizeof(float) * L0.w, L0.h, hipMemcpyDeviceToDevice, c->stream));
+    }
+    for (int k = 0; k < r->ncls; k++) {
+      if (cnt[k] == 0) continue;
+      akz_ctx* c = r->cls[k];
+      akz_status st = akz_process_batch_device_async(c, c->img_in, sizeof(float) * c->lev[0].pitch, c->zs, cnt[k]);
+      if (st != AKZ_OK) return st;
+    }
   }
   r->img_cls = cls;
   r->img_slot = slot;
@@ -1292,6 +1523,7 @@ akz_status akz_ragged_results_device(akz_ragged* r, int i, const int32_t** d_cou
 akz_status akz_ragged_set_graph_mode(akz_ragged* r, int enabled) {
   if (!r) return AKZ_ERR_ARG;
   for (int k = 0; k < r->ncls; k++) akz_set_graph_mode(r->cls[k], enabled);
+  r->graph = enabled != 0;
   return AKZ_OK;
 }
 

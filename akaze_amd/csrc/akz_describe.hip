@@ -40,23 +40,22 @@ constexpr int OR_WAVES = 4;
 // lane b (b < 42) sums bin b in sample order, so the float sums do not depend
 // on scheduling.  The strongest window is found by a wavefront maximum: the
 // first window reaching it, as the reference's sequential scan keeps.
-__global__ __launch_bounds__(64 * OR_WAVES) void k_orientation(akz_desc_params p, akz_kp* __restrict__ kps,
-                                                               const int32_t* nkps_p, OrientWeights wt) {
+// Keypoint kp of image z (kps: that image's keypoints), one wavefront per
+// keypoint: every wavefront of the block calls it once per round (active or
+// not), so the block's barriers are reached by all of them.  The wavefronts
+// of one block may work on different images (p is the wavefront's).
+__device__ __forceinline__ void orient_one(const akz_desc_params& p, akz_kp* __restrict__ kps, int kp, bool active,
+                                           const OrientWeights& wt, int z) {
   __shared__ int8_t s_bin[OR_WAVES][OR_NS + 3];
   __shared__ float s_dx[OR_WAVES][OR_NS + 3], s_dy[OR_WAVES][OR_NS + 3];
   __shared__ float s_rx[OR_WAVES][42], s_ry[OR_WAVES][42];
   __shared__ float s_ex[OR_WAVES][42], s_ey[OR_WAVES][42];
   const int lane = threadIdx.x & 63, wv = akz_wave();
-  kps = akz_zoff(kps, p.zs);
-  const int n = *akz_zoff(nkps_p, p.zs);
-  // the loop bound is uniform over the block so every barrier is reached by all waves
-  for (int base = blockIdx.x * OR_WAVES; base < n; base += gridDim.x * OR_WAVES) {
-    const int kp = base + wv;
-    const bool active = kp < n;
+  {
     if (active) {
       akz_kp k = kps[kp];
       const int lev = k.class_id;
-      const float2* lxy = akz_zoff(p.Lxy[lev], p.zs);
+      const float2* lxy = akz_zoff(p.Lxy[lev], p.zs, z);
       const int pitch = p.pitch[lev], w = p.w[lev], h = p.h[lev];
       const int octave = k.octave;
       const int step = (int)(0.5f * k.size + 0.5f) >> octave;
@@ -126,6 +125,72 @@ __global__ __launch_bounds__(64 * OR_WAVES) void k_orientation(akz_desc_params p
   }
 }
 
+// The keypoints of image z (grid stride over blocks bx of nb); every thread of
+// the block calls it with the same arguments.
+__device__ __forceinline__ void orient_image(const akz_desc_params& p, akz_kp* __restrict__ kps,
+                                             const int32_t* nkps_p, const OrientWeights& wt, int z, int bx, int nb) {
+  const int wv = akz_wave();
+  kps = akz_zoff(kps, p.zs, z);
+  const int n = *akz_zoff(nkps_p, p.zs, z);
+  // the loop bound is uniform over the block so every barrier is reached by all waves
+  for (int base = bx * OR_WAVES; base < n; base += nb * OR_WAVES) {
+    const int kp = base + wv;
+    orient_one(p, kps, kp, kp < n, wt, z);
+  }
+}
+
+__global__ __launch_bounds__(64 * OR_WAVES) void k_orientation(akz_desc_params p, akz_kp* __restrict__ kps,
+                                                               const int32_t* nkps_p, OrientWeights wt) {
+  orient_image(p, kps, nkps_p, wt, blockIdx.z, blockIdx.x, gridDim.x);
+}
+
+// Persistent form over a ragged batch: the keypoints of all images of the
+// work list form one flat list (image i's after the counts of images 0..i-1,
+// summed per block into LDS), walked with the grid stride.
+__device__ __forceinline__ int list_counts(const akz_desc_worklist& wl, int32_t* s_start) {
+  if (threadIdx.x < 64) {
+    const int i = threadIdx.x;
+    int v = 0;
+    if (i < wl.nimg) v = *akz_zoff(wl.nkps[wl.cls[i]], wl.P[wl.cls[i]].zs, wl.z[i]);
+    int inc = v;
+    for (int d = 1; d < 64; d <<= 1) {
+      const int o = __shfl_up(inc, d, 64);
+      if (i >= d) inc += o;
+    }
+    s_start[i] = inc - v;
+    if (i == 63) s_start[64] = inc;
+  }
+  __syncthreads();
+  return s_start[64];
+}
+
+__device__ __forceinline__ int list_image(const int32_t* s_start, int nimg, int item) {
+  int a = 0, e = nimg;
+  while (e - a > 1) {
+    const int m = (a + e) >> 1;
+    if (s_start[m] <= item) a = m; else e = m;
+  }
+  return a;
+}
+
+__global__ __launch_bounds__(64 * OR_WAVES) void k_orientation_list(akz_desc_worklist wl, OrientWeights wt) {
+  __shared__ int32_t s_start[65];
+  const int total = list_counts(wl, s_start);
+  const int wv = akz_wave();
+  // one wavefront per keypoint: wavefront wv of a block takes item base + wv
+  for (int base = blockIdx.x * OR_WAVES; base < total; base += gridDim.x * OR_WAVES) {
+    const int item = base + wv;
+    const bool active = item < total;
+    int i = 0, kp = 0;
+    if (active) {
+      i = __builtin_amdgcn_readfirstlane(list_image(s_start, wl.nimg, item));
+      kp = item - s_start[i];
+    }
+    const int c = wl.cls[i];
+    orient_one(wl.P[c], akz_zoff(wl.kps[c], wl.P[c].zs, wl.z[i]), kp, active, wt, wl.z[i]);
+  }
+}
+
 // MLDB descriptor (ExtractDescriptors + BuildDescriptor, cuda_akaze.cu:
 // 1420-1533, 1786-1807).  Lane l accumulates the samples i = l, l+64, ... of
 // the rotated sampling grid into its own partial sums of the 87 values (29
@@ -157,19 +222,27 @@ __device__ inline mldb_cells mldb_cell(int i, int size2, int size3, int size4, i
   return c;
 }
 
-__global__ __launch_bounds__(64 * MLDB_WAVES) void k_mldb(akz_desc_params p, const akz_kp* __restrict__ kps,
-                                                          const int32_t* nkps_p, uint8_t* __restrict__ desc) {
+// The descriptors of image z (grid stride over blocks bx of nb).
+// Items bx, bx + nb, ... < n: item_of(it) gives the item's image parameters,
+// keypoints and 64-byte descriptor rows (of its own image) and keypoint
+// index.  Pattern size and comparison list are p0's (the same for every
+// image of a call).
+struct mldb_item {
+  const akz_desc_params* p;
+  const akz_kp* kps;
+  uint8_t* desc;
+  int kp, z;
+};
+template <class ItemOf>
+__device__ __forceinline__ void mldb_run(const akz_desc_params& p0, int n, int bx, int nb, ItemOf item_of) {
   // 29 cells x 3 channels, plus cell 29: where a sample outside a grid's
   // cells adds (never read)
   __shared__ float acc[90 * MLDB_VS];
   __shared__ float vals[88];
   const int lane = threadIdx.x & 63;
-  kps = akz_zoff(kps, p.zs);
-  desc = akz_zoff(desc, p.zs);
-  const int n = *akz_zoff(nkps_p, p.zs);
-  const int size2 = p.patsize;
-  const int size3 = (int)ceilf(2.0f * p.patsize / 3.0f);
-  const int size4 = (int)ceilf(0.5f * p.patsize);
+  const int size2 = p0.patsize;
+  const int size3 = (int)ceilf(2.0f * p0.patsize / 3.0f);
+  const int size4 = (int)ceilf(0.5f * p0.patsize);
   const int winsize = akz_imax(3 * size3, 4 * size4);
   constexpr int MS = 7;  // samples per lane (m < 7 covers grids up to 21x21)
   // comparison pairs of the lane's output byte (the variant's list, entries
@@ -179,8 +252,8 @@ __global__ __launch_bounds__(64 * MLDB_WAVES) void k_mldb(akz_desc_params p, con
 #pragma unroll
   for (int b = 0; b < 8; b++) {
     const int c = lane < 61 ? lane * 8 + b : AKZ_MLDB_TABLE - 1;
-    ia[b] = p.cmp[c];
-    ib[b] = p.cmp[AKZ_MLDB_TABLE + c];
+    ia[b] = p0.cmp[c];
+    ib[b] = p0.cmp[AKZ_MLDB_TABLE + c];
   }
   // first accumulator row (3 x cell) of the lane's samples in each grid: the
   // same for every keypoint
@@ -194,11 +267,15 @@ __global__ __launch_bounds__(64 * MLDB_WAVES) void k_mldb(akz_desc_params p, con
     rb[1][m] = 3 * (cc.c3 < 0 ? 29 : cc.c3);
     rb[2][m] = 3 * (cc.c4 < 0 ? 29 : cc.c4);
   }
-  for (int kp = blockIdx.x; kp < n; kp += gridDim.x) {
-    const akz_kp k = kps[kp];
+  for (int it = bx; it < n; it += nb) {
+    const mldb_item itm = item_of(it);
+    const akz_desc_params& p = *itm.p;
+    const int kp = itm.kp, z = itm.z;
+    uint8_t* __restrict__ desc = itm.desc;
+    const akz_kp k = itm.kps[kp];
     const int lev = k.class_id;
-    const float* imd = akz_zoff(p.Lt[lev], p.zs);
-    const float2* lxy = akz_zoff(p.Lxy[lev], p.zs);
+    const float* imd = akz_zoff(p.Lt[lev], p.zs, z);
+    const float2* lxy = akz_zoff(p.Lxy[lev], p.zs, z);
     const int pitch = p.pitch[lev], w = p.w[lev], h = p.h[lev];
     const float iratio = 1.0f / (1 << k.octave);
     const int scale = (int)(0.5f * k.size * iratio + 0.5f);
@@ -281,6 +358,29 @@ __global__ __launch_bounds__(64 * MLDB_WAVES) void k_mldb(akz_desc_params p, con
   }
 }
 
+__global__ __launch_bounds__(64 * MLDB_WAVES) void k_mldb(akz_desc_params p, const akz_kp* __restrict__ kps,
+                                                          const int32_t* nkps_p, uint8_t* __restrict__ desc) {
+  const int z = blockIdx.z;
+  const akz_kp* kz = akz_zoff(kps, p.zs, z);
+  uint8_t* dz = akz_zoff(desc, p.zs, z);
+  const int n = *akz_zoff(nkps_p, p.zs, z);
+  mldb_run(p, n, blockIdx.x, gridDim.x, [&](int it) { return mldb_item{&p, kz, dz, it, z}; });
+}
+
+// Persistent form over a ragged batch: one flat list of the keypoints of all
+// images (see k_orientation_list).
+__global__ __launch_bounds__(64 * MLDB_WAVES) void k_mldb_list(akz_desc_worklist wl) {
+  __shared__ int32_t s_start[65];
+  const int total = list_counts(wl, s_start);
+  mldb_run(wl.P[wl.cls[0]], total, blockIdx.x, gridDim.x, [&](int it) {
+    const int i = __builtin_amdgcn_readfirstlane(list_image(s_start, wl.nimg, it));
+    const int c = __builtin_amdgcn_readfirstlane(wl.cls[i]);
+    const akz_desc_params* p = &wl.P[c];
+    return mldb_item{p, akz_zoff(wl.kps[c], p->zs, wl.z[i]), akz_zoff(wl.desc[c], p->zs, wl.z[i]),
+                     it - s_start[i], wl.z[i]};
+  });
+}
+
 OrientWeights g_weights;
 bool g_weights_ready = false;
 
@@ -306,6 +406,13 @@ void akz_launch_orientation(hipStream_t s, const akz_desc_params& p, akz_kp* kps
   int blocks = (cap + OR_WAVES - 1) / OR_WAVES;
   blocks = std::min(std::max(blocks, 1), per_image_blocks(2048, nz, 256));
   hipLaunchKernelGGL(k_orientation, dim3(blocks, 1, nz), dim3(64 * OR_WAVES), 0, s, p, kps, nkps, g_weights);
+}
+
+void akz_launch_describe_list(hipStream_t s, const akz_desc_worklist& wl, bool orientation, int or_blocks,
+                              int mldb_blocks) {
+  if (orientation)
+    hipLaunchKernelGGL(k_orientation_list, dim3(or_blocks), dim3(64 * OR_WAVES), 0, s, wl, g_weights);
+  hipLaunchKernelGGL(k_mldb_list, dim3(mldb_blocks), dim3(64 * MLDB_WAVES), 0, s, wl);
 }
 
 void akz_launch_mldb(hipStream_t s, const akz_desc_params& p, const akz_kp* kps, const int32_t* nkps, uint8_t* desc64,

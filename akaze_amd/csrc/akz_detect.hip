@@ -183,17 +183,19 @@ __device__ __forceinline__ void detect_interior(const float* s_l, float* s_x, fl
   for (int k = threadIdx.x; k < DF_RD * 2; k += 64 * NW) det(k % DF_RD, 64 + k / DF_RD);
 }
 
-__global__ __launch_bounds__(512) void k_detect_fused(akz_det_params p, int32_t* __restrict__ seg_count,
-                                                      uint8_t* __restrict__ seg_slots) {
-  seg_count = akz_zoff(seg_count, p.zs);
-  seg_slots = akz_zoff(seg_slots, p.zs);
+// One 64 x DF_TH tile `bx` of image `z` (tiles numbered level-major through
+// p.tile_start); the block's 512 threads all call it, with the same bx and z.
+__device__ __forceinline__ void detect_tile(const akz_det_params& p, int bx, int z, int32_t* __restrict__ seg_count,
+                                            uint8_t* __restrict__ seg_slots) {
+  seg_count = akz_zoff(seg_count, p.zs, z);
+  seg_slots = akz_zoff(seg_slots, p.zs, z);
   __shared__ float s_l[DF_R1 * DF_RL];
   __shared__ float s_x[DF_R2 * DF_RX];
   __shared__ float s_y[DF_R2 * DF_RX];
   float* const s_d = s_l;  // DF_RD <= DF_R1 rows
-  const int l = find_level(p, blockIdx.x);
+  const int l = find_level(p, bx);
   const akz_level_desc& L = p.lev[l];
-  const int t = blockIdx.x - p.tile_start[l];
+  const int t = bx - p.tile_start[l];
   const int tx = t % L.nseg_x;
   const int x0 = tx * 64, y0 = (t / L.nseg_x) * DF_TH;
   const int w = L.w, h = L.h, pitch = L.pitch, s = L.deriv_step;
@@ -201,7 +203,7 @@ __global__ __launch_bounds__(512) void k_detect_fused(akz_det_params p, int32_t*
   const int W1 = 64 + 2 * H1, W2 = 64 + 2 * H2;
   const int lane = threadIdx.x & 63, wv = akz_wave();
   constexpr int NW = 8;
-  const float* lsm = akz_zoff(p.Lsm[l], p.zs);
+  const float* lsm = akz_zoff(p.Lsm[l], p.zs, z);
   const bool interior = x0 >= H1 && x0 + 64 + H1 <= w && y0 >= H1 && y0 + DF_TH + H1 <= h;
   // Lsmooth rows [y0-H1, y0+TH+H1), columns [x0-H1, x0+64+H1): every load of
   // a thread is issued before any is written to LDS
@@ -302,8 +304,8 @@ __global__ __launch_bounds__(512) void k_detect_fused(akz_det_params p, int32_t*
   }
   __syncthreads();
   const int x = x0 + lane;
-  float2* Lxy = akz_zoff(p.Lxy[l], p.zs);
-  float* D = akz_zoff(p.det[l], p.zs);
+  float2* Lxy = akz_zoff(p.Lxy[l], p.zs, z);
+  float* D = akz_zoff(p.det[l], p.zs, z);
   // the detector border test splits into a column part and a row part
   const bool col_in = (int)(x - L.border + 0.5f) - 1 >= 0 && (int)(x + L.border + 0.5f) + 1 < w;
   for (int r = wv; r < DF_TH; r += NW) {
@@ -379,9 +381,38 @@ __global__ __launch_bounds__(512) void k_detect_fused(akz_det_params p, int32_t*
       for (int q = 0; q < NW; q++) mm = fmaxf(mm, s_tm[q][k][c]);
       const int pty = pr0 + k, ptx = x0 / AKZ_PROP_TILE + c;
       if (mm > -INFINITY && pty < p.tile_h[l] && ptx < p.tile_w[l])
-        atomicMax(p.tile_key + (size_t)blockIdx.z * p.tiles_per_image + p.tile_base[l] + pty * p.tile_w[l] + ptx,
+        atomicMax(p.tile_key + (size_t)z * p.tiles_per_image + p.tile_base[l] + pty * p.tile_w[l] + ptx,
                   akz_det_key(mm));
     }
+  }
+}
+
+__global__ __launch_bounds__(512) void k_detect_fused(akz_det_params p, int32_t* __restrict__ seg_count,
+                                                      uint8_t* __restrict__ seg_slots) {
+  detect_tile(p, blockIdx.x, blockIdx.z, seg_count, seg_slots);
+}
+
+// Persistent form over a ragged batch: the tiles of every image of every
+// size class form one work list (image i owns items [start[i], start[i+1]),
+// its level-major tiles in order); each block walks the list with the grid
+// stride and runs the tile with its class's parameters.
+// (eight wavefronts per SIMD, as the grid form reaches: four 40 KB blocks per CU)
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k_detect_fused_list(
+    akz_det_worklist wl) {
+  const int total = wl.start[wl.nimg];
+  for (int item = blockIdx.x; item < total; item += gridDim.x) {
+    int a = 0, e = wl.nimg;  // last image with start <= item
+    while (e - a > 1) {
+      const int m = (a + e) >> 1;
+      if (wl.start[m] <= item) a = m; else e = m;
+    }
+    // the item is the block's: keep its decode in scalar registers
+    const int ia = __builtin_amdgcn_readfirstlane(a);
+    const int c = __builtin_amdgcn_readfirstlane(wl.cls[ia]);
+    const int bx = __builtin_amdgcn_readfirstlane(item - wl.start[ia]);
+    const int z = __builtin_amdgcn_readfirstlane(wl.z[ia]);
+    detect_tile(wl.P[c], bx, z, wl.seg_count[c], wl.seg_slots[c]);
+    __syncthreads();  // the next tile's loads overwrite this one's LDS
   }
 }
 
@@ -1076,6 +1107,10 @@ int akz_detect_tile_rows() { return DF_TH; }
 void akz_launch_detect_fused(hipStream_t s, const akz_det_params& p32, const akz_det_bufs& b, int nz) {
   hipLaunchKernelGGL(k_detect_fused, dim3(p32.tile_start[p32.nlev], 1, nz), dim3(512), 0, s, p32, b.seg_count,
                      b.seg_slots);
+}
+
+void akz_launch_detect_fused_list(hipStream_t s, const akz_det_worklist& wl, int blocks) {
+  hipLaunchKernelGGL(k_detect_fused_list, dim3(blocks), dim3(512), 0, s, wl);
 }
 
 void akz_launch_extrema(hipStream_t s, const akz_det_params& p, const akz_det_bufs& b, int nz) {

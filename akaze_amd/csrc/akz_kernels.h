@@ -4,6 +4,7 @@
 #include "akz_common.h"
 
 struct akz_taps { float k[11]; };
+constexpr int AKZ_MAX_CLASSES_DEV = 16;  // size classes of a ragged batch (AKZ_MAX_CLASSES)
 
 // ---- scale space (akz_scale_space.hip) ----
 // Every launcher takes the batch (akz_batch): pointers name image 0's arrays,
@@ -111,6 +112,18 @@ void akz_launch_extrema(hipStream_t s, const akz_det_params& p, const akz_det_bu
 int akz_detect_max_step();
 int akz_detect_tile_rows();
 void akz_launch_detect_fused(hipStream_t s, const akz_det_params& p32, const akz_det_bufs& b, int nz);
+// Work list of a ragged batch for the persistent detector: image i is image
+// z[i] of size class cls[i], whose tiles are items start[i] .. start[i+1]-1;
+// P points to the classes' fused-detector parameters in device memory.
+constexpr int AKZ_LIST_IMAGES = 64;
+struct akz_det_worklist {
+  int nimg;
+  int cls[AKZ_LIST_IMAGES], z[AKZ_LIST_IMAGES], start[AKZ_LIST_IMAGES + 1];
+  const akz_det_params* P;
+  int32_t* seg_count[AKZ_MAX_CLASSES_DEV];
+  uint8_t* seg_slots[AKZ_MAX_CLASSES_DEV];
+};
+void akz_launch_detect_fused_list(hipStream_t s, const akz_det_worklist& wl, int blocks);
 // segment scan, level starts, status reset and candidate materialisation
 void akz_launch_candidates(hipStream_t s, const akz_det_params& p, const akz_det_bufs& b, int nz);
 // CSR neighbour lists and the wave-synchronous suppression with its output
@@ -192,6 +205,19 @@ void akz_launch_orientation(hipStream_t s, const akz_desc_params& p, akz_kp* kps
                             int nz);
 void akz_launch_mldb(hipStream_t s, const akz_desc_params& p, const akz_kp* kps, const int32_t* nkps,
                      uint8_t* desc64, int cap, int nz);
+// Work list of a ragged batch for the persistent orientation and MLDB
+// kernels: image i is image z[i] of class cls[i]; P points to the classes'
+// descriptor parameters in device memory, the per-class arrays are image 0's.
+struct akz_desc_worklist {
+  int nimg;
+  int cls[AKZ_LIST_IMAGES], z[AKZ_LIST_IMAGES];
+  const akz_desc_params* P;
+  akz_kp* kps[AKZ_MAX_CLASSES_DEV];
+  const int32_t* nkps[AKZ_MAX_CLASSES_DEV];
+  uint8_t* desc[AKZ_MAX_CLASSES_DEV];
+};
+void akz_launch_describe_list(hipStream_t s, const akz_desc_worklist& wl, bool orientation, int or_blocks,
+                              int mldb_blocks);
 
 // ---- matcher (akz_match.hip) ----
 int akz_match_splits(int nq, int nt);

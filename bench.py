@@ -189,6 +189,7 @@ def ragged_workload(A, args):
     rg = A.Ragged(A.AKAZEOptions(), RAGGED_SIZES, caps)
     if args.autotune:
         rg.autotune()
+    rg.set_work_lists(args.work_lists)
     rg.set_graph_mode(True)
     sizes = [RAGGED_SIZES[k] for k in picks]
     bufs = [device_image(A, 300 + i, w, h) for i, (w, h) in enumerate(sizes)]
@@ -224,7 +225,7 @@ def ragged_workload(A, args):
             "config": {"workload": "akaze_ragged64_5sizes_detect_mldb_match", "images_per_step": 64,
                        "sizes": ["%dx%d" % s for s in RAGGED_SIZES], "class_counts": caps,
                        "megapixels_per_step": round(mpix, 2), "match_pairs_per_step": npairs[0],
-                       "autotuned": bool(args.autotune), "graph": True}}
+                       "autotuned": bool(args.autotune), "graph": True, "work_lists": bool(args.work_lists)}}
 
 
 def drift_homographies(rng, w, h, n):
@@ -270,6 +271,8 @@ def streams_workload(A, args):
         frames.append(fr)
     rg = A.Ragged(A.AKAZEOptions(), classes, [8, 8])
     audit = A.Ragged(A.AKAZEOptions(), classes, [2, 2])
+    rg.set_work_lists(args.work_lists)
+    audit.set_work_lists(args.work_lists)
     for k in range(2):
         c = ctypes.c_void_p()
         A._check(A.lib().akz_ragged_context(rg.r, k, ctypes.byref(c)), "akz_ragged_context")
@@ -434,6 +437,9 @@ def main():
                     help="plain: the headline 1080p single-image line; the others measure the ragged-batch, "
                          "camera-stream and matcher configurations")
     ap.add_argument("--frames", type=int, default=64, help="frames per stream (streams workload)")
+    ap.add_argument("--work-lists", type=int, default=0,
+                    help="ragged / streams workloads: persistent work-list kernels over the whole call (1) or one "
+                         "batch call per size class on concurrent streams (0, faster on MI355X)")
     ap.add_argument("--autotune", action="store_true", help="autotune FED kernels first")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 256 MiB device copy kernel first (byte-counter calibration for profiles)")

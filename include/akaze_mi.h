@@ -193,6 +193,16 @@ akz_status akz_ragged_destroy(akz_ragged* r);
 /* The batch context of class `cls` (owned by r). */
 akz_status akz_ragged_context(akz_ragged* r, int cls, akz_ctx** ctx);
 akz_status akz_ragged_set_graph_mode(akz_ragged* r, int enabled);
+/* Work lists (1): the scale space runs per class on concurrent streams, then
+ * the detector, the orientation and the MLDB stages each run as ONE
+ * persistent launch over all images of all classes, walking
+ * (image, level, tile) and (image, keypoint) work lists; the candidate and
+ * suppression stages run per class in between.  In graph mode the whole call
+ * is one captured graph per class composition (images per class).
+ * 0 (the default): one batch call per class (each class's own graph), the
+ * classes on concurrent streams, which overlap one class's scale space with
+ * another's detection.  Results are identical either way. */
+akz_status akz_ragged_set_work_lists(akz_ragged* r, int enabled);
 akz_status akz_ragged_autotune(akz_ragged* r, int reps);
 /* Processes n device images (sizes widths[i] x heights[i], each one of the
  * classes; pitch 0 = dense) and waits; counts[i] receives image i's kept

@@ -119,15 +119,24 @@ def test_ragged_five_size_classes_match_oracle(gpu_lib):
     ref = [oracle_run(im.shape[1], im.shape[0], im) for im in imgs]
     caps = [order.count(k) for k in range(len(sizes))]
     rg = A.Ragged(A.AKAZEOptions(), sizes, caps)
-    for graph in (False, True):
-        rg.set_graph_mode(graph)
-        for rep in range(2 if graph else 1):
-            print("ragged 5 classes: graph %s rep %d" % (graph, rep), flush=True)
-            res = rg.process(imgs)
-            for i, (kps, desc) in enumerate(res):
-                assert len(kps) > 0
-                assert kps.tobytes() == ref[i][0].tobytes(), (graph, rep, i, sizes[order[i]])
-                assert np.array_equal(desc, ref[i][1]), (graph, rep, i, sizes[order[i]])
+    # per-class batch calls, then the persistent work-list kernels over the
+    # whole call (eager, then one captured graph replayed), then a different
+    # class composition (a subset of the images) from a second graph
+    for lists in (False, True):
+        rg.set_work_lists(lists)
+        for graph in (False, True):
+            rg.set_graph_mode(graph)
+            for rep in range(2 if graph else 1):
+                print("ragged 5 classes: work lists %s graph %s rep %d" % (lists, graph, rep), flush=True)
+                res = rg.process(imgs)
+                for i, (kps, desc) in enumerate(res):
+                    assert len(kps) > 0
+                    assert kps.tobytes() == ref[i][0].tobytes(), (lists, graph, rep, i, sizes[order[i]])
+                    assert np.array_equal(desc, ref[i][1]), (lists, graph, rep, i, sizes[order[i]])
+        sub = [0, 2, 3, 6]
+        res = rg.process([imgs[i] for i in sub])
+        for j, i in enumerate(sub):
+            assert res[j][0].tobytes() == ref[i][0].tobytes() and np.array_equal(res[j][1], ref[i][1]), (lists, i)
 
 
 def test_every_fed_variant_and_the_autotuner(gpu_lib):

@@ -38,7 +38,8 @@ def drift(rng, w, h, n):
     return Hs
 
 
-def test_camera_streams_with_proposal_audit_and_matching(gpu_lib):
+@pytest.mark.parametrize("work_lists", [False, True])
+def test_camera_streams_with_proposal_audit_and_matching(gpu_lib, work_lists):
     A = gpu_lib
     from akaze_amd.proposal import ProposalNet, keypoint_recall
     rec = json.load(open(os.path.join(REPO, "akaze_amd", "proposal_1080p.json")))
@@ -64,6 +65,8 @@ def test_camera_streams_with_proposal_audit_and_matching(gpu_lib):
         c = ctypes.c_void_p()
         A._check(A.lib().akz_ragged_context(rg.r, k, ctypes.byref(c)), "akz_ragged_context")
         A._check(A.lib().akz_set_proposal(c, net.net, ctypes.c_float(rec["threshold"])), "akz_set_proposal")
+    rg.set_work_lists(work_lists)
+    audit.set_work_lists(work_lists)
     rg.set_graph_mode(True)
     audit.set_graph_mode(True)
     m = A.Matcher()
