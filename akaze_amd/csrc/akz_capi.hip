@@ -290,6 +290,8 @@ akz_status allocate(akz_ctx* c) {
     akz_level_desc& D = c->det.lev[i];
     D.w = L.w; D.h = L.h; D.pitch = L.pitch; D.octave = L.octave; D.sublevel = L.sublevel;
     D.deriv_step = L.deriv_step; D.border = L.border; D.det_size = L.det_size;
+    akz_border_interval(L.border, L.w, &D.xlo, &D.xhi);
+    akz_border_interval(L.border, L.h, &D.ylo, &D.yhi);
     D.nseg_x = (L.w + 63) / 64;
     D.seg_base = seg_total;
     seg_total += D.nseg_x * L.h;

@@ -38,7 +38,25 @@ struct akz_level_desc {
   float border, det_size;   // FindExtrema border and size (AKAZE.cpp:353-367)
   int seg_base;             // offset of this level's row-segment counts
   int nseg_x;               // row segments per row: ceil(w / 64)
+  // the coordinates FindExtrema's border test accepts, (int)(c - border + 0.5f)
+  // - 1 >= 0 && (int)(c + border + 0.5f) + 1 < size, as intervals [lo, hi)
+  // evaluated once on the host with the same float expression
+  int xlo, xhi, ylo, yhi;
 };
+
+// The interval of coordinates c in [0, size) passing FindExtrema's border
+// test (cuda_akaze.cu:749-752); the test is monotone on each side.
+inline void akz_border_interval(float border, int size, int* lo, int* hi) {
+  auto in = [&](int c) {
+    return (int)((float)c - border + 0.5f) - 1 >= 0 && (int)((float)c + border + 0.5f) + 1 < size;
+  };
+  int a = 0;
+  while (a < size && !in(a)) a++;
+  int b = a;
+  while (b < size && in(b)) b++;
+  *lo = a;
+  *hi = b;
+}
 
 // Status block of one image (read back with one copy): kept keypoints,
 // overflow flags, candidates before the capacity cut, suppression waves.

@@ -306,15 +306,16 @@ __device__ __forceinline__ void detect_tile(const akz_det_params& p, int bx, int
   const int x = x0 + lane;
   float2* Lxy = akz_zoff(p.Lxy[l], p.zs, z);
   float* D = akz_zoff(p.det[l], p.zs, z);
-  // the detector border test splits into a column part and a row part
-  const bool col_in = (int)(x - L.border + 0.5f) - 1 >= 0 && (int)(x + L.border + 0.5f) + 1 < w;
+  // the detector border test splits into a column part and a row part,
+  // each an interval precomputed on the host
+  const bool col_in = x >= L.xlo && x < L.xhi;
   for (int r = wv; r < DF_TH; r += NW) {
     const int y = y0 + r;
     if (y >= h) break;
     bool pred = false;
     const int id = (r + 1) * DF_RL + lane + 1;
     const size_t o = (size_t)y * pitch + x;
-    const bool row_in = (int)(y - L.border + 0.5f) - 1 >= 0 && (int)(y + L.border + 0.5f) + 1 < h;
+    const bool row_in = y >= L.ylo && y < L.yhi;
     if (x < w) {
       const int i2 = (r + H2) * DF_RX + lane + H2;
       Lxy[o] = make_float2(s_x[i2], s_y[i2]);
