@@ -534,6 +534,51 @@ __global__ __launch_bounds__(NT) void k_fed_multi_sf(const float* __restrict__ i
 // copy of an old value); both kernels that use them read that lane's result
 // only in their halo, which never reaches an output.
 
+// The steps of one launch on a register region whose rows 0..R-1 are
+// A[OFF..OFF+R-1]; rows 1..R-2 are updated.  Each flux is formed once and
+// serves both pixels it joins: a pixel's west flux is its west neighbour's
+// east flux, Wh(x-1) * (L(x) - L(x-1)), the same product taken by a DPP
+// shift, and its north flux is the south flux of the row above, kept from
+// that row (formed from the row's values before this step, like every
+// operand of the step).  The update is k_fed_step's expression in its
+// order, so the values are those of forming all four fluxes per pixel.  At
+// image borders the outward flux is +0, what NLDStep's clamped neighbour
+// (the pixel itself) gives for a finite conductance.
+// The empty asm makes ytop opaque in every step, so the per-row border tests
+// stay scalar compares inside the loop instead of R hoisted 64-bit masks
+// (which spill).
+template <int R, int OFF, bool BORDER, int NA>
+__device__ __forceinline__ void fed_reg_steps(float (&A)[NA], const float (&Wh)[R], const float (&Wv)[R],
+                                              const akz_fed_steps& st, int gx, int w, int y0, int h) {
+  const bool east_self = BORDER && gx >= w - 1, west_self = BORDER && gx <= 0;
+  for (int s = 0; s < st.n; s++) {
+    const float stepsize = st.half_tau[s];
+    int ytop = y0;
+    if (BORDER) asm volatile("" : "+s"(ytop));
+    // south flux of row 0: the north flux of row 1
+    float yup = Wv[0] * (A[OFF + 1] - A[OFF]);
+#pragma unroll
+    for (int r = 1; r < R - 1; r++) {
+      const float i0 = A[OFF + r];
+      float le = from_right(i0), ls = A[OFF + r + 1];
+      if (BORDER) {
+        if (east_self) le = i0;
+        if (ytop + r >= h - 1) ls = i0;
+      }
+      const float xpos = Wh[r] * (le - i0);
+      float xneg = from_left(xpos);
+      const float ypos = Wv[r] * (ls - i0);
+      float yneg = yup;
+      if (BORDER) {
+        if (west_self) xneg = 0.0f;
+        if (ytop + r <= 0) yneg = 0.0f;
+      }
+      yup = ypos;
+      A[OFF + r] = i0 + stepsize * (xpos - xneg + ypos - yneg);
+    }
+  }
+}
+
 template <int R, bool BORDER>
 __device__ __forceinline__ void fed_reg_body(const float* __restrict__ L, const float* __restrict__ g,
                                              float* __restrict__ out, int pitch, int w, int h,
@@ -552,35 +597,7 @@ __device__ __forceinline__ void fed_reg_body(const float* __restrict__ L, const 
   for (int r = 0; r < R; r++) Wh[r] = Wv[r] + from_right(Wv[r]);
 #pragma unroll
   for (int r = 0; r < R - 1; r++) Wv[r] = Wv[r] + Wv[r + 1];
-  const bool east_self = BORDER && gx >= w - 1, west_self = BORDER && gx <= 0;
-  for (int s = 0; s < st.n; s++) {
-    const float stepsize = st.half_tau[s];
-    // the empty asm makes ytop opaque in every step, so the per-row border
-    // tests stay scalar compares inside the loop instead of R hoisted
-    // 64-bit masks (which spill)
-    int ytop = y0;
-    if (BORDER) asm volatile("" : "+s"(ytop));
-    float up = Lr[0];
-#pragma unroll
-    for (int r = 1; r < R - 1; r++) {
-      const float i0 = Lr[r];
-      float le = from_right(i0), lw = from_left(i0);
-      float ln = up, ls = Lr[r + 1];
-      if (BORDER) {
-        if (east_self) le = i0;
-        if (west_self) lw = i0;
-        if (ytop + r <= 0) ln = i0;
-        if (ytop + r >= h - 1) ls = i0;
-      }
-      const float ww = from_left(Wh[r]);
-      const float xpos = Wh[r] * (le - i0);
-      const float xneg = ww * (i0 - lw);
-      const float ypos = Wv[r] * (ls - i0);
-      const float yneg = Wv[r - 1] * (i0 - ln);
-      up = i0;
-      Lr[r] = i0 + stepsize * (xpos - xneg + ypos - yneg);
-    }
-  }
+  fed_reg_steps<R, 0, BORDER>(Lr, Wh, Wv, st, gx, w, y0, h);
   const int hn = st.n;
   if (lane < hn || lane >= 64 - hn || gx >= w) return;
 #pragma unroll
@@ -803,32 +820,7 @@ __device__ __forceinline__ void fed_reg_sf_body(const float* __restrict__ in, fl
   for (int r = 0; r < R; r++) Wh[r] = Wv[r] + from_right(Wv[r]);
 #pragma unroll
   for (int r = 0; r < R - 1; r++) Wv[r] = Wv[r] + Wv[r + 1];
-  const bool east_self = BORDER && gx >= w - 1, west_self = BORDER && gx <= 0;
-  for (int s = 0; s < st.n; s++) {
-    const float stepsize = st.half_tau[s];
-    int ytop = y0;
-    if (BORDER) asm volatile("" : "+s"(ytop));
-    float up = A[3];
-#pragma unroll
-    for (int r = 1; r < R - 1; r++) {
-      const float i0 = A[r + 3];
-      float le = from_right(i0), lw = from_left(i0);
-      float ln = up, ls = A[r + 4];
-      if (BORDER) {
-        if (east_self) le = i0;
-        if (west_self) lw = i0;
-        if (ytop + r <= 0) ln = i0;
-        if (ytop + r >= h - 1) ls = i0;
-      }
-      const float ww = from_left(Wh[r]);
-      const float xpos = Wh[r] * (le - i0);
-      const float xneg = ww * (i0 - lw);
-      const float ypos = Wv[r] * (ls - i0);
-      const float yneg = Wv[r - 1] * (i0 - ln);
-      up = i0;
-      A[r + 3] = i0 + stepsize * (xpos - xneg + ypos - yneg);
-    }
-  }
+  fed_reg_steps<R, 3, BORDER>(A, Wh, Wv, st, gx, w, y0, h);
   if (!out_col) return;
 #pragma unroll
   for (int r = 1; r < R - 1; r++) {
