@@ -430,6 +430,17 @@ int level_block(const akz_ctx* c, int i) {
   return c->fed_block == 1 ? 1 : std::min(c->fed_block, akz_fed_variant_steps(c->fed_variant[i]));
 }
 
+// Whether level i's first FED launch forms the level's input, an exact
+// HalfSample2 of the previous level, while loading it (fused launches of the
+// LDS variants at an octave's first level).
+bool level_half_in_fed(const akz_ctx* c, int i) {
+  const Level& L = c->lev[i];
+  const Level& P = c->lev[i - 1];
+  const int bs = level_block(c, i), n = L.nsteps;
+  const bool fuse = bs > 1 && n > 0 && akz_fed_variant_fusable(c->fed_variant[i], std::min(bs, n));
+  return fuse && L.octave > P.octave && P.w == 2 * L.w && P.h == 2 * L.h && akz_fed_variant_halves(c->fed_variant[i]);
+}
+
 // Create_Nonlinear_Scale_Space (AKAZE.cpp:211-276) on the context stream.
 inline void record(akz_ctx* c, int e) {
   if (e == EV_START) c->ev_mask = 0;
@@ -460,8 +471,17 @@ void enqueue_scale_space(akz_ctx* c, const float* img, int in_pitch, size_t in_z
     const int bs = level_block(c, i);
     const int nl = (n + bs - 1) / bs;
     float* first_dst = (nl % 2 == 1) ? L.Lt : c->fed_tmp;
+    // the level's first FED launch builds Lsmooth and g where that pays
+    // (akz_fed_variant_fusable); g goes through HBM only when further
+    // launches of the level read it
+    const bool fuse = bs > 1 && n > 0 && akz_fed_variant_fusable(c->fed_variant[i], std::min(bs, n));
+    // an exact halving at an octave's first level is formed inside that
+    // fused launch's load where the variant can (the LDS ones)
+    const bool half_in_fed = level_half_in_fed(c, i);
     const float* base;
-    if (L.octave > P.octave) {
+    if (half_in_fed) {
+      base = nullptr;
+    } else if (L.octave > P.octave) {
       float* hs = (nl % 2 == 1) ? c->fed_tmp : L.Lt;  // must differ from first_dst
       if (n == 0) hs = L.Lt;
       akz_launch_halfsample(s, P.Lt, P.w, P.h, P.pitch, hs, L.w, L.h, L.pitch, bt);
@@ -470,10 +490,6 @@ void enqueue_scale_space(akz_ctx* c, const float* img, int in_pitch, size_t in_z
       base = P.Lt;  // Copy (AKAZE.cpp:258) is folded into the first step's read
       if (n == 0) akz_launch_copy2d(s, P.Lt, P.pitch, L.Lt, L.pitch, L.w, L.h, bt);
     }
-    // the level's first FED launch builds Lsmooth and g where that pays
-    // (akz_fed_variant_fusable); g goes through HBM only when further
-    // launches of the level read it
-    const bool fuse = bs > 1 && n > 0 && akz_fed_variant_fusable(c->fed_variant[i], std::min(bs, n));
     if (!fuse)
       akz_launch_smooth_flow(s, base, L.pitch, L.Lsm, c->g, L.w, L.h, c->opts.diffusivity, c->kc + i, c->taps1, bt);
     const float* cur = base;
@@ -486,8 +502,9 @@ void enqueue_scale_space(akz_ctx* c, const float* img, int in_pitch, size_t in_z
       if (bs == 1)
         akz_launch_fed_step(s, cur, c->g, dst, L.pitch, L.w, L.h, ht[0], bt);
       else if (fuse && j == 0)
-        akz_launch_fed_variant_sf(s, c->fed_variant[i], cur, L.Lsm, nl > 1 ? c->g : nullptr, dst, L.pitch, L.w, L.h,
-                                  c->opts.diffusivity, c->kc + i, c->taps1, ht, k, bt);
+        akz_launch_fed_variant_sf(s, c->fed_variant[i], cur, half_in_fed ? P.Lt : nullptr, P.pitch, L.Lsm,
+                                  nl > 1 ? c->g : nullptr, dst, L.pitch, L.w, L.h, c->opts.diffusivity, c->kc + i,
+                                  c->taps1, ht, k, bt);
       else
         akz_launch_fed_variant(s, c->fed_variant[i], cur, c->g, dst, L.pitch, L.w, L.h, ht, k, bt);
       cur = dst;
@@ -1086,6 +1103,9 @@ akz_status akz_profile_fed_batch(akz_ctx* c, const float* d_imgs, size_t pitch_b
     // and writes Lsmooth, L and (when later launches read it) g
     least += 12.0 * px * nl;
     if (fused && nl > 1) least += 4.0 * px;
+    // a launch that halves the previous level reads it (16 B per pixel of
+    // the level) instead of the halved image
+    if (level_half_in_fed(c, i)) least += 12.0 * px;
     launches += nl;
   }
   if (fed_ms) *fed_ms = total;
@@ -1198,9 +1218,9 @@ akz_status akz_autotune(akz_ctx* c, int nz, int reps) {
           const float* src = even ? L.Lt : c->fed_tmp;
           float* dst = even ? c->fed_tmp : L.Lt;
           if (fuse && j == 0)
-            akz_launch_fed_variant_sf(c->stream, v, src, L.Lsm, nl > 1 ? c->g : nullptr, dst, L.pitch, L.w, L.h,
-                                      c->opts.diffusivity, c->kc + i, c->taps1, ht + j, std::min(bs, L.nsteps - j),
-                                      bt);
+            akz_launch_fed_variant_sf(c->stream, v, src, nullptr, 0, L.Lsm, nl > 1 ? c->g : nullptr, dst, L.pitch,
+                                      L.w, L.h, c->opts.diffusivity, c->kc + i, c->taps1, ht + j,
+                                      std::min(bs, L.nsteps - j), bt);
           else
             akz_launch_fed_variant(c->stream, v, src, c->g, dst, L.pitch, L.w, L.h, ht + j,
                                    std::min(bs, L.nsteps - j), bt);

@@ -48,9 +48,13 @@ void akz_launch_fed_variant(hipStream_t s, int variant, const float* Lin, const 
 // launch instead of two; every LDS variant does, the register variants when
 // the region's halo is small
 bool akz_fed_variant_fusable(int variant, int n);
-void akz_launch_fed_variant_sf(hipStream_t s, int variant, const float* in, float* lsm, float* gout, float* Lout,
-                               int pitch, int w, int h, int type, const float* kc, const akz_taps& taps,
-                               const float* half_tau, int n, akz_batch bt);
+// With `parent` non-null (only for variants where akz_fed_variant_halves
+// holds; `in` is then unused), the input is HalfSample2 of `parent`, an
+// image of exactly twice the level's width and height with row pitch ppitch.
+bool akz_fed_variant_halves(int variant);
+void akz_launch_fed_variant_sf(hipStream_t s, int variant, const float* in, const float* parent, int ppitch,
+                               float* lsm, float* gout, float* Lout, int pitch, int w, int h, int type,
+                               const float* kc, const akz_taps& taps, const float* half_tau, int n, akz_batch bt);
 // every float x in [1, +inf]: akz_rcp_ge1(x) against 1.0f / x, differing
 // results added to *mismatches
 void akz_launch_rcp_check(hipStream_t s, unsigned long long* mismatches);

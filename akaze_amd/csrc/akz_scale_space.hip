@@ -32,6 +32,34 @@ __device__ __forceinline__ void load_window(float* __restrict__ dst, const float
   }
 }
 
+// load_window of the image HalfSample2 makes of `src`, a parent image of
+// exactly twice the size (cuda_akaze.cu:432-440): each element is the 2x2
+// box mean of the parent at twice its clamped coordinates, in k_half2's
+// expression, so the window equals a load of k_half2's output bit for bit.
+template <int AH, int AW, int NT>
+__device__ __forceinline__ void load_window_half(float* __restrict__ dst, const float* __restrict__ src, int ppitch,
+                                                 int x0, int y0, int w, int h) {
+  constexpr int PER = (AH * AW + NT - 1) / NT;
+  float v[PER][4];
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const int idx = threadIdx.x + k * NT;
+    if (idx < AH * AW) {
+      const int r = idx / AW, c = idx - r * AW;
+      const float* p = src + (size_t)(2 * akz_iclamp(y0 + r, 0, h - 1)) * ppitch + 2 * akz_iclamp(x0 + c, 0, w - 1);
+      v[k][0] = p[0];
+      v[k][1] = p[1];
+      v[k][2] = p[ppitch];
+      v[k][3] = p[ppitch + 1];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const int idx = threadIdx.x + k * NT;
+    if (idx < AH * AW) dst[idx] = 0.25f * (v[k][0] + v[k][1] + v[k][2] + v[k][3]);
+  }
+}
+
 struct akz_octaves { int nlev; int oct[AKZ_MAX_LEVELS]; };
 
 // Wavefront lane shifts by DPP with zero fill (one instruction, no copy of an
@@ -259,12 +287,32 @@ __global__ __launch_bounds__(1024) void k_contrast_finish(const int32_t* __restr
   maxbits = akz_zoff(maxbits, zs);
   kc = akz_zoff(kc, zs);
   const int lane = threadIdx.x & 63, wv = akz_wave();
-  if ((int)threadIdx.x < nbins) {
-    const int32_t* row = hist_part + (size_t)threadIdx.x * CT_HIST_BLOCKS;
+  // a bin's nb block totals are contiguous and 16-byte aligned; P adjacent
+  // threads share a bin (every thread of the block busy at up to 256 bins,
+  // two per bin up to 512), each with up to 8 four-int loads in flight
+  static_assert(CT_HIST_BLOCKS % 4 == 0, "aligned bin rows");
+  {
+    const int P = nbins <= 256 ? 4 : 2;
+    const int bin = threadIdx.x / P, part = threadIdx.x % P;
     int sum = 0;
-#pragma unroll 8
-    for (int b = 0; b < nb; b++) sum += row[b];
-    tot[threadIdx.x] = sum;
+    if (bin < nbins) {
+      const int32_t* row = hist_part + (size_t)bin * CT_HIST_BLOCKS;
+      const int4* row4 = reinterpret_cast<const int4*>(row);
+      const int nq = nb >> 2;
+      constexpr int U4 = 8;
+      for (int q0 = part; q0 < nq; q0 += U4 * P) {
+        int4 v[U4];
+#pragma unroll
+        for (int u = 0; u < U4; u++) v[u] = q0 + u * P < nq ? row4[q0 + u * P] : make_int4(0, 0, 0, 0);
+#pragma unroll
+        for (int u = 0; u < U4; u++) sum += v[u].x + v[u].y + v[u].z + v[u].w;
+      }
+      if (part == 0)
+        for (int b = nq * 4; b < nb; b++) sum += row[b];
+    }
+    // the P parts of a bin are adjacent lanes of one wavefront
+    for (int d = 1; d < P; d <<= 1) sum += __shfl_xor(sum, d, 64);
+    if (bin < nbins && part == 0) tot[bin] = sum;
   }
   __syncthreads();
   // percentile: the loop of ContrastPercentile stops at k = 1 + the first bin
@@ -401,11 +449,14 @@ __global__ __launch_bounds__(NT) void k_fed_multi(const float* __restrict__ L, c
 // k_fed_multi.  One launch instead of two, and g never goes through HBM when
 // the level takes one launch.  LDS: input window, row pass and Lsmooth with
 // its ring; the FED buffers reuse the first two once they are dead.
+// With `parent` given (the previous octave's last level, exactly twice the
+// size, row pitch ppitch), the input is that image's HalfSample2, formed
+// while loading: the first launch of an octave needs no separate halving.
 template <int RW, int RH, int NT, int K = FB_K>
-__global__ __launch_bounds__(NT) void k_fed_multi_sf(const float* __restrict__ in, float* __restrict__ lsm,
-                                                     float* __restrict__ gout, float* __restrict__ out, int pitch,
-                                                     int w, int h, int type, const float* kcp, akz_taps taps,
-                                                     akz_fed_steps st, size_t zs) {
+__global__ __launch_bounds__(NT) void k_fed_multi_sf(const float* __restrict__ in, const float* __restrict__ parent,
+                                                     int ppitch, float* __restrict__ lsm, float* __restrict__ gout,
+                                                     float* __restrict__ out, int pitch, int w, int h, int type,
+                                                     const float* kcp, akz_taps taps, akz_fed_steps st, size_t zs) {
   static_assert(RW > 2 * K && RH > 2 * K, "a tile remains inside the halo");
   constexpr int AW = RW + 6, AH = RH + 6;  // input window
   constexpr int BW = RW + 2;               // row pass (AH rows) and Lsmooth ring (RH + 2 rows)
@@ -414,7 +465,6 @@ __global__ __launch_bounds__(NT) void k_fed_multi_sf(const float* __restrict__ i
   static_assert((1 << LW) == RW, "power-of-two region width");
   static_assert(RH * RW <= AH * AW && RH * RW <= AH * BW, "FED buffers fit the dead windows");
   const akz_tile3 tl = akz_xcd_tile();
-  in = akz_zoff(in, zs, tl.z);
   lsm = akz_zoff(lsm, zs, tl.z);
   if (gout) gout = akz_zoff(gout, zs, tl.z);
   out = akz_zoff(out, zs, tl.z);
@@ -428,7 +478,10 @@ __global__ __launch_bounds__(NT) void k_fed_multi_sf(const float* __restrict__ i
   const int tid = threadIdx.x;
   const int hn = st.n;
   const int x0 = tl.x * (RW - 2 * hn) - hn, y0 = tl.y * (RH - 2 * hn) - hn;
-  load_window<AH, AW, NT>(sA, in, pitch, x0 - 3, y0 - 3, w, h);
+  if (parent)
+    load_window_half<AH, AW, NT>(sA, akz_zoff(parent, zs, tl.z), ppitch, x0 - 3, y0 - 3, w, h);
+  else
+    load_window<AH, AW, NT>(sA, akz_zoff(in, zs, tl.z), pitch, x0 - 3, y0 - 3, w, h);
   __syncthreads();
   // L of the region (the window without its 3-pixel ring), and the row pass
   for (int idx = tid; idx < RH * RW; idx += NT) {
@@ -1133,9 +1186,13 @@ static void launch_fed_reg_sf(hipStream_t s, dim3 grid, const float* in, float* 
 // First launch of a level, fused with the level's smooth + flow
 // (k_fed_multi_sf for the LDS variants, k_fed_reg_sf for the register
 // ones); g is written only when `gout` is given.
-void akz_launch_fed_variant_sf(hipStream_t s, int variant, const float* in, float* lsm, float* gout, float* Lout,
-                               int pitch, int w, int h, int type, const float* kc, const akz_taps& taps,
-                               const float* half_tau, int n, akz_batch bt) {
+bool akz_fed_variant_halves(int variant) {
+  return variant >= 2 && variant <= AKZ_FED_VARIANT_MAX && variant != 12 && variant != 13 && variant != 16;
+}
+
+void akz_launch_fed_variant_sf(hipStream_t s, int variant, const float* in, const float* parent, int ppitch,
+                               float* lsm, float* gout, float* Lout, int pitch, int w, int h, int type,
+                               const float* kc, const akz_taps& taps, const float* half_tau, int n, akz_batch bt) {
   const int nz = bt.nz;
   const size_t zs = bt.zs;
   akz_fed_steps st;
@@ -1147,8 +1204,8 @@ void akz_launch_fed_variant_sf(hipStream_t s, int variant, const float* in, floa
     return dim3((w + tw - 1) / tw, (h + th - 1) / th, nz);
   };
 #define AKZ_SF(RW_, RH_, NT_, K_)                                                                               \
-  hipLaunchKernelGGL((k_fed_multi_sf<RW_, RH_, NT_, K_>), grid(RW_, RH_), dim3(NT_), 0, s, in, lsm, gout, Lout, \
-                     pitch, w, h, type, kc, taps, st, zs)
+  hipLaunchKernelGGL((k_fed_multi_sf<RW_, RH_, NT_, K_>), grid(RW_, RH_), dim3(NT_), 0, s, in, parent, ppitch, lsm, \
+                     gout, Lout, pitch, w, h, type, kc, taps, st, zs)
   switch (variant) {
     case 2: AKZ_SF(64, 64, 512, FB_K); break;
     case 3: AKZ_SF(32, 32, 256, FB_K); break;

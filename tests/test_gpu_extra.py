@@ -34,6 +34,25 @@ def test_diffusivities(gpu_lib, diffusivity):
             assert np.array_equal(g, c), (lvl, err)
 
 
+@pytest.mark.parametrize("nbins,perc", [(1, 0.7), (301, 0.7), (512, 0.7), (512, 1.0), (300, 0.05)])
+def test_contrast_histogram_sizes(gpu_lib, nbins, perc):
+    """Histogram sizes across the whole accepted range 1..512, including
+    those whose bins reach past 300 (the histogram's second LDS array), and
+    the percentile at its ends: every level's contrast factor and levels
+    equal the oracle's."""
+    A = gpu_lib
+    W, H = 640, 480
+    img = A.to_float(A.synth_image(21, W, H))
+    kw = dict(kcontrast_nbins=nbins, kcontrast_percentile=perc)
+    ak = A.AKAZE(A.AKAZEOptions(img_width=W, img_height=H, **kw))
+    ak.Create_Nonlinear_Scale_Space(img)
+    orc = O.Pipeline(W, H, **kw)
+    orc.scale_space(img)
+    for lvl in range(ak.nlev):
+        assert ak.read_kcontrast(lvl) == orc.kcontrast(lvl), (lvl, nbins, perc)
+        assert np.array_equal(ak.read_level(lvl, 0), orc.level(lvl, 0)), (lvl, nbins, perc)
+
+
 def test_one_octave_image(gpu_lib):
     """96x80: the second octave (48x40) falls under the 80x40 cut, leaving
     four levels."""
