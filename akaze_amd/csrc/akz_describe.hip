@@ -132,10 +132,15 @@ __device__ __forceinline__ void orient_image(const akz_desc_params& p, akz_kp* _
   const int wv = akz_wave();
   kps = akz_zoff(kps, p.zs, z);
   const int n = *akz_zoff(nkps_p, p.zs, z);
-  // the loop bound is uniform over the block so every barrier is reached by all waves
-  for (int base = bx * OR_WAVES; base < n; base += nb * OR_WAVES) {
+  // the loop bound is uniform over the block so every barrier is reached by
+  // all waves.  Consecutive keypoints (overlapping windows) stay on one XCD
+  // and its L2: the blocks of XCD bx % 8 take the XCD's eighth of the
+  // keypoints, OR_WAVES at a time with the stride of its blocks.
+  const int nx = (nb & 7) == 0 ? 8 : 1, xcd = bx % nx;
+  const int lo = (int)((long long)n * xcd / nx), hi = (int)((long long)n * (xcd + 1) / nx);
+  for (int base = lo + (bx / nx) * OR_WAVES; base < hi; base += (nb / nx) * OR_WAVES) {
     const int kp = base + wv;
-    orient_one(p, kps, kp, kp < n, wt, z);
+    orient_one(p, kps, kp, kp < hi, wt, z);
   }
 }
 
@@ -267,7 +272,12 @@ __device__ __forceinline__ void mldb_run(const akz_desc_params& p0, int n, int b
     rb[1][m] = 3 * (cc.c3 < 0 ? 29 : cc.c3);
     rb[2][m] = 3 * (cc.c4 < 0 ? 29 : cc.c4);
   }
-  for (int it = bx; it < n; it += nb) {
+  // consecutive keypoints (sorted by level and position, so their sampling
+  // windows overlap) stay on one XCD and its L2: the blocks of XCD bx % 8
+  // take the XCD's eighth of the items with the stride of its blocks
+  const int nx = (nb & 7) == 0 ? 8 : 1, xcd = bx % nx;
+  const int lo = (int)((long long)n * xcd / nx), hi = (int)((long long)n * (xcd + 1) / nx);
+  for (int it = lo + bx / nx; it < hi; it += nb / nx) {
     const mldb_item itm = item_of(it);
     const akz_desc_params& p = *itm.p;
     const int kp = itm.kp, z = itm.z;
