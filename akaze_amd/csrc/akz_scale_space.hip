@@ -287,19 +287,21 @@ __global__ __launch_bounds__(1024) void k_contrast_finish(const int32_t* __restr
   maxbits = akz_zoff(maxbits, zs);
   kc = akz_zoff(kc, zs);
   const int lane = threadIdx.x & 63, wv = akz_wave();
-  // a bin's nb block totals are contiguous and 16-byte aligned; P adjacent
-  // threads share a bin (every thread of the block busy at up to 256 bins,
-  // two per bin up to 512), each with up to 8 four-int loads in flight
+  // a bin's nb block totals are contiguous and 16-byte aligned; P = 1024 /
+  // nbins threads (at most 8) share a bin, each with up to 16 four-int loads
+  // in flight together (one round of memory latency for up to 64 x P
+  // blocks), and their partial sums meet in LDS
   static_assert(CT_HIST_BLOCKS % 4 == 0, "aligned bin rows");
+  __shared__ int32_t part_sum[1024];
+  const int P = akz_imin(8, 1024 / akz_imax(nbins, 1));
   {
-    const int P = nbins <= 256 ? 4 : 2;
     const int bin = threadIdx.x / P, part = threadIdx.x % P;
     int sum = 0;
     if (bin < nbins) {
       const int32_t* row = hist_part + (size_t)bin * CT_HIST_BLOCKS;
       const int4* row4 = reinterpret_cast<const int4*>(row);
       const int nq = nb >> 2;
-      constexpr int U4 = 8;
+      constexpr int U4 = 16;
       for (int q0 = part; q0 < nq; q0 += U4 * P) {
         int4 v[U4];
 #pragma unroll
@@ -310,9 +312,13 @@ __global__ __launch_bounds__(1024) void k_contrast_finish(const int32_t* __restr
       if (part == 0)
         for (int b = nq * 4; b < nb; b++) sum += row[b];
     }
-    // the P parts of a bin are adjacent lanes of one wavefront
-    for (int d = 1; d < P; d <<= 1) sum += __shfl_xor(sum, d, 64);
-    if (bin < nbins && part == 0) tot[bin] = sum;
+    part_sum[threadIdx.x] = sum;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < nbins) {
+    int sum = 0;
+    for (int k = 0; k < P; k++) sum += part_sum[threadIdx.x * P + k];
+    tot[threadIdx.x] = sum;
   }
   __syncthreads();
   // percentile: the loop of ContrastPercentile stops at k = 1 + the first bin

@@ -71,6 +71,9 @@ def fed_roofline(avg_ms, alg_bytes, least_bytes, traffic, basis):
         rate = traffic / (avg_ms * 1e-3) / 1e9
         r["traffic_GBs"] = round(rate, 1)
         r["traffic_frac"] = round(rate / HBM_PEAK_GBS, 4)
+        # not measured by this run: the bytes come from the counter runs
+        # committed beside the kernels, divided by this run's launch time
+        r["traffic_source"] = "profiles/fed_plain_1080p_pmc.md (rocprofv3 FETCH_SIZE and WRITE_SIZE runs)"
     return r
 
 
