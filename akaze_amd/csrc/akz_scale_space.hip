@@ -1240,12 +1240,14 @@ void akz_launch_fed_variant_sf(hipStream_t s, int variant, const float* in, cons
 #undef AKZ_SF
 }
 
-// Region shape by level size and images per launch, from the autotuner's
-// choices on MI355X (halo = steps of the launch, XCD-aware tile placement).
-// One 1920x1080 image: 32-row registers at 1920x1080, 64x64/1024t at
-// 960x540, 32x32/1024t below.  Batches of 16: 48-row registers at
-// 1920x1080, 32-row registers at 960x540 and 480x270, 64x64/1024t at
-// 240x135.
+// Region shape by level size and images per launch (halo = steps of the
+// launch, XCD-aware tile placement), measured on MI355X in the whole
+// pipeline.  One 1920x1080 image per call: 32-row registers at 1920x1080 and
+// 960x540, 32x32/1024t below (with three images in flight the register
+// launches at 960x540 leave more of the GPU to the other pipelines than
+// 64x64/1024t regions, which the autotuner, timing one level alone,
+// prefers).  Batches of 16: 48-row registers at 1920x1080, 32-row registers
+// at 960x540 and 480x270, 64x64/1024t at 240x135.
 int akz_fed_variant_for(int w, int h, int nz) {
   const size_t px = (size_t)w * h;
   if (nz > 1) {
@@ -1255,8 +1257,7 @@ int akz_fed_variant_for(int w, int h, int nz) {
     if (tot >= 300000) return 6;
     return 5;
   }
-  if (px >= 1500000) return 16;
-  if (px >= 300000) return 6;
+  if (px >= 300000) return 16;
   return 5;
 }
 
