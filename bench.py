@@ -40,7 +40,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def fed_roofline(avg_ms, alg_bytes, least_bytes, traffic, basis):
+def fed_roofline(avg_ms, alg_bytes, least_bytes, traffic, basis, traffic_profile="fed_plain_1080p_pmc.md"):
     """Roofline object of the FED launches at the measured average launch.
 
     Two byte counts per launch:
@@ -73,7 +73,7 @@ def fed_roofline(avg_ms, alg_bytes, least_bytes, traffic, basis):
         r["traffic_frac"] = round(rate / HBM_PEAK_GBS, 4)
         # not measured by this run: the bytes come from the counter runs
         # committed beside the kernels, divided by this run's launch time
-        r["traffic_source"] = "profiles/fed_plain_1080p_pmc.md (rocprofv3 FETCH_SIZE and WRITE_SIZE runs)"
+        r["traffic_source"] = "profiles/%s (rocprofv3 FETCH_SIZE and WRITE_SIZE runs)" % traffic_profile
     return r
 
 
@@ -129,19 +129,22 @@ def cpu_baseline(img, W, H, budget_s):
                            "sample": "%d images in %.1f s on 1 thread" % (n1, el1)}}
 
 
-def batched_comparison(A, W, H, B=16, reps=20):
+def batched_comparison(A, W, H, B=16, reps=20, autotune=False):
     """Images/s of 16-image batches (graph mode, resident device-rendered
     images) with the full scan and with the committed learned proposal, and
-    the proposal's keypoint recall on those images."""
+    the proposal's keypoint recall on those images.  The FED variants are the
+    built-in batch table unless `autotune` (the autotuner times each level's
+    FED alone; in the whole pipeline its choices measured 2% slower)."""
     from akaze_amd.proposal import ProposalNet, keypoint_recall
     here = os.path.join(REPO, "akaze_amd")
     rec = json.load(open(os.path.join(here, "proposal_1080p.json")))
     net = ProposalNet()
     net.load(os.path.join(here, "proposal_1080p.f32"))
     bk = A.BatchAKAZE(A.AKAZEOptions(img_width=W, img_height=H), B)
-    variants = bk.autotune(B)
+    variants = bk.autotune(B) if autotune else bk.fed_variants()
     bk.synth(20_000_000, B)
-    out = {"images_per_call": B, "threshold": rec["threshold"], "autotuned_fed_variants": [int(v) for v in variants]}
+    out = {"images_per_call": B, "threshold": rec["threshold"], "autotuned": bool(autotune),
+           "fed_variants": [int(v) for v in variants]}
     kps = {}
     for name, nn in (("full_scan", None), ("learned_proposal", net)):
         bk.set_proposal(nn, rec["threshold"])
@@ -164,7 +167,8 @@ def batched_comparison(A, W, H, B=16, reps=20):
     out["fed_roofline"] = fed_roofline(fms / fl, fby / fl, fmin / fl,
                                        fed_traffic("akaze_batched16_1080p_kernel_stats.md",
                                                    "traffic_bytes_per_launch_batched16")
-                                       if (W, H, B) == (1920, 1080, 16) else None, "per_launch")
+                                       if (W, H, B) == (1920, 1080, 16) else None, "per_launch",
+                                       "akaze_batched16_1080p_kernel_stats.md")
     full, prop = kps["full_scan"], kps["learned_proposal"]
     total = sum(len(f) for f in full)
     out["proposal_kp_recall"] = round(sum(len(f) * keypoint_recall(f, p) for f, p in zip(full, prop)) / total, 4)
@@ -546,7 +550,7 @@ def main():
 
     batched = None
     if not args.no_batched and B == 1:
-        batched = batched_comparison(A, W, H)
+        batched = batched_comparison(A, W, H, autotune=args.autotune)
         log("batched: %s" % batched)
 
     cpu = None
