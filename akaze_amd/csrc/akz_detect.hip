@@ -140,14 +140,16 @@ static_assert(DF_RL >= 64 + 2 * DF_H1 && DF_RX >= 64 + 2 * DF_H2 && (DF_RL & 1) 
 constexpr int DF_R1 = DF_TH + 2 * DF_H1, DF_R2 = DF_TH + 2 * DF_H2, DF_RD = DF_TH + 2;
 static_assert(DF_RD <= DF_R1, "the determinant fits the Lsmooth buffer");
 
-// Derivative and determinant phases of a tile whose stencils never reach
-// the image border (every reflect-101 index is the plain one): with the step
-// S known at compile time, position (r, c) of each grid reads rows r, r+S,
+// Derivative and determinant phases of a tile whose stencils read plain
+// LDS positions (interior tiles, and border tiles whose LDS grids hold the
+// reflect-101 images of the positions outside the image): with the step S
+// known at compile time, position (r, c) of each grid reads rows r, r+S,
 // r+2S and columns c, c+S, c+2S of the grid below it, at constant LDS
-// offsets.  Same operands in the same order as the general path.
-template <int S>
+// offsets.  Same operands in the same order as the general path.  `mid`
+// runs between the two phases, after the first barrier.
+template <int S, class Mid>
 __device__ __forceinline__ void detect_interior(const float* s_l, float* s_x, float* s_y, float* s_d, float fac1,
-                                                float fac2, int wv, int lane) {
+                                                float fac2, int wv, int lane, Mid mid) {
   constexpr int NW = 8, H2 = S + 1, R2n = DF_TH + 2 * H2, E2 = 2 * H2, R = DF_RL, X = DF_RX;
   auto deriv = [&](int r, int c) {
     const float* b = s_l + r * R + c;
@@ -164,6 +166,7 @@ __device__ __forceinline__ void detect_interior(const float* s_l, float* s_x, fl
   if (R2n % NW && wv + (R2n / NW) * NW < R2n) deriv(wv + (R2n / NW) * NW, lane);
   for (int k = threadIdx.x; k < R2n * E2; k += 64 * NW) deriv(k % R2n, 64 + k / R2n);
   __syncthreads();
+  mid();
   auto det = [&](int r, int c) {
     const float* bx = s_x + r * X + c;
     const float* by = s_y + r * X + c;
@@ -205,9 +208,38 @@ __device__ __forceinline__ void detect_tile(const akz_det_params& p, int bx, int
   constexpr int NW = 8;
   const float* lsm = akz_zoff(p.Lsm[l], p.zs, z);
   const bool interior = x0 >= H1 && x0 + 64 + H1 <= w && y0 >= H1 && y0 + DF_TH + H1 <= h;
+  // A border tile takes the interior path when its grids can hold the
+  // reflect-101 images of their positions outside the image: Lsmooth is
+  // loaded from reflected coordinates, and every Lx / Ly position outside
+  // the image is then a copy of the reflected one, which lies in the tile's
+  // own derivative grid when the tile keeps at least 33 columns (16 rows)
+  // inside the image at its right (bottom) border.
+  const bool reflected = !interior && s >= 2 && s <= 4 && w > H1 && h > H1 && (x0 + 64 <= w || w - x0 >= 33) &&
+                         (y0 + DF_TH <= h || h - y0 >= 16);
   // Lsmooth rows [y0-H1, y0+TH+H1), columns [x0-H1, x0+64+H1): every load of
   // a thread is issued before any is written to LDS
-  if (interior) {
+  if (reflected) {
+    constexpr int ROWS = (DF_TH + 2 * DF_H1 + NW - 1) / NW;
+    const int R1n = DF_TH + 2 * H1;
+    float v[ROWS][2];
+#pragma unroll
+    for (int k = 0; k < ROWS; k++) {
+      const int r = wv + k * NW;
+      if (r < R1n) {
+        const float* row = lsm + (size_t)akz_reflect101(y0 - H1 + r, h) * pitch;
+        v[k][0] = row[akz_reflect101(x0 - H1 + lane, w)];
+        if (lane < W1 - 64) v[k][1] = row[akz_reflect101(x0 - H1 + 64 + lane, w)];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < ROWS; k++) {
+      const int r = wv + k * NW;
+      if (r < R1n) {
+        s_l[r * DF_RL + lane] = v[k][0];
+        if (lane < W1 - 64) s_l[r * DF_RL + 64 + lane] = v[k][1];
+      }
+    }
+  } else if (interior) {
     constexpr int ROWS = (DF_TH + 2 * DF_H1 + NW - 1) / NW;
     const int R1n = DF_TH + 2 * H1;
     const float* base = lsm + (size_t)(y0 - H1) * pitch + (x0 - H1);
@@ -253,10 +285,26 @@ __device__ __forceinline__ void detect_tile(const akz_det_params& p, int bx, int
   }
   __syncthreads();
   const float fac1 = p.fac1, fac2 = p.fac2;
-  if (interior && s >= 2 && s <= 4) {
-    if (s == 2) detect_interior<2>(s_l, s_x, s_y, s_d, fac1, fac2, wv, lane);
-    else if (s == 3) detect_interior<3>(s_l, s_x, s_y, s_d, fac1, fac2, wv, lane);
-    else detect_interior<4>(s_l, s_x, s_y, s_d, fac1, fac2, wv, lane);
+  if ((interior || reflected) && s >= 2 && s <= 4) {
+    // reflected tiles: the Lx / Ly positions outside the image become copies
+    // of their reflect-101 images (always inside the image and the grid)
+    // before the determinant reads them
+    auto mid = [&] {
+      if (!reflected) return;
+      const int R2n = DF_TH + 2 * H2, W2 = 64 + 2 * H2;
+      for (int k = threadIdx.x; k < R2n * W2; k += 64 * NW) {
+        const int r = k / W2, c = k - r * W2;
+        const int y = y0 - H2 + r, x = x0 - H2 + c;
+        if (y >= 0 && y < h && x >= 0 && x < w) continue;
+        const int src = (akz_reflect101(y, h) - (y0 - H2)) * DF_RX + akz_reflect101(x, w) - (x0 - H2);
+        s_x[r * DF_RX + c] = s_x[src];
+        s_y[r * DF_RX + c] = s_y[src];
+      }
+      __syncthreads();
+    };
+    if (s == 2) detect_interior<2>(s_l, s_x, s_y, s_d, fac1, fac2, wv, lane, mid);
+    else if (s == 3) detect_interior<3>(s_l, s_x, s_y, s_d, fac1, fac2, wv, lane, mid);
+    else detect_interior<4>(s_l, s_x, s_y, s_d, fac1, fac2, wv, lane, mid);
   } else {
     // first derivatives on rows [y0-H2, y0+TH+H2), columns [x0-H2, x0+64+H2):
     // 64 columns per row-pass, then the 2*H2 leftover columns as one flat list
