@@ -213,9 +213,12 @@ __device__ __forceinline__ void detect_tile(const akz_det_params& p, int bx, int
   // loaded from reflected coordinates, and every Lx / Ly position outside
   // the image is then a copy of the reflected one, which lies in the tile's
   // own derivative grid when the tile keeps at least 33 columns (16 rows)
-  // inside the image at its right (bottom) border.
-  const bool reflected = !interior && s >= 2 && s <= 4 && w > H1 && h > H1 && (x0 + 64 <= w || w - x0 >= 33) &&
-                         (y0 + DF_TH <= h || h - y0 >= 16);
+  // inside the image at its right (bottom) border.  One reflection must
+  // bring every loaded coordinate into the image: c in [-(n - 1), 2n - 2]
+  // (small levels, narrower than the tile plus its halo, may not allow it).
+  const bool reflected = !interior && s >= 2 && s <= 4 && H1 <= w - 1 && H1 <= h - 1 &&
+                         x0 + 63 + H1 <= 2 * w - 2 && y0 + DF_TH - 1 + H1 <= 2 * h - 2 &&
+                         (x0 + 64 <= w || w - x0 >= 33) && (y0 + DF_TH <= h || h - y0 >= 16);
   // Lsmooth rows [y0-H1, y0+TH+H1), columns [x0-H1, x0+64+H1): every load of
   // a thread is issued before any is written to LDS
   if (reflected) {

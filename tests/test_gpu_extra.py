@@ -53,6 +53,28 @@ def test_contrast_histogram_sizes(gpu_lib, nbins, perc):
         assert np.array_equal(ak.read_level(lvl, 0), orc.level(lvl, 0)), (lvl, nbins, perc)
 
 
+@pytest.mark.parametrize("W,H", [(37, 19), (40, 20), (70, 31), (100, 47), (160, 17)])
+def test_tiny_images(gpu_lib, W, H):
+    """Images whose levels are narrower or shorter than a detector tile plus
+    its halo: the detector's border tiles take the general path or the
+    reflected one only where one reflection stays inside the image.  Levels,
+    first derivatives and determinants equal the oracle's (these sizes leave
+    no pixel inside the detector border, so there are no keypoints)."""
+    A = gpu_lib
+    img = A.to_float(A.synth_image(31, W, H))
+    ak = A.AKAZE(A.AKAZEOptions(img_width=W, img_height=H))
+    ak.Create_Nonlinear_Scale_Space(img)
+    ak.detect()
+    orc = O.Pipeline(W, H)
+    orc.scale_space(img)
+    orc.detect()
+    for lvl in range(ak.nlev):
+        for which in (0, 2, 3, 4):
+            assert np.array_equal(ak.read_level(lvl, which), orc.level(lvl, which)), (lvl, which)
+    kps, desc = ak.describe()
+    assert kps.tobytes() == orc.keypoints().tobytes() and np.array_equal(desc, orc.describe())
+
+
 def test_one_octave_image(gpu_lib):
     """96x80: the second octave (48x40) falls under the 80x40 cut, leaving
     four levels."""
