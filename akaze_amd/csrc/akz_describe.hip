@@ -90,10 +90,35 @@ __device__ __forceinline__ void orient_one(const akz_desc_params& p, akz_kp* __r
       }
     }
     __syncthreads();
+    // lane b gets the members of bin b as two 64-bit masks (samples 0..63
+    // and 64..108, one ballot per bin and half), then adds them in sample
+    // order: the bins' few members instead of a scan of all 109 samples
+    uint64_t m0 = 0ull, m1 = 0ull;
+    {
+      const int b0 = active ? (int)s_bin[wv][lane] : -1;
+      const int b1 = active && lane + 64 < OR_NS ? (int)s_bin[wv][lane + 64] : -1;
+      for (int b = 0; b < 42; b++) {
+        const uint64_t a0 = __ballot(b0 == b), a1 = __ballot(b1 == b);
+        if (lane == b) {
+          m0 = a0;
+          m1 = a1;
+        }
+      }
+    }
     if (active && lane < 42) {
       float rx = 0.0f, ry = 0.0f;
-      for (int sn = 0; sn < OR_NS; sn++)
-        if (s_bin[wv][sn] == lane) { rx += s_dx[wv][sn]; ry += s_dy[wv][sn]; }
+      while (m0) {
+        const int sn = __ffsll((unsigned long long)m0) - 1;
+        m0 &= m0 - 1ull;
+        rx += s_dx[wv][sn];
+        ry += s_dy[wv][sn];
+      }
+      while (m1) {
+        const int sn = 64 + __ffsll((unsigned long long)m1) - 1;
+        m1 &= m1 - 1ull;
+        rx += s_dx[wv][sn];
+        ry += s_dy[wv][sn];
+      }
       s_rx[wv][lane] = rx;
       s_ry[wv][lane] = ry;
     }
