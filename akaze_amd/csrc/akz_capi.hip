@@ -307,9 +307,9 @@ akz_status allocate(akz_ctx* c) {
     if (i < c->nlev) t32 += c->det.lev[i].nseg_x * ((c->lev[i].h + dr - 1) / dr);
   }
   const int nblk_seg = akz_scan_blocks(seg_total);
-  size_t o_segc = cv.take<int32_t>(seg_total + 4), o_sego = cv.take<int32_t>(seg_total + 4);
+  size_t o_segc = cv.take<int32_t>(seg_total + 4);
   size_t o_slots = cv.take<uint8_t>((size_t)seg_total * 32);
-  size_t o_scan = cv.take<int32_t>(nblk_seg + 4);
+  size_t o_scan = cv.take<int32_t>((size_t)nblk_seg * 16 + 4);
   // candidate capacity: every strict 3x3 maximum (at most one per 2x2 block)
   size_t capz = 0;
   for (int i = 0; i < c->nlev; i++) capz += (size_t)((c->lev[i].w + 1) / 2) * ((c->lev[i].h + 1) / 2);
@@ -354,17 +354,14 @@ akz_status allocate(akz_ctx* c) {
   c->desc = (uint8_t*)(b + o_desc);
   akz_det_bufs& D = c->db;
   D.seg_count = (int32_t*)(b + o_segc);
-  D.seg_off = (int32_t*)(b + o_sego);
   D.seg_slots = (uint8_t*)(b + o_slots);
-  D.scan_bsum = (int32_t*)(b + o_scan);
-  D.scan_total = D.scan_bsum + nblk_seg + 1;
+  D.seg_wsum = (int32_t*)(b + o_scan);
   D.level_start = c->level_start;
   D.ncand = c->ncand;
   D.cand = c->cand;
   D.nb_count = (int32_t*)(b + o_nbc);
   D.nb_off = (int32_t*)(b + o_nbo);
   D.nb_bsum = (int32_t*)(b + o_nbs);
-  D.nb_total = D.nb_bsum + nblk_nb + 1;
   D.nb_list = (int32_t*)(b + o_nbl);
   D.mem = (int32_t*)(b + o_mem);
   D.minnb = (int32_t*)(b + o_minnb);

@@ -385,7 +385,7 @@ __device__ __forceinline__ void detect_tile(const akz_det_params& p, int bx, int
       const float n4 = s_d[id - 1], n5 = s_d[id + 1], n6 = s_d[id + R], n7 = s_d[id - R];
       pred = above & (v > n0) & (v > n1) & (v > n2) & (v > n3) & (v > n4) & (v > n5) & (v > n6) & (v > n7);
     }
-    // without the full determinant image, the subpixel fit of k_scatter
+    // without the full determinant image, the subpixel fit of k_scan_scatter
     // still finds the 3x3 neighbourhood of every candidate (all inside the
     // image: candidates keep the detector border)
     if (pred && !p.det_full) {
@@ -555,36 +555,17 @@ __global__ __launch_bounds__(SCAN_T) void k_xscan_down(const int32_t* __restrict
     if (i + k <= nvalid) off[i + k] = o[k];
 }
 
-// Materialise candidates in keypoint layout (FindExtrema, cuda_akaze.cu:744-772).
-// Block 0 also writes what the later stages read of the segment scan: the
-// per-level start offsets, the candidate count (cut at the capacity) and the
-// cleared overflow flags of this detection run.
-__global__ __launch_bounds__(256) void k_scatter(akz_det_params p, akz_det_bufs b) {
-  if (blockIdx.x == 0) {
-    const int32_t* off = akz_zoff(b.seg_off, b.zs);
-    int32_t* ls = akz_zoff(b.level_start, b.zs);
-    int32_t* st = akz_zoff(b.status, b.zs);
-    const int t = threadIdx.x;
-    if (t < p.nlev) ls[t] = off[p.lev[t].seg_base];
-    if (t == 0) {
-      const int total = off[p.seg_total];
-      ls[p.nlev] = total;
-      st[AKZ_ST_TOTAL] = total;
-      st[AKZ_ST_FLAGS] = 0;
-      *akz_zoff(b.ncand, b.zs) = akz_imin(total, b.cap);
-    }
-  }
-  const int seg = blockIdx.x * 256 + threadIdx.x;
-  if (seg >= p.seg_total) return;
-  const int32_t* seg_count = akz_zoff(b.seg_count, b.zs);
-  const int c = seg_count[seg];
-  if (c == 0) return;
+// The c candidates of row segment `seg` (ranks in seg_slots) in keypoint
+// layout at cand[base ...] (FindExtrema, cuda_akaze.cu:744-772): the subpixel
+// fit on the determinant's 3x3 neighbourhood; slots at or past the capacity
+// are dropped.
+__device__ __forceinline__ void scatter_segment(const akz_det_params& p, const akz_det_bufs& b, int seg, int c,
+                                                int base) {
   int l = 0;
   while (l + 1 < p.nlev && seg >= p.lev[l + 1].seg_base) l++;
   const akz_level_desc& L = p.lev[l];
   const int ls = seg - L.seg_base;
   const int y = ls / L.nseg_x, tx = ls - y * L.nseg_x;
-  const int base = akz_zoff(b.seg_off, b.zs)[seg];
   const uint8_t* slots = akz_zoff(b.seg_slots, b.zs);
   akz_kp* cand = akz_zoff(b.cand, b.zs);
   const float* imd = akz_zoff(p.det[l], p.zs);
@@ -617,6 +598,53 @@ __global__ __launch_bounds__(256) void k_scatter(akz_det_params p, akz_det_bufs 
     k.angle = dst1;
     cand[idx] = k;
   }
+}
+
+// Sums of the row segments' candidate counts over runs of 256 segments
+// (one per wave: 64 lanes x 4 counts), the carries of k_scan_scatter's blocks.
+__global__ __launch_bounds__(SCAN_T) void k_seg_sums(const int32_t* __restrict__ cnt, int n,
+                                                      int32_t* __restrict__ wsum, size_t zs) {
+  cnt = akz_zoff(cnt, zs);
+  wsum = akz_zoff(wsum, zs);
+  const int i = blockIdx.x * SCAN_BLK + threadIdx.x * SCAN_PER;
+  const int4 v = load4(cnt, i, n);
+  int t = v.x + v.y + v.z + v.w;
+  for (int d = 32; d >= 1; d >>= 1) t += __shfl_xor(t, d, 64);
+  if ((threadIdx.x & 63) == 0) wsum[blockIdx.x * (SCAN_T / 64) + akz_wave()] = t;
+}
+
+// The scan of the row segments' candidate counts fused with the scatter of
+// the candidates, one segment per thread: a segment's offset is the sum of
+// the 256-segment runs before its block (k_seg_sums) plus the block's
+// exclusive scan, and its candidates are written there, so the candidate
+// order is (level, y, x).  The thread of a level's first segment writes the
+// level's start; the thread of index seg_total (the grid covers it) writes
+// the total, the candidate count (cut at the capacity), and clears the
+// overflow flags of this detection run.
+constexpr int SS_T = 256;
+__global__ __launch_bounds__(SS_T) void k_scan_scatter(akz_det_params p, akz_det_bufs b) {
+  __shared__ int32_t s_w[17];
+  const int32_t* wsum = akz_zoff(b.seg_wsum, b.zs);
+  const int n = p.seg_total;
+  const int seg = blockIdx.x * SS_T + threadIdx.x;
+  const int c = seg < n ? akz_zoff(b.seg_count, b.zs)[seg] : 0;
+  int part = 0;
+  for (int j = threadIdx.x; j < (int)blockIdx.x; j += SS_T) part += wsum[j];
+  int ex;
+  const int carry = block_exclusive_scan(part, &ex, s_w);
+  block_exclusive_scan(c, &ex, s_w);
+  const int o = carry + ex;
+  int32_t* ls = akz_zoff(b.level_start, b.zs);
+  for (int l = 0; l < p.nlev; l++)
+    if (seg == p.lev[l].seg_base) ls[l] = o;
+  if (seg == n) {
+    int32_t* st = akz_zoff(b.status, b.zs);
+    ls[p.nlev] = o;
+    st[AKZ_ST_TOTAL] = o;
+    st[AKZ_ST_FLAGS] = 0;
+    *akz_zoff(b.ncand, b.zs) = akz_imin(o, b.cap);
+  }
+  if (c > 0) scatter_segment(p, b, seg, c, o);
 }
 
 // Neighbour lists of FindNeighbors (cuda_akaze.cu:1005-1070): earlier
@@ -1150,6 +1178,7 @@ void xscan(hipStream_t s, const int32_t* cnt, int len, const int32_t* n_ptr, int
 }  // namespace
 
 int akz_scan_blocks(int len) { return (len + 1 + SCAN_BLK - 1) / SCAN_BLK; }
+int akz_seg_runs(int seg_total) { return seg_total / SS_T + 1; }
 
 void akz_launch_hessian(hipStream_t s, const akz_det_params& p, int nz) {
   const int nb = p.tile_start[p.nlev];
@@ -1175,8 +1204,9 @@ void akz_launch_extrema(hipStream_t s, const akz_det_params& p, const akz_det_bu
 }
 
 void akz_launch_candidates(hipStream_t s, const akz_det_params& p, const akz_det_bufs& b, int nz) {
-  xscan(s, b.seg_count, p.seg_total, nullptr, b.scan_bsum, b.seg_off, nz, b.zs);
-  hipLaunchKernelGGL(k_scatter, dim3((p.seg_total + 255) / 256, 1, nz), dim3(256), 0, s, p, b);
+  hipLaunchKernelGGL(k_seg_sums, dim3(akz_scan_blocks(p.seg_total), 1, nz), dim3(SCAN_T), 0, s, b.seg_count,
+                     p.seg_total, b.seg_wsum, b.zs);
+  hipLaunchKernelGGL(k_scan_scatter, dim3(akz_seg_runs(p.seg_total), 1, nz), dim3(SS_T), 0, s, p, b);
 }
 
 void akz_launch_suppression(hipStream_t s, const akz_det_bufs& b, int nz) {

@@ -87,16 +87,13 @@ struct akz_det_params {
 struct akz_det_bufs {
   int32_t* seg_count;    // seg_total (+4) in-segment candidate counts
   uint8_t* seg_slots;    // 32 x-offsets per segment
-  int32_t* seg_off;      // seg_total + 1 offsets
-  int32_t* scan_bsum;    // akz_scan_blocks(seg_total) block sums
-  int32_t* scan_total;
+  int32_t* seg_wsum;     // akz_scan_blocks(seg_total) * 16 sums of 256 counts
   int32_t* level_start;  // nlev + 1
   int32_t* ncand;        // candidates after the capacity cut
   akz_kp* cand;          // cap
   int32_t* nb_count;     // cap
   int32_t* nb_off;       // cap + 1 CSR offsets
   int32_t* nb_bsum;      // akz_scan_blocks(cap)
-  int32_t* nb_total;
   int32_t* nb_list;      // nb_cap
   int32_t *mem, *minnb, *code;  // cap each (suppression state beyond LDS)
   int32_t* act;          // 2 x cap
@@ -108,6 +105,7 @@ struct akz_det_bufs {
   size_t zs;
 };
 int akz_scan_blocks(int len);
+int akz_seg_runs(int seg_total);  // blocks of k_scan_scatter: 256 segments each, index seg_total included
 void akz_launch_hessian(hipStream_t s, const akz_det_params& p, int nz);
 void akz_launch_extrema(hipStream_t s, const akz_det_params& p, const akz_det_bufs& b, int nz);
 // Hessian response + extrema in one kernel (derivative steps up to
