@@ -317,11 +317,11 @@ akz_status allocate(akz_ctx* c) {
   if (capz > (size_t)(1u << 22) - 1) capz = (1u << 22) - 1;
   c->cap = (int)std::max<size_t>(capz, 1);
   const size_t nb_cap = (size_t)c->cap * AKZ_NB_PER_CAND;
-  const int nblk_nb = akz_scan_blocks(c->cap);
   size_t o_cand = cv.take<akz_kp>(c->cap), o_ls = cv.take<int32_t>(AKZ_MAX_LEVELS + 1);
+  size_t o_cxy = cv.take<float2>(c->cap);
   size_t o_nc = cv.take<int32_t>(4), o_status = cv.take<int32_t>(4);
   size_t o_nbc = cv.take<int32_t>(c->cap + 4), o_nbo = cv.take<int32_t>(c->cap + 4);
-  size_t o_nbs = cv.take<int32_t>(nblk_nb + 4);
+  size_t o_nbs = cv.take<int32_t>(4);  // 64-bit pool counter
   size_t o_nbl = cv.take<int32_t>(nb_cap);
   size_t o_mem = cv.take<int32_t>(c->cap), o_minnb = cv.take<int32_t>(c->cap), o_code = cv.take<int32_t>(c->cap);
   size_t o_act = cv.take<int32_t>((size_t)c->cap * 2);
@@ -359,9 +359,10 @@ akz_status allocate(akz_ctx* c) {
   D.level_start = c->level_start;
   D.ncand = c->ncand;
   D.cand = c->cand;
+  D.cand_xy = (float2*)(b + o_cxy);
   D.nb_count = (int32_t*)(b + o_nbc);
   D.nb_off = (int32_t*)(b + o_nbo);
-  D.nb_bsum = (int32_t*)(b + o_nbs);
+  D.nb_pool = (int32_t*)(b + o_nbs);
   D.nb_list = (int32_t*)(b + o_nbl);
   D.mem = (int32_t*)(b + o_mem);
   D.minnb = (int32_t*)(b + o_minnb);

@@ -473,12 +473,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
   }
 }
 
-// Exclusive scans over per-image count arrays in two launches: block totals
-// (k_xscan_reduce), then each block adds the totals of the blocks before it
-// and scans its 4096 counts (k_xscan_down).  Counts at indices >= nvalid
-// count as zero, where nvalid is *n_ptr when given and `len` otherwise;
-// offsets are written for indices 0..nvalid, so off[nvalid] is the total.
-// Each thread owns four consecutive counts, read as one int4.
+// Scan blocks of k_seg_sums: 1024 threads, each owning four consecutive
+// counts read as one int4.
 constexpr int SCAN_T = 1024, SCAN_PER = 4, SCAN_BLK = SCAN_T * SCAN_PER;
 
 __device__ __forceinline__ int4 load4(const int32_t* a, int i, int n) {
@@ -512,49 +508,6 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* excl, int32_t* s
   return tot;
 }
 
-__global__ __launch_bounds__(SCAN_T) void k_xscan_reduce(const int32_t* __restrict__ cnt, int len, const int32_t* n_ptr,
-                                                         int32_t* __restrict__ bsum, size_t zs) {
-  __shared__ int32_t s_w[17];
-  cnt = akz_zoff(cnt, zs);
-  bsum = akz_zoff(bsum, zs);
-  const int nvalid = n_ptr ? *akz_zoff(n_ptr, zs) : len;
-  // the grid covers the capacity; blocks past the device-side count sum to 0
-  if ((int)blockIdx.x * SCAN_BLK > nvalid) {
-    if (threadIdx.x == 0) bsum[blockIdx.x] = 0;
-    return;
-  }
-  const int i = blockIdx.x * SCAN_BLK + threadIdx.x * SCAN_PER;
-  int4 v = load4(cnt, i, nvalid);
-  int ex;
-  int tot = block_exclusive_scan(v.x + v.y + v.z + v.w, &ex, s_w);
-  if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
-}
-
-__global__ __launch_bounds__(SCAN_T) void k_xscan_down(const int32_t* __restrict__ cnt, int len, const int32_t* n_ptr,
-                                                       const int32_t* __restrict__ bsum, int32_t* __restrict__ off,
-                                                       size_t zs) {
-  __shared__ int32_t s_w[17];
-  cnt = akz_zoff(cnt, zs);
-  bsum = akz_zoff(bsum, zs);
-  off = akz_zoff(off, zs);
-  const int nvalid = n_ptr ? *akz_zoff(n_ptr, zs) : len;
-  if ((int)blockIdx.x * SCAN_BLK > nvalid) return;  // no offset to write
-  const int i = blockIdx.x * SCAN_BLK + threadIdx.x * SCAN_PER;
-  int4 v = load4(cnt, i, nvalid);
-  // the block's carry: the totals of the blocks before it (a few hundred at
-  // most), summed here instead of in a separate launch
-  int part = 0;
-  for (int j = threadIdx.x; j < (int)blockIdx.x; j += SCAN_T) part += bsum[j];
-  int ex;
-  const int carry = block_exclusive_scan(part, &ex, s_w);
-  block_exclusive_scan(v.x + v.y + v.z + v.w, &ex, s_w);
-  const int o0 = carry + ex;
-  const int o[4] = {o0, o0 + v.x, o0 + v.x + v.y, o0 + v.x + v.y + v.z};
-#pragma unroll
-  for (int k = 0; k < 4; k++)
-    if (i + k <= nvalid) off[i + k] = o[k];
-}
-
 // The c candidates of row segment `seg` (ranks in seg_slots) in keypoint
 // layout at cand[base ...] (FindExtrema, cuda_akaze.cu:744-772): the subpixel
 // fit on the determinant's 3x3 neighbourhood; slots at or past the capacity
@@ -568,6 +521,7 @@ __device__ __forceinline__ void scatter_segment(const akz_det_params& p, const a
   const int y = ls / L.nseg_x, tx = ls - y * L.nseg_x;
   const uint8_t* slots = akz_zoff(b.seg_slots, b.zs);
   akz_kp* cand = akz_zoff(b.cand, b.zs);
+  float2* cand_xy = akz_zoff(b.cand_xy, b.zs);
   const float* imd = akz_zoff(p.det[l], p.zs);
   const size_t pitch = L.pitch;
   for (int r = 0; r < c; r++) {
@@ -597,6 +551,7 @@ __device__ __forceinline__ void scatter_segment(const akz_det_params& p, const a
     k.y = ratio * (y);
     k.angle = dst1;
     cand[idx] = k;
+    cand_xy[idx] = make_float2(k.x, k.y);
   }
 }
 
@@ -643,73 +598,140 @@ __global__ __launch_bounds__(SS_T) void k_scan_scatter(akz_det_params p, akz_det
     st[AKZ_ST_TOTAL] = o;
     st[AKZ_ST_FLAGS] = 0;
     *akz_zoff(b.ncand, b.zs) = akz_imin(o, b.cap);
+    *reinterpret_cast<unsigned long long*>(akz_zoff(b.nb_pool, b.zs)) = 0ull;  // k_nb_pool's allocator
   }
   if (c > 0) scatter_segment(p, b, seg, c, o);
 }
 
-// Neighbour lists of FindNeighbors (cuda_akaze.cu:1005-1070): earlier
-// candidates of the same level and all candidates of the previous level
-// within half the candidate's size.  Weak candidates (negative size) get
-// none, as the reference's early break gives.  The previous level is entered
-// at the last candidate whose y is within reach (binary search) instead of
-// being walked from its end.  FILL = false counts, FILL = true writes the
-// CSR list at the scanned offsets.  Entries past nb_cap are dropped and flag
-// an overflow; the suppression reads the lists cut at nb_cap, so every entry
-// it reads was written by this run.
-template <bool FILL>
-__global__ __launch_bounds__(256) void k_neighbors(akz_det_bufs b) {
-  const akz_kp* __restrict__ cand = akz_zoff(b.cand, b.zs);
-  const int32_t* __restrict__ level_start = akz_zoff(b.level_start, b.zs);
-  int32_t* nb_count = akz_zoff(b.nb_count, b.zs);
-  const int32_t* nb_off = akz_zoff(b.nb_off, b.zs);
-  int32_t* nb_list = akz_zoff(b.nb_list, b.zs);
-  const int n = *akz_zoff(b.ncand, b.zs);
-  for (int c = blockIdx.x * 256 + threadIdx.x; c < n; c += gridDim.x * 256) {
-    const akz_kp k = cand[c];
-    const int lvl = k.class_id;
-    const int start = level_start[lvl];
-    const float size = k.size;
-    const float half = size * .5f;
-    const double r2 = (double)(size * size) * 0.25;
-    int cnt = 0;
-    const int o = FILL ? nb_off[c] : 0;
-    for (int i = c - 1; i >= start; i--) {
-      const akz_kp q = cand[i];
-      if (k.y - q.y > half) break;
-      float dist = (k.x - q.x) * (k.x - q.x) + (k.y - q.y) * (k.y - q.y);
-      if ((double)dist < r2) {
-        if (FILL && o + cnt < b.nb_cap) nb_list[o + cnt] = i;
-        cnt++;
-      }
-    }
-    if (lvl > 0) {
-      const int lo = level_start[lvl - 1], hi = start;
-      // first index in [lo, hi) with q.y - k.y > half (monotone in sorted order)
-      int a = lo, e = hi;
-      while (a < e) {
-        int m = (a + e) >> 1;
-        if (cand[m].y - k.y > half) e = m; else a = m + 1;
-      }
-      for (int i = a - 1; i >= lo; i--) {
-        const akz_kp q = cand[i];
-        if (k.y - q.y > half) break;
-        float dist = (k.x - q.x) * (k.x - q.x) + (k.y - q.y) * (k.y - q.y);
-        if ((double)dist < r2) {
-          if (FILL && o + cnt < b.nb_cap) nb_list[o + cnt] = i;
-          cnt++;
-        }
-      }
-    }
-    if (!FILL) nb_count[c] = cnt;
-    else if (o + cnt > b.nb_cap) atomicOr(akz_zoff(b.status, b.zs) + AKZ_ST_FLAGS, 1);
+// Neighbour list of FindNeighbors (cuda_akaze.cu:1005-1070) for candidate c
+// at (kx, ky) of size `size`: earlier candidates of its level (indices
+// start .. c-1, walked downward) and the candidates of the previous level
+// (prev_lo .. start-1, when has_prev) within half the size.  A weak
+// candidate (negative size) gets none, as the reference's early break
+// gives.  The previous level is entered at the last candidate whose y is
+// within reach (binary search) instead of being walked from its end.  xy(i)
+// is candidate i's position; emit(k, i) takes the k-th neighbour i in list
+// order.  Returns the count.
+template <class XY, class Emit>
+__device__ __forceinline__ int nb_search(int c, float kx, float ky, float size, int start, int prev_lo,
+                                         bool has_prev, XY xy, Emit emit) {
+  const float half = size * .5f;
+  const double r2 = (double)(size * size) * 0.25;
+  int cnt = 0;
+  for (int i = c - 1; i >= start; i--) {
+    const float2 q = xy(i);
+    if (ky - q.y > half) break;
+    float dist = (kx - q.x) * (kx - q.x) + (ky - q.y) * (ky - q.y);
+    if ((double)dist < r2) emit(cnt++, i);
   }
+  if (has_prev) {
+    // first index in [prev_lo, start) with q.y - ky > half (monotone in the
+    // sorted order)
+    int a = prev_lo, e = start;
+    while (a < e) {
+      const int m = (a + e) >> 1;
+      if (xy(m).y - ky > half) e = m; else a = m + 1;
+    }
+    for (int i = a - 1; i >= prev_lo; i--) {
+      const float2 q = xy(i);
+      if (ky - q.y > half) break;
+      float dist = (kx - q.x) * (kx - q.x) + (ky - q.y) * (ky - q.y);
+      if ((double)dist < r2) emit(cnt++, i);
+    }
+  }
+  return cnt;
 }
 
 constexpr int NMS_THREADS = 1024;
-constexpr int NMS_LDS_N = 8192;     // candidates whose suppression state fits LDS
-constexpr int NMS_LDS_NB = 16384;   // neighbour entries that fit LDS beside it
+constexpr int NMS_LDS_N = 8192;     // candidates whose suppression runs in LDS
+constexpr int NMS_PER = NMS_LDS_N / NMS_THREADS;  // candidates (or list entries) per thread
+constexpr int NMS_FEXT = 4096;      // LDS pool of the neighbours past the first, for lists of three or more
+constexpr int NMS_CNT_MAX = 63;     // longest list the packed form holds
+constexpr int NMS_TAIL = 64;        // pending candidates from which one wavefront runs the waves
+static_assert(NMS_TAIL % 64 == 0, "the tail keeps NMS_TAIL / 64 candidates per lane");
+// A neighbour list packed in 32 bits: the first neighbour (13 bits), the
+// second neighbour or, for three or more, the offset of neighbours 1.. in the
+// pool (13 bits), and the count (6 bits).
+static_assert(NMS_LDS_N <= 8192 && NMS_FEXT <= 8192, "13-bit fields");
+__device__ __forceinline__ uint32_t fwd_pack(int q0, int f2, int cnt) {
+  return (uint32_t)q0 | ((uint32_t)f2 << 13) | ((uint32_t)cnt << 26);
+}
 constexpr int CODE_WAIT = -1, CODE_ADD = -2;
 constexpr int NMS_BITMAP_WORDS = 2048;  // bitmap ranking covers up to 65536 candidates
+
+// Neighbour lists of every candidate, for the suppression kernel: each
+// candidate's list is written by one thread, in list order, into space taken
+// from a pool (nb_off = start, nb_count = length); a wavefront takes the
+// space of its 64 lists with one atomic add, so no scan over the counts is
+// needed, and where the lists sit in the pool does not matter.  Positions
+// come from the compact cand_xy array; a block's threads take consecutive
+// candidates, whose searches cover overlapping ranges that stay in the CU's
+// cache.  Entries past nb_cap are dropped and flag an overflow; the
+// suppression reads only entries written in this run.
+constexpr int NB_KEEP = 8;
+__global__ __launch_bounds__(256) void k_nb_lists(akz_det_bufs b) {
+  const int n = *akz_zoff(b.ncand, b.zs);
+  const akz_kp* __restrict__ cand = akz_zoff(b.cand, b.zs);
+  const float2* __restrict__ cxy = akz_zoff(b.cand_xy, b.zs);
+  const int32_t* __restrict__ level_start = akz_zoff(b.level_start, b.zs);
+  int32_t* nb_count = akz_zoff(b.nb_count, b.zs);
+  int32_t* nb_off = akz_zoff(b.nb_off, b.zs);
+  int32_t* nb_list = akz_zoff(b.nb_list, b.zs);
+  unsigned long long* pool = reinterpret_cast<unsigned long long*>(akz_zoff(b.nb_pool, b.zs));
+  const int lane = threadIdx.x & 63;
+  auto xy = [&](int i) { return cxy[i]; };
+  // the loop runs while any lane of the wavefront has a candidate
+  for (int c0 = blockIdx.x * 256 + (threadIdx.x & ~63); c0 < n; c0 += gridDim.x * 256) {
+    const int c = c0 + lane;
+    int cnt = 0, lvl = 0, start = 0, lo = 0;
+    float2 p = make_float2(0.0f, 0.0f);
+    float size = 0.0f;
+    // the first NB_KEEP neighbours found stay in registers; longer lists are
+    // searched again for the write
+    int r[NB_KEEP];
+#pragma unroll
+    for (int u = 0; u < NB_KEEP; u++) r[u] = 0;
+    if (c < n) {
+      p = cxy[c];
+      size = cand[c].size;
+      lvl = cand[c].class_id;
+      start = level_start[lvl];
+      lo = lvl > 0 ? level_start[lvl - 1] : 0;
+      cnt = nb_search(c, p.x, p.y, size, start, lo, lvl > 0, xy, [&](int k, int i) {
+#pragma unroll
+        for (int u = 0; u < NB_KEEP; u++) r[u] = (k == u) ? i : r[u];
+      });
+    }
+    // the wavefront's space: inclusive prefix of the counts, one atomic add
+    int inc = cnt;
+    for (int d = 1; d < 64; d <<= 1) {
+      const int o = __shfl_up(inc, d, 64);
+      if (lane >= d) inc += o;
+    }
+    const int total = __shfl(inc, 63, 64);
+    unsigned long long wbase = 0;
+    if (lane == 63 && total > 0) wbase = atomicAdd(pool, (unsigned long long)total);
+    wbase = __shfl(wbase, 63, 64);
+    if (c < n) {
+      const unsigned long long base = wbase + (unsigned long long)(inc - cnt);
+      const long long room_ll = (long long)b.nb_cap - (long long)base;
+      const int room = room_ll <= 0 ? 0 : (room_ll < cnt ? (int)room_ll : cnt);
+      int32_t* dst = nb_list + base;
+      if (cnt <= NB_KEEP) {
+#pragma unroll
+        for (int u = 0; u < NB_KEEP; u++)
+          if (u < room) dst[u] = r[u];
+      } else if (room > 0) {
+        nb_search(c, p.x, p.y, size, start, lo, lvl > 0, xy, [&](int j, int i) {
+          if (j < room) dst[j] = i;
+        });
+      }
+      nb_off[c] = room > 0 ? (int)base : 0;
+      nb_count[c] = room;
+      if (room < cnt) atomicOr(akz_zoff(b.status, b.zs) + AKZ_ST_FLAGS, 1);
+    }
+  }
+}
 
 // block-wide exclusive scan for the 1024-thread suppression kernel
 __device__ __forceinline__ int nms_block_scan(int v, int* excl, int32_t* s_w) {
@@ -874,306 +896,443 @@ __device__ int nms_output_flat(const akz_kp* __restrict__ cand, int n, MemAt mem
   return kept;
 }
 
-// Wave-synchronous greedy suppression of FilterExtrema_kernel
-// (cuda_akaze.cu:1072-1224), one workgroup per image.  Per wave: a candidate
-// whose neighbours are all processed is "ready"; a ready candidate with no
-// kept neighbour is kept; otherwise it contends for its kept neighbour with
-// the smallest member value, the lowest contending index wins (atomicMin
-// here, a racy min in the reference), and the winner replaces that neighbour
-// if its response is higher, else is dropped.  No step of a wave depends on
-// the order in which its candidates are visited.
-//
-// Up to NMS_LDS_N candidates with at most NMS_LDS_NB neighbour entries, the
-// whole state lives in LDS (16-bit member values, codes and offsets, the CSR
-// lists and the responses) and every wave visits all candidates, skipping
-// the processed ones: a wave costs three barriers and LDS reads only.  Beyond
-// that the state is in global memory and each wave visits a compacted list
-// of the unprocessed candidates.
-__global__ __launch_bounds__(NMS_THREADS) void k_nms(akz_det_bufs b) {
-  __shared__ int16_t s_mem[NMS_LDS_N];
-  __shared__ int32_t s_minnb[NMS_LDS_N];
-  __shared__ uint16_t s_off[NMS_LDS_N + 2];
-  __shared__ float s_resp[NMS_LDS_N];
-  __shared__ uint8_t s_strong[NMS_LDS_N];  // 1: size >= 0 (emitted when kept)
-  __shared__ __align__(16) int16_t s_nb[NMS_LDS_NB];  // the output phase reuses it (24 KB)
-  __shared__ int32_t s_w[17];
-  __shared__ int32_t s_nact[2];
-  __shared__ int32_t s_left[2];  // work left after a wave, by the parity of the wave
-  __shared__ int32_t s_run;
-  int32_t* s_sort = reinterpret_cast<int32_t*>(s_nb);
-  uint32_t* s_bits = reinterpret_cast<uint32_t*>(s_nb) + 2048;
-  int32_t* s_pre = reinterpret_cast<int32_t*>(s_nb) + 2048 + NMS_BITMAP_WORDS;
-  const akz_kp* __restrict__ cand = akz_zoff(b.cand, b.zs);
+// Wave loop of the suppression with its state in global memory (more than
+// NMS_LDS_N candidates, or neighbour lists too long for LDS), lists at
+// (nb_off, nb_count): each wave visits a compacted list of the candidates
+// still pending.  Returns the number of waves.
+__device__ int nms_waves_global(const akz_det_bufs& b, const akz_kp* __restrict__ cand, int n, int32_t* s_nact) {
+  const int t = threadIdx.x;
   const int32_t* __restrict__ nb_off = akz_zoff(b.nb_off, b.zs);
+  const int32_t* __restrict__ nb_cnt = akz_zoff(b.nb_count, b.zs);
   const int32_t* __restrict__ nb_list = akz_zoff(b.nb_list, b.zs);
+  int32_t* mem = akz_zoff(b.mem, b.zs);
+  int32_t* minnb = akz_zoff(b.minnb, b.zs);
+  int32_t* g_code = akz_zoff(b.code, b.zs);
+  int32_t* act[2] = {akz_zoff(b.act, b.zs), akz_zoff(b.act, b.zs) + b.cap};
+  for (int i = t; i < n; i += NMS_THREADS) {
+    mem[i] = -1;
+    minnb[i] = INT32_MAX;
+    act[0][i] = i;
+  }
+  if (t == 0) { s_nact[0] = n; s_nact[1] = 0; }
+  __syncthreads();
+  int cur = 0, wave = 0;
+  while (wave < b.max_waves) {
+    const int na = s_nact[cur];
+    if (na == 0) break;
+    const int32_t* A = act[cur];
+    for (int j = t; j < na; j += NMS_THREADS) {
+      const int c = A[j];
+      const int e0 = nb_off[c], e1 = e0 + nb_cnt[c];
+      bool ready = true, anykept = false;
+      int best = -1, bestv = INT32_MAX;
+      for (int k = e0; k < e1; k++) {
+        const int q = nb_list[k];
+        const int m = mem[q];
+        if (m == -1) { ready = false; break; }
+        if (m >= 0) {
+          anykept = true;
+          if (m < bestv) { bestv = m; best = q; }
+        }
+      }
+      int code = CODE_WAIT;
+      if (ready) {
+        if (!anykept) code = CODE_ADD;
+        else { code = best; atomicMin(&minnb[best], c); }
+      }
+      g_code[c] = code;
+    }
+    __syncthreads();
+    for (int j = t; j < na; j += NMS_THREADS) {
+      const int c = A[j];
+      const int code = g_code[c];
+      if (code == CODE_ADD) {
+        mem[c] = c;
+      } else if (code >= 0 && minnb[code] == c) {
+        if (cand[c].response > cand[code].response) {
+          mem[c] = code;
+          mem[code] = -2;
+        } else {
+          mem[c] = -2;
+        }
+      }
+    }
+    __syncthreads();
+    const int nxt = cur ^ 1;
+    for (int j = t; j < na; j += NMS_THREADS) {
+      const int c = A[j];
+      const int code = g_code[c];
+      if (code >= 0) minnb[code] = INT32_MAX;
+      if (mem[c] == -1) {
+        int pos = atomicAdd(&s_nact[nxt], 1);
+        act[nxt][pos] = c;
+      }
+    }
+    __syncthreads();
+    if (t == 0) s_nact[cur] = 0;
+    cur = nxt;
+    wave++;
+    __syncthreads();
+  }
+  return wave;
+}
+
+// One wave's decision for a pending candidate with packed list f whose first
+// two neighbours have member values m0 and m1 (m1 is unused for lists of one
+// or of three and more): CODE_WAIT while a neighbour is pending, CODE_ADD
+// when no neighbour is kept, else the kept neighbour with the smallest
+// member value, first in list order on ties.
+__device__ __forceinline__ int nms_classify(uint32_t f, int m0, int m1, const int16_t* s_mem,
+                                            const int16_t* s_fext) {
+  const int cnt = (int)(f >> 26);
+  const int q0 = (int)(f & 0x1fffu), f2 = (int)((f >> 13) & 0x1fffu);
+  bool ready = true;
+  int best = -1, bestv = INT32_MAX;
+  if (cnt >= 1) {
+    if (m0 == -1) ready = false;
+    else if (m0 >= 0) { bestv = m0; best = q0; }
+  }
+  if (cnt == 2) {
+    if (m1 == -1) ready = false;
+    else if (m1 >= 0 && m1 < bestv) { bestv = m1; best = f2; }
+  } else if (cnt > 2) {
+    for (int i = 0; i < cnt - 1; i++) {
+      const int q = s_fext[f2 + i];
+      const int m = s_mem[q];
+      if (m == -1) ready = false;
+      else if (m >= 0 && m < bestv) { bestv = m; best = q; }
+    }
+  }
+  if (!ready) return CODE_WAIT;
+  return best < 0 ? CODE_ADD : best;
+}
+
+// The second phase of a wave for candidate c with decision `code`: an
+// addition, or for a contender that won its target's arbitration a
+// replacement (response rc above the target's rq) or a drop.  Returns
+// whether c is processed.
+__device__ __forceinline__ bool nms_resolve(int c, int code, bool won, float rc, float rq, int16_t* s_mem) {
+  if (code == CODE_ADD) {
+    s_mem[c] = (int16_t)c;
+    return true;
+  }
+  if (code >= 0 && won) {
+    if (rc > rq) {
+      s_mem[c] = (int16_t)code;
+      s_mem[code] = -2;
+    } else {
+      s_mem[c] = -2;
+    }
+    return true;
+  }
+  return false;
+}
+
+// Wave-synchronous greedy suppression of FilterExtrema_kernel
+// (cuda_akaze.cu:1072-1224) and its output, one workgroup per image, on the
+// neighbour lists of k_nb_lists.  Per wave: a candidate whose neighbours are
+// all processed is "ready"; a ready candidate with no kept neighbour is kept;
+// otherwise it contends for its kept neighbour with the smallest member
+// value, the lowest contending index wins (atomicMin here, a racy min in the
+// reference), and the winner replaces that neighbour if its response is
+// higher, else is dropped.  No step of a wave depends on the order in which
+// its candidates are visited.
+//
+// Up to NMS_LDS_N candidates everything is in LDS, the neighbour lists
+// packed per candidate in 32 bits (the first neighbour, the second or the
+// offset of the rest in a small pool, the count: almost every candidate has
+// at most two).  Wave 0 is part of the set-up: nothing is processed before
+// it, so exactly the candidates without neighbours are ready, and all of
+// them are kept.  The candidates still pending are kept in a compacted list,
+// so a wave costs work only for them.  While the list is longer than a
+// wavefront, a wave is: list entries, packed lists and the neighbours'
+// member values loaded in three rounds, classification and arbitration, a
+// barrier, the arbitration results and responses loaded in one round,
+// additions and replacements, the candidates left pending appended to the
+// next list (one LDS atomic per wavefront), a barrier.  From then on one
+// wavefront runs the remaining waves with one candidate per lane and no
+// barriers: a wavefront's LDS instructions complete in order, so each step
+// sees the previous one.  Beyond NMS_LDS_N candidates, or when a list does
+// not fit the packed form or the pool, the state is in global memory.
+__global__ __launch_bounds__(NMS_THREADS) void k_suppress(akz_det_bufs b) {
+  __shared__ int16_t s_mem[NMS_LDS_N];                   // member values (-1 pending, -2 dropped)
+  __shared__ __align__(16) int32_t s_minnb[NMS_LDS_N];   // arbitration slots; the global path's output scratch
+  __shared__ float s_resp[NMS_LDS_N];                    // responses
+  __shared__ uint32_t s_fwd[NMS_LDS_N];                  // packed neighbour lists
+  __shared__ int16_t s_fext[NMS_FEXT];                   // neighbours 1.. of lists of three or more
+  __shared__ __align__(16) int16_t s_list[NMS_LDS_N];    // pending candidates, then the output's scratch
+  __shared__ uint32_t s_strong[NMS_LDS_N / 32];          // bit i: candidate i is strong (size >= 0)
+  __shared__ int32_t s_w[17];
+  __shared__ int32_t s_nact[2];  // list lengths, by the parity of the wave
+  __shared__ int32_t s_run;
+  __shared__ int32_t s_ftop;     // pool entries taken
+  __shared__ int32_t s_spill;    // a list does not fit the packed form or the pool
+  const akz_kp* __restrict__ cand = akz_zoff(b.cand, b.zs);
   akz_kp* __restrict__ out = akz_zoff(b.kps, b.zs);
   int32_t* __restrict__ kidx = akz_zoff(b.kidx, b.zs);
   int32_t* st = akz_zoff(b.status, b.zs);
   const int cap = b.cap;
   const int t = threadIdx.x;
+  const int lane = t & 63;
   const int n = *akz_zoff(b.ncand, b.zs);
-  const int ntot = n > 0 ? akz_imin(nb_off[n], b.nb_cap) : 0;
-  const bool lds = n <= NMS_LDS_N && ntot <= NMS_LDS_NB;
   int wave = 0;
-  int kept;
-  if (lds) {
-    // thread t owns candidates t + j * NMS_THREADS; their pending bits, the
-    // readiness-scan resume points, list ends and codes stay in registers
-    constexpr int PER = NMS_LDS_N / NMS_THREADS;
-    int rs[PER], e1[PER], cd[PER];
-    uint32_t pend = 0;
-    // every global load of a thread is issued before its LDS stores
-    {
-      float rv[PER], sz[PER];
-      int ov[PER + 1];
+  int kept = 0;
+  bool global_lists = n > NMS_LDS_N;
+  if (t == 0) { s_ftop = 0; s_spill = 0; }
+  if (!global_lists) {
+    // the lists of k_nb_lists packed into LDS: counts and starts, then the
+    // first two entries (one round of loads each); entries 1.. of lists of
+    // three or more into the pool at space taken with one LDS atomic
+    const int32_t* nb_off = akz_zoff(b.nb_off, b.zs);
+    const int32_t* nb_count = akz_zoff(b.nb_count, b.zs);
+    const int32_t* nb_list = akz_zoff(b.nb_list, b.zs);
+    int lo_[NMS_PER], lc_[NMS_PER], q0_[NMS_PER], q1_[NMS_PER];
+    float rv[NMS_PER];
+    bool sg[NMS_PER];
 #pragma unroll
-      for (int j = 0; j < PER; j++) {
-        const int i = t + j * NMS_THREADS;
-        if (i < n) { rv[j] = cand[i].response; sz[j] = cand[i].size; }
+    for (int j = 0; j < NMS_PER; j++) {
+      const int c = t + j * NMS_THREADS;
+      lo_[j] = lc_[j] = 0;
+      rv[j] = 0.0f;
+      sg[j] = false;
+      if (c < n) {
+        lo_[j] = nb_off[c];
+        lc_[j] = nb_count[c];
+        rv[j] = cand[c].response;
+        sg[j] = !(cand[c].size < 0);
       }
+    }
 #pragma unroll
-      for (int j = 0; j <= PER; j++) {
-        const int i = t + j * NMS_THREADS;
-        if (i <= n) ov[j] = nb_off[i];
-      }
-      constexpr int PER4 = NMS_LDS_NB / 4 / NMS_THREADS;
-      int4 lv[PER4];
-      const int n4 = (ntot + 3) >> 2;
+    for (int j = 0; j < NMS_PER; j++) {
+      q0_[j] = lc_[j] > 0 ? nb_list[lo_[j]] : 0;
+      q1_[j] = lc_[j] > 1 ? nb_list[lo_[j] + 1] : 0;
+    }
+    const bool wave0 = b.max_waves > 0 && n > 0;
+    if (t == 0) s_nact[0] = 0;
+    __syncthreads();
+    // Wave 0 is part of the set-up: the candidates without neighbours are
+    // kept, the others form the list of pending candidates.
 #pragma unroll
-      for (int j = 0; j < PER4; j++) {
-        const int k4 = t + j * NMS_THREADS;
-        if (4 * k4 + 3 < ntot) {
-          lv[j] = reinterpret_cast<const int4*>(nb_list)[k4];
-        } else if (k4 < n4) {
-          lv[j].x = nb_list[4 * k4];
-          lv[j].y = 4 * k4 + 1 < ntot ? nb_list[4 * k4 + 1] : 0;
-          lv[j].z = 4 * k4 + 2 < ntot ? nb_list[4 * k4 + 2] : 0;
-          lv[j].w = 0;
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < PER; j++) {
-        const int i = t + j * NMS_THREADS;
-        if (i < n) {
-          s_mem[i] = -1;
-          s_minnb[i] = INT32_MAX;
-          s_resp[i] = rv[j];
-          s_strong[i] = !(sz[j] < 0);
-        }
-      }
-#pragma unroll
-      for (int j = 0; j <= PER; j++) {
-        const int i = t + j * NMS_THREADS;
-        if (i <= n) s_off[i] = (uint16_t)akz_imin(ov[j], b.nb_cap);
-        if (j < PER) {
-          if (i < n) {
-            rs[j] = akz_imin(ov[j], b.nb_cap);
-            pend |= 1u << j;
+    for (int j = 0; j < NMS_PER; j++) {
+      if (j * NMS_THREADS >= n) break;
+      const int c = t + j * NMS_THREADS;
+      bool pending = false;
+      if (c < n) {
+        const int cnt = lc_[j];
+        uint32_t f2 = (uint32_t)q1_[j];
+        if (cnt >= 3) {
+          int o = NMS_FEXT;
+          if (cnt <= NMS_CNT_MAX) o = atomicAdd(&s_ftop, cnt - 1);
+          if (o + cnt - 1 <= NMS_FEXT) {
+            for (int k = 1; k < cnt; k++) s_fext[o + k - 1] = (int16_t)nb_list[lo_[j] + k];
+            f2 = (uint32_t)o;
           } else {
-            rs[j] = 0;
+            s_spill = 1;
           }
-          cd[j] = CODE_WAIT;
         }
+        s_fwd[c] = (uint32_t)q0_[j] | (f2 << 13) | ((uint32_t)akz_imin(cnt, NMS_CNT_MAX) << 26);
+        const bool add = wave0 && cnt == 0;
+        s_mem[c] = add ? (int16_t)c : (int16_t)-1;
+        s_minnb[c] = INT32_MAX;
+        s_resp[c] = rv[j];
+        pending = !add;
       }
-#pragma unroll
-      for (int j = 0; j < PER4; j++) {
-        const int k4 = t + j * NMS_THREADS;
-        if (k4 < n4) {
-          // 4 * k4 + 3 < 4 * n4 <= NMS_LDS_NB: inside s_nb
-          s_nb[4 * k4] = (int16_t)lv[j].x;
-          s_nb[4 * k4 + 1] = (int16_t)lv[j].y;
-          s_nb[4 * k4 + 2] = (int16_t)lv[j].z;
-          s_nb[4 * k4 + 3] = (int16_t)lv[j].w;
-        }
+      // candidates c - lane .. c - lane + 63 are this wavefront's slot j
+      const uint64_t sb = __ballot(sg[j]);
+      if (lane == 0) {
+        const int w0 = (c - lane) >> 5;
+        s_strong[w0] = (uint32_t)sb;
+        s_strong[w0 + 1] = (uint32_t)(sb >> 32);
+      }
+      const uint64_t bm = __ballot(pending);
+      if (bm != 0ull) {
+        const int leader = __ffsll((unsigned long long)bm) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&s_nact[0], __popcll(bm));
+        base = __shfl(base, leader, 64);
+        if (pending) s_list[base + __popcll(bm & ((1ull << lane) - 1ull))] = (int16_t)c;
       }
     }
     __syncthreads();
-    // list ends: the next candidate's start
-#pragma unroll
-    for (int j = 0; j < PER; j++) {
-      const int i = t + j * NMS_THREADS;
-      e1[j] = i < n ? (int)s_off[i + 1] : 0;
-    }
-    bool any = n > 0;
-    while (wave < b.max_waves && any) {
-      // the flag of this wave's parity was last read before the previous
-      // wave's first barrier; thread 0 clears it, wavefronts with work left
-      // set it after this wave's second phase
-      if (t == 0) s_left[wave & 1] = 0;
-      // Arbitration key of this wave: every neighbour of a candidate has a
+    // a list that does not fit the packed form or the pool: the whole
+    // suppression on the lists in global memory
+    if (s_spill) global_lists = true;
+    if (!global_lists) {
+      if (wave0) wave = 1;
+      int cur = 0;
+      int P = s_nact[0];
+      // Arbitration key of a wave: every neighbour of a candidate has a
       // lower index, so the lowest pending candidate is always ready and
-      // wins; a wave with work left therefore has wave < n <= NMS_LDS_N and
+      // wins; a wave with work left therefore has wave < n <= NMS_LDS_N, and
       // keys of later waves are smaller than every key left from earlier
       // ones, which makes a reset of the slots unnecessary.
-      const int tag = (NMS_LDS_N - 1 - wave) << 13;
-      // phase 1: classify ready candidates and arbitrate contention
+      while (wave < b.max_waves && P > NMS_TAIL) {
+        const int nxt = cur ^ 1;
+        // the other length was last read before the previous wave's barriers
+        if (t == 0) s_nact[nxt] = 0;
+        const int tag = (NMS_LDS_N - 1 - wave) << 13;
+        // list entries per thread (uniform); wavefronts past the list skip
+        // both phases
+        const int ne = __builtin_amdgcn_readfirstlane(((t & ~63) < P) ? (P + NMS_THREADS - 1) / NMS_THREADS : 0);
+        int cc[NMS_PER], cd[NMS_PER];
+        uint32_t fw[NMS_PER];
+        int m0[NMS_PER], m1[NMS_PER];
+        // phase 1: entries, their packed lists, their neighbours' member
+        // values (the second slot also for longer lists: an in-range pool
+        // offset whose value is not used), one round each (the loops end at
+        // the uniform ne: scalar branches skip the rest)
 #pragma unroll
-      for (int j = 0; j < PER; j++) {
-        if (!(pend & (1u << j))) continue;
-        const int c = t + j * NMS_THREADS;
-        // a processed neighbour never becomes pending again: resume the
-        // readiness scan at the first neighbour that was pending
-        // (two neighbours per step: both loads in flight together)
-        int k = rs[j];
-        const int e = e1[j];
-        for (; k + 1 < e; k += 2) {
-          const int ma = s_mem[s_nb[k]], mb = s_mem[s_nb[k + 1]];
-          if (ma == -1) break;
-          if (mb == -1) { k++; break; }
+        for (int k = 0; k < NMS_PER; k++) {
+          if (k >= ne) break;
+          cc[k] = -1;
+          if (t + k * NMS_THREADS < P) cc[k] = s_list[t + k * NMS_THREADS];
         }
-        if (k == e - 1 && s_mem[s_nb[k]] != -1) k = e;
-        rs[j] = k;
-        int code = CODE_WAIT;
-        if (k == e) {
-          // the kept neighbour with the smallest member value, first in
-          // list order on ties; four neighbours per step
-          int best = -1, bestv = INT32_MAX;
-          int q0 = s_off[c];
-          for (; q0 + 3 < e; q0 += 4) {
-            const int qa = s_nb[q0], qb = s_nb[q0 + 1], qc = s_nb[q0 + 2], qd = s_nb[q0 + 3];
-            const int ma = s_mem[qa], mb = s_mem[qb], mc = s_mem[qc], md = s_mem[qd];
-            if (ma >= 0 && ma < bestv) { bestv = ma; best = qa; }
-            if (mb >= 0 && mb < bestv) { bestv = mb; best = qb; }
-            if (mc >= 0 && mc < bestv) { bestv = mc; best = qc; }
-            if (md >= 0 && md < bestv) { bestv = md; best = qd; }
-          }
-          for (; q0 < e; q0++) {
-            const int q = s_nb[q0];
-            const int m = s_mem[q];
-            if (m >= 0 && m < bestv) { bestv = m; best = q; }
-          }
-          if (best < 0) code = CODE_ADD;
-          else { code = best; atomicMin(&s_minnb[best], tag | c); }
-        }
-        cd[j] = code;
-      }
-      __syncthreads();
-      // phase 2: additions and replacements (targets are kept, never pending)
 #pragma unroll
-      for (int j = 0; j < PER; j++) {
-        if (!(pend & (1u << j))) continue;
-        const int c = t + j * NMS_THREADS;
-        const int code = cd[j];
-        if (code == CODE_ADD) {
-          s_mem[c] = (int16_t)c;
-          pend &= ~(1u << j);
-        } else if (code >= 0 && s_minnb[code] == (tag | c)) {
-          if (s_resp[c] > s_resp[code]) {
-            s_mem[c] = (int16_t)code;
-            s_mem[code] = -2;
-          } else {
-            s_mem[c] = -2;
-          }
-          pend &= ~(1u << j);
+        for (int k = 0; k < NMS_PER; k++) {
+          if (k >= ne) break;
+          fw[k] = cc[k] >= 0 ? s_fwd[cc[k]] : 0u;
         }
-      }
-      const bool wave_left = __ballot(pend != 0) != 0;
-      if (wave_left && (t & 63) == 0) s_left[wave & 1] = 1;
-      __syncthreads();
-      any = s_left[wave & 1] != 0;
-      wave++;
-    }
-    // n <= NMS_LDS_N: at most 4 chunks of 256 words, side by side in s_bits
-    static_assert(((NMS_LDS_N + 2047) / 2048) * (NMS_LDS_N / 32) <= NMS_BITMAP_WORDS, "chunk bitmaps fit");
-    kept = nms_output_flat(
-        cand, n, [&](int i) { return (int)s_mem[i]; }, [&](int v) { return s_strong[v] != 0; }, out, kidx, cap,
-        s_bits, s_pre, s_w);
-  } else {
-    int32_t* mem = akz_zoff(b.mem, b.zs);
-    int32_t* minnb = akz_zoff(b.minnb, b.zs);
-    int32_t* g_code = akz_zoff(b.code, b.zs);
-    int32_t* act[2] = {akz_zoff(b.act, b.zs), akz_zoff(b.act, b.zs) + cap};
-    for (int i = t; i < n; i += NMS_THREADS) {
-      mem[i] = -1;
-      minnb[i] = INT32_MAX;
-      act[0][i] = i;
-    }
-    if (t == 0) { s_nact[0] = n; s_nact[1] = 0; }
-    __syncthreads();
-    int cur = 0;
-    while (wave < b.max_waves) {
-      const int na = s_nact[cur];
-      if (na == 0) break;
-      const int32_t* A = act[cur];
-      for (int j = t; j < na; j += NMS_THREADS) {
-        const int c = A[j];
-        const int e0 = akz_imin(nb_off[c], b.nb_cap), e1 = akz_imin(nb_off[c + 1], b.nb_cap);
-        bool ready = true, anykept = false;
-        int best = -1, bestv = INT32_MAX;
-        for (int k = e0; k < e1; k++) {
-          const int q = nb_list[k];
-          const int m = mem[q];
-          if (m == -1) { ready = false; break; }
-          if (m >= 0) {
-            anykept = true;
-            if (m < bestv) { bestv = m; best = q; }
+#pragma unroll
+        for (int k = 0; k < NMS_PER; k++) {
+          if (k >= ne) break;
+          m0[k] = m1[k] = 0;
+          if (cc[k] >= 0) {
+            m0[k] = s_mem[fw[k] & 0x1fffu];
+            m1[k] = s_mem[(fw[k] >> 13) & 0x1fffu];
           }
         }
-        int code = CODE_WAIT;
-        if (ready) {
-          if (!anykept) code = CODE_ADD;
-          else { code = best; atomicMin(&minnb[best], c); }
+        // classification in list order (smallest member value, first on
+        // ties) and arbitration
+#pragma unroll
+        for (int k = 0; k < NMS_PER; k++) {
+          if (k >= ne) break;
+          cd[k] = CODE_WAIT;
+          if (cc[k] < 0) continue;
+          const int c = cc[k];
+          const int code = nms_classify(fw[k], m0[k], m1[k], s_mem, s_fext);
+          if (code >= 0) atomicMin(&s_minnb[code], tag | c);
+          cd[k] = code;
         }
-        g_code[c] = code;
-      }
-      __syncthreads();
-      for (int j = t; j < na; j += NMS_THREADS) {
-        const int c = A[j];
-        const int code = g_code[c];
-        if (code == CODE_ADD) {
-          mem[c] = c;
-        } else if (code >= 0 && minnb[code] == c) {
-          if (cand[c].response > cand[code].response) {
-            mem[c] = code;
-            mem[code] = -2;
-          } else {
-            mem[c] = -2;
+        __syncthreads();
+        // phase 2: arbitration results and responses in one round, then
+        // additions and replacements (targets are kept, never pending)
+        int mn[NMS_PER];
+        float rc[NMS_PER], rq[NMS_PER];
+#pragma unroll
+        for (int k = 0; k < NMS_PER; k++) {
+          if (k >= ne) break;
+          mn[k] = -1;
+          rc[k] = rq[k] = 0.0f;
+          if (cd[k] >= 0) {
+            mn[k] = s_minnb[cd[k]];
+            rc[k] = s_resp[cc[k]];
+            rq[k] = s_resp[cd[k]];
           }
         }
+#pragma unroll
+        for (int k = 0; k < NMS_PER; k++) {
+          if (k >= ne) break;
+          const int c = cc[k];
+          const bool pending = c >= 0 && !nms_resolve(c, cd[k], mn[k] == (tag | c), rc[k], rq[k], s_mem);
+          // the candidates left pending go to the next list: one LDS atomic
+          // per wavefront, lanes at their rank among the wavefront's
+          const uint64_t bm = __ballot(pending);
+          if (bm != 0ull) {
+            const int leader = __ffsll((unsigned long long)bm) - 1;
+            int base = 0;
+            if (lane == leader) base = atomicAdd(&s_nact[nxt], __popcll(bm));
+            base = __shfl(base, leader, 64);
+            if (pending) s_list[base + __popcll(bm & ((1ull << lane) - 1ull))] = (int16_t)c;
+          }
+        }
+        __syncthreads();
+        P = s_nact[nxt];
+        cur = nxt;
+        wave++;
       }
-      __syncthreads();
-      const int nxt = cur ^ 1;
-      for (int j = t; j < na; j += NMS_THREADS) {
-        const int c = A[j];
-        const int code = g_code[c];
-        if (code >= 0) minnb[code] = INT32_MAX;
-        if (mem[c] == -1) {
-          int pos = atomicAdd(&s_nact[nxt], 1);
-          act[nxt][pos] = c;
+      // the last waves: wavefront 0, up to NMS_TAIL / 64 pending candidates
+      // per lane, no barriers
+      if (t < 64 && wave < b.max_waves && P > 0) {
+        constexpr int TE = NMS_TAIL / 64;
+        int c[TE];
+        uint32_t f[TE];
+#pragma unroll
+        for (int u = 0; u < TE; u++) c[u] = lane + 64 * u < P ? (int)s_list[lane + 64 * u] : -1;
+#pragma unroll
+        for (int u = 0; u < TE; u++) f[u] = c[u] >= 0 ? s_fwd[c[u]] : 0u;
+        auto any_left = [&]() {
+          bool a = false;
+#pragma unroll
+          for (int u = 0; u < TE; u++) a |= c[u] >= 0;
+          return __ballot(a) != 0ull;
+        };
+        while (wave < b.max_waves && any_left()) {
+          const int tag = (NMS_LDS_N - 1 - wave) << 13;
+          int m0[TE], m1[TE], code[TE];
+#pragma unroll
+          for (int u = 0; u < TE; u++) {
+            m0[u] = m1[u] = 0;
+            if (c[u] >= 0) { m0[u] = s_mem[f[u] & 0x1fffu]; m1[u] = s_mem[(f[u] >> 13) & 0x1fffu]; }
+          }
+#pragma unroll
+          for (int u = 0; u < TE; u++) {
+            code[u] = CODE_WAIT;
+            if (c[u] >= 0) {
+              code[u] = nms_classify(f[u], m0[u], m1[u], s_mem, s_fext);
+              if (code[u] >= 0) atomicMin(&s_minnb[code[u]], tag | c[u]);
+            }
+          }
+          // the atomics of every lane are done before these reads (in-order
+          // LDS instructions of one wavefront)
+          __builtin_amdgcn_wave_barrier();
+          int mn[TE];
+          float rc[TE], rq[TE];
+#pragma unroll
+          for (int u = 0; u < TE; u++) {
+            mn[u] = -1;
+            rc[u] = rq[u] = 0.0f;
+            if (c[u] >= 0 && code[u] >= 0) { mn[u] = s_minnb[code[u]]; rc[u] = s_resp[c[u]]; rq[u] = s_resp[code[u]]; }
+          }
+#pragma unroll
+          for (int u = 0; u < TE; u++)
+            if (c[u] >= 0 && nms_resolve(c[u], code[u], mn[u] == (tag | c[u]), rc[u], rq[u], s_mem)) c[u] = -1;
+          __builtin_amdgcn_wave_barrier();
+          wave++;
         }
       }
       __syncthreads();
-      if (t == 0) s_nact[cur] = 0;
-      cur = nxt;
-      wave++;
-      __syncthreads();
+      // n <= NMS_LDS_N: at most 4 chunks of 256 words, side by side
+      static_assert(((NMS_LDS_N + 2047) / 2048) * (NMS_LDS_N / 32) <= NMS_BITMAP_WORDS, "chunk bitmaps fit");
+      static_assert(2 * NMS_BITMAP_WORDS * 4 <= NMS_LDS_N * 2, "output scratch fits the list");
+      int32_t* scr = reinterpret_cast<int32_t*>(s_list);
+      kept = nms_output_flat(
+          cand, n, [&](int i) { return (int)s_mem[i]; },
+          [&](int v) { return ((s_strong[v >> 5] >> (v & 31)) & 1u) != 0; }, out, kidx, cap,
+          reinterpret_cast<uint32_t*>(scr), scr + NMS_BITMAP_WORDS, s_w);
     }
-    kept = nms_output(
-        cand, n, [&](int i) { return mem[i]; }, [&](int v) { return !(cand[v].size < 0); }, out, kidx, cap, s_sort,
-        s_bits, s_pre, s_w, &s_run);
   }
+  if (global_lists) {
+    const int32_t* mem = akz_zoff(b.mem, b.zs);
+    int32_t* scr = s_minnb;
+    static_assert(2048 + 2 * NMS_BITMAP_WORDS <= NMS_LDS_N, "output scratch fits");
+    wave = nms_waves_global(b, cand, n, s_nact);
+    kept = nms_output(
+        cand, n, [&](int i) { return mem[i]; }, [&](int v) { return !(cand[v].size < 0); }, out, kidx, cap, scr,
+        reinterpret_cast<uint32_t*>(scr) + 2048, scr + 2048 + NMS_BITMAP_WORDS, s_w, &s_run);
+  }
+  __syncthreads();
   if (t == 0) {
+    const int flags = st[AKZ_ST_FLAGS];
     st[AKZ_ST_WAVES] = wave;
     st[AKZ_ST_KEPT] = akz_imin(kept, cap);
-    // the whole block (flags and total come from earlier kernels) into the
+    // the whole block (the total comes from the candidate kernel) into the
     // contiguous per-batch array the host reads with one copy
-    int4 blk = make_int4(akz_imin(kept, cap), st[AKZ_ST_FLAGS], st[AKZ_ST_TOTAL], wave);
+    int4 blk = make_int4(akz_imin(kept, cap), flags, st[AKZ_ST_TOTAL], wave);
     reinterpret_cast<int4*>(b.status_all)[blockIdx.z] = blk;
   }
 }
 
-inline int grid_stride_blocks(int cap) {
-  int b = (cap + 255) / 256;
-  return b < 2048 ? (b > 0 ? b : 1) : 2048;
-}
-
-void xscan(hipStream_t s, const int32_t* cnt, int len, const int32_t* n_ptr, int32_t* bsum, int32_t* off, int nz,
-           size_t zs) {
-  // grid covers indices 0..len (len + 1 offsets)
-  const int nblk = (len + 1 + SCAN_BLK - 1) / SCAN_BLK;
-  hipLaunchKernelGGL(k_xscan_reduce, dim3(nblk, 1, nz), dim3(SCAN_T), 0, s, cnt, len, n_ptr, bsum, zs);
-  hipLaunchKernelGGL(k_xscan_down, dim3(nblk, 1, nz), dim3(SCAN_T), 0, s, cnt, len, n_ptr, bsum, off, zs);
-}
+constexpr int NB_LIST_BLOCKS = 64;  // k_nb_lists' grid: 16384 threads walk the candidates
 
 }  // namespace
 
@@ -1210,9 +1369,6 @@ void akz_launch_candidates(hipStream_t s, const akz_det_params& p, const akz_det
 }
 
 void akz_launch_suppression(hipStream_t s, const akz_det_bufs& b, int nz) {
-  const int g = grid_stride_blocks(b.cap);
-  hipLaunchKernelGGL(k_neighbors<false>, dim3(g, 1, nz), dim3(256), 0, s, b);
-  xscan(s, b.nb_count, b.cap, b.ncand, b.nb_bsum, b.nb_off, nz, b.zs);
-  hipLaunchKernelGGL(k_neighbors<true>, dim3(g, 1, nz), dim3(256), 0, s, b);
-  hipLaunchKernelGGL(k_nms, dim3(1, 1, nz), dim3(NMS_THREADS), 0, s, b);
+  hipLaunchKernelGGL(k_nb_lists, dim3(NB_LIST_BLOCKS, 1, nz), dim3(256), 0, s, b);
+  hipLaunchKernelGGL(k_suppress, dim3(1, 1, nz), dim3(NMS_THREADS), 0, s, b);
 }

@@ -91,9 +91,10 @@ struct akz_det_bufs {
   int32_t* level_start;  // nlev + 1
   int32_t* ncand;        // candidates after the capacity cut
   akz_kp* cand;          // cap
-  int32_t* nb_count;     // cap
-  int32_t* nb_off;       // cap + 1 CSR offsets
-  int32_t* nb_bsum;      // akz_scan_blocks(cap)
+  float2* cand_xy;       // cap: the candidates' positions (x, y), the neighbour search's input
+  int32_t* nb_count;     // cap: list lengths (lists of k_nb_pool or of the suppression's global mode)
+  int32_t* nb_off;       // cap: list starts in nb_list
+  int32_t* nb_pool;      // 64-bit allocator of nb_list entries (k_nb_pool), 8-byte aligned
   int32_t* nb_list;      // nb_cap
   int32_t *mem, *minnb, *code;  // cap each (suppression state beyond LDS)
   int32_t* act;          // 2 x cap
