@@ -85,6 +85,7 @@ SIGNATURES = {
     "akz_input_buffer": [_P, _P, _P],
     "akz_set_graph_mode": [_P, _I],
     "akz_set_keep_determinant": [_P, _I],
+    "akz_set_suppression_pool": [_P, _I],
     "akz_read_level": [_P, _I, _I, _P],
     "akz_read_kcontrast": [_P, _I, _P],
     "akz_read_candidates": [_P, _P, _I, _P, _P, _P],
@@ -393,6 +394,11 @@ class AKAZE:
                                     ctypes.byref(s), ctypes.byref(e)), "akz_level_size")
         return w.value, h.value, o.value, s.value, e.value
 
+    def set_suppression_pool(self, entries):
+        """Cap the suppression's LDS pool for long neighbour lists (0..4096);
+        images whose lists do not fit take the global-memory path."""
+        _check(lib().akz_set_suppression_pool(self.ctx, int(entries)), "akz_set_suppression_pool")
+
     # --- reference API
     def Create_Nonlinear_Scale_Space(self, img):
         img = np.ascontiguousarray(img, np.float32)
@@ -636,6 +642,11 @@ class BatchAKAZE:
         """Write full determinant images in process_async runs (for read_level(.., 4)
         and extract); by default only the candidates' neighbourhoods are written."""
         _check(lib().akz_set_keep_determinant(self.ctx, int(bool(keep))), "akz_set_keep_determinant")
+
+    def set_suppression_pool(self, entries):
+        """Cap the suppression's LDS pool for long neighbour lists (0..4096);
+        images whose lists do not fit take the global-memory path."""
+        _check(lib().akz_set_suppression_pool(self.ctx, int(entries)), "akz_set_suppression_pool")
 
     def set_proposal(self, net, threshold=0.5):
         """Detect with the learned tile proposal `net` (None: full scan)."""

@@ -375,6 +375,7 @@ akz_status allocate(akz_ctx* c) {
   D.cap = c->cap;
   D.nb_cap = (int)nb_cap;
   D.max_waves = c->opts.nms_max_waves;
+  D.nms_pool = akz_suppression_pool_max();
   D.zs = c->zs;
   AKZ_HIP_TRY(hipMemsetAsync(c->slab, 0, c->slab_bytes, c->stream));
   AKZ_HIP_TRY(hipHostMalloc((void**)&c->h_small, 16 * (size_t)c->batch, hipHostMallocDefault));
@@ -967,6 +968,14 @@ akz_status akz_set_keep_determinant(akz_ctx* c, int keep) {
   if (!c) return AKZ_ERR_ARG;
   AKZ_HIP_TRY(hipStreamSynchronize(c->stream));
   c->keep_det = keep != 0;
+  drop_graphs(c);
+  return AKZ_OK;
+}
+
+akz_status akz_set_suppression_pool(akz_ctx* c, int entries) {
+  if (!c || entries < 0 || entries > akz_suppression_pool_max()) return AKZ_ERR_ARG;
+  AKZ_HIP_TRY(hipStreamSynchronize(c->stream));
+  c->db.nms_pool = entries;
   drop_graphs(c);
   return AKZ_OK;
 }

@@ -1123,7 +1123,7 @@ __global__ __launch_bounds__(NMS_THREADS) void k_suppress(akz_det_bufs b) {
         if (cnt >= 3) {
           int o = NMS_FEXT;
           if (cnt <= NMS_CNT_MAX) o = atomicAdd(&s_ftop, cnt - 1);
-          if (o + cnt - 1 <= NMS_FEXT) {
+          if (o + cnt - 1 <= b.nms_pool) {
             for (int k = 1; k < cnt; k++) s_fext[o + k - 1] = (int16_t)nb_list[lo_[j] + k];
             f2 = (uint32_t)o;
           } else {
@@ -1336,6 +1336,7 @@ constexpr int NB_LIST_BLOCKS = 64;  // k_nb_lists' grid: 16384 threads walk the 
 
 }  // namespace
 
+int akz_suppression_pool_max() { return NMS_FEXT; }
 int akz_scan_blocks(int len) { return (len + 1 + SCAN_BLK - 1) / SCAN_BLK; }
 int akz_seg_runs(int seg_total) { return seg_total / SS_T + 1; }
 

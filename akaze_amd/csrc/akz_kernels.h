@@ -103,9 +103,11 @@ struct akz_det_bufs {
   int32_t* status;       // AKZ_ST_* block
   int32_t* status_all;   // contiguous copies of every image's block (4 ints per image, not z-strided)
   int cap, nb_cap, max_waves;
+  int nms_pool;          // LDS pool entries the suppression may take for lists of three or more
   size_t zs;
 };
 int akz_scan_blocks(int len);
+int akz_suppression_pool_max();  // the suppression's LDS pool of neighbour entries
 int akz_seg_runs(int seg_total);  // blocks of k_scan_scatter: 256 segments each, index seg_total included
 void akz_launch_hessian(hipStream_t s, const akz_det_params& p, int nz);
 void akz_launch_extrema(hipStream_t s, const akz_det_params& p, const akz_det_bufs& b, int nz);

@@ -153,6 +153,7 @@ def test_batch_ragged_and_proposal_entry_points_validate_arguments():
     assert L.akz_set_proposal(None, None, ctypes.c_float(0.5)) == 1
     assert L.akz_synth_device_batch(ctypes.c_uint64(1), 0, 640, 480, None, 2560, 0) == 1
     assert L.akz_autotune(None, 1, 1) == 1
+    assert L.akz_set_suppression_pool(None, 0) == 1
 
 
 def test_bench_refuses_more_than_one_gpu():

@@ -138,3 +138,41 @@ def test_many_candidates(gpu_lib, W, H, thr):
     odesc = orc.describe()
     assert n == on and kps.tobytes() == orc.keypoints().tobytes()
     assert np.array_equal(desc, odesc)
+
+
+@pytest.mark.parametrize("W,H,seed,pool", [(640, 480, 0, 0), (1920, 1080, 1, 0), (1920, 1080, 1, 8)])
+def test_suppression_pool_spill(gpu_lib, W, H, seed, pool):
+    """Neighbour lists of three or more that do not fit the suppression's LDS
+    pool send the image down the global-memory path: same keypoints."""
+    A = gpu_lib
+    ak, orc, _ = run_both(A, W, H, seed)
+    ak.set_suppression_pool(pool)
+    n = ak.detect()
+    orc.detect()
+    cand, ls, waves = ak.read_candidates()
+    assert waves == orc.waves()
+    kps, desc = ak.describe()
+    odesc = orc.describe()
+    assert n == len(orc.keypoints()) and kps.tobytes() == orc.keypoints().tobytes()
+    assert np.array_equal(desc, odesc)
+
+
+@pytest.mark.parametrize("max_waves", [0, 1, 2, 5, 13])
+def test_suppression_wave_limit(gpu_lib, max_waves):
+    """A wave limit stops the suppression where the oracle's stops: wave 0
+    alone, a few block waves, and a stop inside the single-wavefront tail."""
+    A = gpu_lib
+    W, H, seed = 1920, 1080, 1
+    img = A.to_float(A.synth_image(seed, W, H))
+    ak = A.AKAZE(A.AKAZEOptions(img_width=W, img_height=H, nms_max_waves=max_waves))
+    ak.Create_Nonlinear_Scale_Space(img)
+    n = ak.detect()
+    orc = O.Pipeline(W, H, nms_max_waves=max_waves)
+    orc.scale_space(img)
+    on = orc.detect()
+    _, _, waves = ak.read_candidates()
+    assert waves == orc.waves() == min(max_waves, waves)
+    kps, desc = ak.describe()
+    odesc = orc.describe()
+    assert n == on and kps.tobytes() == orc.keypoints().tobytes()
+    assert np.array_equal(desc, odesc)
