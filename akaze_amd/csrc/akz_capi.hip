@@ -432,11 +432,11 @@ int level_block(const akz_ctx* c, int i) {
 // Whether level i's first FED launch forms the level's input, an exact
 // HalfSample2 of the previous level, while loading it (fused launches of the
 // LDS variants at an octave's first level).
-bool level_half_in_fed(const akz_ctx* c, int i) {
+bool level_half_in_fed(const akz_ctx* c, int i, int nz) {
   const Level& L = c->lev[i];
   const Level& P = c->lev[i - 1];
   const int bs = level_block(c, i), n = L.nsteps;
-  const bool fuse = bs > 1 && n > 0 && akz_fed_variant_fusable(c->fed_variant[i], std::min(bs, n));
+  const bool fuse = bs > 1 && n > 0 && akz_fed_variant_fusable(c->fed_variant[i], std::min(bs, n), nz);
   return fuse && L.octave > P.octave && P.w == 2 * L.w && P.h == 2 * L.h && akz_fed_variant_halves(c->fed_variant[i]);
 }
 
@@ -473,10 +473,10 @@ void enqueue_scale_space(akz_ctx* c, const float* img, int in_pitch, size_t in_z
     // the level's first FED launch builds Lsmooth and g where that pays
     // (akz_fed_variant_fusable); g goes through HBM only when further
     // launches of the level read it
-    const bool fuse = bs > 1 && n > 0 && akz_fed_variant_fusable(c->fed_variant[i], std::min(bs, n));
+    const bool fuse = bs > 1 && n > 0 && akz_fed_variant_fusable(c->fed_variant[i], std::min(bs, n), nz);
     // an exact halving at an octave's first level is formed inside that
     // fused launch's load where the variant can (the LDS ones)
-    const bool half_in_fed = level_half_in_fed(c, i);
+    const bool half_in_fed = level_half_in_fed(c, i, nz);
     const float* base;
     if (half_in_fed) {
       base = nullptr;
@@ -1099,7 +1099,7 @@ akz_status akz_profile_fed_batch(akz_ctx* c, const float* d_imgs, size_t pitch_b
     total += ms;
     const double px = (double)c->lev[i].w * c->lev[i].h * n;
     const int bs = level_block(c, i), ns = c->lev[i].nsteps, nl = (ns + bs - 1) / bs;
-    const bool fused = bs > 1 && ns > 0 && akz_fed_variant_fusable(c->fed_variant[i], std::min(bs, ns));
+    const bool fused = bs > 1 && ns > 0 && akz_fed_variant_fusable(c->fed_variant[i], std::min(bs, ns), n);
     // algorithmic traffic of one step: read L and g, write L (4 B each per
     // pixel); a level whose first launch also builds Lsmooth and g adds that
     // pass's (read L, write Lsmooth and g)
@@ -1112,7 +1112,7 @@ akz_status akz_profile_fed_batch(akz_ctx* c, const float* d_imgs, size_t pitch_b
     if (fused && nl > 1) least += 4.0 * px;
     // a launch that halves the previous level reads it (16 B per pixel of
     // the level) instead of the halved image
-    if (level_half_in_fed(c, i)) least += 12.0 * px;
+    if (level_half_in_fed(c, i, n)) least += 12.0 * px;
     launches += nl;
   }
   if (fed_ms) *fed_ms = total;
@@ -1212,7 +1212,7 @@ akz_status akz_autotune(akz_ctx* c, int nz, int reps) {
       // once to warm up, then `reps` timed times, ping-ponging Lt and the scratch
       const int bs = std::min(c->fed_block, akz_fed_variant_steps(v));
       const int nl = (L.nsteps + bs - 1) / bs;
-      const bool fuse = bs > 1 && akz_fed_variant_fusable(v, std::min(bs, L.nsteps));
+      const bool fuse = bs > 1 && akz_fed_variant_fusable(v, std::min(bs, L.nsteps), nz);
       // as in the pipeline: smooth + flow, fused into the first launch of
       // the blocked variants
       auto run_level = [&](int parity) {

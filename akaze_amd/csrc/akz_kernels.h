@@ -45,9 +45,9 @@ void akz_launch_fed_variant(hipStream_t s, int variant, const float* Lin, const 
                             int w, int h, const float* half_tau, int n, akz_batch bt);
 // whether a level's first launch of n steps with this variant also builds
 // the level's Lsmooth and conductance (k_fed_multi_sf, k_fed_reg_sf): one
-// launch instead of two; every LDS variant does, the register variants when
-// the region's halo is small
-bool akz_fed_variant_fusable(int variant, int n);
+// launch instead of two; every LDS variant does, the register variants for
+// one image per launch (nz = 1) and otherwise when the region's halo is small
+bool akz_fed_variant_fusable(int variant, int n, int nz);
 // With `parent` non-null (only for variants where akz_fed_variant_halves
 // holds; `in` is then unused), the input is HalfSample2 of `parent`, an
 // image of exactly twice the level's width and height with row pitch ppitch.
