@@ -806,21 +806,33 @@ __global__ __launch_bounds__(64) void k_smooth_flow_reg(const float* __restrict_
 // the image the conductance is computed from clamped input: finite, and
 // only ever multiplied by a zero flux difference at the border.
 template <int R, int TYPE, bool BORDER>
-__device__ __forceinline__ void fed_reg_sf_body(const float* __restrict__ in, float* __restrict__ lsm,
-                                                float* __restrict__ gout, float* __restrict__ out, int pitch, int w,
-                                                int h, float invk, const akz_taps& taps, const akz_fed_steps& st,
-                                                int x0, int y0) {
+__device__ __forceinline__ void fed_reg_sf_body(const float* __restrict__ in, const float* __restrict__ parent,
+                                                int ppitch, float* __restrict__ lsm, float* __restrict__ gout,
+                                                float* __restrict__ out, int pitch, int w, int h, float invk,
+                                                const akz_taps& taps, const akz_fed_steps& st, int x0, int y0) {
   constexpr int NA = R + 6;
   const int lane = threadIdx.x;
   const int gx = x0 + lane;
   const int cx = BORDER ? akz_iclamp(gx, 0, w - 1) : gx;
   const int hn = st.n;
-  // A[i]: input row y0 - 3 + i; rows 3..R+2 are the FED rows
+  // A[i]: input row y0 - 3 + i; rows 3..R+2 are the FED rows.  With a
+  // parent (the previous level, exactly twice the size) the input is its
+  // 2x2 box mean at twice the clamped coordinates, k_half2's expression, so
+  // the rows equal a load of k_half2's output bit for bit.
   float A[NA];
+  if (parent) {
 #pragma unroll
-  for (int i = 0; i < NA; i++) {
-    const int gy = BORDER ? akz_iclamp(y0 - 3 + i, 0, h - 1) : y0 - 3 + i;
-    A[i] = in[(size_t)gy * pitch + cx];
+    for (int i = 0; i < NA; i++) {
+      const int gy = BORDER ? akz_iclamp(y0 - 3 + i, 0, h - 1) : y0 - 3 + i;
+      const float* p = parent + (size_t)(2 * gy) * ppitch + 2 * cx;
+      A[i] = 0.25f * (p[0] + p[1] + p[ppitch] + p[ppitch + 1]);
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < NA; i++) {
+      const int gy = BORDER ? akz_iclamp(y0 - 3 + i, 0, h - 1) : y0 - 3 + i;
+      A[i] = in[(size_t)gy * pitch + cx];
+    }
   }
   const float k0 = taps.k[0], k1 = taps.k[1], k2 = taps.k[2], k3 = taps.k[3], k4 = taps.k[4];
   const bool first_col = BORDER && gx == 0, last_col = BORDER && gx == w - 1;
@@ -890,11 +902,12 @@ __device__ __forceinline__ void fed_reg_sf_body(const float* __restrict__ in, fl
 
 template <int R, int TYPE>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(R == 48 ? 3 : 1)))
-void k_fed_reg_sf(const float* __restrict__ in, float* __restrict__ lsm, float* __restrict__ gout,
-                  float* __restrict__ out, int pitch, int w, int h, const float* kcp, akz_taps taps,
-                  akz_fed_steps st, size_t zs) {
+void k_fed_reg_sf(const float* __restrict__ in, const float* __restrict__ parent, int ppitch,
+                  float* __restrict__ lsm, float* __restrict__ gout, float* __restrict__ out, int pitch, int w,
+                  int h, const float* kcp, akz_taps taps, akz_fed_steps st, size_t zs) {
   const akz_tile3 tl = akz_xcd_tile();
-  in = akz_zoff(in, zs, tl.z);
+  if (parent) parent = akz_zoff(parent, zs, tl.z);
+  else in = akz_zoff(in, zs, tl.z);
   lsm = akz_zoff(lsm, zs, tl.z);
   if (gout) gout = akz_zoff(gout, zs, tl.z);
   out = akz_zoff(out, zs, tl.z);
@@ -907,9 +920,9 @@ void k_fed_reg_sf(const float* __restrict__ in, float* __restrict__ lsm, float* 
   // pixel on or beyond an image border
   const bool interior = x0 >= 1 && x0 + 65 <= w && y0 >= 3 && y0 + R + 3 <= h;
   if (interior)
-    fed_reg_sf_body<R, TYPE, false>(in, lsm, gout, out, pitch, w, h, invk, taps, st, x0, y0);
+    fed_reg_sf_body<R, TYPE, false>(in, parent, ppitch, lsm, gout, out, pitch, w, h, invk, taps, st, x0, y0);
   else
-    fed_reg_sf_body<R, TYPE, true>(in, lsm, gout, out, pitch, w, h, invk, taps, st, x0, y0);
+    fed_reg_sf_body<R, TYPE, true>(in, parent, ppitch, lsm, gout, out, pitch, w, h, invk, taps, st, x0, y0);
 }
 
 // One FED step, LDS-tiled: a 64x16 output tile plus a one-pixel halo of L and
@@ -1180,25 +1193,23 @@ bool akz_fed_variant_fusable(int variant, int n, int nz) {
 }
 
 template <int R>
-static void launch_fed_reg_sf(hipStream_t s, dim3 grid, const float* in, float* lsm, float* gout, float* Lout,
-                              int pitch, int w, int h, int type, const float* kc, const akz_taps& taps,
-                              const akz_fed_steps& st, size_t zs) {
-  if (type == 0)
-    hipLaunchKernelGGL((k_fed_reg_sf<R, 0>), grid, dim3(64), 0, s, in, lsm, gout, Lout, pitch, w, h, kc, taps, st, zs);
-  else if (type == 1)
-    hipLaunchKernelGGL((k_fed_reg_sf<R, 1>), grid, dim3(64), 0, s, in, lsm, gout, Lout, pitch, w, h, kc, taps, st, zs);
-  else if (type == 2)
-    hipLaunchKernelGGL((k_fed_reg_sf<R, 2>), grid, dim3(64), 0, s, in, lsm, gout, Lout, pitch, w, h, kc, taps, st, zs);
-  else
-    hipLaunchKernelGGL((k_fed_reg_sf<R, 3>), grid, dim3(64), 0, s, in, lsm, gout, Lout, pitch, w, h, kc, taps, st, zs);
+static void launch_fed_reg_sf(hipStream_t s, dim3 grid, const float* in, const float* parent, int ppitch,
+                              float* lsm, float* gout, float* Lout, int pitch, int w, int h, int type,
+                              const float* kc, const akz_taps& taps, const akz_fed_steps& st, size_t zs) {
+#define AKZ_REG_SF(TYPE_)                                                                                        \
+  hipLaunchKernelGGL((k_fed_reg_sf<R, TYPE_>), grid, dim3(64), 0, s, in, parent, ppitch, lsm, gout, Lout, pitch, w, \
+                     h, kc, taps, st, zs)
+  if (type == 0) AKZ_REG_SF(0);
+  else if (type == 1) AKZ_REG_SF(1);
+  else if (type == 2) AKZ_REG_SF(2);
+  else AKZ_REG_SF(3);
+#undef AKZ_REG_SF
 }
 
 // First launch of a level, fused with the level's smooth + flow
 // (k_fed_multi_sf for the LDS variants, k_fed_reg_sf for the register
 // ones); g is written only when `gout` is given.
-bool akz_fed_variant_halves(int variant) {
-  return variant >= 2 && variant <= AKZ_FED_VARIANT_MAX && variant != 12 && variant != 13 && variant != 16;
-}
+bool akz_fed_variant_halves(int variant) { return variant >= 2 && variant <= AKZ_FED_VARIANT_MAX; }
 
 void akz_launch_fed_variant_sf(hipStream_t s, int variant, const float* in, const float* parent, int ppitch,
                                float* lsm, float* gout, float* Lout, int pitch, int w, int h, int type,
@@ -1234,9 +1245,10 @@ void akz_launch_fed_variant_sf(hipStream_t s, int variant, const float* in, cons
       // 58 columns carry g; the tile is those minus the halo of n steps
       const int R = variant == 12 ? 64 : variant == 13 ? 48 : 32;
       const dim3 g((w + 57 - 2 * n) / (58 - 2 * n), (h + R - 1 - 2 * n) / (R - 2 * n), nz);
-      if (R == 64) launch_fed_reg_sf<64>(s, g, in, lsm, gout, Lout, pitch, w, h, type, kc, taps, st, zs);
-      else if (R == 48) launch_fed_reg_sf<48>(s, g, in, lsm, gout, Lout, pitch, w, h, type, kc, taps, st, zs);
-      else launch_fed_reg_sf<32>(s, g, in, lsm, gout, Lout, pitch, w, h, type, kc, taps, st, zs);
+      if (R == 64) launch_fed_reg_sf<64>(s, g, in, parent, ppitch, lsm, gout, Lout, pitch, w, h, type, kc, taps, st, zs);
+      else if (R == 48)
+        launch_fed_reg_sf<48>(s, g, in, parent, ppitch, lsm, gout, Lout, pitch, w, h, type, kc, taps, st, zs);
+      else launch_fed_reg_sf<32>(s, g, in, parent, ppitch, lsm, gout, Lout, pitch, w, h, type, kc, taps, st, zs);
       break;
     }
     default: AKZ_SF(64, 32, 256, FB_K); break;
