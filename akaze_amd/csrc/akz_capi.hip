@@ -323,8 +323,10 @@ akz_status allocate(akz_ctx* c) {
   size_t o_nbc = cv.take<int32_t>(c->cap + 4), o_nbo = cv.take<int32_t>(c->cap + 4);
   size_t o_nbs = cv.take<int32_t>(4);  // 64-bit pool counter
   size_t o_nbl = cv.take<int32_t>(nb_cap);
+  size_t o_nbp = cv.take<uint2>(c->cap);
   size_t o_mem = cv.take<int32_t>(c->cap), o_minnb = cv.take<int32_t>(c->cap), o_code = cv.take<int32_t>(c->cap);
   size_t o_act = cv.take<int32_t>((size_t)c->cap * 2);
+  size_t o_arb = cv.take<unsigned long long>(std::min(c->cap, 32768));
   size_t o_kps = cv.take<akz_kp>(c->cap), o_kidx = cv.take<int32_t>(c->cap);
   size_t o_desc = cv.take<uint8_t>((size_t)c->cap * AKZ_DESC_STRIDE);
   c->zs = align_up(cv.off, ALIGN);
@@ -364,10 +366,12 @@ akz_status allocate(akz_ctx* c) {
   D.nb_off = (int32_t*)(b + o_nbo);
   D.nb_pool = (int32_t*)(b + o_nbs);
   D.nb_list = (int32_t*)(b + o_nbl);
+  D.nb_pack = (uint2*)(b + o_nbp);
   D.mem = (int32_t*)(b + o_mem);
   D.minnb = (int32_t*)(b + o_minnb);
   D.code = (int32_t*)(b + o_code);
   D.act = (int32_t*)(b + o_act);
+  D.arb = (unsigned long long*)(b + o_arb);
   D.kps = c->kps;
   D.kidx = (int32_t*)(b + o_kidx);
   D.status = c->status;

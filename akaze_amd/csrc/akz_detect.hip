@@ -728,6 +728,9 @@ __global__ __launch_bounds__(256) void k_nb_lists(akz_det_bufs b) {
       }
       nb_off[c] = room > 0 ? (int)base : 0;
       nb_count[c] = room;
+      // the first two entries and min(count, 3) in one 8-byte word
+      akz_zoff(b.nb_pack, b.zs)[c] = make_uint2((uint32_t)(room > 0 ? r[0] : 0) | ((uint32_t)akz_imin(room, 3) << 30),
+                                                (uint32_t)(room > 1 ? r[1] : 0));
       if (room < cnt) atomicOr(akz_zoff(b.status, b.zs) + AKZ_ST_FLAGS, 1);
     }
   }
@@ -1028,6 +1031,198 @@ __device__ __forceinline__ bool nms_resolve(int c, int code, bool won, float rc,
   return false;
 }
 
+constexpr int NMS_MID_N = 32768;    // candidates whose member values and pending list fit LDS as 16-bit values
+constexpr int NMS_MID_CH = 8;       // list entries per round of staged loads
+constexpr int NMS_MID_E = NMS_MID_N / NMS_THREADS;  // list entries per thread at most
+static_assert(NMS_MID_N <= 32768 && NMS_MID_E % NMS_MID_CH == 0, "16-bit indices, whole rounds");
+
+// The waves for NMS_LDS_N < n <= NMS_MID_N candidates: member values and the
+// compacted list of pending candidates in LDS (16-bit), the neighbour lists
+// of k_nb_lists read from global memory, arbitration by a 64-bit atomic
+// minimum in global memory whose upper half carries the wave (as the LDS
+// path's keys: no reset of the slots).  Phase 1 leaves each list entry's
+// candidate index and code in one word of global memory (by list position,
+// read back by the same thread in phase 2), so the registers stay free for
+// the LDS path's waves; each round of NMS_MID_CH entries issues its loads
+// together.
+// Wave 0 is part of the set-up; the last waves (at most NMS_TAIL pending) run
+// on one wavefront without barriers.  Returns the number of waves.
+__device__ int nms_waves_mid(const akz_det_bufs& b, const akz_kp* __restrict__ cand, int n, int16_t* s_mem,
+                             int16_t* s_list, int32_t* s_nact) {
+  const int t = threadIdx.x, lane = t & 63;
+  const int32_t* __restrict__ nb_off = akz_zoff(b.nb_off, b.zs);
+  const int32_t* __restrict__ nb_count = akz_zoff(b.nb_count, b.zs);
+  const int32_t* __restrict__ nb_list = akz_zoff(b.nb_list, b.zs);
+  const uint2* __restrict__ nb_pack = akz_zoff(b.nb_pack, b.zs);
+  unsigned long long* arb = akz_zoff(b.arb, b.zs);
+  int32_t* g_pk = akz_zoff(b.code, b.zs);  // entry e: candidate | (code + 2) << 16, -1: none
+  const bool wave0 = b.max_waves > 0 && n > 0;
+  if (t == 0) s_nact[0] = 0;
+  __syncthreads();
+  for (int base = 0; base < n; base += NMS_THREADS) {
+    const int c = base + t;
+    bool pending = false;
+    if (c < n) {
+      const bool add = wave0 && nb_count[c] == 0;
+      s_mem[c] = add ? (int16_t)c : (int16_t)-1;
+      arb[c] = ~0ull;
+      pending = !add;
+    }
+    const uint64_t bm = __ballot(pending);
+    if (bm != 0ull) {
+      const int leader = __ffsll((unsigned long long)bm) - 1;
+      int o = 0;
+      if (lane == leader) o = atomicAdd(&s_nact[0], __popcll(bm));
+      o = __shfl(o, leader, 64);
+      if (pending) s_list[o + __popcll(bm & ((1ull << lane) - 1ull))] = (int16_t)c;
+    }
+  }
+  __syncthreads();
+  int wave = wave0 ? 1 : 0;
+  int cur = 0;
+  int P = s_nact[0];
+  // classification of candidate c (neighbour count cnt at nb_list[o]) whose
+  // first two neighbours q0, q1 have member values m0, m1
+  auto classify = [&](int cnt, int o, int q0, int q1, int m0, int m1) {
+    bool ready = true;
+    int best = -1, bestv = INT32_MAX;
+    if (cnt >= 1) {
+      if (m0 == -1) ready = false;
+      else if (m0 >= 0) { bestv = m0; best = q0; }
+    }
+    if (cnt >= 2) {
+      if (m1 == -1) ready = false;
+      else if (m1 >= 0 && m1 < bestv) { bestv = m1; best = q1; }
+    }
+    for (int k = 2; k < cnt; k++) {
+      const int q = nb_list[o + k];
+      const int m = s_mem[q];
+      if (m == -1) ready = false;
+      else if (m >= 0 && m < bestv) { bestv = m; best = q; }
+    }
+    if (!ready) return CODE_WAIT;
+    return best < 0 ? CODE_ADD : best;
+  };
+  while (wave < b.max_waves && P > NMS_TAIL) {
+    const int nxt = cur ^ 1;
+    if (t == 0) s_nact[nxt] = 0;
+    const unsigned long long tag = (unsigned long long)(0xffffffffu - (unsigned)wave) << 32;
+    const int ne = __builtin_amdgcn_readfirstlane(((t & ~63) < P) ? (P + NMS_THREADS - 1) / NMS_THREADS : 0);
+#pragma unroll 1
+    for (int r = 0; r < NMS_MID_E / NMS_MID_CH; r++) {
+      if (r * NMS_MID_CH >= ne) break;
+      int c[NMS_MID_CH], o[NMS_MID_CH], cn[NMS_MID_CH], q0[NMS_MID_CH], q1[NMS_MID_CH];
+#pragma unroll
+      for (int k = 0; k < NMS_MID_CH; k++) {
+        const int e = t + (r * NMS_MID_CH + k) * NMS_THREADS;
+        c[k] = e < P ? (int)s_list[e] : -1;
+      }
+      uint2 pw[NMS_MID_CH];
+#pragma unroll
+      for (int k = 0; k < NMS_MID_CH; k++) pw[k] = c[k] >= 0 ? nb_pack[c[k]] : make_uint2(0u, 0u);
+#pragma unroll
+      for (int k = 0; k < NMS_MID_CH; k++) {
+        q0[k] = (int)(pw[k].x & 0x3fffffffu);
+        q1[k] = (int)pw[k].y;
+        cn[k] = (int)(pw[k].x >> 30);
+        o[k] = 0;
+        // lists of three or more: the count and start (rare)
+        if (cn[k] == 3) { o[k] = nb_off[c[k]]; cn[k] = nb_count[c[k]]; }
+      }
+#pragma unroll
+      for (int k = 0; k < NMS_MID_CH; k++) {
+        int code = CODE_WAIT;
+        if (c[k] >= 0) {
+          code = classify(cn[k], o[k], q0[k], q1[k], s_mem[q0[k]], s_mem[q1[k]]);
+          if (code >= 0) atomicMin(&arb[code], tag | (unsigned long long)c[k]);
+        }
+        const int e = t + (r * NMS_MID_CH + k) * NMS_THREADS;
+        if (e < P) g_pk[e] = c[k] < 0 ? -1 : (c[k] | ((code + 2) << 16));
+      }
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int r = 0; r < NMS_MID_E / NMS_MID_CH; r++) {
+      if (r * NMS_MID_CH >= ne) break;
+      unsigned long long mn[NMS_MID_CH];
+      float rc[NMS_MID_CH], rq[NMS_MID_CH];
+      int pv[NMS_MID_CH];
+#pragma unroll
+      for (int k = 0; k < NMS_MID_CH; k++) {
+        const int e = t + (r * NMS_MID_CH + k) * NMS_THREADS;
+        pv[k] = e < P ? g_pk[e] : -1;
+      }
+#pragma unroll
+      for (int k = 0; k < NMS_MID_CH; k++) {
+        const int v = pv[k];
+        const int c = v & 0xffff, code = (v >> 16) - 2;
+        mn[k] = 0ull;
+        rc[k] = rq[k] = 0.0f;
+        if (v >= 0 && code >= 0) {
+          mn[k] = __hip_atomic_load(&arb[code], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          rc[k] = cand[c].response;
+          rq[k] = cand[code].response;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < NMS_MID_CH; k++) {
+        const int v = pv[k];
+        const int c = v & 0xffff, code = (v >> 16) - 2;
+        bool pending = false;
+        if (v >= 0) {
+          const bool won = mn[k] == (tag | (unsigned long long)c);
+          pending = !nms_resolve(c, code, won, rc[k], rq[k], s_mem);
+        }
+        const uint64_t bm = __ballot(pending);
+        if (bm != 0ull) {
+          const int leader = __ffsll((unsigned long long)bm) - 1;
+          int o = 0;
+          if (lane == leader) o = atomicAdd(&s_nact[nxt], __popcll(bm));
+          o = __shfl(o, leader, 64);
+          if (pending) s_list[o + __popcll(bm & ((1ull << lane) - 1ull))] = (int16_t)c;
+        }
+      }
+    }
+    __syncthreads();
+    P = s_nact[nxt];
+    cur = nxt;
+    wave++;
+  }
+  // the last waves: wavefront 0, one pending candidate per lane
+  if (t < 64 && wave < b.max_waves && P > 0) {
+    static_assert(NMS_TAIL == 64, "one candidate per lane");
+    int c = lane < P ? (int)s_list[lane] : -1;
+    const uint2 pw = c >= 0 ? nb_pack[c] : make_uint2(0u, 0u);
+    int o = 0, cn = (int)(pw.x >> 30);
+    const int q0 = (int)(pw.x & 0x3fffffffu), q1 = (int)pw.y;
+    if (cn == 3) { o = nb_off[c]; cn = nb_count[c]; }
+    const float rc = c >= 0 ? cand[c].response : 0.0f;
+    while (wave < b.max_waves && __ballot(c >= 0) != 0ull) {
+      const unsigned long long tag = (unsigned long long)(0xffffffffu - (unsigned)wave) << 32;
+      int code = CODE_WAIT;
+      if (c >= 0) {
+        code = classify(cn, o, q0, q1, s_mem[q0], s_mem[q1]);
+        if (code >= 0) atomicMin(&arb[code], tag | (unsigned long long)c);
+      }
+      // every lane's atomic is done before the arbitration slots are read
+      __threadfence_block();
+      if (c >= 0 && code != CODE_WAIT) {
+        unsigned long long mn = 0ull;
+        float rq = 0.0f;
+        if (code >= 0) {
+          mn = __hip_atomic_load(&arb[code], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          rq = cand[code].response;
+        }
+        if (nms_resolve(c, code, mn == (tag | (unsigned long long)c), rc, rq, s_mem)) c = -1;
+      }
+      __builtin_amdgcn_wave_barrier();
+      wave++;
+    }
+  }
+  __syncthreads();
+  return wave;
+}
+
 // Wave-synchronous greedy suppression of FilterExtrema_kernel
 // (cuda_akaze.cu:1072-1224) and its output, one workgroup per image, on the
 // neighbour lists of k_nb_lists.  Per wave: a candidate whose neighbours are
@@ -1055,13 +1250,24 @@ __device__ __forceinline__ bool nms_resolve(int c, int code, bool won, float rc,
 // sees the previous one.  Beyond NMS_LDS_N candidates, or when a list does
 // not fit the packed form or the pool, the state is in global memory.
 __global__ __launch_bounds__(NMS_THREADS) void k_suppress(akz_det_bufs b) {
-  __shared__ int16_t s_mem[NMS_LDS_N];                   // member values (-1 pending, -2 dropped)
-  __shared__ __align__(16) int32_t s_minnb[NMS_LDS_N];   // arbitration slots; the global path's output scratch
-  __shared__ float s_resp[NMS_LDS_N];                    // responses
-  __shared__ uint32_t s_fwd[NMS_LDS_N];                  // packed neighbour lists
-  __shared__ int16_t s_fext[NMS_FEXT];                   // neighbours 1.. of lists of three or more
-  __shared__ __align__(16) int16_t s_list[NMS_LDS_N];    // pending candidates, then the output's scratch
-  __shared__ uint32_t s_strong[NMS_LDS_N / 32];          // bit i: candidate i is strong (size >= 0)
+  // One LDS buffer in two layouts.  Up to NMS_LDS_N candidates: arbitration
+  // slots, responses, packed neighbour lists, 16-bit member values, the
+  // pending list (then the output's scratch), the pool of long lists and a
+  // bitmap of strong candidates.  Up to NMS_MID_N: 16-bit member values and
+  // the pending list (then the output's scratch).  Beyond: the output's
+  // scratch.
+  constexpr int O_MINNB = 0, O_RESP = O_MINNB + 4 * NMS_LDS_N, O_FWD = O_RESP + 4 * NMS_LDS_N;
+  constexpr int O_MEM = O_FWD + 4 * NMS_LDS_N, O_LIST = O_MEM + 2 * NMS_LDS_N, O_FEXT = O_LIST + 2 * NMS_LDS_N;
+  constexpr int O_STRONG = O_FEXT + 2 * NMS_FEXT, SMALL_BYTES = O_STRONG + NMS_LDS_N / 8;
+  constexpr int MID_BYTES = 4 * NMS_MID_N;
+  __shared__ __align__(16) unsigned char s_raw[SMALL_BYTES > MID_BYTES ? SMALL_BYTES : MID_BYTES];
+  int32_t* s_minnb = reinterpret_cast<int32_t*>(s_raw + O_MINNB);
+  float* s_resp = reinterpret_cast<float*>(s_raw + O_RESP);
+  uint32_t* s_fwd = reinterpret_cast<uint32_t*>(s_raw + O_FWD);
+  int16_t* s_mem = reinterpret_cast<int16_t*>(s_raw + O_MEM);
+  int16_t* s_list = reinterpret_cast<int16_t*>(s_raw + O_LIST);
+  int16_t* s_fext = reinterpret_cast<int16_t*>(s_raw + O_FEXT);
+  uint32_t* s_strong = reinterpret_cast<uint32_t*>(s_raw + O_STRONG);
   __shared__ int32_t s_w[17];
   __shared__ int32_t s_nact[2];  // list lengths, by the parity of the wave
   __shared__ int32_t s_run;
@@ -1086,26 +1292,31 @@ __global__ __launch_bounds__(NMS_THREADS) void k_suppress(akz_det_bufs b) {
     const int32_t* nb_off = akz_zoff(b.nb_off, b.zs);
     const int32_t* nb_count = akz_zoff(b.nb_count, b.zs);
     const int32_t* nb_list = akz_zoff(b.nb_list, b.zs);
+    const uint2* nb_pack = akz_zoff(b.nb_pack, b.zs);
     int lo_[NMS_PER], lc_[NMS_PER], q0_[NMS_PER], q1_[NMS_PER];
     float rv[NMS_PER];
     bool sg[NMS_PER];
 #pragma unroll
     for (int j = 0; j < NMS_PER; j++) {
       const int c = t + j * NMS_THREADS;
-      lo_[j] = lc_[j] = 0;
+      uint2 pw = make_uint2(0u, 0u);
       rv[j] = 0.0f;
       sg[j] = false;
       if (c < n) {
-        lo_[j] = nb_off[c];
-        lc_[j] = nb_count[c];
+        pw = nb_pack[c];
         rv[j] = cand[c].response;
         sg[j] = !(cand[c].size < 0);
       }
+      q0_[j] = (int)(pw.x & 0x3fffffffu);
+      q1_[j] = (int)pw.y;
+      lc_[j] = (int)(pw.x >> 30);
+      lo_[j] = 0;
     }
+    // lists of three or more: the count and start (rare)
 #pragma unroll
     for (int j = 0; j < NMS_PER; j++) {
-      q0_[j] = lc_[j] > 0 ? nb_list[lo_[j]] : 0;
-      q1_[j] = lc_[j] > 1 ? nb_list[lo_[j] + 1] : 0;
+      const int c = t + j * NMS_THREADS;
+      if (lc_[j] == 3) { lo_[j] = nb_off[c]; lc_[j] = nb_count[c]; }
     }
     const bool wave0 = b.max_waves > 0 && n > 0;
     if (t == 0) s_nact[0] = 0;
@@ -1311,10 +1522,19 @@ __global__ __launch_bounds__(NMS_THREADS) void k_suppress(akz_det_bufs b) {
           reinterpret_cast<uint32_t*>(scr), scr + NMS_BITMAP_WORDS, s_w);
     }
   }
-  if (global_lists) {
+  if (global_lists && n <= NMS_MID_N) {
+    int16_t* m16 = reinterpret_cast<int16_t*>(s_raw);
+    int16_t* l16 = m16 + NMS_MID_N;
+    wave = nms_waves_mid(b, cand, n, m16, l16, s_nact);
+    int32_t* scr = reinterpret_cast<int32_t*>(l16);
+    static_assert(2048 + 2 * NMS_BITMAP_WORDS <= NMS_MID_N / 2, "output scratch fits the list");
+    kept = nms_output(
+        cand, n, [&](int i) { return (int)m16[i]; }, [&](int v) { return !(cand[v].size < 0); }, out, kidx, cap,
+        scr, reinterpret_cast<uint32_t*>(scr) + 2048, scr + 2048 + NMS_BITMAP_WORDS, s_w, &s_run);
+  } else if (global_lists) {
     const int32_t* mem = akz_zoff(b.mem, b.zs);
-    int32_t* scr = s_minnb;
-    static_assert(2048 + 2 * NMS_BITMAP_WORDS <= NMS_LDS_N, "output scratch fits");
+    int32_t* scr = reinterpret_cast<int32_t*>(s_raw);
+    static_assert(4 * (2048 + 2 * NMS_BITMAP_WORDS) <= SMALL_BYTES, "output scratch fits");
     wave = nms_waves_global(b, cand, n, s_nact);
     kept = nms_output(
         cand, n, [&](int i) { return mem[i]; }, [&](int v) { return !(cand[v].size < 0); }, out, kidx, cap, scr,

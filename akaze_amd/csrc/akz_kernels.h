@@ -96,8 +96,10 @@ struct akz_det_bufs {
   int32_t* nb_off;       // cap: list starts in nb_list
   int32_t* nb_pool;      // 64-bit allocator of nb_list entries (k_nb_pool), 8-byte aligned
   int32_t* nb_list;      // nb_cap
+  uint2* nb_pack;        // cap: first two list entries and min(count, 3) << 30 (x)
   int32_t *mem, *minnb, *code;  // cap each (suppression state beyond LDS)
   int32_t* act;          // 2 x cap
+  unsigned long long* arb;  // min(cap, 32768) 64-bit arbitration slots of the suppression's middle path
   akz_kp* kps;           // cap output keypoints
   int32_t* kidx;         // cap: candidate index of each output keypoint
   int32_t* status;       // AKZ_ST_* block
