@@ -618,26 +618,44 @@ __device__ __forceinline__ int nb_search(int c, float kx, float ky, float size, 
   const float half = size * .5f;
   const double r2 = (double)(size * size) * 0.25;
   int cnt = 0;
-  for (int i = c - 1; i >= start; i--) {
-    const float2 q = xy(i);
-    if (ky - q.y > half) break;
-    float dist = (kx - q.x) * (kx - q.x) + (ky - q.y) * (ky - q.y);
-    if ((double)dist < r2) emit(cnt++, i);
-  }
+  // walk i0, i0 - 1, ... down to lo, stopping at the first candidate more
+  // than half above; four positions per round of loads, examined in order
+  auto scan = [&](int i0, int lo) {
+    for (int i = i0; i >= lo; i -= 4) {
+      float2 q[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) q[u] = (i - u >= lo) ? xy(i - u) : make_float2(0.0f, 0.0f);
+      bool stop = false;
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        if (stop) break;
+        if (i - u < lo || ky - q[u].y > half) { stop = true; break; }
+        float dist = (kx - q[u].x) * (kx - q[u].x) + (ky - q[u].y) * (ky - q[u].y);
+        if ((double)dist < r2) emit(cnt++, i - u);
+      }
+      if (stop) break;
+    }
+  };
+  scan(c - 1, start);
   if (has_prev) {
     // first index in [prev_lo, start) with q.y - ky > half (monotone in the
-    // sorted order)
+    // sorted order; indices below a are false, from e on true): a four-way
+    // search, three pivots loaded together
     int a = prev_lo, e = start;
+    while (e - a >= 4) {
+      const int d = (e - a) >> 2;
+      const int m1 = a + d, m2 = a + 2 * d, m3 = a + 3 * d;
+      const bool p1 = xy(m1).y - ky > half, p2 = xy(m2).y - ky > half, p3 = xy(m3).y - ky > half;
+      if (p1) e = m1;
+      else if (p2) { a = m1 + 1; e = m2; }
+      else if (p3) { a = m2 + 1; e = m3; }
+      else a = m3 + 1;
+    }
     while (a < e) {
       const int m = (a + e) >> 1;
       if (xy(m).y - ky > half) e = m; else a = m + 1;
     }
-    for (int i = a - 1; i >= prev_lo; i--) {
-      const float2 q = xy(i);
-      if (ky - q.y > half) break;
-      float dist = (kx - q.x) * (kx - q.x) + (ky - q.y) * (ky - q.y);
-      if ((double)dist < r2) emit(cnt++, i);
-    }
+    scan(a - 1, prev_lo);
   }
   return cnt;
 }
@@ -663,13 +681,17 @@ constexpr int NMS_BITMAP_WORDS = 2048;  // bitmap ranking covers up to 65536 can
 // candidate's list is written by one thread, in list order, into space taken
 // from a pool (nb_off = start, nb_count = length); a wavefront takes the
 // space of its 64 lists with one atomic add, so no scan over the counts is
-// needed, and where the lists sit in the pool does not matter.  Positions
-// come from the compact cand_xy array; a block's threads take consecutive
-// candidates, whose searches cover overlapping ranges that stay in the CU's
-// cache.  Entries past nb_cap are dropped and flag an overflow; the
-// suppression reads only entries written in this run.
+// needed, and where the lists sit in the pool does not matter.  A block
+// takes 256 consecutive candidates; every position their searches can read
+// lies between the start of the previous level of the first one and the
+// last one, and that window is staged in LDS from the compact cand_xy array
+// when it fits (NB_WIN positions; else the searches read cand_xy).  Entries
+// past nb_cap are dropped and flag an overflow; the suppression reads only
+// entries written in this run.
 constexpr int NB_KEEP = 8;
+constexpr int NB_WIN = 8192;
 __global__ __launch_bounds__(256) void k_nb_lists(akz_det_bufs b) {
+  __shared__ float2 s_win[NB_WIN];
   const int n = *akz_zoff(b.ncand, b.zs);
   const akz_kp* __restrict__ cand = akz_zoff(b.cand, b.zs);
   const float2* __restrict__ cxy = akz_zoff(b.cand_xy, b.zs);
@@ -679,10 +701,17 @@ __global__ __launch_bounds__(256) void k_nb_lists(akz_det_bufs b) {
   int32_t* nb_list = akz_zoff(b.nb_list, b.zs);
   unsigned long long* pool = reinterpret_cast<unsigned long long*>(akz_zoff(b.nb_pool, b.zs));
   const int lane = threadIdx.x & 63;
-  auto xy = [&](int i) { return cxy[i]; };
-  // the loop runs while any lane of the wavefront has a candidate
-  for (int c0 = blockIdx.x * 256 + (threadIdx.x & ~63); c0 < n; c0 += gridDim.x * 256) {
-    const int c = c0 + lane;
+  // the loop runs while the block has candidates (uniform)
+  for (int cb = blockIdx.x * 256; cb < n; cb += gridDim.x * 256) {
+    const int ce = akz_imin(cb + 256, n);
+    const int lv0 = cand[cb].class_id;
+    const int win_lo = level_start[lv0 > 0 ? lv0 - 1 : 0];
+    const bool staged = ce - win_lo <= NB_WIN;
+    if (staged)
+      for (int i = win_lo + (int)threadIdx.x; i < ce; i += 256) s_win[i - win_lo] = cxy[i];
+    __syncthreads();
+    auto xy = [&](int i) { return staged ? s_win[i - win_lo] : cxy[i]; };
+    const int c = cb + (int)threadIdx.x;
     int cnt = 0, lvl = 0, start = 0, lo = 0;
     float2 p = make_float2(0.0f, 0.0f);
     float size = 0.0f;
@@ -733,6 +762,7 @@ __global__ __launch_bounds__(256) void k_nb_lists(akz_det_bufs b) {
                                                 (uint32_t)(room > 1 ? r[1] : 0));
       if (room < cnt) atomicOr(akz_zoff(b.status, b.zs) + AKZ_ST_FLAGS, 1);
     }
+    __syncthreads();  // the next block of candidates overwrites the window
   }
 }
 
