@@ -392,6 +392,75 @@ constexpr int FB_K = 8;      // steps per launch of most variants
 constexpr int FB_KMAX = 16;  // the deep-halo variants
 struct akz_fed_steps { float half_tau[FB_KMAX]; int n; };
 
+// The FED steps of one launch on an RW x RH LDS region (L in sa, g in sg;
+// sb the second buffer; the region's origin is image pixel (x0, y0)).
+// Thread t owns region pixels t + k * NT throughout the launch: g does not
+// change within it, so each owned pixel keeps its conductance pair sums
+// (f0 + gE, f0 + gW, f0 + gS, f0 + gN: the operands of k_fed_step's
+// fluxes, formed once) and its border clamps in registers, and a step
+// reads L only.  Step s updates the pixels at least s from the region edge
+// inside the image (one pixel of halo is consumed per step); at image
+// borders the outward neighbour is the pixel itself, as in NLDStep.
+// Returns the buffer holding the last step.
+template <int RW, int RH, int NT>
+__device__ __forceinline__ float* fed_lds_steps(float* sa, float* sb, const float* sg, const akz_fed_steps& st,
+                                                int x0, int y0, int w, int h) {
+  constexpr int LW = (RW == 128) ? 7 : (RW == 64) ? 6 : (RW == 32 ? 5 : 4);
+  static_assert((1 << LW) == RW, "power-of-two region width");
+  constexpr int PX = (RH * RW + NT - 1) / NT;
+  const int tid = threadIdx.x;
+  // per owned pixel: margin to the region edge (-1 outside the image) in
+  // bits 4.., self-neighbour flags E, W, S, N in bits 0..3
+  int pk[PX];
+  float se[PX], sw[PX], ss[PX], sn[PX];
+#pragma unroll
+  for (int k = 0; k < PX; k++) {
+    const int idx = tid + k * NT;
+    pk[k] = -1 << 4;
+    se[k] = sw[k] = ss[k] = sn[k] = 0.0f;
+    if (idx < RH * RW) {
+      const int r = idx >> LW, c = idx & (RW - 1);
+      const int gx = x0 + c, gy = y0 + r;
+      if (gx >= 0 && gy >= 0 && gx < w && gy < h) {
+        const int m = akz_imin(akz_imin(r, RH - 1 - r), akz_imin(c, RW - 1 - c));
+        const bool ee = gx >= w - 1, ww = gx <= 0, es = gy >= h - 1, en = gy <= 0;
+        pk[k] = (m << 4) | (ee ? 1 : 0) | (ww ? 2 : 0) | (es ? 4 : 0) | (en ? 8 : 0);
+        const float f0 = sg[idx];
+        se[k] = f0 + sg[ee ? idx : idx + 1];
+        sw[k] = f0 + sg[ww ? idx : idx - 1];
+        ss[k] = f0 + sg[es ? idx : idx + RW];
+        sn[k] = f0 + sg[en ? idx : idx - RW];
+      }
+    }
+  }
+  float* src = sa;
+  float* dst = sb;
+  for (int s = 1; s <= st.n; s++) {
+    const float stepsize = st.half_tau[s - 1];
+    const float* __restrict__ cs = src;
+    float* __restrict__ cd = dst;
+#pragma unroll
+    for (int k = 0; k < PX; k++) {
+      if ((pk[k] >> 4) < s) continue;
+      const int idx = tid + k * NT;
+      const int e = (pk[k] & 1) ? idx : idx + 1, wi = (pk[k] & 2) ? idx : idx - 1;
+      const int so = (pk[k] & 4) ? idx : idx + RW, no = (pk[k] & 8) ? idx : idx - RW;
+      const float i0 = cs[idx];
+      const float le = cs[e], lw = cs[wi], ls = cs[so], ln = cs[no];
+      float xpos = se[k] * (le - i0);
+      float xneg = sw[k] * (i0 - lw);
+      float ypos = ss[k] * (ls - i0);
+      float yneg = sn[k] * (i0 - ln);
+      cd[idx] = i0 + stepsize * (xpos - xneg + ypos - yneg);
+    }
+    __syncthreads();
+    float* t = src;
+    src = dst;
+    dst = t;
+  }
+  return src;
+}
+
 template <int RW, int RH, int NT, int K = FB_K>
 __global__ __launch_bounds__(NT) void k_fed_multi(const float* __restrict__ L, const float* __restrict__ g,
                                                   float* __restrict__ out, int pitch, int w, int h,
@@ -412,32 +481,7 @@ __global__ __launch_bounds__(NT) void k_fed_multi(const float* __restrict__ L, c
   load_window<RH, RW, NT>(sa, L, pitch, x0, y0, w, h);
   load_window<RH, RW, NT>(sg, g, pitch, x0, y0, w, h);
   __syncthreads();
-  float* src = sa;
-  float* dst = sb;
-  for (int s = 1; s <= st.n; s++) {
-    const float stepsize = st.half_tau[s - 1];
-    const float* __restrict__ cs = src;
-    float* __restrict__ cd = dst;
-    for (int idx = s * RW + tid; idx < (RH - s) * RW; idx += NT) {
-      const int r = idx >> LW, c = idx & (RW - 1);
-      const int gx = x0 + c, gy = y0 + r;
-      if (c < s || c >= RW - s || gx < 0 || gy < 0 || gx >= w || gy >= h) continue;
-      const int e = (gx < w - 1) ? idx + 1 : idx, wi = (gx > 0) ? idx - 1 : idx;
-      const int so = (gy < h - 1) ? idx + RW : idx, no = (gy > 0) ? idx - RW : idx;
-      const float i0 = cs[idx], f0 = sg[idx];
-      const float le = cs[e], lw = cs[wi], ls = cs[so], ln = cs[no];
-      const float ge = sg[e], gw = sg[wi], gs = sg[so], gn = sg[no];
-      float xpos = (f0 + ge) * (le - i0);
-      float xneg = (f0 + gw) * (i0 - lw);
-      float ypos = (f0 + gs) * (ls - i0);
-      float yneg = (f0 + gn) * (i0 - ln);
-      cd[idx] = i0 + stepsize * (xpos - xneg + ypos - yneg);
-    }
-    __syncthreads();
-    float* t = src;
-    src = dst;
-    dst = t;
-  }
+  const float* src = fed_lds_steps<RW, RH, NT>(sa, sb, sg, st, x0, y0, w, h);
   for (int idx = hn * RW + tid; idx < (RH - hn) * RW; idx += NT) {
     const int r = idx >> LW, c = idx & (RW - 1);
     const int gx = x0 + c, gy = y0 + r;
@@ -543,32 +587,7 @@ __global__ __launch_bounds__(NT) void k_fed_multi_sf(const float* __restrict__ i
     }
   }
   __syncthreads();
-  float* src = sa;
-  float* dst = sb;
-  for (int s = 1; s <= st.n; s++) {
-    const float stepsize = st.half_tau[s - 1];
-    const float* __restrict__ cs = src;
-    float* __restrict__ cd = dst;
-    for (int idx = s * RW + tid; idx < (RH - s) * RW; idx += NT) {
-      const int r = idx >> LW, c = idx & (RW - 1);
-      const int gx = x0 + c, gy = y0 + r;
-      if (c < s || c >= RW - s || gx < 0 || gy < 0 || gx >= w || gy >= h) continue;
-      const int e = (gx < w - 1) ? idx + 1 : idx, wi = (gx > 0) ? idx - 1 : idx;
-      const int so = (gy < h - 1) ? idx + RW : idx, no = (gy > 0) ? idx - RW : idx;
-      const float i0 = cs[idx], f0 = sg[idx];
-      const float le = cs[e], lw = cs[wi], ls = cs[so], ln = cs[no];
-      const float ge = sg[e], gw = sg[wi], gs = sg[so], gn = sg[no];
-      float xpos = (f0 + ge) * (le - i0);
-      float xneg = (f0 + gw) * (i0 - lw);
-      float ypos = (f0 + gs) * (ls - i0);
-      float yneg = (f0 + gn) * (i0 - ln);
-      cd[idx] = i0 + stepsize * (xpos - xneg + ypos - yneg);
-    }
-    __syncthreads();
-    float* t = src;
-    src = dst;
-    dst = t;
-  }
+  const float* src = fed_lds_steps<RW, RH, NT>(sa, sb, sg, st, x0, y0, w, h);
   for (int idx = hn * RW + tid; idx < (RH - hn) * RW; idx += NT) {
     const int r = idx >> LW, c = idx & (RW - 1);
     const int gx = x0 + c, gy = y0 + r;
