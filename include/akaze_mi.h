@@ -149,8 +149,8 @@ akz_status akz_set_keep_determinant(akz_ctx* ctx, int keep);
 /* No counterpart in the reference: a test and tuning hook of the
  * suppression.  Up to 8192 candidates it keeps the neighbour lists in LDS,
  * the entries past the first of lists of three or more in a pool of at most
- * 4096; a list that does not fit sends the image's suppression to its
- * global-memory path (same result).  `entries` (0..4096, default 4096) caps
+ * 4096; a list that does not fit sends the image's suppression to the
+ * path that reads the lists from global memory (same result).  `entries` (0..4096, default 4096) caps
  * that pool; 0 sends every image with such a list down the global path. */
 akz_status akz_set_suppression_pool(akz_ctx* ctx, int entries);
 
