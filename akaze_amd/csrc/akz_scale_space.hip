@@ -705,6 +705,110 @@ void k_fed_reg(const float* __restrict__ L, const float* __restrict__ g,
   else fed_reg_body<R, true>(L, g, out, pitch, w, h, st, x0, y0);
 }
 
+// FED steps with temporal blocking on a workgroup of NW wavefronts stacked
+// down one region of 64 columns x NW * RPW rows: wavefront w keeps region
+// rows w * RPW .. w * RPW + RPW - 1 in registers (lanes are columns, as in
+// k_fed_reg), so vertical neighbours are registers and horizontal ones DPP
+// shifts; per step only the first and last row of every wavefront go
+// through LDS (double-buffered: one barrier per step) to the wavefronts
+// above and below.  The conductance pair sums of the region are formed once
+// per launch, a wavefront's sums with the rows just outside its strip from
+// g itself.  The arithmetic per pixel and step is k_fed_reg's (k_fed_step's
+// expression in its order); region rows 0 and NW * RPW - 1 are not updated,
+// and the halo is the launch's step count.  A launch of 480x270 takes 60
+// workgroups of 8 wavefronts where 32x32 LDS regions take 510 of 16.
+template <int RPW, int NW, bool BORDER>
+__device__ __forceinline__ void fed_strip_body(const float* __restrict__ L, const float* __restrict__ g,
+                                               float* __restrict__ out, int pitch, int w, int h,
+                                               const akz_fed_steps& st, int x0, int y0, float* s_x) {
+  constexpr int RH = RPW * NW;
+  const int lane = threadIdx.x & 63, wv = akz_wave();
+  const int gx = x0 + lane;
+  const int cx = BORDER ? akz_iclamp(gx, 0, w - 1) : gx;
+  const int q0 = wv * RPW;  // the wavefront's first region row
+  auto row = [&](int q) {   // image row of region row q (clamped at the borders)
+    const int gy = y0 + q;
+    return (size_t)(BORDER ? akz_iclamp(gy, 0, h - 1) : gy) * pitch + cx;
+  };
+  float A[RPW], Wv[RPW], Wh[RPW];
+#pragma unroll
+  for (int r = 0; r < RPW; r++) {
+    A[r] = L[row(q0 + r)];
+    Wv[r] = g[row(q0 + r)];  // g for now
+  }
+  // g of the rows just above and below the strip (inside the image for an
+  // interior region, clamped otherwise)
+  const float gup = g[row(q0 - 1)], gdn = g[row(q0 + RPW)];
+#pragma unroll
+  for (int r = 0; r < RPW; r++) Wh[r] = Wv[r] + from_right(Wv[r]);
+  const float wup = gup + Wv[0];  // the sum of the pair (q0 - 1, q0)
+#pragma unroll
+  for (int r = 0; r < RPW - 1; r++) Wv[r] = Wv[r] + Wv[r + 1];
+  Wv[RPW - 1] = Wv[RPW - 1] + gdn;
+  const bool east_self = BORDER && gx >= w - 1, west_self = BORDER && gx <= 0;
+  const bool first = wv == 0, last = wv == NW - 1;
+  for (int s = 0; s < st.n; s++) {
+    const float stepsize = st.half_tau[s];
+    float* xb = s_x + (s & 1) * (NW * 128);
+    xb[wv * 128 + lane] = A[0];
+    xb[wv * 128 + 64 + lane] = A[RPW - 1];
+    __syncthreads();
+    const float above = first ? 0.0f : xb[(wv - 1) * 128 + 64 + lane];
+    const float below = last ? 0.0f : xb[(wv + 1) * 128 + lane];
+    int ytop = y0 + q0;
+    if (BORDER) asm volatile("" : "+s"(ytop));
+    // the north flux of the strip's first row: the south flux of the row
+    // above, from the values before this step
+    float yup = wup * (A[0] - above);
+#pragma unroll
+    for (int r = 0; r < RPW; r++) {
+      const float i0 = A[r];
+      float le = from_right(i0), ls = r + 1 < RPW ? A[r + 1] : below;
+      if (BORDER) {
+        if (east_self) le = i0;
+        if (ytop + r >= h - 1) ls = i0;
+      }
+      const float xpos = Wh[r] * (le - i0);
+      float xneg = from_left(xpos);
+      const float ypos = Wv[r] * (ls - i0);
+      float yneg = yup;
+      if (BORDER) {
+        if (west_self) xneg = 0.0f;
+        if (ytop + r <= 0) yneg = 0.0f;
+      }
+      yup = ypos;
+      const bool edge = (r == 0 && first) || (r == RPW - 1 && last);
+      if (!edge) A[r] = i0 + stepsize * (xpos - xneg + ypos - yneg);
+    }
+  }
+  const int hn = st.n;
+  if (lane < hn || lane >= 64 - hn || gx >= w) return;
+#pragma unroll
+  for (int r = 0; r < RPW; r++) {
+    const int q = q0 + r, gy = y0 + q;
+    if (q >= hn && q < RH - hn && gy < h) out[(size_t)gy * pitch + gx] = A[r];
+  }
+}
+
+template <int RPW, int NW>
+__global__ __launch_bounds__(64 * NW) void k_fed_strip(const float* __restrict__ L, const float* __restrict__ g,
+                                                      float* __restrict__ out, int pitch, int w, int h,
+                                                      akz_fed_steps st, size_t zs) {
+  __shared__ float s_x[2 * NW * 128];
+  const akz_tile3 tl = akz_xcd_tile();
+  L = akz_zoff(L, zs, tl.z);
+  g = akz_zoff(g, zs, tl.z);
+  out = akz_zoff(out, zs, tl.z);
+  constexpr int RH = RPW * NW;
+  const int hn = st.n;  // halo: one pixel per step
+  const int x0 = tl.x * (64 - 2 * hn) - hn, y0 = tl.y * (RH - 2 * hn) - hn;
+  // no pixel of the region is on or beyond an image border (so the rows
+  // just above and below it are inside); uniform over the workgroup
+  const bool interior = x0 >= 1 && x0 + 64 <= w - 1 && y0 >= 1 && y0 + RH <= h - 1;
+  if (interior) fed_strip_body<RPW, NW, false>(L, g, out, pitch, w, h, st, x0, y0, s_x);
+  else fed_strip_body<RPW, NW, true>(L, g, out, pitch, w, h, st, x0, y0, s_x);
+}
+
 // Diffusivity of Flow (cuda_akaze.cu:240-287) from dif2 = (Lx^2 + Ly^2) / k^2:
 // 0 Perona-Malik g1, 1 Perona-Malik g2, 2 Weickert, 3 Charbonnier.
 template <int TYPE>
@@ -1191,6 +1295,8 @@ void akz_launch_fed_variant(hipStream_t s, int variant, const float* Lin, const 
   } else if (variant == 11) {
     hipLaunchKernelGGL((k_fed_multi<128, 32, 512>), grid(128, 32), dim3(512), 0, s, Lin, g, Lout, pitch, w, h, st,
                        zs);
+  } else if (variant == 17) {
+    hipLaunchKernelGGL((k_fed_strip<8, 8>), grid(64, 64), dim3(512), 0, s, Lin, g, Lout, pitch, w, h, st, zs);
   } else {
     hipLaunchKernelGGL((k_fed_multi<64, 32, 256>), grid(64, 32), dim3(256), 0, s, Lin, g, Lout, pitch, w, h, st, zs);
   }
@@ -1257,6 +1363,8 @@ void akz_launch_fed_variant_sf(hipStream_t s, int variant, const float* in, cons
     case 10: AKZ_SF(64, 128, 1024, FB_K); break;
     case 11: AKZ_SF(128, 32, 512, FB_K); break;
     case 14: AKZ_SF(64, 64, 1024, FB_KMAX); break;
+    // the strip variant's first launch: 64x48 LDS regions
+    case 17: AKZ_SF(64, 48, 512, FB_K); break;
     case 15: AKZ_SF(128, 64, 1024, FB_KMAX); break;
     case 12:
     case 13:
@@ -1278,11 +1386,14 @@ void akz_launch_fed_variant_sf(hipStream_t s, int variant, const float* in, cons
 // Region shape by level size and images per launch (halo = steps of the
 // launch, XCD-aware tile placement), measured on MI355X in the whole
 // pipeline.  One 1920x1080 image per call: 32-row registers at 1920x1080 and
-// 960x540, 32x32/1024t below (with three images in flight the register
-// launches at 960x540 leave more of the GPU to the other pipelines than
-// 64x64/1024t regions, which the autotuner, timing one level alone,
-// prefers).  Batches of 16: 48-row registers at 1920x1080, 32-row registers
-// at 960x540 and 480x270, 64x64/1024t at 240x135.
+// 960x540, wavefront strips (first launch 64x48/512t LDS) at 480x270,
+// 32x32/1024t at 240x135.  With three images in flight a launch's share of
+// the GPU counts as much as its duration: the register launches at 960x540
+// leave more of it to the other pipelines than 64x64/1024t regions (which
+// the autotuner, timing one level alone, prefers), and at 480x270 the strips'
+// 60 workgroups of 8 wavefronts more than 32x32/1024t's 510 of 16, although
+// each launch takes longer.  Batches of 16: 48-row registers at 1920x1080,
+// 32-row registers at 960x540 and 480x270, 64x64/1024t at 240x135.
 int akz_fed_variant_for(int w, int h, int nz) {
   const size_t px = (size_t)w * h;
   if (nz > 1) {
@@ -1293,6 +1404,7 @@ int akz_fed_variant_for(int w, int h, int nz) {
     return 5;
   }
   if (px >= 300000) return 16;
+  if (px >= 100000) return 17;
   return 5;
 }
 

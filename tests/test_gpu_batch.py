@@ -140,15 +140,16 @@ def test_ragged_five_size_classes_match_oracle(gpu_lib):
 
 
 def test_every_fed_variant_and_the_autotuner(gpu_lib):
-    """Each blocked FED kernel variant (region shapes 2..16) on every level,
-    and the autotuner's choice, give the oracle's levels bit for bit."""
+    """Each blocked FED kernel variant (ids 2..17: LDS regions, register
+    regions, wavefront strips) on every level, and the autotuner's choice,
+    give the oracle's levels bit for bit."""
     A = gpu_lib
     W, H = 900, 675
     img = A.to_float(A.synth_image(91, W, H))
     orc = O.Pipeline(W, H)
     okp, odesc = orc.run(img)
     bk = A.BatchAKAZE(A.AKAZEOptions(img_width=W, img_height=H), 2)
-    for v in range(2, 17):
+    for v in range(2, 18):
         bk.set_fed_variants([v] * bk.nlev)
         bk.upload([img, img])
         bk.process_async(2)
@@ -156,7 +157,7 @@ def test_every_fed_variant_and_the_autotuner(gpu_lib):
             check_image(bk, z, okp, odesc, "variant %d" % v)
         assert np.array_equal(bk.read_level(1, bk.nlev - 1, 0), orc.level(bk.nlev - 1, 0)), v
     chosen = bk.autotune(2, reps=2)
-    assert all(2 <= v <= 16 for v in chosen[1:])
+    assert all(2 <= v <= 17 for v in chosen[1:])
     bk.upload([img, img])
     bk.process_async(2)
     check_image(bk, 1, okp, odesc, "autotuned %s" % list(chosen))
