@@ -717,6 +717,53 @@ void k_fed_reg(const float* __restrict__ L, const float* __restrict__ g,
 // expression in its order); region rows 0 and NW * RPW - 1 are not updated,
 // and the halo is the launch's step count.  A launch of 480x270 takes 60
 // workgroups of 8 wavefronts where 32x32 LDS regions take 510 of 16.
+// The steps of one launch on wavefront wv's strip, rows A[OFF..OFF+RPW-1]
+// (image row ytop for the first), with the pair sums of the strip's rows
+// (Wv[r]: rows r, r + 1 of the region, the last with the row below the
+// strip) and wup (the pair above the strip).
+template <int RPW, int NW, int OFF, bool BORDER, int NA>
+__device__ __forceinline__ void fed_strip_steps(float (&A)[NA], const float (&Wh)[RPW], const float (&Wv)[RPW],
+                                                float wup, const akz_fed_steps& st, int gx, int w, int ytop0,
+                                                int h, float* s_x) {
+  const int lane = threadIdx.x & 63, wv = akz_wave();
+  const bool east_self = BORDER && gx >= w - 1, west_self = BORDER && gx <= 0;
+  const bool first = wv == 0, last = wv == NW - 1;
+  for (int s = 0; s < st.n; s++) {
+    const float stepsize = st.half_tau[s];
+    float* xb = s_x + (s & 1) * (NW * 128);
+    xb[wv * 128 + lane] = A[OFF];
+    xb[wv * 128 + 64 + lane] = A[OFF + RPW - 1];
+    __syncthreads();
+    const float above = first ? 0.0f : xb[(wv - 1) * 128 + 64 + lane];
+    const float below = last ? 0.0f : xb[(wv + 1) * 128 + lane];
+    int ytop = ytop0;
+    if (BORDER) asm volatile("" : "+s"(ytop));
+    // the north flux of the strip's first row: the south flux of the row
+    // above, from the values before this step
+    float yup = wup * (A[OFF] - above);
+#pragma unroll
+    for (int r = 0; r < RPW; r++) {
+      const float i0 = A[OFF + r];
+      float le = from_right(i0), ls = r + 1 < RPW ? A[OFF + r + 1] : below;
+      if (BORDER) {
+        if (east_self) le = i0;
+        if (ytop + r >= h - 1) ls = i0;
+      }
+      const float xpos = Wh[r] * (le - i0);
+      float xneg = from_left(xpos);
+      const float ypos = Wv[r] * (ls - i0);
+      float yneg = yup;
+      if (BORDER) {
+        if (west_self) xneg = 0.0f;
+        if (ytop + r <= 0) yneg = 0.0f;
+      }
+      yup = ypos;
+      const bool edge = (r == 0 && first) || (r == RPW - 1 && last);
+      if (!edge) A[OFF + r] = i0 + stepsize * (xpos - xneg + ypos - yneg);
+    }
+  }
+}
+
 template <int RPW, int NW, bool BORDER>
 __device__ __forceinline__ void fed_strip_body(const float* __restrict__ L, const float* __restrict__ g,
                                                float* __restrict__ out, int pitch, int w, int h,
@@ -745,42 +792,7 @@ __device__ __forceinline__ void fed_strip_body(const float* __restrict__ L, cons
 #pragma unroll
   for (int r = 0; r < RPW - 1; r++) Wv[r] = Wv[r] + Wv[r + 1];
   Wv[RPW - 1] = Wv[RPW - 1] + gdn;
-  const bool east_self = BORDER && gx >= w - 1, west_self = BORDER && gx <= 0;
-  const bool first = wv == 0, last = wv == NW - 1;
-  for (int s = 0; s < st.n; s++) {
-    const float stepsize = st.half_tau[s];
-    float* xb = s_x + (s & 1) * (NW * 128);
-    xb[wv * 128 + lane] = A[0];
-    xb[wv * 128 + 64 + lane] = A[RPW - 1];
-    __syncthreads();
-    const float above = first ? 0.0f : xb[(wv - 1) * 128 + 64 + lane];
-    const float below = last ? 0.0f : xb[(wv + 1) * 128 + lane];
-    int ytop = y0 + q0;
-    if (BORDER) asm volatile("" : "+s"(ytop));
-    // the north flux of the strip's first row: the south flux of the row
-    // above, from the values before this step
-    float yup = wup * (A[0] - above);
-#pragma unroll
-    for (int r = 0; r < RPW; r++) {
-      const float i0 = A[r];
-      float le = from_right(i0), ls = r + 1 < RPW ? A[r + 1] : below;
-      if (BORDER) {
-        if (east_self) le = i0;
-        if (ytop + r >= h - 1) ls = i0;
-      }
-      const float xpos = Wh[r] * (le - i0);
-      float xneg = from_left(xpos);
-      const float ypos = Wv[r] * (ls - i0);
-      float yneg = yup;
-      if (BORDER) {
-        if (west_self) xneg = 0.0f;
-        if (ytop + r <= 0) yneg = 0.0f;
-      }
-      yup = ypos;
-      const bool edge = (r == 0 && first) || (r == RPW - 1 && last);
-      if (!edge) A[r] = i0 + stepsize * (xpos - xneg + ypos - yneg);
-    }
-  }
+  fed_strip_steps<RPW, NW, 0, BORDER>(A, Wh, Wv, wup, st, gx, w, y0 + q0, h, s_x);
   const int hn = st.n;
   if (lane < hn || lane >= 64 - hn || gx >= w) return;
 #pragma unroll
@@ -1048,6 +1060,151 @@ void k_fed_reg_sf(const float* __restrict__ in, const float* __restrict__ parent
     fed_reg_sf_body<R, TYPE, true>(in, parent, ppitch, lsm, gout, out, pitch, w, h, invk, taps, st, x0, y0);
 }
 
+// The first FED launch of a level on wavefront strips, fused with the
+// level's smooth + flow (k_fed_reg_sf's streaming passes, per wavefront):
+// wavefront w loads the input rows q0 - 4 .. q0 + RPW + 3 of its strip
+// (region rows q0 .. q0 + RPW - 1), streams the row pass, the column pass
+// and Scharr/diffusivity down them, and so has the conductance of its rows
+// and of the rows just above and below the strip; its input rows
+// q0 .. q0 + RPW - 1 become the FED rows.  Lsmooth (and g when later
+// launches read it) is written for the tile.  g is right on lanes 3..60, so
+// the tile is (58 - 2n) x (NW * RPW - 2n).  With a parent (the previous
+// level, exactly twice the size) the input is its 2x2 box mean in
+// k_half2's expression.
+template <int RPW, int NW, int TYPE, bool BORDER>
+__device__ __forceinline__ void fed_strip_sf_body(const float* __restrict__ in, const float* __restrict__ parent,
+                                                  int ppitch, float* __restrict__ lsm, float* __restrict__ gout,
+                                                  float* __restrict__ out, int pitch, int w, int h, float invk,
+                                                  const akz_taps& taps, const akz_fed_steps& st, int x0, int y0,
+                                                  float* s_x) {
+  constexpr int RH = RPW * NW, NA = RPW + 8;
+  const int lane = threadIdx.x & 63, wv = akz_wave();
+  const int gx = x0 + lane;
+  const int cx = BORDER ? akz_iclamp(gx, 0, w - 1) : gx;
+  const int hn = st.n;
+  const int q0 = wv * RPW, ys = y0 + q0;  // the strip's first row in the image
+  // A[i]: input row ys - 4 + i; A[4..RPW+3] are the FED rows
+  float A[NA];
+  if (parent) {
+#pragma unroll
+    for (int i = 0; i < NA; i++) {
+      const int gy = BORDER ? akz_iclamp(ys - 4 + i, 0, h - 1) : ys - 4 + i;
+      const float* p = parent + (size_t)(2 * gy) * ppitch + 2 * cx;
+      A[i] = 0.25f * (p[0] + p[1] + p[ppitch] + p[ppitch + 1]);
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < NA; i++) {
+      const int gy = BORDER ? akz_iclamp(ys - 4 + i, 0, h - 1) : ys - 4 + i;
+      A[i] = in[(size_t)gy * pitch + cx];
+    }
+  }
+  const float k0 = taps.k[0], k1 = taps.k[1], k2 = taps.k[2], k3 = taps.k[3], k4 = taps.k[4];
+  const bool first_col = BORDER && gx == 0, last_col = BORDER && gx == w - 1;
+  const bool out_col = lane >= 3 + hn && lane < 61 - hn && gx < w;
+  float Wv[RPW], Wh[RPW];
+  float gup = 0.0f, gdn = 0.0f;
+  // rings: B rows i-4..i, C / CL / CR rows j-2..j (C[j]: row ys - 2 + j)
+  float B[5], C[3], CL[3], CR[3];
+#pragma unroll
+  for (int i = 0; i < NA; i++) {
+    {
+      const float m1 = from_left(A[i]), p1 = from_right(A[i]);
+      const float m2 = from_left(m1), p2 = from_right(p1);
+      float sum = 0.0f;
+      sum += m2 * k0;
+      sum += m1 * k1;
+      sum += A[i] * k2;
+      sum += p1 * k3;
+      sum += p2 * k4;
+      B[i % 5] = sum;
+    }
+    const int j = i - 4;
+    if (j < 0) continue;
+    {
+      float sum = 0.0f;
+      sum += B[(j + 0) % 5] * k0;
+      sum += B[(j + 1) % 5] * k1;
+      sum += B[(j + 2) % 5] * k2;
+      sum += B[(j + 3) % 5] * k3;
+      sum += B[(j + 4) % 5] * k4;
+      C[j % 3] = sum;
+      const float fl = from_left(sum), fr = from_right(sum);
+      CL[j % 3] = first_col ? fr : fl;
+      CR[j % 3] = last_col ? fl : fr;
+    }
+    const int rb = j - 2;  // C rows rb, rb+1, rb+2 exist: Scharr of strip row rr
+    if (rb < 0) continue;
+    const int rr = rb - 1;  // -1 .. RPW
+    int ytop = ys, hk = hn;
+    asm volatile("" : "+s"(ytop), "+s"(hk));
+    const int y = ytop + rr;
+    const bool top = BORDER && y == 0, bot = BORDER && y == h - 1;
+    const int im = (top ? rb + 2 : rb) % 3, ip = (bot ? rb : rb + 2) % 3, ic = (rb + 1) % 3;
+    const float ul = CL[im], ur = CR[im], ll = CL[ip], lr = CR[ip];
+    const float lx = 3.0f * (lr - ll + ur - ul) + 10.0f * (CR[ic] - CL[ic]);
+    const float ly = 3.0f * (lr + ll - ur - ul) + 10.0f * (C[ip] - C[im]);
+    const float res = diffusivity<TYPE>(invk * (lx * lx + ly * ly));
+    if (rr < 0) {
+      gup = res;
+    } else if (rr >= RPW) {
+      gdn = res;
+    } else {
+      Wv[rr] = res;  // g for now
+      const int q = q0 + rr;
+      if (out_col && q >= hk && q < RH - hk && (!BORDER || y < h)) {
+        const size_t o = (size_t)y * pitch + gx;
+        lsm[o] = C[ic];
+        if (gout) gout[o] = res;
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < RPW; r++) Wh[r] = Wv[r] + from_right(Wv[r]);
+  const float wup = gup + Wv[0];
+#pragma unroll
+  for (int r = 0; r < RPW - 1; r++) Wv[r] = Wv[r] + Wv[r + 1];
+  Wv[RPW - 1] = Wv[RPW - 1] + gdn;
+  fed_strip_steps<RPW, NW, 4, BORDER>(A, Wh, Wv, wup, st, gx, w, ys, h, s_x);
+  if (!out_col) return;
+#pragma unroll
+  for (int r = 0; r < RPW; r++) {
+    const int q = q0 + r, gy = ys + r;
+    if (q >= hn && q < RH - hn && gy < h) out[(size_t)gy * pitch + gx] = A[r + 4];
+  }
+}
+
+template <int RPW, int NW, int TYPE>
+__global__ __launch_bounds__(64 * NW) void k_fed_strip_sf(const float* __restrict__ in,
+                                                         const float* __restrict__ parent, int ppitch,
+                                                         float* __restrict__ lsm, float* __restrict__ gout,
+                                                         float* __restrict__ out, int pitch, int w, int h,
+                                                         const float* kcp, akz_taps taps, akz_fed_steps st,
+                                                         size_t zs) {
+  __shared__ float s_x[2 * NW * 128];
+  const akz_tile3 tl = akz_xcd_tile();
+  if (parent) parent = akz_zoff(parent, zs, tl.z);
+  else in = akz_zoff(in, zs, tl.z);
+  lsm = akz_zoff(lsm, zs, tl.z);
+  if (gout) gout = akz_zoff(gout, zs, tl.z);
+  out = akz_zoff(out, zs, tl.z);
+  kcp = akz_zoff(kcp, zs, tl.z);
+  const float kcontrast = *kcp;
+  const float invk = 1.0f / (kcontrast * kcontrast);
+  constexpr int RH = RPW * NW;
+  const int hn = st.n;
+  const int x0 = tl.x * (58 - 2 * hn) - 3 - hn, y0 = tl.y * (RH - 2 * hn) - hn;
+  // every input row (4 above and below the region) inside the image, no
+  // Scharr reflection and no region pixel on or beyond an image border
+  const bool interior = x0 >= 1 && x0 + 65 <= w && y0 >= 4 && y0 + RH + 4 <= h;
+  if (interior)
+    fed_strip_sf_body<RPW, NW, TYPE, false>(in, parent, ppitch, lsm, gout, out, pitch, w, h, invk, taps, st, x0, y0,
+                                            s_x);
+  else
+    fed_strip_sf_body<RPW, NW, TYPE, true>(in, parent, ppitch, lsm, gout, out, pitch, w, h, invk, taps, st, x0, y0,
+                                           s_x);
+}
+
 // One FED step, LDS-tiled: a 64x16 output tile plus a one-pixel halo of L and
 // g is read once with coalesced rows; arithmetic as k_fed_step.
 __global__ __launch_bounds__(256) void k_fed_tile(const float* __restrict__ L, const float* __restrict__ g,
@@ -1295,7 +1452,7 @@ void akz_launch_fed_variant(hipStream_t s, int variant, const float* Lin, const 
   } else if (variant == 11) {
     hipLaunchKernelGGL((k_fed_multi<128, 32, 512>), grid(128, 32), dim3(512), 0, s, Lin, g, Lout, pitch, w, h, st,
                        zs);
-  } else if (variant == 17) {
+  } else if (variant == 17 || variant == 18) {
     hipLaunchKernelGGL((k_fed_strip<8, 8>), grid(64, 64), dim3(512), 0, s, Lin, g, Lout, pitch, w, h, st, zs);
   } else {
     hipLaunchKernelGGL((k_fed_multi<64, 32, 256>), grid(64, 32), dim3(256), 0, s, Lin, g, Lout, pitch, w, h, st, zs);
@@ -1331,6 +1488,20 @@ static void launch_fed_reg_sf(hipStream_t s, dim3 grid, const float* in, const f
 #undef AKZ_REG_SF
 }
 
+template <int RPW, int NW>
+static void launch_fed_strip_sf(hipStream_t s, dim3 grid, const float* in, const float* parent, int ppitch,
+                                float* lsm, float* gout, float* Lout, int pitch, int w, int h, int type,
+                                const float* kc, const akz_taps& taps, const akz_fed_steps& st, size_t zs) {
+#define AKZ_STRIP_SF(TYPE_)                                                                                  \
+  hipLaunchKernelGGL((k_fed_strip_sf<RPW, NW, TYPE_>), grid, dim3(64 * NW), 0, s, in, parent, ppitch, lsm, gout, \
+                     Lout, pitch, w, h, kc, taps, st, zs)
+  if (type == 0) AKZ_STRIP_SF(0);
+  else if (type == 1) AKZ_STRIP_SF(1);
+  else if (type == 2) AKZ_STRIP_SF(2);
+  else AKZ_STRIP_SF(3);
+#undef AKZ_STRIP_SF
+}
+
 // First launch of a level, fused with the level's smooth + flow
 // (k_fed_multi_sf for the LDS variants, k_fed_reg_sf for the register
 // ones); g is written only when `gout` is given.
@@ -1363,8 +1534,13 @@ void akz_launch_fed_variant_sf(hipStream_t s, int variant, const float* in, cons
     case 10: AKZ_SF(64, 128, 1024, FB_K); break;
     case 11: AKZ_SF(128, 32, 512, FB_K); break;
     case 14: AKZ_SF(64, 64, 1024, FB_KMAX); break;
-    // the strip variant's first launch: 64x48 LDS regions
+    // the strip variants' first launch: 64x48 LDS regions, or strips
     case 17: AKZ_SF(64, 48, 512, FB_K); break;
+    case 18: {
+      const dim3 g((w + 57 - 2 * n) / (58 - 2 * n), (h + 63 - 2 * n) / (64 - 2 * n), nz);
+      launch_fed_strip_sf<8, 8>(s, g, in, parent, ppitch, lsm, gout, Lout, pitch, w, h, type, kc, taps, st, zs);
+      break;
+    }
     case 15: AKZ_SF(128, 64, 1024, FB_KMAX); break;
     case 12:
     case 13:
@@ -1385,14 +1561,14 @@ void akz_launch_fed_variant_sf(hipStream_t s, int variant, const float* in, cons
 
 // Region shape by level size and images per launch (halo = steps of the
 // launch, XCD-aware tile placement), measured on MI355X in the whole
-// pipeline.  One 1920x1080 image per call: 32-row registers at 1920x1080 and
-// 960x540, wavefront strips (first launch 64x48/512t LDS) at 480x270,
-// 32x32/1024t at 240x135.  With three images in flight a launch's share of
-// the GPU counts as much as its duration: the register launches at 960x540
-// leave more of it to the other pipelines than 64x64/1024t regions (which
-// the autotuner, timing one level alone, prefers), and at 480x270 the strips'
-// 60 workgroups of 8 wavefronts more than 32x32/1024t's 510 of 16, although
-// each launch takes longer.  Batches of 16: 48-row registers at 1920x1080,
+// pipeline.  One 1920x1080 image per call: wavefront strips with the smooth +
+// flow in their first launch from 1920x1080 to 480x270, 32x32/1024t at
+// 240x135.  With three images in flight a launch's share of the GPU counts
+// as much as its duration: at 480x270 the strips' 60 workgroups of 8
+// wavefronts leave more of it to the other pipelines than 32x32/1024t's 510
+// of 16, and at 960x540 and 1920x1080 they replace single-wavefront register
+// regions of 32 rows whose launches were latency-bound (one or two
+// wavefronts per SIMD).  Batches of 16: 48-row registers at 1920x1080,
 // 32-row registers at 960x540 and 480x270, 64x64/1024t at 240x135.
 int akz_fed_variant_for(int w, int h, int nz) {
   const size_t px = (size_t)w * h;
@@ -1403,8 +1579,7 @@ int akz_fed_variant_for(int w, int h, int nz) {
     if (tot >= 300000) return 6;
     return 5;
   }
-  if (px >= 300000) return 16;
-  if (px >= 100000) return 17;
+  if (px >= 100000) return 18;
   return 5;
 }
 

@@ -140,7 +140,7 @@ def test_ragged_five_size_classes_match_oracle(gpu_lib):
 
 
 def test_every_fed_variant_and_the_autotuner(gpu_lib):
-    """Each blocked FED kernel variant (ids 2..17: LDS regions, register
+    """Each blocked FED kernel variant (ids 2..18: LDS regions, register
     regions, wavefront strips) on every level, and the autotuner's choice,
     give the oracle's levels bit for bit."""
     A = gpu_lib
@@ -149,7 +149,7 @@ def test_every_fed_variant_and_the_autotuner(gpu_lib):
     orc = O.Pipeline(W, H)
     okp, odesc = orc.run(img)
     bk = A.BatchAKAZE(A.AKAZEOptions(img_width=W, img_height=H), 2)
-    for v in range(2, 18):
+    for v in range(2, 19):
         bk.set_fed_variants([v] * bk.nlev)
         bk.upload([img, img])
         bk.process_async(2)
@@ -157,10 +157,33 @@ def test_every_fed_variant_and_the_autotuner(gpu_lib):
             check_image(bk, z, okp, odesc, "variant %d" % v)
         assert np.array_equal(bk.read_level(1, bk.nlev - 1, 0), orc.level(bk.nlev - 1, 0)), v
     chosen = bk.autotune(2, reps=2)
-    assert all(2 <= v <= 17 for v in chosen[1:])
+    assert all(2 <= v <= 18 for v in chosen[1:])
     bk.upload([img, img])
     bk.process_async(2)
     check_image(bk, 1, okp, odesc, "autotuned %s" % list(chosen))
+
+
+def test_strip_variants_on_exactly_halving_levels(gpu_lib):
+    """The wavefront-strip FED variants (17: LDS first launch, 18: strips
+    with smooth + flow) on sizes whose octaves halve exactly, where the first
+    launch of an octave forms the halved input while loading, with one and
+    with three images per launch; every level against the oracle's."""
+    A = gpu_lib
+    for (W, H, seed) in [(640, 480, 97), (1280, 720, 98)]:
+        img = A.to_float(A.synth_image(seed, W, H))
+        orc = O.Pipeline(W, H)
+        okp, odesc = orc.run(img)
+        for nz in (1, 3):
+            bk = A.BatchAKAZE(A.AKAZEOptions(img_width=W, img_height=H), nz)
+            for v in (17, 18):
+                bk.set_fed_variants([v] * bk.nlev)
+                bk.upload([img] * nz)
+                bk.process_async(nz)
+                for z in range(nz):
+                    check_image(bk, z, okp, odesc, "variant %d %dx%d nz %d" % (v, W, H, nz))
+                for lvl in range(1, bk.nlev):
+                    assert np.array_equal(bk.read_level(nz - 1, lvl, 0), orc.level(lvl, 0)), (v, W, H, nz, lvl)
+                    assert np.array_equal(bk.read_level(nz - 1, lvl, 1), orc.level(lvl, 1)), (v, W, H, nz, lvl)
 
 
 def test_determinant_images_on_request(gpu_lib):
