@@ -1062,40 +1062,41 @@ void k_fed_reg_sf(const float* __restrict__ in, const float* __restrict__ parent
 
 // The first FED launch of a level on wavefront strips, fused with the
 // level's smooth + flow (k_fed_reg_sf's streaming passes, per wavefront):
-// wavefront w loads the input rows q0 - 4 .. q0 + RPW + 3 of its strip
+// wavefront w loads the input rows q0 - 3 .. q0 + RPW + 2 of its strip
 // (region rows q0 .. q0 + RPW - 1), streams the row pass, the column pass
-// and Scharr/diffusivity down them, and so has the conductance of its rows
-// and of the rows just above and below the strip; its input rows
-// q0 .. q0 + RPW - 1 become the FED rows.  Lsmooth (and g when later
-// launches read it) is written for the tile.  g is right on lanes 3..60, so
-// the tile is (58 - 2n) x (NW * RPW - 2n).  With a parent (the previous
-// level, exactly twice the size) the input is its 2x2 box mean in
-// k_half2's expression.
+// and Scharr/diffusivity down them to the conductance of its own rows, and
+// takes the conductance of the rows just above and below the strip from
+// the neighbouring wavefronts through LDS (the region's first and last
+// rows are not updated, so the outermost wavefronts need none); its input
+// rows become the FED rows.  Lsmooth (and g when later launches read it) is
+// written for the tile.  g is right on lanes 3..60, so the tile is
+// (58 - 2n) x (NW * RPW - 2n).  With a parent (the previous level, exactly
+// twice the size) the input is its 2x2 box mean in k_half2's expression.
 template <int RPW, int NW, int TYPE, bool BORDER>
 __device__ __forceinline__ void fed_strip_sf_body(const float* __restrict__ in, const float* __restrict__ parent,
                                                   int ppitch, float* __restrict__ lsm, float* __restrict__ gout,
                                                   float* __restrict__ out, int pitch, int w, int h, float invk,
                                                   const akz_taps& taps, const akz_fed_steps& st, int x0, int y0,
                                                   float* s_x) {
-  constexpr int RH = RPW * NW, NA = RPW + 8;
+  constexpr int RH = RPW * NW, NA = RPW + 6;
   const int lane = threadIdx.x & 63, wv = akz_wave();
   const int gx = x0 + lane;
   const int cx = BORDER ? akz_iclamp(gx, 0, w - 1) : gx;
   const int hn = st.n;
   const int q0 = wv * RPW, ys = y0 + q0;  // the strip's first row in the image
-  // A[i]: input row ys - 4 + i; A[4..RPW+3] are the FED rows
+  // A[i]: input row ys - 3 + i; A[3..RPW+2] are the FED rows
   float A[NA];
   if (parent) {
 #pragma unroll
     for (int i = 0; i < NA; i++) {
-      const int gy = BORDER ? akz_iclamp(ys - 4 + i, 0, h - 1) : ys - 4 + i;
+      const int gy = BORDER ? akz_iclamp(ys - 3 + i, 0, h - 1) : ys - 3 + i;
       const float* p = parent + (size_t)(2 * gy) * ppitch + 2 * cx;
       A[i] = 0.25f * (p[0] + p[1] + p[ppitch] + p[ppitch + 1]);
     }
   } else {
 #pragma unroll
     for (int i = 0; i < NA; i++) {
-      const int gy = BORDER ? akz_iclamp(ys - 4 + i, 0, h - 1) : ys - 4 + i;
+      const int gy = BORDER ? akz_iclamp(ys - 3 + i, 0, h - 1) : ys - 3 + i;
       A[i] = in[(size_t)gy * pitch + cx];
     }
   }
@@ -1103,8 +1104,7 @@ __device__ __forceinline__ void fed_strip_sf_body(const float* __restrict__ in, 
   const bool first_col = BORDER && gx == 0, last_col = BORDER && gx == w - 1;
   const bool out_col = lane >= 3 + hn && lane < 61 - hn && gx < w;
   float Wv[RPW], Wh[RPW];
-  float gup = 0.0f, gdn = 0.0f;
-  // rings: B rows i-4..i, C / CL / CR rows j-2..j (C[j]: row ys - 2 + j)
+  // rings: B rows i-4..i, C / CL / CR rows j-2..j (C[j]: row ys - 1 + j)
   float B[5], C[3], CL[3], CR[3];
 #pragma unroll
   for (int i = 0; i < NA; i++) {
@@ -1133,44 +1133,44 @@ __device__ __forceinline__ void fed_strip_sf_body(const float* __restrict__ in, 
       CL[j % 3] = first_col ? fr : fl;
       CR[j % 3] = last_col ? fl : fr;
     }
-    const int rb = j - 2;  // C rows rb, rb+1, rb+2 exist: Scharr of strip row rr
-    if (rb < 0) continue;
-    const int rr = rb - 1;  // -1 .. RPW
+    const int r = j - 2;  // strip row r: C rows r, r+1, r+2 exist
+    if (r < 0) continue;
     int ytop = ys, hk = hn;
     asm volatile("" : "+s"(ytop), "+s"(hk));
-    const int y = ytop + rr;
+    const int y = ytop + r;
     const bool top = BORDER && y == 0, bot = BORDER && y == h - 1;
-    const int im = (top ? rb + 2 : rb) % 3, ip = (bot ? rb : rb + 2) % 3, ic = (rb + 1) % 3;
+    const int im = (top ? r + 2 : r) % 3, ip = (bot ? r : r + 2) % 3, ic = (r + 1) % 3;
     const float ul = CL[im], ur = CR[im], ll = CL[ip], lr = CR[ip];
     const float lx = 3.0f * (lr - ll + ur - ul) + 10.0f * (CR[ic] - CL[ic]);
     const float ly = 3.0f * (lr + ll - ur - ul) + 10.0f * (C[ip] - C[im]);
     const float res = diffusivity<TYPE>(invk * (lx * lx + ly * ly));
-    if (rr < 0) {
-      gup = res;
-    } else if (rr >= RPW) {
-      gdn = res;
-    } else {
-      Wv[rr] = res;  // g for now
-      const int q = q0 + rr;
-      if (out_col && q >= hk && q < RH - hk && (!BORDER || y < h)) {
-        const size_t o = (size_t)y * pitch + gx;
-        lsm[o] = C[ic];
-        if (gout) gout[o] = res;
-      }
+    Wv[r] = res;  // g for now
+    const int q = q0 + r;
+    if (out_col && q >= hk && q < RH - hk && (!BORDER || y < h)) {
+      const size_t o = (size_t)y * pitch + gx;
+      lsm[o] = C[ic];
+      if (gout) gout[o] = res;
     }
   }
+  // the conductance of the rows next to the strip, from the neighbouring
+  // wavefronts (s_x[0, NW * 128); the steps use the rest)
+  s_x[wv * 128 + lane] = Wv[0];
+  s_x[wv * 128 + 64 + lane] = Wv[RPW - 1];
+  __syncthreads();
+  const float gup = wv > 0 ? s_x[(wv - 1) * 128 + 64 + lane] : 0.0f;
+  const float gdn = wv < NW - 1 ? s_x[(wv + 1) * 128 + lane] : 0.0f;
 #pragma unroll
   for (int r = 0; r < RPW; r++) Wh[r] = Wv[r] + from_right(Wv[r]);
   const float wup = gup + Wv[0];
 #pragma unroll
   for (int r = 0; r < RPW - 1; r++) Wv[r] = Wv[r] + Wv[r + 1];
   Wv[RPW - 1] = Wv[RPW - 1] + gdn;
-  fed_strip_steps<RPW, NW, 4, BORDER>(A, Wh, Wv, wup, st, gx, w, ys, h, s_x);
+  fed_strip_steps<RPW, NW, 3, BORDER>(A, Wh, Wv, wup, st, gx, w, ys, h, s_x + NW * 128);
   if (!out_col) return;
 #pragma unroll
   for (int r = 0; r < RPW; r++) {
     const int q = q0 + r, gy = ys + r;
-    if (q >= hn && q < RH - hn && gy < h) out[(size_t)gy * pitch + gx] = A[r + 4];
+    if (q >= hn && q < RH - hn && gy < h) out[(size_t)gy * pitch + gx] = A[r + 3];
   }
 }
 
@@ -1181,7 +1181,7 @@ __global__ __launch_bounds__(64 * NW) void k_fed_strip_sf(const float* __restric
                                                          float* __restrict__ out, int pitch, int w, int h,
                                                          const float* kcp, akz_taps taps, akz_fed_steps st,
                                                          size_t zs) {
-  __shared__ float s_x[2 * NW * 128];
+  __shared__ float s_x[3 * NW * 128];  // the conductance exchange, then the steps' two buffers
   const akz_tile3 tl = akz_xcd_tile();
   if (parent) parent = akz_zoff(parent, zs, tl.z);
   else in = akz_zoff(in, zs, tl.z);
@@ -1194,9 +1194,9 @@ __global__ __launch_bounds__(64 * NW) void k_fed_strip_sf(const float* __restric
   constexpr int RH = RPW * NW;
   const int hn = st.n;
   const int x0 = tl.x * (58 - 2 * hn) - 3 - hn, y0 = tl.y * (RH - 2 * hn) - hn;
-  // every input row (4 above and below the region) inside the image, no
+  // every input row (3 above and below the region) inside the image, no
   // Scharr reflection and no region pixel on or beyond an image border
-  const bool interior = x0 >= 1 && x0 + 65 <= w && y0 >= 4 && y0 + RH + 4 <= h;
+  const bool interior = x0 >= 1 && x0 + 65 <= w && y0 >= 3 && y0 + RH + 3 <= h;
   if (interior)
     fed_strip_sf_body<RPW, NW, TYPE, false>(in, parent, ppitch, lsm, gout, out, pitch, w, h, invk, taps, st, x0, y0,
                                             s_x);
