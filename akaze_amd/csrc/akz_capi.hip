@@ -1189,7 +1189,7 @@ akz_status akz_prop_read_probs(akz_ctx* c, int z, float* probs) {
 // ---- autotuner ----
 akz_status akz_autotune(akz_ctx* c, int nz, int reps) {
   if (!c || nz < 1 || nz > c->batch || reps < 1) return AKZ_ERR_ARG;
-  static const int kVariants[] = {2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18};
+  static const int kVariants[] = {2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19};
   hipEvent_t a, b;
   AKZ_HIP_TRY(hipEventCreate(&a));
   AKZ_HIP_TRY(hipEventCreate(&b));

@@ -25,7 +25,7 @@ void akz_launch_contrast(hipStream_t s, const float* img, int in_pitch, size_t i
 // FED kernel variant chosen for a level size and the images per launch
 int akz_fed_variant_for(int w, int h, int nz);
 // blocked FED variants are 2..AKZ_FED_VARIANT_MAX (akz_launch_fed_variant)
-constexpr int AKZ_FED_VARIANT_MAX = 18;
+constexpr int AKZ_FED_VARIANT_MAX = 19;
 // LowPass(sigma=1) + Flow of one level in one kernel; taps: 5 taps of sigma 1
 void akz_launch_smooth_flow(hipStream_t s, const float* in, int pitch, float* lsm, float* g, int w, int h, int type,
                             const float* kc, const akz_taps& taps, akz_batch bt);

@@ -1380,7 +1380,7 @@ int akz_fed_block_steps() { return FB_KMAX; }
 
 int akz_fed_variant_steps(int variant) {
   if (variant <= 1) return 1;
-  return (variant == 14 || variant == 15) ? FB_KMAX : FB_K;
+  return (variant == 14 || variant == 15 || variant == 19) ? FB_KMAX : FB_K;
 }
 
 // Kernel variant per FED launch: 0 k_fed_step, 1 k_fed_tile; blocked regions
@@ -1452,7 +1452,7 @@ void akz_launch_fed_variant(hipStream_t s, int variant, const float* Lin, const 
   } else if (variant == 11) {
     hipLaunchKernelGGL((k_fed_multi<128, 32, 512>), grid(128, 32), dim3(512), 0, s, Lin, g, Lout, pitch, w, h, st,
                        zs);
-  } else if (variant == 17 || variant == 18) {
+  } else if (variant == 17 || variant == 18 || variant == 19) {
     hipLaunchKernelGGL((k_fed_strip<8, 8>), grid(64, 64), dim3(512), 0, s, Lin, g, Lout, pitch, w, h, st, zs);
   } else {
     hipLaunchKernelGGL((k_fed_multi<64, 32, 256>), grid(64, 32), dim3(256), 0, s, Lin, g, Lout, pitch, w, h, st, zs);
@@ -1536,7 +1536,8 @@ void akz_launch_fed_variant_sf(hipStream_t s, int variant, const float* in, cons
     case 14: AKZ_SF(64, 64, 1024, FB_KMAX); break;
     // the strip variants' first launch: 64x48 LDS regions, or strips
     case 17: AKZ_SF(64, 48, 512, FB_K); break;
-    case 18: {
+    case 18:
+    case 19: {
       const dim3 g((w + 57 - 2 * n) / (58 - 2 * n), (h + 63 - 2 * n) / (64 - 2 * n), nz);
       launch_fed_strip_sf<8, 8>(s, g, in, parent, ppitch, lsm, gout, Lout, pitch, w, h, type, kc, taps, st, zs);
       break;
@@ -1562,13 +1563,13 @@ void akz_launch_fed_variant_sf(hipStream_t s, int variant, const float* in, cons
 // Region shape by level size and images per launch (halo = steps of the
 // launch, XCD-aware tile placement), measured on MI355X in the whole
 // pipeline.  One 1920x1080 image per call: wavefront strips with the smooth +
-// flow in their first launch from 1920x1080 to 480x270, 32x32/1024t at
-// 240x135.  With three images in flight a launch's share of the GPU counts
-// as much as its duration: at 480x270 the strips' 60 workgroups of 8
-// wavefronts leave more of it to the other pipelines than 32x32/1024t's 510
-// of 16, and at 960x540 and 1920x1080 they replace single-wavefront register
-// regions of 32 rows whose launches were latency-bound (one or two
-// wavefronts per SIMD).  Batches of 16: 48-row registers at 1920x1080,
+// flow in their first launch from 1920x1080 to 480x270 (16 steps per launch
+// at 480x270), 32x32/1024t at 240x135.  With three images in flight a
+// launch's share of the GPU counts as much as its duration: at 480x270 the
+// strips' workgroups of 8 wavefronts leave more of it to the other pipelines
+// than 32x32/1024t's 510 of 16, and at 960x540 and 1920x1080 they replace
+// single-wavefront register regions of 32 rows whose launches were
+// latency-bound (one or two wavefronts per SIMD).  Batches of 16: 48-row registers at 1920x1080,
 // 32-row registers at 960x540 and 480x270, 64x64/1024t at 240x135.
 int akz_fed_variant_for(int w, int h, int nz) {
   const size_t px = (size_t)w * h;
@@ -1579,7 +1580,8 @@ int akz_fed_variant_for(int w, int h, int nz) {
     if (tot >= 300000) return 6;
     return 5;
   }
-  if (px >= 100000) return 18;
+  if (px >= 300000) return 18;
+  if (px >= 100000) return 19;
   return 5;
 }
 

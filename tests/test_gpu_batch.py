@@ -140,7 +140,7 @@ def test_ragged_five_size_classes_match_oracle(gpu_lib):
 
 
 def test_every_fed_variant_and_the_autotuner(gpu_lib):
-    """Each blocked FED kernel variant (ids 2..18: LDS regions, register
+    """Each blocked FED kernel variant (ids 2..19: LDS regions, register
     regions, wavefront strips) on every level, and the autotuner's choice,
     give the oracle's levels bit for bit."""
     A = gpu_lib
@@ -149,7 +149,7 @@ def test_every_fed_variant_and_the_autotuner(gpu_lib):
     orc = O.Pipeline(W, H)
     okp, odesc = orc.run(img)
     bk = A.BatchAKAZE(A.AKAZEOptions(img_width=W, img_height=H), 2)
-    for v in range(2, 19):
+    for v in range(2, 20):
         bk.set_fed_variants([v] * bk.nlev)
         bk.upload([img, img])
         bk.process_async(2)
@@ -157,7 +157,7 @@ def test_every_fed_variant_and_the_autotuner(gpu_lib):
             check_image(bk, z, okp, odesc, "variant %d" % v)
         assert np.array_equal(bk.read_level(1, bk.nlev - 1, 0), orc.level(bk.nlev - 1, 0)), v
     chosen = bk.autotune(2, reps=2)
-    assert all(2 <= v <= 18 for v in chosen[1:])
+    assert all(2 <= v <= 19 for v in chosen[1:])
     bk.upload([img, img])
     bk.process_async(2)
     check_image(bk, 1, okp, odesc, "autotuned %s" % list(chosen))
@@ -165,9 +165,10 @@ def test_every_fed_variant_and_the_autotuner(gpu_lib):
 
 def test_strip_variants_on_exactly_halving_levels(gpu_lib):
     """The wavefront-strip FED variants (17: LDS first launch, 18: strips
-    with smooth + flow) on sizes whose octaves halve exactly, where the first
-    launch of an octave forms the halved input while loading, with one and
-    with three images per launch; every level against the oracle's."""
+    with smooth + flow, 19: the same with 16 steps per launch) on sizes
+    whose octaves halve exactly, where the first launch of an octave forms
+    the halved input while loading, with one and with three images per
+    launch; every level against the oracle's."""
     A = gpu_lib
     for (W, H, seed) in [(640, 480, 97), (1280, 720, 98)]:
         img = A.to_float(A.synth_image(seed, W, H))
@@ -175,7 +176,7 @@ def test_strip_variants_on_exactly_halving_levels(gpu_lib):
         okp, odesc = orc.run(img)
         for nz in (1, 3):
             bk = A.BatchAKAZE(A.AKAZEOptions(img_width=W, img_height=H), nz)
-            for v in (17, 18):
+            for v in (17, 18, 19):
                 bk.set_fed_variants([v] * bk.nlev)
                 bk.upload([img] * nz)
                 bk.process_async(nz)
