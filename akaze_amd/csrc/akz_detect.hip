@@ -508,12 +508,10 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* excl, int32_t* s
   return tot;
 }
 
-// The c candidates of row segment `seg` (ranks in seg_slots) in keypoint
-// layout at cand[base ...] (FindExtrema, cuda_akaze.cu:744-772): the subpixel
-// fit on the determinant's 3x3 neighbourhood; slots at or past the capacity
-// are dropped.
-__device__ __forceinline__ void scatter_segment(const akz_det_params& p, const akz_det_bufs& b, int seg, int c,
-                                                int base) {
+// Candidate r of row segment `seg` (its rank r in seg_slots) in keypoint
+// layout at cand[idx] (FindExtrema, cuda_akaze.cu:744-772): the subpixel fit
+// on the determinant's 3x3 neighbourhood.
+__device__ __forceinline__ void scatter_one(const akz_det_params& p, const akz_det_bufs& b, int seg, int r, int idx) {
   int l = 0;
   while (l + 1 < p.nlev && seg >= p.lev[l + 1].seg_base) l++;
   const akz_level_desc& L = p.lev[l];
@@ -524,9 +522,7 @@ __device__ __forceinline__ void scatter_segment(const akz_det_params& p, const a
   float2* cand_xy = akz_zoff(b.cand_xy, b.zs);
   const float* imd = akz_zoff(p.det[l], p.zs);
   const size_t pitch = L.pitch;
-  for (int r = 0; r < c; r++) {
-    const int idx = base + r;
-    if (idx >= b.cap) return;
+  {
     const int x = tx * 64 + slots[(size_t)seg * 32 + r];
     const size_t q = (size_t)y * pitch + x;
     float v = imd[q];
@@ -572,13 +568,18 @@ __global__ __launch_bounds__(SCAN_T) void k_seg_sums(const int32_t* __restrict__
 // the candidates, one segment per thread: a segment's offset is the sum of
 // the 256-segment runs before its block (k_seg_sums) plus the block's
 // exclusive scan, and its candidates are written there, so the candidate
-// order is (level, y, x).  The thread of a level's first segment writes the
-// level's start; the thread of index seg_total (the grid covers it) writes
-// the total, the candidate count (cut at the capacity), and clears the
-// overflow flags of this detection run.
+// order is (level, y, x).  The candidates of a wavefront's 64 segments are
+// contiguous; its lanes take them in turn (candidate j of the wavefront to
+// lane j % 64, its segment from an LDS table), so a segment with many
+// candidates does not hold one lane for all of them.  Candidates at or past
+// the capacity are dropped.  The thread of a level's first segment writes
+// the level's start; the thread of index seg_total (the grid covers it)
+// writes the total, the candidate count (cut at the capacity), and clears
+// the overflow flags of this detection run.
 constexpr int SS_T = 256;
 __global__ __launch_bounds__(SS_T) void k_scan_scatter(akz_det_params p, akz_det_bufs b) {
   __shared__ int32_t s_w[17];
+  __shared__ uint8_t s_tab[SS_T / 64][64 * 32];  // the segment (lane) of each candidate of a wavefront
   const int32_t* wsum = akz_zoff(b.seg_wsum, b.zs);
   const int n = p.seg_total;
   const int seg = blockIdx.x * SS_T + threadIdx.x;
@@ -600,7 +601,21 @@ __global__ __launch_bounds__(SS_T) void k_scan_scatter(akz_det_params p, akz_det
     *akz_zoff(b.ncand, b.zs) = akz_imin(o, b.cap);
     *reinterpret_cast<unsigned long long*>(akz_zoff(b.nb_pool, b.zs)) = 0ull;  // k_nb_pool's allocator
   }
-  if (c > 0) scatter_segment(p, b, seg, c, o);
+  // the wavefront's candidates: [o0, o0 + T)
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int o0 = __shfl(o, 0, 64);
+  const int T = __shfl(o + c, 63, 64) - o0;
+  uint8_t* tab = s_tab[wv];
+  for (int r = 0; r < c; r++) tab[o - o0 + r] = (uint8_t)lane;
+  // the table's writes precede its reads in the wavefront's LDS order
+  __builtin_amdgcn_wave_barrier();
+  for (int j0 = 0; j0 < T; j0 += 64) {
+    const int j = j0 + lane;
+    int sl = 0;
+    if (j < T) sl = tab[j];
+    const int os = __shfl(o, sl, 64);  // the segment's offset (all lanes take part)
+    if (j < T && o0 + j < b.cap) scatter_one(p, b, seg - lane + sl, o0 + j - os, o0 + j);
+  }
 }
 
 // Neighbour list of FindNeighbors (cuda_akaze.cu:1005-1070) for candidate c
