@@ -178,12 +178,13 @@ akz_status akz_batch_results_device(akz_ctx* ctx, int z, const int32_t** d_count
                                     const uint8_t** d_desc64);
 akz_status akz_read_batch_level(akz_ctx* ctx, int z, int lvl, int which, float* dst);
 /* Autotuner (no reference counterpart): times every blocked FED kernel
- * variant (LDS or register region shape, threads per block, 8 or 16 steps
- * per launch) over each level's smooth + flow and whole FED run, as the
+ * variant (LDS or register region shape, wavefront strips, threads per
+ * block, 8 or 16 steps per launch) over each level's smooth + flow and
+ * whole FED run, as the
  * pipeline runs them (the LDS variants build Lsmooth and the conductance in
  * their first launch), on the level's own buffers with batches of nz images,
  * and keeps the fastest per level size.  Overwrites the level contents; captured graphs are rebuilt on
- * the next call.  Variants are readable and settable per level (2..16). */
+ * the next call.  Variants are readable and settable per level (2..19). */
 akz_status akz_autotune(akz_ctx* ctx, int nz, int reps);
 akz_status akz_get_fed_variants(const akz_ctx* ctx, int* variants);
 akz_status akz_set_fed_variants(akz_ctx* ctx, const int* variants);
