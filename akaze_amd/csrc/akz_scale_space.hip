@@ -1298,9 +1298,10 @@ inline dim3 grid64x4(int w, int h, int nz) { return dim3((w + 63) / 64, (h + 3) 
 
 
 // k_contrast_front: 64 - 2 max(R0, 3) columns per wavefront (54 at least:
-// R0 <= 5) and TH rows: 32 for batches, fewer for one image, whose strips
-// would otherwise be too few to fill the GPU
-constexpr int CFR_TH = 32, CFR_TH1 = 8;
+// R0 <= 5) and TH rows: 32 for batches, 16 for one image (more strips to
+// fill the GPU; 8 rows repeat more halo rows, which costs more than it gains
+// with three images in flight)
+constexpr int CFR_TH = 32, CFR_TH1 = 16;
 template <int R0, int TH>
 static dim3 cfr_grid(int w, int h, int nz) {
   constexpr int TW = 64 - 2 * (R0 > 3 ? R0 : 3);
