@@ -280,7 +280,7 @@ akz_status allocate(akz_ctx* c) {
   const Level& L0 = c->lev[0];
   size_t n0 = (size_t)L0.pitch * (L0.h + 1);
   size_t o_img = cv.take<float>(n0), o_g = cv.take<float>(n0), o_tmp = cv.take<float>(n0);
-  size_t o_max = cv.take<uint32_t>(1), o_hist = cv.take<int32_t>(512 * 512);
+  size_t o_max = cv.take<uint32_t>(1), o_hist = cv.take<int32_t>(512);
   size_t o_bmax = cv.take<uint32_t>(akz_contrast_front_blocks(L0.w, L0.h));
   size_t o_kc = cv.take<float>(AKZ_MAX_LEVELS);
   // row segments of 64 pixels, level-major, raster order

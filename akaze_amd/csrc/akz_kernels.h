@@ -16,11 +16,11 @@ constexpr int AKZ_MAX_CLASSES_DEV = 16;  // size classes of a ragged batch (AKZ_
 // its sigma=1 blur (taps_c) into `grad` in one launch, with the tile maxima
 // in `blockmax` (akz_contrast_front_blocks(w, h) entries); then the
 // histogram and the percentile (bin totals included), leaving the per-level factor
-// in kc[0..nlev).  hist_part: 512 x 512 ints; maxbits: one word.
+// in kc[0..nlev).  hist_tot: 512 ints (the histogram); maxbits: one word.
 int akz_contrast_front_blocks(int w, int h);
 void akz_launch_contrast(hipStream_t s, const float* img, int in_pitch, size_t in_zs, float* lt, float* grad,
                          int pitch, int w, int h, int R0, const akz_taps& taps0, const akz_taps& taps_c, int nbins,
-                         float perc, uint32_t* maxbits, uint32_t* blockmax, int32_t* hist_part, float* kc, int nlev,
+                         float perc, uint32_t* maxbits, uint32_t* blockmax, int32_t* hist_tot, float* kc, int nlev,
                          const int* octaves_host, akz_batch bt);
 // FED kernel variant chosen for a level size and the images per launch
 int akz_fed_variant_for(int w, int h, int nz);

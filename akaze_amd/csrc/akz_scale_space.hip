@@ -93,7 +93,8 @@ __global__ __launch_bounds__(64) void k_contrast_front(const float* __restrict__
                                                            size_t in_zs, float* __restrict__ lt,
                                                            float* __restrict__ grad, int pitch, int w, int h,
                                                            akz_taps taps_c, akz_taps taps0,
-                                                           uint32_t* __restrict__ blockmax, size_t zs) {
+                                                           uint32_t* __restrict__ blockmax,
+                                                           int32_t* __restrict__ hist_tot, size_t zs) {
   constexpr int M = R0 > 3 ? R0 : 3, TW = 64 - 2 * M, NA = TH + 2 * M;
   constexpr int K0 = 2 * R0 + 1;
   const akz_tile3 tl = akz_xcd_tile();
@@ -101,6 +102,11 @@ __global__ __launch_bounds__(64) void k_contrast_front(const float* __restrict__
   lt = akz_zoff(lt, zs, tl.z);
   grad = akz_zoff(grad, zs, tl.z);
   blockmax = akz_zoff(blockmax, zs, tl.z);
+  // the image's histogram, which k_contrast_hist adds to, starts at zero
+  if (tl.x == 0 && tl.y == 0) {
+    int32_t* ht = akz_zoff(hist_tot, zs, tl.z);
+    for (int i = threadIdx.x; i < 512; i += 64) ht[i] = 0;
+  }
   const int lane = threadIdx.x;
   const int x0 = tl.x * TW - M, y0 = tl.y * TH;
   const int gx = x0 + lane;
@@ -196,11 +202,12 @@ constexpr int CT_HIST_THREADS = 512;
 // maximum (block 0 stores it for the percentile).  Bins are kept in 16
 // columns (LDS hist[bin][lane % 16]): at most four lanes of a wavefront add
 // to one counter, and the 19 KB table clears and sums four times faster
-// than one column per lane; each wavefront walks whole rows; a block stores
-// its bin totals in hist_part.
+// than one column per lane; each wavefront walks whole rows; a block adds
+// its bin totals to the image's histogram (integer counts: the sums do not
+// depend on the order), which k_contrast_front cleared.
 __global__ __launch_bounds__(CT_HIST_THREADS) void k_contrast_hist(const float* __restrict__ grad, int pitch,
                                                                    int w, int h, const uint32_t* blockmax, int nbm,
-                                                                   int nbins, int32_t* __restrict__ hist_part,
+                                                                   int nbins, int32_t* __restrict__ hist_tot,
                                                                    uint32_t* maxbits, size_t zs) {
   // 16 columns per bin, each shared by lanes l, l+16, l+32, l+48; bins >= 300
   // (nbins <= 512) go to `tail`
@@ -209,7 +216,7 @@ __global__ __launch_bounds__(CT_HIST_THREADS) void k_contrast_hist(const float* 
   __shared__ uint32_t s_w[CT_HIST_THREADS / 64];
   grad = akz_zoff(grad, zs);
   blockmax = akz_zoff(blockmax, zs);
-  hist_part = akz_zoff(hist_part, zs);
+  hist_tot = akz_zoff(hist_tot, zs);
   maxbits = akz_zoff(maxbits, zs);
   const int lane = threadIdx.x & 63, wv = akz_wave();
   constexpr int NW = CT_HIST_THREADS / 64;
@@ -270,56 +277,23 @@ __global__ __launch_bounds__(CT_HIST_THREADS) void k_contrast_hist(const float* 
     } else {
       sum = tail[i - 300];
     }
-    hist_part[(size_t)i * CT_HIST_BLOCKS + blockIdx.x] = sum;  // bin-major for the final reduction
+    if (sum != 0) atomicAdd(&hist_tot[i], sum);
   }
 }
 
-// Percentile search of ContrastPercentile (:637-642) over the summed block
-// histograms, on the device so no value round-trips through the host; the
+// Percentile search of ContrastPercentile (:637-642) over the image's
+// histogram, on the device so no value round-trips through the host; the
 // per-level factor decays by 0.75 at each new octave (AKAZE.cpp:254-256).
-// Thread i adds bin i over the nb block histograms (counts are integers: the
-// result does not depend on the order of the additions).
-__global__ __launch_bounds__(1024) void k_contrast_finish(const int32_t* __restrict__ hist_part, int nb, int nbins,
+__global__ __launch_bounds__(1024) void k_contrast_finish(const int32_t* __restrict__ hist_tot, int nbins,
                                                           const uint32_t* maxbits, int w, int h, float perc, float* kc,
                                                           akz_octaves octs, size_t zs) {
   __shared__ int32_t tot[512];
-  hist_part = akz_zoff(hist_part, zs);
+  hist_tot = akz_zoff(hist_tot, zs);
   maxbits = akz_zoff(maxbits, zs);
   kc = akz_zoff(kc, zs);
   const int lane = threadIdx.x & 63, wv = akz_wave();
-  // a bin's nb block totals are contiguous and 16-byte aligned; P = 1024 /
-  // nbins threads (at most 8) share a bin, each with up to 16 four-int loads
-  // in flight together (one round of memory latency for up to 64 x P
-  // blocks), and their partial sums meet in LDS
-  static_assert(CT_HIST_BLOCKS % 4 == 0, "aligned bin rows");
-  __shared__ int32_t part_sum[1024];
-  const int P = akz_imin(8, 1024 / akz_imax(nbins, 1));
-  {
-    const int bin = threadIdx.x / P, part = threadIdx.x % P;
-    int sum = 0;
-    if (bin < nbins) {
-      const int32_t* row = hist_part + (size_t)bin * CT_HIST_BLOCKS;
-      const int4* row4 = reinterpret_cast<const int4*>(row);
-      const int nq = nb >> 2;
-      constexpr int U4 = 16;
-      for (int q0 = part; q0 < nq; q0 += U4 * P) {
-        int4 v[U4];
-#pragma unroll
-        for (int u = 0; u < U4; u++) v[u] = q0 + u * P < nq ? row4[q0 + u * P] : make_int4(0, 0, 0, 0);
-#pragma unroll
-        for (int u = 0; u < U4; u++) sum += v[u].x + v[u].y + v[u].z + v[u].w;
-      }
-      if (part == 0)
-        for (int b = nq * 4; b < nb; b++) sum += row[b];
-    }
-    part_sum[threadIdx.x] = sum;
-  }
-  __syncthreads();
-  if ((int)threadIdx.x < nbins) {
-    int sum = 0;
-    for (int k = 0; k < P; k++) sum += part_sum[threadIdx.x * P + k];
-    tot[threadIdx.x] = sum;
-  }
+  // the bin totals k_contrast_hist's blocks added up
+  if ((int)threadIdx.x < nbins) tot[threadIdx.x] = hist_tot[threadIdx.x];
   __syncthreads();
   // percentile: the loop of ContrastPercentile stops at k = 1 + the first bin
   // whose cumulative count reaches nthreshold (k = 0 when nthreshold <= 0);
@@ -1311,7 +1285,7 @@ int akz_contrast_front_blocks(int w, int h) { return ((w + 53) / 54) * ((h + CFR
 
 void akz_launch_contrast(hipStream_t s, const float* img, int in_pitch, size_t in_zs, float* lt, float* grad,
                          int pitch, int w, int h, int R0, const akz_taps& taps0, const akz_taps& taps_c, int nbins,
-                         float perc, uint32_t* maxbits, uint32_t* blockmax, int32_t* hist_part, float* kc, int nlev,
+                         float perc, uint32_t* maxbits, uint32_t* blockmax, int32_t* hist_tot, float* kc, int nlev,
                          const int* octaves_host, akz_batch bt) {
   akz_octaves o;
   o.nlev = nlev;
@@ -1322,7 +1296,7 @@ void akz_launch_contrast(hipStream_t s, const float* img, int in_pitch, size_t i
 #define AKZ_CF(R, TH)                                                                                    \
   fgrid = cfr_grid<R, TH>(w, h, nz);                                                                     \
   hipLaunchKernelGGL((k_contrast_front<R, TH>), fgrid, dim3(64), 0, s, img, in_pitch, in_zs, lt, grad, \
-                     pitch, w, h, taps_c, taps0, blockmax, zs)
+                     pitch, w, h, taps_c, taps0, blockmax, hist_tot, zs)
 #define AKZ_CF2(R)    \
   if (nz > 1) {       \
     AKZ_CF(R, CFR_TH);  \
@@ -1343,8 +1317,8 @@ void akz_launch_contrast(hipStream_t s, const float* img, int in_pitch, size_t i
   const int hb = akz_imin((h - 2 + CT_HIST_THREADS / 64 - 1) / (CT_HIST_THREADS / 64),
                           akz_imax(CT_HIST_BLOCKS / per, 32));
   hipLaunchKernelGGL(k_contrast_hist, dim3(hb, 1, nz), dim3(CT_HIST_THREADS), 0, s, grad, pitch, w, h, blockmax,
-                     (int)(fgrid.x * fgrid.y), nbins, hist_part, maxbits, zs);
-  hipLaunchKernelGGL(k_contrast_finish, dim3(1, 1, nz), dim3(1024), 0, s, hist_part, hb, nbins, maxbits, w, h, perc,
+                     (int)(fgrid.x * fgrid.y), nbins, hist_tot, maxbits, zs);
+  hipLaunchKernelGGL(k_contrast_finish, dim3(1, 1, nz), dim3(1024), 0, s, hist_tot, nbins, maxbits, w, h, perc,
                      kc, o, zs);
 }
 
