@@ -428,9 +428,16 @@ akz_status allocate(akz_ctx* c) {
   return AKZ_OK;
 }
 
-// FED steps per launch at level i: the variant's, capped by the option
+// FED steps per launch at level i: at most the variant's (its halo), capped
+// by the option; a level needing several launches gets them balanced (21
+// steps as 7 + 7 + 7, not 8 + 8 + 5): the same launches with smaller halos.
 int level_block(const akz_ctx* c, int i) {
-  return c->fed_block == 1 ? 1 : std::min(c->fed_block, akz_fed_variant_steps(c->fed_variant[i]));
+  if (c->fed_block == 1) return 1;
+  const int cap = std::min(c->fed_block, akz_fed_variant_steps(c->fed_variant[i]));
+  const int n = c->lev[i].nsteps;
+  if (n <= cap) return cap;
+  const int nl = (n + cap - 1) / cap;
+  return (n + nl - 1) / nl;
 }
 
 // Whether level i's first FED launch forms the level's input, an exact
